@@ -1,0 +1,164 @@
+/*
+ * nextla_hip.h — C ABI of libnextla_hip.so, the MI355X (gfx950) implementation of
+ * NextLA.jl's tile-QR kernels and recursive TRSM/TRMM.
+ *
+ * Conventions (all entry points):
+ *  - Matrices are column-major DEVICE pointers with an explicit leading dimension,
+ *    exactly what a Julia view with stride(A,1) == 1 hands over (pointer(A), stride(A,2)).
+ *  - Sizes are int64_t.  Char flags take the reference's values ('L'/'R', 'N'/'C'/'T', ...).
+ *    For real types 'T' means the same as 'C' (src/larfb.jl:79-85, src/parfb.jl:96-100).
+ *  - `stream` is a hipStream_t passed as void*; NULL means the default stream.  Every
+ *    call is stream-ordered and returns without synchronising; a caller that needs the
+ *    reference's synchronous semantics calls nla_synchronize(stream).
+ *  - Return status: 0 = success; -i = the i-th argument is illegal (the reference would
+ *    throw ArgumentError; nla_last_error() returns the reference's message text);
+ *    > 0 = a HIP runtime error code.
+ *  - The caller owns every buffer.  Workspace arguments accept the reference's sizes;
+ *    the library never needs more than the reference asks for, and it keeps any extra
+ *    device scratch in a per-device pool of its own.
+ *  - Quick returns and k clamping are those of the reference routine cited on each line.
+ * 'd' = Float64, 's' = Float32.
+ */
+#ifndef NEXTLA_HIP_H
+#define NEXTLA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last non-zero status on this thread (the reference's ArgumentError text). */
+const char *nla_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int nla_version(void);
+/* hipStreamSynchronize(stream); returns a HIP error code or 0. */
+int nla_synchronize(void *stream);
+/* Bytes of device scratch the library holds on the current device. */
+int64_t nla_scratch_bytes(void);
+
+/* ---- Householder generation ------------------------------------------------------ */
+/* larfg!(n, alpha, x, incx, tau): src/larfg.jl:123-186.  alpha and tau are DEVICE scalars;
+ * on return *alpha = beta, x = v[2:n], *tau = tau.  (Julia shim returns (alpha, tau).) */
+int nla_dlarfg(int64_t n, double *alpha, double *x, int64_t incx, double *tau, void *stream);
+int nla_slarfg(int64_t n, float *alpha, float *x, int64_t incx, float *tau, void *stream);
+
+/* ---- Panel QR and T factor --------------------------------------------------------- */
+/* geqr2!(m, n, A, tau, work): src/geqr2.jl:107-121.  work (length n) is accepted, unused. */
+int nla_dgeqr2(int64_t m, int64_t n, double *A, int64_t lda, double *tau, double *work, void *stream);
+int nla_sgeqr2(int64_t m, int64_t n, float *A, int64_t lda, float *tau, float *work, void *stream);
+
+/* larft!(direct, storev, n, k, V, tau, T): src/larft.jl:205-213.  direct='F', storev='C'
+ * are implemented (the forms the QR path uses); the others return -1 / -2. */
+int nla_dlarft(char direct, char storev, int64_t n, int64_t k, const double *V, int64_t ldv,
+               const double *tau, double *T, int64_t ldt, void *stream);
+int nla_slarft(char direct, char storev, int64_t n, int64_t k, const float *V, int64_t ldv,
+               const float *tau, float *T, int64_t ldt, void *stream);
+
+/* geqrt!(m, n, ib, A, T, tau, work): src/geqrt.jl:22-68.  T is ib x min(m,n) (ldt >= ib). */
+int nla_dgeqrt(int64_t m, int64_t n, int64_t ib, double *A, int64_t lda, double *T, int64_t ldt,
+               double *tau, double *work, void *stream);
+int nla_sgeqrt(int64_t m, int64_t n, int64_t ib, float *A, int64_t lda, float *T, int64_t ldt,
+               float *tau, float *work, void *stream);
+
+/* ---- Block-reflector application ----------------------------------------------------- */
+/* larfb!(side, trans, direct, storev, m, n, k, V, ldv, T, C, work): src/larfb.jl:37-142.
+ * direct='F', storev='C' only, as in the reference (src/larfb.jl:42-44). */
+int nla_dlarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
+               const double *V, int64_t ldv, const double *T, int64_t ldt, double *C, int64_t ldc,
+               double *work, int64_t ldwork, void *stream);
+int nla_slarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
+               const float *V, int64_t ldv, const float *T, int64_t ldt, float *C, int64_t ldc,
+               float *work, int64_t ldwork, void *stream);
+
+/* unmqr!(side, trans, m, n, k, ib, A, lda, T, C, work): src/unmqr.jl:52-149.
+ * a_cols = size(A,2), used by the clamp k = min(k, nq, size(A,2)) (src/unmqr.jl:83). */
+int nla_dunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const double *A,
+               int64_t lda, int64_t a_cols, const double *T, int64_t ldt, double *C, int64_t ldc,
+               double *work, void *stream);
+int nla_sunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const float *A,
+               int64_t lda, int64_t a_cols, const float *T, int64_t ldt, float *C, int64_t ldc,
+               float *work, void *stream);
+
+/* ---- Triangle-on-square kernels --------------------------------------------------- */
+/* tsqrt!(m, n, ib, A1, A2, T, tau, work): src/tsqrt.jl:128-156.  m1 = size(A1,1);
+ * k = min(m1, n) reflectors (src/tsqrt.jl:139-140). */
+int nla_dtsqrt(int64_t m, int64_t n, int64_t ib, double *A1, int64_t lda1, int64_t m1, double *A2,
+               int64_t lda2, double *T, int64_t ldt, double *tau, double *work, void *stream);
+int nla_stsqrt(int64_t m, int64_t n, int64_t ib, float *A1, int64_t lda1, int64_t m1, float *A2,
+               int64_t lda2, float *T, int64_t ldt, float *tau, float *work, void *stream);
+
+/* tsmqr!(side, trans, m1, n1, m2, n2, k, ib, A1, A2, V, T, work): src/tsmqr.jl:47-146.
+ * v_cols = size(V,2), t_cols = size(T,2) for the clamp at src/tsmqr.jl:94-96. */
+int nla_dtsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k, int64_t ib,
+               double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V, int64_t ldv,
+               int64_t v_cols, const double *T, int64_t ldt, int64_t t_cols, double *work, void *stream);
+int nla_stsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k, int64_t ib,
+               float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V, int64_t ldv,
+               int64_t v_cols, const float *T, int64_t ldt, int64_t t_cols, float *work, void *stream);
+
+/* parfb!(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, A2, V, T, work):
+ * src/parfb.jl:44-191.  direct='F', storev='C', l=0 are implemented (the tsmqr! path);
+ * other forms return -3 / -4 / -10.  work is k x n2 (side L) or m2 x k (side R). */
+int nla_dparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2,
+               int64_t n2, int64_t k, int64_t l, double *A1, int64_t lda1, double *A2, int64_t lda2,
+               const double *V, int64_t ldv, const double *T, int64_t ldt, double *work, int64_t ldwork,
+               void *stream);
+int nla_sparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2,
+               int64_t n2, int64_t k, int64_t l, float *A1, int64_t lda1, float *A2, int64_t lda2,
+               const float *V, int64_t ldv, const float *T, int64_t ldt, float *work, int64_t ldwork,
+               void *stream);
+
+/* ---- Recursive TRSM / TRMM ------------------------------------------------------ */
+/* unified_rectrxm!(side, uplo, trans, alpha, func, A, B): src/rectrxm.jl:43-76.
+ * func 'S' = solve op(A) X = alpha B / X op(A) = alpha B, 'M' = B := alpha op(A) B / alpha B op(A).
+ * n = size(A,1); m = the other dimension of B (B is n x m for side 'L', m x n for 'R').
+ * cutoff = recursion threshold (src/rectrxm.jl:52,63); 0 selects the reference's
+ * 256 ('S') / 16 ('M').  Only the uplo triangle of A is read. */
+int nla_drectrxm(char side, char uplo, char trans, double alpha, char func, int64_t n, int64_t m,
+                 const double *A, int64_t lda, double *B, int64_t ldb, int64_t cutoff, void *stream);
+int nla_srectrxm(char side, char uplo, char trans, float alpha, char func, int64_t n, int64_t m,
+                 const float *A, int64_t lda, float *B, int64_t ldb, int64_t cutoff, void *stream);
+
+/* Base-case kernels LeftLowerTRSM!/LeftUpperTRSM!/RightLowerTRSM!/RightUpperTRSM!
+ * (src/trsm.jl:128-150) and the TRMM counterparts (src/trmm.jl:332-389): one diagonal
+ * block, B n x m (side 'L') or m x n (side 'R'); n <= 1024 as in src/trsm.jl:10-12. */
+int nla_dtrsm_base(char side, char uplo, int64_t n, int64_t m, const double *A, int64_t lda, double *B,
+                   int64_t ldb, void *stream);
+int nla_strsm_base(char side, char uplo, int64_t n, int64_t m, const float *A, int64_t lda, float *B,
+                   int64_t ldb, void *stream);
+int nla_dtrmm_base(char side, char uplo, int64_t n, int64_t m, const double *A, int64_t lda, double *B,
+                   int64_t ldb, void *stream);
+int nla_strmm_base(char side, char uplo, int64_t n, int64_t m, const float *A, int64_t lda, float *B,
+                   int64_t ldb, void *stream);
+
+/* GEMM_ADD!(A,B,C): C += A*B and GEMM_SUB!(A,B,C): A -= B*C (src/matmul.jl:69-81) as one
+ * primitive: C[N x M] += alpha * X[N x R] * Y[R x M].
+ * GEMM_ADD!(A,B,C) = nla_dgemm_acc(N=size(A,1), R=size(B,1), M=size(B,2),  1, A, B, C);
+ * GEMM_SUB!(A,B,C) = nla_dgemm_acc(N=size(A,1), R=size(C,1), M=size(A,2), -1, B, C, A). */
+int nla_dgemm_acc(int64_t N, int64_t R, int64_t M, double alpha, const double *X, int64_t ldx,
+                  const double *Y, int64_t ldy, double *C, int64_t ldc, void *stream);
+int nla_sgemm_acc(int64_t N, int64_t R, int64_t M, float alpha, const float *X, int64_t ldx,
+                  const float *Y, int64_t ldy, float *C, int64_t ldc, void *stream);
+
+/* ---- Tile-QR driver (no reference function: the composition of geqrt!/unmqr!/tsqrt!/
+ * tsmqr! that config "tile QR" describes, flat-tree PLASMA order) ------------------------- */
+/* Factor A (m x n, both multiples of nb) in nb x nb tiles with inner block ib.
+ * T receives one ib x nb block per tile: tile (i,k) at T[i*ib, k*nb], ldt >= (m/nb)*ib. */
+int nla_dtile_geqrf(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
+                    int64_t ldt, void *stream);
+int nla_stile_geqrf(int64_t m, int64_t n, int64_t nb, int64_t ib, float *A, int64_t lda, float *T,
+                    int64_t ldt, void *stream);
+/* Apply Q (trans 'N') or Q^T ('T'/'C') of nla_?tile_geqrf from the left to C (m x nrhs). */
+int nla_dtile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, int64_t ib,
+                    const double *A, int64_t lda, const double *T, int64_t ldt, double *C, int64_t ldc,
+                    void *stream);
+int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, int64_t ib,
+                    const float *A, int64_t lda, const float *T, int64_t ldt, float *C, int64_t ldc,
+                    void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEXTLA_HIP_H */

@@ -1,0 +1,183 @@
+// MFMA GEMM for the dense updates of the path: GEMM_ADD!/GEMM_SUB! of the recursive
+// TRSM/TRMM (src/matmul.jl:5-81) and the large products of the QR apply routines.
+//
+//   C[M x N] = alpha * op(A)[M x K] * op(B)[K x N] + beta * C     (column-major)
+//
+// Design (CDNA4, one 256-thread workgroup = 4 wave64s per BM x BN output tile):
+//  - operands are staged through LDS as k-major panels As[BK][BM], Bs[BK][BN]; the next
+//    BK slab is fetched into registers while the current one feeds the matrix cores;
+//  - each wave owns a (BM/2) x (BN/2) quadrant made of 16x16 MFMA blocks
+//    (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32);
+//  - the MFMA's A slot takes the op(B) fragment and its B slot the op(A) fragment, so the
+//    accumulator's fastest lane index runs down a column of C and stores coalesce;
+//  - workgroups are renumbered so that the tiles an XCD (one L2) executes are adjacent.
+#include "nla_common.hpp"
+#include "nla_gemm.hpp"
+
+namespace nla {
+
+template <typename T> struct GemmPad { static constexpr int v = 2; };
+
+template <typename T, int BM, int BN, int BK, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T *__restrict__ A,
+                                                   i64 lda, const T *__restrict__ B, i64 ldb, T beta,
+                                                   T *__restrict__ C, i64 ldc, int tiles_m) {
+    constexpr int PAD = GemmPad<T>::v;
+    constexpr int LA = (BM * BK) / 256;  // elements of the A slab per thread
+    constexpr int LB = (BK * BN) / 256;
+    constexpr int FM = BM / 32;          // 16-row blocks per wave (2 waves along M)
+    constexpr int FN = BN / 32;
+    __shared__ T As[BK][BM + PAD];
+    __shared__ T Bs[BK][BN + PAD];
+
+    // XCD-aware tile numbering: hardware deals workgroups round-robin over 8 XCDs.
+    int bid = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    const int tm = bid % tiles_m, tn = bid / tiles_m;
+    const int m0 = tm * BM, n0 = tn * BN;
+
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w & 1, wn = w >> 1;
+
+    T ra[LA], rb[LB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            int idx = t + 256 * i, m, k;
+            if (!TA) { m = idx % BM; k = idx / BM; } else { k = idx % BK; m = idx / BK; }
+            int gm = m0 + m, gk = k0 + k;
+            T v = 0;
+            if (gm < M && gk < K) v = TA ? A[gk + (i64)gm * lda] : A[gm + (i64)gk * lda];
+            ra[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            int idx = t + 256 * i, n, k;
+            if (!TB) { k = idx % BK; n = idx / BK; } else { n = idx % BN; k = idx / BN; }
+            int gn = n0 + n, gk = k0 + k;
+            T v = 0;
+            if (gn < N && gk < K) v = TB ? B[gn + (i64)gk * ldb] : B[gk + (i64)gn * ldb];
+            rb[i] = v;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < LA; ++i) {
+            int idx = t + 256 * i, m, k;
+            if (!TA) { m = idx % BM; k = idx / BM; } else { k = idx % BK; m = idx / BK; }
+            As[k][m] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            int idx = t + 256 * i, n, k;
+            if (!TB) { k = idx % BK; n = idx / BK; } else { n = idx % BN; k = idx / BN; }
+            Bs[k][n] = rb[i];
+        }
+    };
+
+    typename Acc<T>::t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_zero<T>();
+
+    const int lr = lane & 15, lk = lane >> 4;
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (k0 + BK < K) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) a[i] = As[kk + lk][wm * (BM / 2) + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) b[j] = Bs[kk + lk][wn * (BN / 2) + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) mfma16(acc[i][j], b[j], a[i]);
+        }
+    }
+
+    // acc[i][j] element r of lane l is C[m0 + wm*BM/2 + 16 i + (l%16), n0 + wn*BN/2 + 16 j + 4(l/16) + r]
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int gm = m0 + wm * (BM / 2) + i * 16 + lr;
+        if (gm >= M) continue;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gn = n0 + wn * (BN / 2) + j * 16 + 4 * lk + r;
+                if (gn < N) {
+                    T *c = &C[gm + (i64)gn * ldc];
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * *c;
+                    *c = v;
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int BM, int BN, int BK>
+static hipError_t launch_tiles(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, i64 lda, const T *B,
+                               i64 ldb, T beta, T *C, i64 ldc, hipStream_t s) {
+    const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+    dim3 grid(tm * tn), block(256);
+    if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else gemm_kernel<T, BM, BN, BK, true, true><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t gemm(char ta, char tb, i64 M, i64 N, i64 K, T alpha, const T *A, i64 lda, const T *B, i64 ldb, T beta,
+                T *C, i64 ldc, hipStream_t s) {
+    if (M <= 0 || N <= 0) return hipSuccess;
+    const bool TA = is_trans(ta), TB = is_trans(tb);
+    if (K <= 0) {
+        if (beta == T(1)) return hipSuccess;
+        // C = beta * C through a K=0 product: the kernel reads nothing from A/B.
+    }
+    const i64 big_tiles = ((M + 127) / 128) * ((N + 127) / 128);
+    if (big_tiles >= 512)
+        return launch_tiles<T, 128, 128, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch_tiles<T, 64, 64, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+template hipError_t gemm<double>(char, char, i64, i64, i64, double, const double *, i64, const double *, i64, double,
+                                 double *, i64, hipStream_t);
+template hipError_t gemm<float>(char, char, i64, i64, i64, float, const float *, i64, const float *, i64, float,
+                                float *, i64, hipStream_t);
+
+template <typename T>
+static int gemm_acc_entry(i64 N, i64 R, i64 M, T alpha, const T *X, i64 ldx, const T *Y, i64 ldy, T *C, i64 ldc,
+                          void *stream) {
+    if (N < 0) return arg_error(1, "illegal value of N: %lld", (long long)N);
+    if (R < 0) return arg_error(2, "illegal value of R: %lld", (long long)R);
+    if (M < 0) return arg_error(3, "illegal value of M: %lld", (long long)M);
+    if (N == 0 || M == 0 || R == 0) return 0;
+    if (ldx < N) return arg_error(6, "ldx must be >= N");
+    if (ldy < R) return arg_error(8, "ldy must be >= R");
+    if (ldc < N) return arg_error(10, "ldc must be >= N");
+    if (N > INT32_MAX || M > INT32_MAX || R > INT32_MAX) return arg_error(1, "dimension exceeds 2^31-1");
+    NLA_HIP(gemm<T>('N', 'N', N, M, R, alpha, X, ldx, Y, ldy, T(1), C, ldc, as_stream(stream)));
+    return 0;
+}
+
+}  // namespace nla
+
+NLA_API int nla_dgemm_acc(int64_t N, int64_t R, int64_t M, double alpha, const double *X, int64_t ldx,
+                          const double *Y, int64_t ldy, double *C, int64_t ldc, void *stream) {
+    return nla::gemm_acc_entry<double>(N, R, M, alpha, X, ldx, Y, ldy, C, ldc, stream);
+}
+NLA_API int nla_sgemm_acc(int64_t N, int64_t R, int64_t M, float alpha, const float *X, int64_t ldx,
+                          const float *Y, int64_t ldy, float *C, int64_t ldc, void *stream) {
+    return nla::gemm_acc_entry<float>(N, R, M, alpha, X, ldx, Y, ldy, C, ldc, stream);
+}

@@ -1,0 +1,818 @@
+// Tile-QR kernels of NextLA.jl on CDNA4: larfg!, geqr2!, larft!, larfb!, geqrt!, unmqr!,
+// tsqrt!, tsmqr!, parfb! (src/larfg.jl, src/geqr2.jl, src/larft.jl, src/larfb.jl,
+// src/geqrt.jl, src/unmqr.jl, src/tsqrt.jl, src/tsmqr.jl, src/parfb.jl).
+//
+// Two kernel families:
+//  * Panel kernels (one 256-thread workgroup): the column-by-column Householder work of
+//    geqr2!/tsqrt! with block reductions for the norms, the trailing in-panel update and
+//    the T column fused into one pass per column (the reference runs these on a single
+//    work-item, src/geqr2.jl:119, src/tsqrt.jl:148).
+//  * Strip kernels: a block reflector applied from the left acts on every column of C
+//    independently, so a workgroup owns a strip of CW columns and runs the *whole*
+//    unmqr!/tsmqr! block loop on it in one launch (the reference issues one larfb!/parfb!
+//    per ib-block, src/unmqr.jl:127, src/tsmqr.jl:123).  Right-side calls are the left
+//    form on the transposed view with trans flipped: C op(H) = (op(H)^T C^T)^T.
+//  Blocks wider than 64 reflectors are applied as consecutive 64-wide sub-blocks, whose
+//  T factors are the diagonal sub-blocks of the block's T (larft's column recurrence makes
+//  every leading/trailing diagonal block of T the T factor of those reflectors).
+#include "nla_common.hpp"
+#include "nla_qr.hpp"
+
+namespace nla {
+
+// ------------------------------------------------------------------------------------
+// larfg!: src/larfg.jl:123-186 (safe rescale loop kept; beta via hypot as LAPACK dlarfg).
+template <typename T> __device__ __forceinline__ T lamch_sfmin();
+template <> __device__ __forceinline__ double lamch_sfmin<double>() { return 2.2250738585072014e-308; }
+template <> __device__ __forceinline__ float lamch_sfmin<float>() { return 1.17549435e-38f; }
+template <typename T> __device__ __forceinline__ T lamch_eps();
+template <> __device__ __forceinline__ double lamch_eps<double>() { return 1.1102230246251565e-16; }
+template <> __device__ __forceinline__ float lamch_eps<float>() { return 5.96046448e-08f; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void larfg_kernel(int n, T *alpha, T *x, i64 incx, T *tau) {
+    __shared__ T red[16];
+    __shared__ T sh_scale;
+    const int t = threadIdx.x;
+    const int nx = n - 1;
+    // two-pass overflow-safe norm: max |x|, then sum (x/max)^2
+    T mx = 0;
+    for (int i = t; i < nx; i += 256) mx = fmax(mx, fabs(x[i * incx]));
+    mx = wave_max(mx);
+    __syncthreads();
+    if ((t & 63) == 0) red[t >> 6] = mx;
+    __syncthreads();
+    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    T ss = 0;
+    if (mx > 0)
+        for (int i = t; i < nx; i += 256) { T v = x[i * incx] / mx; ss += v * v; }
+    ss = block_sum(ss, red);
+    T xnorm = mx * sqrt(ss);
+    const T a0 = *alpha;
+    if (xnorm == T(0)) {
+        if (t == 0) *tau = 0;
+        return;
+    }
+    T alphr = a0;
+    T beta = -copysign(hypot(alphr, xnorm), alphr);
+    const T safmin = lamch_sfmin<T>() / lamch_eps<T>();
+    const T rsafmn = T(1) / safmin;
+    int knt = 0;
+    T a = a0;
+    if (fabs(beta) < safmin) {
+        while (fabs(beta) < safmin && knt < 20) {
+            ++knt;
+            beta *= rsafmn; alphr *= rsafmn; a *= rsafmn;
+        }
+        T f = 1;
+        for (int j = 0; j < knt; ++j) f *= rsafmn;
+        for (int i = t; i < nx; i += 256) x[i * incx] *= f;
+        __syncthreads();
+        mx = 0;
+        for (int i = t; i < nx; i += 256) mx = fmax(mx, fabs(x[i * incx]));
+        mx = wave_max(mx);
+        __syncthreads();
+        if ((t & 63) == 0) red[t >> 6] = mx;
+        __syncthreads();
+        mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        ss = 0;
+        for (int i = t; i < nx; i += 256) { T v = x[i * incx] / mx; ss += v * v; }
+        ss = block_sum(ss, red);
+        xnorm = mx * sqrt(ss);
+        beta = -copysign(hypot(alphr, xnorm), alphr);
+    }
+    if (t == 0) sh_scale = T(1) / (a - beta);
+    __syncthreads();
+    const T s = sh_scale;
+    for (int i = t; i < nx; i += 256) x[i * incx] *= s;
+    if (t == 0) {
+        *tau = (beta - alphr) / beta;
+        for (int j = 0; j < knt; ++j) beta *= safmin;
+        *alpha = beta;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Panel QR (geqr2! + larft!('F','C') fused).  geqr2_kernel! semantics (src/geqr2.jl:23-105):
+// plain sum-of-squares norm, beta = -sign(alpha) sqrt(alpha^2 + |x|^2), tau = (beta-alpha)/beta,
+// x /= (alpha - beta); the in-panel update uses w = A^T v with v(1) = 1.
+// If Tm != nullptr also writes T (larft_kernel! F/C, src/larft.jl:55-126): for column i,
+// T[0:i, i] = -tau_i * T[0:i,0:i] * (V^T v_i), T[i,i] = tau_i.  LAPACK's lastv trimming
+// only skips exact-zero products, so the dense sums here give the same T.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_qr_kernel(int m, int n, T *__restrict__ A, i64 lda, T *__restrict__ tau,
+                                                       T *__restrict__ Tm, i64 ldt) {
+    __shared__ T red[16];
+    __shared__ T sh[4];
+    extern __shared__ __align__(16) unsigned char dyn_raw[];
+    T *z = reinterpret_cast<T *>(dyn_raw);  // z[n]: v^T A[:, j] for every panel column j
+    T *tcol = z + n;                         // tcol[n] (only with Tm; n <= 256 then)
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int k = min(m, n);
+    for (int i = 0; i < k; ++i) {
+        const int len = m - i;
+        T *col = A + i + (i64)i * lda;  // column i from row i
+        T xs = 0;
+        for (int p = 1 + t; p < len; p += 256) { T v = col[p]; xs += v * v; }
+        xs = block_sum(xs, red);
+        if (t == 0) {
+            T alpha = col[0], tv = 0, beta = alpha, scale = 0;
+            if (len > 1 && xs != T(0)) {
+                beta = -copysign(sqrt(alpha * alpha + xs), alpha);
+                tv = (beta - alpha) / beta;
+                scale = T(1) / (alpha - beta);
+            }
+            sh[0] = tv; sh[1] = beta; sh[2] = scale;
+        }
+        __syncthreads();
+        const T tv = sh[0], beta = sh[1], scale = sh[2];
+        if (tv != T(0)) {
+            for (int p = 1 + t; p < len; p += 256) col[p] *= scale;
+        }
+        __syncthreads();
+        if (t == 0) { col[0] = beta; tau[i] = tv; }
+        // z_j = A[i, j] + sum_{p>=1} A[i+p, j] v_p for all j != i (j < i feeds T, j > i the update)
+        const int jlo = Tm ? 0 : i + 1;
+        if (tv != T(0)) {
+            for (int j = jlo + w; j < n; j += 4) {
+                if (j == i) continue;
+                const T *cj = A + i + (i64)j * lda;
+                T s = 0;
+                for (int p = 1 + lane; p < len; p += 64) s += cj[p] * col[p];
+                s = wave_sum(s);
+                if (lane == 0) z[j] = s + cj[0];
+            }
+        }
+        __syncthreads();
+        if (tv != T(0)) {
+            // trailing in-panel update A[i+p, j] -= tau v_p z_j
+            const int nc = n - i - 1;
+            for (int idx = t; idx < nc * len; idx += 256) {
+                const int p = idx % len, j = i + 1 + idx / len;
+                const T vp = p == 0 ? T(1) : col[p];
+                A[i + p + (i64)j * lda] -= tv * vp * z[j];
+            }
+        }
+        if (Tm) {
+            // T[0:i, i] = -tau * T[0:i, 0:i] * z[0:i]
+            if (t < i) tcol[t] = tv != T(0) ? -tv * z[t] : T(0);
+            __syncthreads();
+            if (t < i) {
+                T s = 0;
+                if (tv != T(0))
+                    for (int q = t; q < i; ++q) s += Tm[t + (i64)q * ldt] * tcol[q];
+                Tm[t + (i64)i * ldt] = s;
+            }
+            if (t == 0) Tm[i + (i64)i * ldt] = tv;
+        }
+        __syncthreads();
+    }
+    // rows/columns beyond k: geqr2 leaves tau[k..] untouched (k = min(m, n)).
+}
+
+// ------------------------------------------------------------------------------------
+// TS panel: one ib-block of tsqrt_block_kernel! (src/tsqrt.jl:46-126).
+// Column r = ii + i of [A1 (upper); A2 (m x n)]: alpha = A1[r,r], x = A2[:, r].
+template <typename T>
+__global__ __launch_bounds__(256) void ts_panel_kernel(int m, int ii, int sb, T *__restrict__ A1, i64 lda1,
+                                                       T *__restrict__ A2, i64 lda2, T *__restrict__ Tm, i64 ldt,
+                                                       T *__restrict__ tau) {
+    __shared__ T red[16];
+    __shared__ T z[256];     // block width sb <= 256
+    __shared__ T tcol[256];
+    __shared__ T sh[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int i = 0; i < sb; ++i) {
+        const int r = ii + i;
+        T *x = A2 + (i64)r * lda2;
+        T xs = 0;
+        for (int p = t; p < m; p += 256) { T v = x[p]; xs += v * v; }
+        xs = block_sum(xs, red);
+        if (t == 0) {
+            T alpha = A1[r + (i64)r * lda1], tv = 0, beta = alpha, scale = 0;
+            if (xs != T(0)) {
+                beta = -copysign(sqrt(alpha * alpha + xs), alpha);
+                tv = (beta - alpha) / beta;
+                scale = T(1) / (alpha - beta);
+            }
+            sh[0] = tv; sh[1] = beta; sh[2] = scale;
+        }
+        __syncthreads();
+        const T tv = sh[0], beta = sh[1], scale = sh[2];
+        if (tv != T(0))
+            for (int p = t; p < m; p += 256) x[p] *= scale;
+        __syncthreads();
+        if (t == 0) { A1[r + (i64)r * lda1] = beta; tau[r] = tv; }
+        // z_j = sum_q A2[q, ii+j] x[q] for block columns j != i; A1 row term only for j > i
+        if (tv != T(0)) {
+            for (int j = w; j < sb; j += 4) {
+                if (j == i) continue;
+                const T *cj = A2 + (i64)(ii + j) * lda2;
+                T s = 0;
+                for (int p = lane; p < m; p += 64) s += cj[p] * x[p];
+                s = wave_sum(s);
+                if (lane == 0) z[j] = s;
+            }
+        }
+        __syncthreads();
+        if (tv != T(0)) {
+            const int nc = sb - i - 1;
+            // A1[r, r+1+j] -= tau * (A1[r, r+1+j] + z) ; A2[:, r+1+j] -= tau x (A1 + z)
+            for (int idx = t; idx < nc * m; idx += 256) {
+                const int p = idx % m, j = i + 1 + idx / m;
+                const T wj = A1[r + (i64)(ii + j) * lda1] + z[j];
+                A2[p + (i64)(ii + j) * lda2] -= tv * x[p] * wj;
+            }
+            __syncthreads();
+            for (int j = i + 1 + t; j < sb; j += 256) {
+                T *a1 = &A1[r + (i64)(ii + j) * lda1];
+                *a1 -= tv * (*a1 + z[j]);
+            }
+        }
+        // T column: T[p, r] = -tau * sum_q T[p, ii+q] z_q  (p < i), T[i, r] = tau
+        if (t < i) tcol[t] = tv != T(0) ? -tv * z[t] : T(0);
+        __syncthreads();
+        if (t < i) {
+            T s = 0;
+            if (tv != T(0))
+                for (int q = t; q < i; ++q) s += Tm[t + (i64)(ii + q) * ldt] * tcol[q];
+            Tm[t + (i64)r * ldt] = s;
+        }
+        if (t == 0) Tm[i + (i64)r * ldt] = tv;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// larft!('F','C') standalone: T = upper k x k with T[0:i,i] = -tau_i T[0:i,0:i] V^T v_i.
+template <typename T>
+__global__ __launch_bounds__(256) void larft_kernel(int n, int k, const T *__restrict__ V, i64 ldv,
+                                                    const T *__restrict__ tau, T *__restrict__ Tm, i64 ldt) {
+    extern __shared__ __align__(16) unsigned char dyn_raw[];
+    T *z = reinterpret_cast<T *>(dyn_raw);  // z[k]
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int i = 0; i < k; ++i) {
+        const T ti = tau[i];
+        // z_p = V[i, p] + sum_{r > i} V[r, p] V[r, i] for p < i
+        for (int p = w; p < i; p += 4) {
+            T s = 0;
+            for (int r = i + 1 + lane; r < n; r += 64) s += V[r + (i64)p * ldv] * V[r + (i64)i * ldv];
+            s = wave_sum(s);
+            if (lane == 0) z[p] = ti == T(0) ? T(0) : -ti * (V[i + (i64)p * ldv] + s);
+        }
+        __syncthreads();
+        for (int p = t; p < i; p += 256) {
+            T s = 0;
+            if (ti != T(0))
+                for (int q = p; q < i; ++q) s += Tm[p + (i64)q * ldt] * z[q];
+            Tm[p + (i64)i * ldt] = s;
+        }
+        if (t == 0) Tm[i + (i64)i * ldt] = ti;
+        __syncthreads();
+    }
+}
+
+
+// ------------------------------------------------------------------------------------
+// Strip kernels.  View element (r, c) of a matrix stored at p with leading dimension ld:
+// TR ? p[c + r*ld] : p[r + c*ld].
+template <typename T, bool TR> __device__ __forceinline__ T &vat(T *p, i64 ld, int r, int c) {
+    return TR ? p[c + (i64)r * ld] : p[r + (i64)c * ld];
+}
+template <typename T, bool TR> __device__ __forceinline__ T vget(const T *p, i64 ld, int r, int c) {
+    return TR ? p[c + (i64)r * ld] : p[r + (i64)c * ld];
+}
+
+constexpr int CW = 16;    // strip width (columns of the left-form view)
+constexpr int KBM = 64;   // reflectors per applied sub-block
+constexpr int RC = 64;    // rows per staged chunk
+
+// Apply op(H) = I - Vh op(T) Vh^T of kb reflectors from the left to rows [Top (kb); Bot (mb)]
+// of the strip c0..c0+cw-1 (left-form views, TRC = stored transposed).
+//   GE: Vh = [V1 unit lower (kb x kb) ; V2 (mb x kb)]   (larfb!, src/larfb.jl:57-98)
+//   TS: Vh = [I ; V2 (mb x kb)]                         (parfb!, src/parfb.jl:121-129,189)
+// W0 = Vh^T C, W = op(T) W0 (op(T) = T for notrans, T^T otherwise), C -= Vh W.
+template <typename T, bool GE, bool TRC>
+__device__ void apply_strip(int kb, int mb, const T *__restrict__ V1, const T *__restrict__ V2, i64 ldv,
+                            const T *__restrict__ Tb, i64 ldt, bool notrans, T *Top, i64 ldtop, T *Bot, i64 ldbot,
+                            int c0, int cw, T *sV, T *sC, T *sW0, T *sW) {
+    const int t = threadIdx.x;
+    const int oc = t % CW, oj = t / CW;  // thread -> W0 entries (oj + 16 q, oc)
+    T acc[KBM / 16];
+#pragma unroll
+    for (int q = 0; q < KBM / 16; ++q) acc[q] = 0;
+    for (int r0 = 0; r0 < mb; r0 += RC) {
+        const int rc = min(RC, mb - r0);
+        __syncthreads();
+        for (int idx = t; idx < rc * kb; idx += 256) {
+            const int r = idx % rc, j = idx / rc;
+            sV[j * RC + r] = V2[(r0 + r) + (i64)j * ldv];
+        }
+        for (int idx = t; idx < rc * CW; idx += 256) {
+            int r, c;
+            if (TRC) { c = idx % CW; r = idx / CW; } else { r = idx % rc; c = idx / rc; }
+            sC[c * RC + r] = (c < cw) ? vget<T, TRC>(Bot, ldbot, r0 + r, c0 + c) : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KBM / 16; ++q) {
+            const int j = oj + 16 * q;
+            if (j < kb) {
+                T s = 0;
+                for (int r = 0; r < rc; ++r) s += sV[j * RC + r] * sC[oc * RC + r];
+                acc[q] += s;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KBM / 16; ++q) {
+        const int j = oj + 16 * q;
+        if (j < kb) {
+            T s = acc[q];
+            if (oc < cw) {
+                s += vget<T, TRC>(Top, ldtop, j, c0 + oc);
+                if (GE)
+                    for (int r = j + 1; r < kb; ++r) s += V1[r + (i64)j * ldv] * vget<T, TRC>(Top, ldtop, r, c0 + oc);
+            }
+            sW0[j * CW + oc] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KBM / 16; ++q) {
+        const int j = oj + 16 * q;
+        if (j < kb) {
+            T s = 0;
+            if (notrans) for (int p = j; p < kb; ++p) s += Tb[j + (i64)p * ldt] * sW0[p * CW + oc];
+            else         for (int p = 0; p <= j; ++p) s += Tb[p + (i64)j * ldt] * sW0[p * CW + oc];
+            sW[j * CW + oc] = s;
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < kb * cw; idx += 256) {
+        int r, c;
+        if (TRC) { c = idx % cw; r = idx / cw; } else { r = idx % kb; c = idx / kb; }
+        T s = sW[r * CW + c];
+        if (GE)
+            for (int j = 0; j < r; ++j) s += V1[r + (i64)j * ldv] * sW[j * CW + c];
+        vat<T, TRC>(Top, ldtop, r, c0 + c) -= s;
+    }
+    for (int idx = t; idx < mb * cw; idx += 256) {
+        int r, c;
+        if (TRC) { c = idx % cw; r = idx / cw; } else { r = idx % mb; c = idx / mb; }
+        T s = 0;
+        for (int j = 0; j < kb; ++j) s += V2[r + (i64)j * ldv] * sW[j * CW + c];
+        vat<T, TRC>(Bot, ldbot, r, c0 + c) -= s;
+    }
+    __syncthreads();
+}
+
+// unmqr!/larfb!/geqrt! trailing update (left form): k reflectors of V (unit lower, ldv) in
+// blocks of ib; block i acts on rows i..m-1 of C (m x ncols view).
+template <typename T, bool TRC>
+__global__ __launch_bounds__(256) void ge_apply_kernel(int m, int ncols, int k, int ib, const T *__restrict__ V,
+                                                       i64 ldv, const T *__restrict__ Tm, i64 ldt, bool notrans,
+                                                       bool forward, T *C, i64 ldc) {
+    __shared__ T sV[KBM * RC];
+    __shared__ T sC[CW * RC];
+    __shared__ T sW0[KBM * CW];
+    __shared__ T sW[KBM * CW];
+    const int c0 = blockIdx.x * CW;
+    const int cw = min(CW, ncols - c0);
+    const int nblk = (k + ib - 1) / ib;
+    for (int bb = 0; bb < nblk; ++bb) {
+        const int b = forward ? bb : nblk - 1 - bb;
+        const int i = b * ib;
+        const int kb = min(ib, k - i);
+        const int nsub = (kb + KBM - 1) / KBM;
+        for (int ss = 0; ss < nsub; ++ss) {
+            const int s = forward ? ss : nsub - 1 - ss;
+            const int j0 = s * KBM, kk = min(KBM, kb - j0);
+            const int row = i + j0;  // first row touched by this sub-block
+            const T *V1 = V + row + (i64)row * ldv;
+            const T *Tb = Tm + j0 + (i64)(i + j0) * ldt;
+            T *Top = TRC ? C + (i64)row * ldc : C + row;
+            T *Bot = TRC ? C + (i64)(row + kk) * ldc : C + row + kk;
+            apply_strip<T, true, TRC>(kk, m - row - kk, V1, V1 + kk, ldv, Tb, ldt, notrans, Top, ldc, Bot, ldc, c0, cw,
+                                      sV, sC, sW0, sW);
+        }
+    }
+}
+
+// tsmqr! (left form): k reflectors of V (m2 x k) in blocks of ib; block i updates rows
+// i..i+kb-1 of A1 and all m2 rows of A2 (parfb!, l = 0).
+template <typename T, bool TRC>
+__global__ __launch_bounds__(256) void ts_apply_kernel(int m2, int ncols, int k, int ib, const T *__restrict__ V,
+                                                       i64 ldv, const T *__restrict__ Tm, i64 ldt, bool notrans,
+                                                       bool forward, T *A1, i64 lda1, T *A2, i64 lda2) {
+    __shared__ T sV[KBM * RC];
+    __shared__ T sC[CW * RC];
+    __shared__ T sW0[KBM * CW];
+    __shared__ T sW[KBM * CW];
+    const int c0 = blockIdx.x * CW;
+    const int cw = min(CW, ncols - c0);
+    const int nblk = (k + ib - 1) / ib;
+    for (int bb = 0; bb < nblk; ++bb) {
+        const int b = forward ? bb : nblk - 1 - bb;
+        const int i = b * ib;
+        const int kb = min(ib, k - i);
+        const int nsub = (kb + KBM - 1) / KBM;
+        for (int ss = 0; ss < nsub; ++ss) {
+            const int s = forward ? ss : nsub - 1 - ss;
+            const int j0 = s * KBM, kk = min(KBM, kb - j0);
+            const int row = i + j0;
+            const T *Vb = V + (i64)row * ldv;
+            const T *Tb = Tm + j0 + (i64)(i + j0) * ldt;
+            T *Top = TRC ? A1 + (i64)row * lda1 : A1 + row;
+            apply_strip<T, false, TRC>(kk, m2, nullptr, Vb, ldv, Tb, ldt, notrans, Top, lda1, A2, lda2, c0, cw, sV, sC,
+                                       sW0, sW);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Host launchers (no validation; used by the entry points below and by the tile driver).
+template <typename T>
+hipError_t launch_panel_qr(i64 m, i64 n, T *A, i64 lda, T *tau, T *Tm, i64 ldt, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    size_t sh = (size_t)n * sizeof(T) * (Tm ? 2 : 1);
+    panel_qr_kernel<T><<<1, 256, sh, s>>>((int)m, (int)n, A, lda, tau, Tm, ldt);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ts_panel(i64 m, i64 ii, i64 sb, T *A1, i64 lda1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau,
+                           hipStream_t s) {
+    ts_panel_kernel<T><<<1, 256, 0, s>>>((int)m, (int)ii, (int)sb, A1, lda1, A2, lda2, Tm, ldt, tau);
+    return hipGetLastError();
+}
+
+// Left-form GE apply on an m x ncols view (trc: C stored transposed).
+template <typename T>
+hipError_t launch_ge_apply(bool trc, i64 m, i64 ncols, i64 k, i64 ib, const T *V, i64 ldv, const T *Tm, i64 ldt,
+                           bool notrans, bool forward, T *C, i64 ldc, hipStream_t s) {
+    if (m <= 0 || ncols <= 0 || k <= 0) return hipSuccess;
+    dim3 grid((unsigned)((ncols + CW - 1) / CW)), block(256);
+    if (trc) ge_apply_kernel<T, true><<<grid, block, 0, s>>>((int)m, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, C, ldc);
+    else ge_apply_kernel<T, false><<<grid, block, 0, s>>>((int)m, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, C, ldc);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ts_apply(bool trc, i64 m2, i64 ncols, i64 k, i64 ib, const T *V, i64 ldv, const T *Tm, i64 ldt,
+                           bool notrans, bool forward, T *A1, i64 lda1, T *A2, i64 lda2, hipStream_t s) {
+    if (m2 <= 0 || ncols <= 0 || k <= 0) return hipSuccess;
+    dim3 grid((unsigned)((ncols + CW - 1) / CW)), block(256);
+    if (trc) ts_apply_kernel<T, true><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2);
+    else ts_apply_kernel<T, false><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2);
+    return hipGetLastError();
+}
+
+// geqrt! body: src/geqrt.jl:44-67 (geqr2 + larft fused in the panel kernel, then larfb).
+template <typename T>
+hipError_t geqrt_async(i64 m, i64 n, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, T *tau, hipStream_t s) {
+    const i64 k = m < n ? m : n;
+    for (i64 i = 0; i < k; i += ib) {
+        const i64 sb = ib < k - i ? ib : k - i;
+        T *panel = A + i + i * lda;
+        hipError_t e = launch_panel_qr<T>(m - i, sb, panel, lda, tau + i, Tm + i * ldt, ldt, s);
+        if (e != hipSuccess) return e;
+        if (n > i + sb) {
+            // larfb!('L','C','F','C', m-i, n-i-sb, sb, ...)
+            e = launch_ge_apply<T>(false, m - i, n - i - sb, sb, sb, panel, lda, Tm + i * ldt, ldt, false, true,
+                                   A + i + (i + sb) * lda, lda, s);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+// tsqrt! body: src/tsqrt.jl:146-155.
+template <typename T>
+hipError_t tsqrt_async(i64 m, i64 n, i64 ib, i64 k, T *A1, i64 lda1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau,
+                       hipStream_t s) {
+    for (i64 ii = 0; ii < k; ii += ib) {
+        const i64 sb = ib < k - ii ? ib : k - ii;
+        hipError_t e = launch_ts_panel<T>(m, ii, sb, A1, lda1, A2, lda2, Tm, ldt, tau, s);
+        if (e != hipSuccess) return e;
+        if (n > ii + sb) {
+            // tsmqr!('L','C', sb, n-ii-sb, m, n-ii-sb, sb, ib, A1[ii:, ii+sb:], A2[:, ii+sb:], A2[:, ii:], T[:, ii:])
+            e = launch_ts_apply<T>(false, m, n - ii - sb, sb, ib, A2 + ii * lda2, lda2, Tm + ii * ldt, ldt, false, true,
+                                   A1 + ii + (ii + sb) * lda1, lda1, A2 + (ii + sb) * lda2, lda2, s);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+template hipError_t launch_ge_apply<double>(bool, i64, i64, i64, i64, const double *, i64, const double *, i64, bool,
+                                            bool, double *, i64, hipStream_t);
+template hipError_t launch_ge_apply<float>(bool, i64, i64, i64, i64, const float *, i64, const float *, i64, bool, bool,
+                                           float *, i64, hipStream_t);
+template hipError_t launch_ts_apply<double>(bool, i64, i64, i64, i64, const double *, i64, const double *, i64, bool,
+                                            bool, double *, i64, double *, i64, hipStream_t);
+template hipError_t launch_ts_apply<float>(bool, i64, i64, i64, i64, const float *, i64, const float *, i64, bool, bool,
+                                           float *, i64, float *, i64, hipStream_t);
+template hipError_t geqrt_async<double>(i64, i64, i64, double *, i64, double *, i64, double *, hipStream_t);
+template hipError_t geqrt_async<float>(i64, i64, i64, float *, i64, float *, i64, float *, hipStream_t);
+template hipError_t tsqrt_async<double>(i64, i64, i64, i64, double *, i64, double *, i64, double *, i64, double *,
+                                        hipStream_t);
+template hipError_t tsqrt_async<float>(i64, i64, i64, i64, float *, i64, float *, i64, float *, i64, float *,
+                                       hipStream_t);
+
+// ------------------------------------------------------------------------------------
+// Entry points: validation mirrors the reference's ArgumentError checks (file:line on each).
+static bool valid_trans(char t) { return t == 'N' || t == 'C' || t == 'T'; }
+
+template <typename T> static int larfg_entry(i64 n, T *alpha, T *x, i64 incx, T *tau, void *stream) {
+    hipStream_t s = as_stream(stream);
+    if (n <= 1) {  // src/larfg.jl:127-130
+        NLA_HIP(hipMemsetAsync(tau, 0, sizeof(T), s));
+        return 0;
+    }
+    if (incx <= 0) return arg_error(4, "incx must be positive, got %lld", (long long)incx);
+    if (n > INT32_MAX) return arg_error(1, "n exceeds 2^31-1");
+    larfg_kernel<T><<<1, 256, 0, s>>>((int)n, alpha, x, incx, tau);
+    NLA_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename T> static int geqr2_entry(i64 m, i64 n, T *A, i64 lda, T *tau, void *stream) {
+    if (m < 0) return arg_error(1, "illegal value of m: %lld", (long long)m);   // src/geqr2.jl:108-110
+    if (n < 0) return arg_error(2, "illegal value of n: %lld", (long long)n);   // src/geqr2.jl:111-113
+    if (m == 0 || n == 0) return 0;
+    if (lda < m) return arg_error(4, "lda must be >= max(1,m)");
+    if (n > 8192) return arg_error(2, "geqr2 panel holds at most 8192 columns, got %lld", (long long)n);
+    if (m > INT32_MAX) return arg_error(1, "m exceeds 2^31-1");
+    NLA_HIP(launch_panel_qr<T>(m, n, A, lda, tau, (T *)nullptr, 0, as_stream(stream)));
+    return 0;
+}
+
+template <typename T>
+static int larft_entry(char direct, char storev, i64 n, i64 k, const T *V, i64 ldv, const T *tau, T *Tm, i64 ldt,
+                       void *stream) {
+    direct = up(direct); storev = up(storev);
+    if (n == 0) return 0;  // src/larft.jl:206-208
+    if (direct != 'F' && direct != 'B') return arg_error(1, "direct must be 'F' or 'B', got '%c'", direct);
+    if (storev != 'C' && storev != 'R') return arg_error(2, "storev must be 'C' or 'R', got '%c'", storev);
+    if (direct != 'F') return arg_error(1, "direct='B' is not implemented (the QR path uses 'F')");
+    if (storev != 'C') return arg_error(2, "storev='R' is not implemented (the QR path uses 'C')");
+    if (n < 0) return arg_error(3, "n must be non-negative");
+    if (k < 0) return arg_error(4, "k must be non-negative");
+    if (k == 0) return 0;
+    if (ldv < n) return arg_error(6, "ldv must be >= n");
+    if (ldt < k) return arg_error(9, "ldt must be >= k");
+    if (k > 4096) return arg_error(4, "k must be <= 4096");
+    larft_kernel<T><<<1, 256, (size_t)k * sizeof(T), as_stream(stream)>>>((int)n, (int)k, V, ldv, tau, Tm, ldt);
+    NLA_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename T>
+static int larfb_entry(char side, char trans, char direct, char storev, i64 m, i64 n, i64 k, const T *V, i64 ldv,
+                       const T *Tm, i64 ldt, T *C, i64 ldc, void *stream) {
+    side = up(side); trans = up(trans); direct = up(direct); storev = up(storev);
+    if (m <= 0 || n <= 0 || k <= 0) return 0;  // src/larfb.jl:38-40
+    if (storev != 'C' || direct != 'F')         // src/larfb.jl:42-44
+        return arg_error(storev != 'C' ? 4 : 3, "larfb! currently supports only forward, columnwise reflectors");
+    if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);
+    if (!valid_trans(trans)) return arg_error(2, "trans must be 'N', 'C', or 'T', got '%c'", trans);
+    const i64 q = side == 'L' ? m : n;
+    if (k > q) return arg_error(7, "k must be <= %lld", (long long)q);
+    if (ldv < q) return arg_error(9, "ldv must be >= %lld", (long long)q);
+    if (ldt < k) return arg_error(11, "ldt must be >= k");
+    if (ldc < m) return arg_error(13, "ldc must be >= m");
+    const bool notrans = trans == 'N';
+    hipStream_t s = as_stream(stream);
+    if (side == 'L') NLA_HIP(launch_ge_apply<T>(false, m, n, k, k, V, ldv, Tm, ldt, notrans, true, C, ldc, s));
+    else NLA_HIP(launch_ge_apply<T>(true, n, m, k, k, V, ldv, Tm, ldt, !notrans, true, C, ldc, s));
+    return 0;
+}
+
+template <typename T>
+static int geqrt_entry(i64 m, i64 n, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, T *tau, void *stream) {
+    if (m < 0) return arg_error(1, "illegal value of m: %lld", (long long)m);   // src/geqrt.jl:24-26
+    if (n < 0) return arg_error(2, "illegal value of n: %lld", (long long)n);   // src/geqrt.jl:28-30
+    if (ib < 0 || (ib == 0 && m > 0 && n > 0))                                 // src/geqrt.jl:32-34
+        return arg_error(3, "illegal value of ib: %lld", (long long)ib);
+    if (m == 0 || n == 0 || ib == 0) return 0;
+    if (lda < m) return arg_error(5, "lda must be >= max(1,m)");
+    const i64 k = m < n ? m : n;
+    if (ldt < (ib < k ? ib : k)) return arg_error(7, "ldt must be >= min(ib, min(m,n))");
+    if (ib > 256) return arg_error(3, "ib must be <= 256 (one panel workgroup), got %lld", (long long)ib);
+    if (m > INT32_MAX || n > INT32_MAX) return arg_error(1, "dimension exceeds 2^31-1");
+    NLA_HIP(geqrt_async<T>(m, n, ib, A, lda, Tm, ldt, tau, as_stream(stream)));
+    return 0;
+}
+
+template <typename T>
+static int unmqr_entry(char side, char trans, i64 m, i64 n, i64 k, i64 ib, const T *A, i64 lda, i64 a_cols,
+                       const T *Tm, i64 ldt, T *C, i64 ldc, void *stream) {
+    side = up(side); trans = up(trans);
+    if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);  // src/unmqr.jl:54-56
+    const i64 nq = side == 'L' ? m : n;
+    if (!valid_trans(trans)) return arg_error(2, "trans must be 'N', 'C', or 'T', got '%c'", trans);  // :66-68
+    if (m < 0) return arg_error(3, "m must be non-negative, got %lld", (long long)m);                 // :70-72
+    if (n < 0) return arg_error(4, "n must be non-negative, got %lld", (long long)n);                 // :74-76
+    if (k < 0) return arg_error(5, "k must be non-negative, got %lld", (long long)k);                 // :78-80
+    if (k > nq) k = nq;                                                                                  // :83
+    if (k > a_cols) k = a_cols;
+    if (ib < 0) return arg_error(6, "ib must be non-negative, got %lld", (long long)ib);               // :85-87
+    if (lda < (nq > 1 ? nq : 1) && nq > 0)                                                               // :89-91
+        return arg_error(8, "lda must be >= max(1,%lld), got %lld", (long long)nq, (long long)lda);
+    if (m == 0 || n == 0 || k == 0) return 0;                                                            // :94-96
+    if (ib == 0) return arg_error(6, "step cannot be zero");  // the block loop at src/unmqr.jl:127 has step ib
+    if (ldt < (ib < k ? ib : k)) return arg_error(11, "ldt must be >= min(ib, k)");
+    if (ldc < m) return arg_error(13, "ldc must be >= m");
+    const bool notrans = trans == 'N';
+    const bool fwd = (side == 'L' && !notrans) || (side == 'R' && notrans);  // src/unmqr.jl:99-109
+    hipStream_t s = as_stream(stream);
+    if (side == 'L') NLA_HIP(launch_ge_apply<T>(false, m, n, k, ib, A, lda, Tm, ldt, notrans, fwd, C, ldc, s));
+    else NLA_HIP(launch_ge_apply<T>(true, n, m, k, ib, A, lda, Tm, ldt, !notrans, fwd, C, ldc, s));
+    return 0;
+}
+
+template <typename T>
+static int tsqrt_entry(i64 m, i64 n, i64 ib, T *A1, i64 lda1, i64 m1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau,
+                       void *stream) {
+    if (m < 0) return arg_error(1, "m must be non-negative, got %lld", (long long)m);    // src/tsqrt.jl:129-131
+    if (n < 0) return arg_error(2, "n must be non-negative, got %lld", (long long)n);    // :132-134
+    if (ib < 0) return arg_error(3, "ib must be non-negative, got %lld", (long long)ib); // :135-137
+    const i64 k = m1 < n ? m1 : n;                                                         // :139-140
+    if (m == 0 || n == 0 || ib == 0 || k <= 0) return 0;                                   // :141-143
+    if (lda1 < m1) return arg_error(5, "lda1 must be >= m1");
+    if (lda2 < m) return arg_error(8, "lda2 must be >= m");
+    if (ldt < (ib < k ? ib : k)) return arg_error(10, "ldt must be >= min(ib, k)");
+    if (ib > 256) return arg_error(3, "ib must be <= 256 (one panel workgroup), got %lld", (long long)ib);
+    if (m > INT32_MAX || n > INT32_MAX) return arg_error(1, "dimension exceeds 2^31-1");
+    NLA_HIP(tsqrt_async<T>(m, n, ib, k, A1, lda1, A2, lda2, Tm, ldt, tau, as_stream(stream)));
+    return 0;
+}
+
+template <typename T>
+static int tsmqr_entry(char side, char trans, i64 m1, i64 n1, i64 m2, i64 n2, i64 k, i64 ib, T *A1, i64 lda1, T *A2,
+                       i64 lda2, const T *V, i64 ldv, i64 v_cols, const T *Tm, i64 ldt, i64 t_cols, void *stream) {
+    side = up(side); trans = up(trans);
+    if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);  // src/tsmqr.jl:51-53
+    if (!valid_trans(trans)) return arg_error(2, "trans must be 'N', 'C', or 'T', got '%c'", trans);  // :55-57
+    if (m1 < 0) return arg_error(3, "m1 must be non-negative, got %lld", (long long)m1);              // :59-61
+    if (n1 < 0) return arg_error(4, "n1 must be non-negative, got %lld", (long long)n1);              // :63-65
+    if (m2 < 0 || (m2 != m1 && side == 'R')) {                                                        // :67-73
+        if (side == 'R')
+            return arg_error(5, "For side='R', m2 must equal m1. Got m1=%lld, m2=%lld", (long long)m1, (long long)m2);
+        return arg_error(5, "m2 must be non-negative, got %lld", (long long)m2);
+    }
+    if (n2 < 0 || (n2 != n1 && side == 'L')) {                                                        // :75-81
+        if (side == 'L')
+            return arg_error(6, "For side='L', n2 must equal n1. Got n1=%lld, n2=%lld", (long long)n1, (long long)n2);
+        return arg_error(6, "n2 must be non-negative, got %lld", (long long)n2);
+    }
+    if (k < 0) return arg_error(7, "k must be non-negative, got %lld", (long long)k);                 // :83-85
+    if (ib < 0) return arg_error(8, "ib must be non-negative, got %lld", (long long)ib);              // :87-89
+    const i64 kmax = side == 'L' ? m1 : n1;                                                           // :93-96
+    const i64 stored = v_cols < t_cols ? v_cols : t_cols;
+    if (k > kmax) k = kmax;
+    if (k > stored) k = stored;
+    if (m1 == 0 || n1 == 0 || m2 == 0 || n2 == 0 || k <= 0 || ib == 0) return 0;                      // :99-101
+    if (lda1 < m1) return arg_error(10, "lda1 must be >= m1");
+    if (lda2 < m2) return arg_error(12, "lda2 must be >= m2");
+    const i64 vr = side == 'L' ? m2 : n2;
+    if (ldv < vr) return arg_error(14, "ldv must be >= %lld", (long long)vr);
+    if (ldt < (ib < k ? ib : k)) return arg_error(17, "ldt must be >= min(ib, k)");
+    const bool notrans = trans == 'N';
+    const bool fwd = (side == 'L' && !notrans) || (side == 'R' && notrans);  // src/tsmqr.jl:104-114
+    hipStream_t s = as_stream(stream);
+    if (side == 'L')
+        NLA_HIP(launch_ts_apply<T>(false, m2, n1, k, ib, V, ldv, Tm, ldt, notrans, fwd, A1, lda1, A2, lda2, s));
+    else
+        NLA_HIP(launch_ts_apply<T>(true, n2, m1, k, ib, V, ldv, Tm, ldt, !notrans, fwd, A1, lda1, A2, lda2, s));
+    return 0;
+}
+
+template <typename T>
+static int parfb_entry(char side, char trans, char direct, char storev, i64 m1, i64 n1, i64 m2, i64 n2, i64 k, i64 l,
+                       T *A1, i64 lda1, T *A2, i64 lda2, const T *V, i64 ldv, const T *Tm, i64 ldt, void *stream) {
+    side = up(side); trans = up(trans); direct = up(direct); storev = up(storev);
+    if (side != 'L' && side != 'R') return arg_error(1, "illegal value of side");          // src/parfb.jl:47-49
+    if (!valid_trans(trans)) return arg_error(2, "illegal value of trans");                 // :51-53
+    if (direct != 'F' && direct != 'B') return arg_error(3, "illegal value of direct");     // :55-57
+    if (storev != 'C' && storev != 'R') return arg_error(4, "illegal value of storev");     // :59-61
+    if (m1 < 0) return arg_error(5, "illegal value of m1");                                  // :63-65
+    if (n1 < 0) return arg_error(6, "illegal value of n1");                                  // :67-69
+    if (m2 < 0 || (side == 'R' && m1 != m2)) return arg_error(7, "illegal value of m2");     // :71-73
+    if (n2 < 0 || (side == 'L' && n1 != n2)) return arg_error(8, "illegal value of n2");     // :75-77
+    if (k < 0) return arg_error(9, "illegal value of k");                                    // :79-81
+    if (l < 0 || l > k) return arg_error(10, "illegal value of l");                          // :83-85
+    if (m1 == 0 || n1 == 0 || n2 == 0 || k == 0) return 0;                                   // :87-89
+    if (direct != 'F') return arg_error(3, "direct='B' is not implemented (tsmqr! uses 'F')");
+    if (storev != 'C') return arg_error(4, "storev='R' is not implemented (tsmqr! uses 'C')");
+    if (l != 0) return arg_error(10, "l > 0 (triangular V, ttmqr!) is not implemented");
+    if (m2 == 0) return 0;
+    if (lda1 < m1) return arg_error(12, "lda1 must be >= m1");
+    if (lda2 < m2) return arg_error(14, "lda2 must be >= m2");
+    const bool notrans = trans == 'N';
+    hipStream_t s = as_stream(stream);
+    // one block of k reflectors: side L uses rows 0..k-1 of A1, side R columns 0..k-1
+    if (side == 'L') NLA_HIP(launch_ts_apply<T>(false, m2, n2, k, k, V, ldv, Tm, ldt, notrans, true, A1, lda1, A2, lda2, s));
+    else NLA_HIP(launch_ts_apply<T>(true, n2, m2, k, k, V, ldv, Tm, ldt, !notrans, true, A1, lda1, A2, lda2, s));
+    return 0;
+}
+
+}  // namespace nla
+
+using namespace nla;
+
+NLA_API int nla_dlarfg(int64_t n, double *alpha, double *x, int64_t incx, double *tau, void *stream) {
+    return larfg_entry<double>(n, alpha, x, incx, tau, stream);
+}
+NLA_API int nla_slarfg(int64_t n, float *alpha, float *x, int64_t incx, float *tau, void *stream) {
+    return larfg_entry<float>(n, alpha, x, incx, tau, stream);
+}
+NLA_API int nla_dgeqr2(int64_t m, int64_t n, double *A, int64_t lda, double *tau, double *work, void *stream) {
+    (void)work;
+    return geqr2_entry<double>(m, n, A, lda, tau, stream);
+}
+NLA_API int nla_sgeqr2(int64_t m, int64_t n, float *A, int64_t lda, float *tau, float *work, void *stream) {
+    (void)work;
+    return geqr2_entry<float>(m, n, A, lda, tau, stream);
+}
+NLA_API int nla_dlarft(char direct, char storev, int64_t n, int64_t k, const double *V, int64_t ldv, const double *tau,
+                       double *T, int64_t ldt, void *stream) {
+    return larft_entry<double>(direct, storev, n, k, V, ldv, tau, T, ldt, stream);
+}
+NLA_API int nla_slarft(char direct, char storev, int64_t n, int64_t k, const float *V, int64_t ldv, const float *tau,
+                       float *T, int64_t ldt, void *stream) {
+    return larft_entry<float>(direct, storev, n, k, V, ldv, tau, T, ldt, stream);
+}
+NLA_API int nla_dgeqrt(int64_t m, int64_t n, int64_t ib, double *A, int64_t lda, double *T, int64_t ldt, double *tau,
+                       double *work, void *stream) {
+    (void)work;
+    return geqrt_entry<double>(m, n, ib, A, lda, T, ldt, tau, stream);
+}
+NLA_API int nla_sgeqrt(int64_t m, int64_t n, int64_t ib, float *A, int64_t lda, float *T, int64_t ldt, float *tau,
+                       float *work, void *stream) {
+    (void)work;
+    return geqrt_entry<float>(m, n, ib, A, lda, T, ldt, tau, stream);
+}
+NLA_API int nla_dlarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
+                       const double *V, int64_t ldv, const double *T, int64_t ldt, double *C, int64_t ldc, double *work,
+                       int64_t ldwork, void *stream) {
+    (void)work; (void)ldwork;
+    return larfb_entry<double>(side, trans, direct, storev, m, n, k, V, ldv, T, ldt, C, ldc, stream);
+}
+NLA_API int nla_slarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
+                       const float *V, int64_t ldv, const float *T, int64_t ldt, float *C, int64_t ldc, float *work,
+                       int64_t ldwork, void *stream) {
+    (void)work; (void)ldwork;
+    return larfb_entry<float>(side, trans, direct, storev, m, n, k, V, ldv, T, ldt, C, ldc, stream);
+}
+NLA_API int nla_dunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const double *A, int64_t lda,
+                       int64_t a_cols, const double *T, int64_t ldt, double *C, int64_t ldc, double *work,
+                       void *stream) {
+    (void)work;
+    return unmqr_entry<double>(side, trans, m, n, k, ib, A, lda, a_cols, T, ldt, C, ldc, stream);
+}
+NLA_API int nla_sunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const float *A, int64_t lda,
+                       int64_t a_cols, const float *T, int64_t ldt, float *C, int64_t ldc, float *work, void *stream) {
+    (void)work;
+    return unmqr_entry<float>(side, trans, m, n, k, ib, A, lda, a_cols, T, ldt, C, ldc, stream);
+}
+NLA_API int nla_dtsqrt(int64_t m, int64_t n, int64_t ib, double *A1, int64_t lda1, int64_t m1, double *A2,
+                       int64_t lda2, double *T, int64_t ldt, double *tau, double *work, void *stream) {
+    (void)work;
+    return tsqrt_entry<double>(m, n, ib, A1, lda1, m1, A2, lda2, T, ldt, tau, stream);
+}
+NLA_API int nla_stsqrt(int64_t m, int64_t n, int64_t ib, float *A1, int64_t lda1, int64_t m1, float *A2, int64_t lda2,
+                       float *T, int64_t ldt, float *tau, float *work, void *stream) {
+    (void)work;
+    return tsqrt_entry<float>(m, n, ib, A1, lda1, m1, A2, lda2, T, ldt, tau, stream);
+}
+NLA_API int nla_dtsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k, int64_t ib,
+                       double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V, int64_t ldv,
+                       int64_t v_cols, const double *T, int64_t ldt, int64_t t_cols, double *work, void *stream) {
+    (void)work;
+    return tsmqr_entry<double>(side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, v_cols, T, ldt, t_cols,
+                               stream);
+}
+NLA_API int nla_stsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k, int64_t ib,
+                       float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V, int64_t ldv, int64_t v_cols,
+                       const float *T, int64_t ldt, int64_t t_cols, float *work, void *stream) {
+    (void)work;
+    return tsmqr_entry<float>(side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, v_cols, T, ldt, t_cols,
+                              stream);
+}
+NLA_API int nla_dparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2, int64_t n2,
+                       int64_t k, int64_t l, double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V,
+                       int64_t ldv, const double *T, int64_t ldt, double *work, int64_t ldwork, void *stream) {
+    (void)work; (void)ldwork;
+    return parfb_entry<double>(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, lda1, A2, lda2, V, ldv, T, ldt,
+                               stream);
+}
+NLA_API int nla_sparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2, int64_t n2,
+                       int64_t k, int64_t l, float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V,
+                       int64_t ldv, const float *T, int64_t ldt, float *work, int64_t ldwork, void *stream) {
+    (void)work; (void)ldwork;
+    return parfb_entry<float>(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, lda1, A2, lda2, V, ldv, T, ldt,
+                              stream);
+}

@@ -1,0 +1,92 @@
+"""ctypes binding of libnextla_hip.so (the C ABI declared in include/nextla_hip.h).
+
+The library is loaded from nextla.jl_amd/lib/.  There is no fallback: if the shared
+object is missing or cannot be loaded, every call raises.
+"""
+import ctypes
+import os
+import re
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libnextla_hip.so")
+HEADER = os.path.join(os.path.dirname(_PKG), "include", "nextla_hip.h")
+
+i64 = ctypes.c_int64
+ch = ctypes.c_char
+vp = ctypes.c_void_p
+dbl = ctypes.c_double
+flt = ctypes.c_float
+
+# argument kinds per routine (type-generic; 'R' = the real scalar type)
+_SIGS = {
+    "larfg": [i64, vp, vp, i64, vp, vp],
+    "geqr2": [i64, i64, vp, i64, vp, vp, vp],
+    "larft": [ch, ch, i64, i64, vp, i64, vp, vp, i64, vp],
+    "geqrt": [i64, i64, i64, vp, i64, vp, i64, vp, vp, vp],
+    "larfb": [ch, ch, ch, ch, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp],
+    "unmqr": [ch, ch, i64, i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, vp, vp],
+    "tsqrt": [i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, vp, vp, vp],
+    "tsmqr": [ch, ch, i64, i64, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, i64, vp, i64, i64, vp, vp],
+    "parfb": [ch, ch, ch, ch, i64, i64, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp],
+    "rectrxm": [ch, ch, ch, "R", ch, i64, i64, vp, i64, vp, i64, i64, vp],
+    "trsm_base": [ch, ch, i64, i64, vp, i64, vp, i64, vp],
+    "trmm_base": [ch, ch, i64, i64, vp, i64, vp, i64, vp],
+    "gemm_acc": [i64, i64, i64, "R", vp, i64, vp, i64, vp, i64, vp],
+    "tile_geqrf": [i64, i64, i64, i64, vp, i64, vp, i64, vp],
+    "tile_ormqr": [ch, i64, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp],
+}
+
+_lib = None
+
+
+class ArgumentError(ValueError):
+    """Raised where NextLA.jl throws ArgumentError (negative status from the library)."""
+
+
+class HipError(RuntimeError):
+    """A HIP runtime error reported by the library (positive status)."""
+
+
+def load():
+    """Load libnextla_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libnextla_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.nla_last_error.restype = ctypes.c_char_p
+    lib.nla_version.restype = ctypes.c_int
+    lib.nla_synchronize.argtypes = [vp]
+    lib.nla_scratch_bytes.restype = i64
+    for name, sig in _SIGS.items():
+        for p, real in (("d", dbl), ("s", flt)):
+            f = getattr(lib, f"nla_{p}{name}")
+            f.argtypes = [real if a == "R" else a for a in sig]
+            f.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def header_symbols():
+    """Every function name declared in include/nextla_hip.h."""
+    with open(HEADER) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(nla_[a-z0-9_]+)\s*\(", text)))
+
+
+def check(status, fn):
+    if status == 0:
+        return
+    msg = load().nla_last_error().decode(errors="replace")
+    if status < 0:
+        raise ArgumentError(msg or f"{fn}: illegal argument {-status}")
+    raise HipError(f"{fn}: {msg} (status {status})")
+
+
+def call(prefix, name, *args):
+    lib = load()
+    fn = f"nla_{prefix}{name}"
+    st = getattr(lib, fn)(*args)
+    check(st, fn)
+    return st

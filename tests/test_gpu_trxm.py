@@ -1,0 +1,123 @@
+"""GPU parity of the recursive TRSM/TRMM (unified_rectrxm!), its base kernels and the
+GEMM updates, against BLAS golden vectors (test/unified_rectrxm.jl, test/trsm.jl) and the
+CPU oracle; large sizes are checked through backward error and cut-off independence."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import nextla_hip as nl  # noqa: E402
+import oracle as O  # noqa: E402
+from conftest import relerr  # noqa: E402
+from golden.gen_golden import TRXM_CASES, tri_matrix  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DTYPES = [np.float64, np.float32]
+P = {np.float64: "d", np.float32: "s"}
+
+
+def F(a):
+    return np.asfortranarray(a)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return nl.to_host(t)
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("case", TRXM_CASES)
+def test_rectrxm_golden_and_oracle(golden, dt, case):
+    g = golden("trxm")
+    key = f"{P[dt]}_{TRXM_CASES.index(case)}"
+    func, side, uplo, trans, n, m, alpha, cut = case
+    dB = nl.to_device(g[key + "_B"])
+    nl.unified_rectrxm_(side, uplo, trans, alpha, func, nl.to_device(g[key + "_A"]), dB, cutoff=cut)
+    B = g[key + "_B"].copy(order="F")
+    O.rectrxm(side, uplo, trans, alpha, func, g[key + "_A"], B, threshold=cut)
+    X = host(dB)
+    # test/unified_rectrxm.jl:8,42-43 (1e-14 relative error for Float64 at n <= 256)
+    tol = 1e-12 if dt == np.float64 else 1e-4
+    assert relerr(X, g[key + "_X"]) < tol
+    assert relerr(X, B) < tol
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("func", "SM")
+@pytest.mark.parametrize("side", "LR")
+@pytest.mark.parametrize("uplo", "LU")
+def test_base_kernels_vs_oracle(dt, func, side, uplo):
+    # LeftLowerTRSM! ... RightUpperTRMM! (src/trsm.jl:128-150, src/trmm.jl:332-389), test/trsm.jl
+    rng = np.random.default_rng(hash((func, side, uplo)) % 1000)
+    n, m = 200, 45
+    A = tri_matrix(rng, n, uplo, dt)
+    B0 = F(rng.standard_normal((n, m) if side == "L" else (m, n)).astype(dt))
+    dB = nl.to_device(B0)
+    fn = getattr(nl, f"{'Left' if side == 'L' else 'Right'}{'Lower' if uplo == 'L' else 'Upper'}"
+                     f"{'TRSM' if func == 'S' else 'TRMM'}_")
+    fn(nl.to_device(A), dB)
+    B = B0.copy(order="F")
+    O.rectrxm(side, uplo, "N", 1.0, func, A, B, threshold=1024)
+    assert relerr(host(dB), B) < (1e-12 if dt == np.float64 else 1e-4)
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+def test_gemm_add_sub(dt):
+    rng = np.random.default_rng(1)
+    A = F(rng.standard_normal((130, 70)).astype(dt))
+    B = F(rng.standard_normal((70, 45)).astype(dt))
+    C = F(rng.standard_normal((130, 45)).astype(dt))
+    dC = nl.to_device(C)
+    nl.GEMM_ADD_(nl.to_device(A), nl.to_device(B), dC)   # C += A B
+    want = C.astype(np.float64) + A.astype(np.float64) @ B
+    assert relerr(host(dC), want) < (1e-13 if dt == np.float64 else 1e-5)
+    dA2 = nl.to_device(C)
+    nl.GEMM_SUB_(dA2, nl.to_device(A), nl.to_device(B))  # A2 -= A B
+    assert relerr(host(dA2), C.astype(np.float64) - A.astype(np.float64) @ B) < (1e-13 if dt == np.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("cut", [64, 128, 256, 512])
+def test_trsm_config_backward_error(dt, cut):
+    # TRSM configuration at reduced size: A n x n lower, off-diag N(0,1)/sqrt(n), diag U[1,2];
+    # property checks: backward error and agreement across cut-offs
+    rng = np.random.Generator(np.random.PCG64(2))
+    n, m = 2048, 256
+    A = np.tril(rng.standard_normal((n, n)) / np.sqrt(n), -1)
+    A[np.diag_indices(n)] = rng.uniform(1, 2, n)
+    A = F(A.astype(dt))
+    B0 = F(rng.standard_normal((n, m)).astype(dt))
+    dA, dB = nl.to_device(A), nl.to_device(B0)
+    nl.unified_rectrxm_("L", "L", "N", 1.0, "S", dA, dB, cutoff=cut)
+    X = host(dB).astype(np.float64)
+    A64 = np.tril(A.astype(np.float64))
+    be = np.linalg.norm(A64 @ X - B0) / (np.linalg.norm(A64) * np.linalg.norm(X))
+    assert be < 10 * np.finfo(dt).eps
+    # TRMM round trip: A * X == B
+    dX = nl.to_device(X.astype(dt))
+    nl.unified_rectrxm_("L", "L", "N", 1.0, "M", dA, dX, cutoff=min(cut, 64))
+    assert relerr(host(dX), B0) < (1e-12 if dt == np.float64 else 1e-4)
+
+
+def test_rectrxm_mid_split_non_power_of_two():
+    # src/rectrxm.jl:129-134: mid = 2^floor(log2 n) for n not a power of two
+    rng = np.random.default_rng(6)
+    for n in (300, 777, 1000):
+        A = tri_matrix(rng, n, "U", np.float64)
+        B0 = F(rng.standard_normal((n, 9)))
+        dB = nl.to_device(B0)
+        nl.unified_rectrxm_("L", "U", "N", 0.5, "S", nl.to_device(A), dB, cutoff=64)
+        B = B0.copy(order="F")
+        O.rectrxm("L", "U", "N", 0.5, "S", A, B, threshold=64)
+        assert relerr(host(dB), B) < 1e-12
+
+
+def test_nextlamatrix_trsm():
+    rng = np.random.default_rng(10)
+    n = 96
+    A = tri_matrix(rng, n, "L", np.float64)
+    B = nl.NextLAMatrix(rng.standard_normal((n, 7)))
+    B0 = B.data.copy()
+    nl.NextLAMatrix.trsm("L", "L", "N", "N", 1.0, nl.NextLAMatrix(A), B)
+    assert relerr(np.tril(A) @ B.data, B0) < 1e-13
+    assert B.size() == (n, 7) and len(B) == n * 7
