@@ -1,0 +1,228 @@
+"""Benchmark of the NextLA.jl hot path on one MI355X.
+
+Workload (`value`): the recursive TRSM configuration -- unified_rectrxm!('L','L','N', 1,
+'S', A, B) with A 8192 x 8192 lower triangular (off-diagonal N(0,1)/sqrt(n), diagonal
+U[1,2]), B 8192 x 4096 N(0,1), Float64, recursion cut-off 256 (the reference default).
+One step = restore B from its pristine copy (device to device) + the solve; inputs are
+resident in HBM before timing starts.  TFLOP/s use the LAPACK count n^2 m.
+
+Extra fields: Float32 TRSM and Float64 TRMM of the same shape, the solve's backward
+error, tile QR (flat-tree geqrt/unmqr/tsqrt/tsmqr) TFLOP/s, the roofline of the
+dominant kernel (the MFMA GEMM of the off-diagonal updates, timed live with hipEvents on
+its launch stream) and the CPU oracle timed on a bounded sample of the same solve.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--qr-n N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(REPO, "nextla.jl_amd"), os.path.join(REPO, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK = {"f64": 78.6, "f32": 157.3}   # TFLOP/s, MI355X dense (vector = matrix for these types)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gemm_flops_of_recursion(n, m, cut):
+    """Flops of the GEMM updates unified_rec issues (src/rectrxm.jl:101-198)."""
+    if n <= cut:
+        return 0
+    mid = n // 2 if (n & (n - 1)) == 0 else 1 << (n.bit_length() - 1)
+    rem = n - mid
+    return 2 * mid * rem * m + gemm_flops_of_recursion(mid, m, cut) + gemm_flops_of_recursion(rem, m, cut)
+
+
+def make_tri(torch, nl, n, dtype, gen):
+    X = torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64) / (n ** 0.5)
+    L = torch.tril(X, -1)
+    L.diagonal().copy_(torch.rand(n, generator=gen, device="cuda", dtype=torch.float64) + 1.0)
+    return nl.colmajor(L.to(dtype))
+
+
+def time_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(n, cut, target_s=12.0):
+    """The oracle's recursive TRSM (a C/OpenMP restatement of unified_rectrxm!) on the host
+    cores, on n x n with a reduced number of right-hand sides chosen for ~target_s seconds."""
+    import numpy as np
+
+    import oracle as O
+
+    rng = np.random.default_rng(7)
+    A = np.tril(rng.standard_normal((n, n)) / np.sqrt(n), -1)
+    A[np.diag_indices(n)] = rng.uniform(1, 2, n)
+    A = np.asfortranarray(A)
+    m = 32
+    B = np.asfortranarray(rng.standard_normal((n, m)))
+    t0 = time.perf_counter()
+    O.rectrxm("L", "L", "N", 1.0, "S", A, B, threshold=cut)
+    dt = time.perf_counter() - t0
+    rate = n * n * m / dt
+    m2 = int(max(32, min(4096, rate * target_s / (n * n))))
+    B = np.asfortranarray(rng.standard_normal((n, m2)))
+    t0 = time.perf_counter()
+    O.rectrxm("L", "L", "N", 1.0, "S", A, B, threshold=cut)
+    dt = time.perf_counter() - t0
+    return {"value": round(n * n * m2 / dt / 1e12, 6), "unit": "TFLOP/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"oracle unified_rectrxm TRSM L/L/N Float64 n={n} with {m2} of the 4096 right-hand sides, "
+                      f"cut-off {cut}, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--m", type=int, default=4096)
+    ap.add_argument("--cutoff", type=int, default=256)
+    ap.add_argument("--qr-n", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import nextla_hip as nl
+
+    nl.load()
+    n, m, cut = args.n, args.m, args.cutoff
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(8192)
+    A = make_tri(torch, nl, n, torch.float64, gen)
+    B0 = nl.colmajor(torch.randn((m, n), generator=gen, device="cuda", dtype=torch.float64).t())
+    B = nl.colmajor(torch.empty_like(B0))
+    flops = float(n) * n * m
+
+    def step():
+        B.copy_(B0)
+        nl.unified_rectrxm_("L", "L", "N", 1.0, "S", A, B, cutoff=cut)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    nl.prof_reset()
+    nl.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    nl.prof_enable(False)
+    gemm_ms, gemm_n = nl.prof_read("gemm")
+    base_ms, base_n = nl.prof_read("trsm_base")
+    nl.prof_reset()
+    value = flops / sec / 1e12
+    gemm_flops = gemm_flops_of_recursion(n, m, cut)
+    achieved = gemm_flops * args.steps / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    log(f"[bench] TRSM f64 n={n} m={m} cut={cut}: {sec*1e3:.3f} ms/step, {value:.2f} TFLOP/s; "
+        f"gemm {gemm_ms/args.steps:.3f} ms/step over {gemm_n//args.steps} launches ({achieved} TFLOP/s); "
+        f"base {base_ms/args.steps:.3f} ms/step")
+
+    # backward error of the timed solve: ||A X - B|| / (||A|| ||X||)
+    Lf = torch.tril(A)
+    R = Lf @ B - B0
+    berr = (torch.linalg.norm(R) / (torch.linalg.norm(Lf) * torch.linalg.norm(B))).item()
+    del R, Lf
+
+    extras = {}
+    if not args.no_extras:
+        # Float32 TRSM, same shape
+        A32 = A.to(torch.float32)
+        A32 = nl.colmajor(A32)
+        B032 = nl.colmajor(B0.to(torch.float32))
+        B32 = nl.colmajor(torch.empty_like(B032))
+
+        def s32():
+            B32.copy_(B032)
+            nl.unified_rectrxm_("L", "L", "N", 1.0, "S", A32, B32, cutoff=cut)
+        t32 = time_steps(torch, s32, max(3, args.steps // 2), 1)
+        extras["trsm_f32_tflops"] = round(flops / t32 / 1e12, 3)
+        del A32, B032, B32
+
+        def trmm():
+            B.copy_(B0)
+            nl.unified_rectrxm_("L", "L", "N", 1.0, "M", A, B, cutoff=0)
+        tm = time_steps(torch, trmm, max(3, args.steps // 2), 1)
+        extras["trmm_f64_tflops"] = round(flops / tm / 1e12, 3)
+
+        # tile QR, Float64, nb = 256, ib = 32
+        qn = args.qr_n
+        nb, ib = (256, 32) if qn >= 4096 else (128, 32)
+        Q0 = nl.colmajor(torch.randn((qn, qn), generator=gen, device="cuda", dtype=torch.float64))
+        Qw = nl.colmajor(torch.empty_like(Q0))
+        holder = {}
+
+        def qr():
+            Qw.copy_(Q0)
+            holder["T"] = nl.tile_geqrf(Qw, nb, ib)
+        tq = time_steps(torch, qr, 2, 1)
+        qflops = 2.0 * qn ** 3 - 2.0 / 3.0 * qn ** 3
+        extras["tile_qr"] = {"n": qn, "nb": nb, "ib": ib, "dtype": "f64", "ms": round(tq * 1e3, 2),
+                             "tflops": round(qflops / tq / 1e12, 3),
+                             "frac_of_peak": round(qflops / tq / 1e12 / PEAK["f64"], 4)}
+        del Q0, Qw, holder
+
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "trsm_f64_gemm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "tile-QR & TRSM TFLOP/s (% of dtype peak), mixed vs FP64 at equal backward err",
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"rectrxm TRSM left/lower/N Float64 n={n}, B {n}x{m}, cut-off {cut}",
+                   "n": n, "m": m, "cutoff": cut, "frac_of_peak": round(value / PEAK["f64"], 4),
+                   "backward_error": berr},
+        "roofline": {"bound": "mfma", "achieved": None if achieved is None else round(achieved, 3),
+                     "peak": PEAK["f64"], "unit": "TFLOP/s",
+                     "frac": None if achieved is None else round(achieved / PEAK["f64"], 4),
+                     "traffic": traffic, "kernel": "gemm_kernel<double> (off-diagonal GEMM_SUB! updates)",
+                     "launches_per_step": gemm_n // max(1, args.steps),
+                     "avg_launch_ms": round(gemm_ms / max(1, gemm_n), 4)},
+        "extras": extras,
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(n, cut)
+        except Exception as e:  # the baseline must not hide the GPU result
+            out["cpu_baseline"] = {"value": None, "unit": "TFLOP/s", "cores": os.cpu_count(), "kind": "port",
+                                   "sample": f"failed: {e}"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -37,6 +37,14 @@ int nla_synchronize(void *stream);
 /* Bytes of device scratch the library holds on the current device. */
 int64_t nla_scratch_bytes(void);
 
+/* Launch profiler (no reference counterpart; measurement hook for bench.py): while enabled,
+ * each kernel family brackets its launches with hipEvents on the launch stream.
+ * nla_prof_read sums elapsed time and launch count for one family: "gemm", "trsm_base",
+ * "trmm_base", "ge_apply", "ts_apply", "panel_qr", "ts_panel", "tile_qr". */
+int nla_prof_enable(int on);
+void nla_prof_reset(void);
+int nla_prof_read(const char *kernel, double *total_ms, int64_t *launches);
+
 /* ---- Householder generation ------------------------------------------------------ */
 /* larfg!(n, alpha, x, incx, tau): src/larfg.jl:123-186.  alpha and tau are DEVICE scalars;
  * on return *alpha = beta, x = v[2:n], *tau = tau.  (Julia shim returns (alpha, tau).) */

@@ -2,6 +2,8 @@
 // the small utility entry points of the C ABI.
 #include "nla_common.hpp"
 
+#include <vector>
+
 namespace nla {
 
 static thread_local std::string g_error;
@@ -71,7 +73,67 @@ void *scratch(ScratchSlot slot, size_t bytes, hipStream_t stream, int *status) {
     return p.ptr[slot];
 }
 
+// ---------------------------------------------------------------------------------
+namespace {
+std::mutex g_prof_mu;
+bool g_prof_enabled = false;
+struct ProfRec { int kind; hipEvent_t a, b; };
+std::vector<ProfRec> g_prof;
+std::vector<hipEvent_t> g_prof_free;
+const char *g_prof_names[PK_COUNT] = {"gemm", "trsm_base", "trmm_base", "ge_apply", "ts_apply",
+                                      "panel_qr", "ts_panel", "tile_qr", "other"};
+}  // namespace
+
+bool prof_on() { return g_prof_enabled; }
+
+hipEvent_t prof_event() {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    if (!g_prof_free.empty()) { hipEvent_t e = g_prof_free.back(); g_prof_free.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void prof_push(int kind, hipEvent_t a, hipEvent_t b) {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    g_prof.push_back({kind, a, b});
+}
+
 }  // namespace nla
+
+NLA_API int nla_prof_enable(int on) {
+    nla::g_prof_enabled = on != 0;
+    return 0;
+}
+
+NLA_API void nla_prof_reset(void) {
+    std::lock_guard<std::mutex> lock(nla::g_prof_mu);
+    for (auto &r : nla::g_prof) {
+        (void)hipEventSynchronize(r.b);
+        nla::g_prof_free.push_back(r.a);
+        nla::g_prof_free.push_back(r.b);
+    }
+    nla::g_prof.clear();
+}
+
+NLA_API int nla_prof_read(const char *kernel, double *total_ms, int64_t *launches) {
+    std::lock_guard<std::mutex> lock(nla::g_prof_mu);
+    double ms = 0;
+    int64_t n = 0;
+    for (auto &r : nla::g_prof) {
+        if (strcmp(nla::g_prof_names[r.kind], kernel) != 0) continue;
+        hipError_t e = hipEventSynchronize(r.b);
+        if (e != hipSuccess) return nla::hip_error(e, "hipEventSynchronize(prof)");
+        float t = 0;
+        e = hipEventElapsedTime(&t, r.a, r.b);
+        if (e != hipSuccess) return nla::hip_error(e, "hipEventElapsedTime(prof)");
+        ms += t;
+        ++n;
+    }
+    *total_ms = ms;
+    *launches = n;
+    return 0;
+}
 
 NLA_API const char *nla_last_error(void) { return nla::g_error.c_str(); }
 

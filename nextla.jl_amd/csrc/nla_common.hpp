@@ -40,6 +40,28 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 enum ScratchSlot { SLOT_W = 0, SLOT_AUX = 1, SLOT_SCHED = 2, SLOT_MIXED = 3, NUM_SLOTS = 4 };
 void *scratch(ScratchSlot slot, size_t bytes, hipStream_t stream, int *status);
 
+// Launch-time profiler: when enabled, every kernel family brackets its launches with a
+// pair of hipEvents on the launch stream; nla_prof_read() sums the elapsed times.
+bool prof_on();
+void prof_push(int kind, hipEvent_t a, hipEvent_t b);
+hipEvent_t prof_event();
+enum ProfKind { PK_GEMM = 0, PK_TRSM_BASE, PK_TRMM_BASE, PK_GE_APPLY, PK_TS_APPLY, PK_PANEL, PK_TS_PANEL,
+                PK_TILE_QR, PK_OTHER, PK_COUNT };
+struct ProfScope {
+    int kind;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    ProfScope(int k, hipStream_t st) : kind(k), s(st) {
+        if (prof_on()) { a = prof_event(); if (a) (void)hipEventRecord(a, s); }
+    }
+    ~ProfScope() {
+        if (a) {
+            hipEvent_t b = prof_event();
+            if (b) { (void)hipEventRecord(b, s); prof_push(kind, a, b); }
+        }
+    }
+};
+
 inline bool is_trans(char t) { return t == 'T' || t == 'C' || t == 't' || t == 'c'; }
 inline char up(char c) { return (c >= 'a' && c <= 'z') ? char(c - 32) : c; }
 
@@ -78,10 +100,12 @@ template <typename T> __device__ __forceinline__ T block_sum(T v, T *red) {
     return s;
 }
 
-// MFMA 16x16x4 (FP64 / FP32).  Fragment layouts (one 64-lane wave):
+// MFMA 16x16x4 (FP64 / FP32).  Fragment layouts (one 64-lane wave), measured on gfx950:
 //   A (16 x 4):  lane l holds A[l % 16][l / 16]
 //   B (4 x 16):  lane l holds B[l / 16][l % 16]
-//   D (16 x 16): lane l, element r holds D[4 * (l / 16) + r][l % 16]
+//   D (16 x 16): lane l, element r holds D[acc_row(l / 16, r)][l % 16] with
+//                acc_row = l/16 + 4 r      for v_mfma_f64_16x16x4f64
+//                acc_row = 4 (l/16) + r    for v_mfma_f32_16x16x4f32
 typedef double dx4 __attribute__((ext_vector_type(4)));
 typedef float fx4 __attribute__((ext_vector_type(4)));
 template <typename T> struct Acc;
@@ -94,6 +118,10 @@ __device__ __forceinline__ void mfma16(dx4 &c, double a, double b) {
 __device__ __forceinline__ void mfma16(fx4 &c, float a, float b) {
     c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+
+template <typename T> __device__ __forceinline__ int acc_row(int lk, int r);
+template <> __device__ __forceinline__ int acc_row<double>(int lk, int r) { return lk + 4 * r; }
+template <> __device__ __forceinline__ int acc_row<float>(int lk, int r) { return 4 * lk + r; }
 
 template <typename T> __device__ __forceinline__ typename Acc<T>::t acc_zero() {
     typename Acc<T>::t z;

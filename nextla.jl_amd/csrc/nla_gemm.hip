@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
         }
     }
 
-    // acc[i][j] element r of lane l is C[m0 + wm*BM/2 + 16 i + (l%16), n0 + wn*BN/2 + 16 j + 4(l/16) + r]
+    // acc[i][j] element r of lane l is C[m0 + wm*BM/2 + 16 i + (l%16), n0 + wn*BN/2 + 16 j + acc_row(l/16, r)]
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int gm = m0 + wm * (BM / 2) + i * 16 + lr;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gn = n0 + wn * (BN / 2) + j * 16 + 4 * lk + r;
+                const int gn = n0 + wn * (BN / 2) + j * 16 + acc_row<T>(lk, r);
                 if (gn < N) {
                     T *c = &C[gm + (i64)gn * ldc];
                     T v = alpha * acc[i][j][r];
@@ -145,6 +145,7 @@ hipError_t gemm(char ta, char tb, i64 M, i64 N, i64 K, T alpha, const T *A, i64 
         if (beta == T(1)) return hipSuccess;
         // C = beta * C through a K=0 product: the kernel reads nothing from A/B.
     }
+    ProfScope prof(PK_GEMM, s);
     const i64 big_tiles = ((M + 127) / 128) * ((N + 127) / 128);
     if (big_tiles >= 512)
         return launch_tiles<T, 128, 128, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);

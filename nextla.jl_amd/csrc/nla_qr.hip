@@ -437,6 +437,7 @@ template <typename T>
 hipError_t launch_panel_qr(i64 m, i64 n, T *A, i64 lda, T *tau, T *Tm, i64 ldt, hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;
     size_t sh = (size_t)n * sizeof(T) * (Tm ? 2 : 1);
+    ProfScope prof(PK_PANEL, s);
     panel_qr_kernel<T><<<1, 256, sh, s>>>((int)m, (int)n, A, lda, tau, Tm, ldt);
     return hipGetLastError();
 }
@@ -444,6 +445,7 @@ hipError_t launch_panel_qr(i64 m, i64 n, T *A, i64 lda, T *tau, T *Tm, i64 ldt, 
 template <typename T>
 hipError_t launch_ts_panel(i64 m, i64 ii, i64 sb, T *A1, i64 lda1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau,
                            hipStream_t s) {
+    ProfScope prof(PK_TS_PANEL, s);
     ts_panel_kernel<T><<<1, 256, 0, s>>>((int)m, (int)ii, (int)sb, A1, lda1, A2, lda2, Tm, ldt, tau);
     return hipGetLastError();
 }
@@ -454,6 +456,7 @@ hipError_t launch_ge_apply(bool trc, i64 m, i64 ncols, i64 k, i64 ib, const T *V
                            bool notrans, bool forward, T *C, i64 ldc, hipStream_t s) {
     if (m <= 0 || ncols <= 0 || k <= 0) return hipSuccess;
     dim3 grid((unsigned)((ncols + CW - 1) / CW)), block(256);
+    ProfScope prof(PK_GE_APPLY, s);
     if (trc) ge_apply_kernel<T, true><<<grid, block, 0, s>>>((int)m, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, C, ldc);
     else ge_apply_kernel<T, false><<<grid, block, 0, s>>>((int)m, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, C, ldc);
     return hipGetLastError();
@@ -464,6 +467,7 @@ hipError_t launch_ts_apply(bool trc, i64 m2, i64 ncols, i64 k, i64 ib, const T *
                            bool notrans, bool forward, T *A1, i64 lda1, T *A2, i64 lda2, hipStream_t s) {
     if (m2 <= 0 || ncols <= 0 || k <= 0) return hipSuccess;
     dim3 grid((unsigned)((ncols + CW - 1) / CW)), block(256);
+    ProfScope prof(PK_TS_APPLY, s);
     if (trc) ts_apply_kernel<T, true><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2);
     else ts_apply_kernel<T, false><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2);
     return hipGetLastError();

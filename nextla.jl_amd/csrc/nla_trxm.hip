@@ -144,6 +144,7 @@ static hipError_t base_launch(char func, bool up, bool trA, bool trB, int n, int
     const int cw = base_cw<T>(n);
     const dim3 grid((m + cw - 1) / cw), block(256);
     size_t sh = (size_t)cw * n * sizeof(T);
+    ProfScope prof(func == 'S' ? PK_TRSM_BASE : PK_TRMM_BASE, s);
     if (func == 'S') {
         sh += 64 * 65 * sizeof(T);
 #define NLA_TRSM_CASE(U, TA, TB)                                                                      \

@@ -59,6 +59,9 @@ def load():
     lib.nla_version.restype = ctypes.c_int
     lib.nla_synchronize.argtypes = [vp]
     lib.nla_scratch_bytes.restype = i64
+    lib.nla_prof_enable.argtypes = [ctypes.c_int]
+    lib.nla_prof_reset.restype = None
+    lib.nla_prof_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(dbl), ctypes.POINTER(i64)]
     for name, sig in _SIGS.items():
         for p, real in (("d", dbl), ("s", flt)):
             f = getattr(lib, f"nla_{p}{name}")
@@ -90,3 +93,19 @@ def call(prefix, name, *args):
     st = getattr(lib, fn)(*args)
     check(st, fn)
     return st
+
+
+def prof_enable(on=True):
+    load().nla_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    load().nla_prof_reset()
+
+
+def prof_read(kernel):
+    """(total milliseconds, launches) recorded for one kernel family while profiling was on."""
+    ms = dbl(0)
+    n = i64(0)
+    check(load().nla_prof_read(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)), "nla_prof_read")
+    return ms.value, n.value
