@@ -119,6 +119,151 @@ __global__ __launch_bounds__(256) void trmm_base_kernel(int n, int m, int cwmax,
     }
 }
 
+// ------------------------------------------------------------------------------------
+// Blocked base solve (cut-off >= 32).  The recursion peels powers of two (>= 32) off the
+// front, so every base block starts at a multiple of 32 of op(A) and only the block that
+// ends at n can have a ragged tail.  Hence the 32 x 32 diagonal sub-blocks of every base
+// call are the 32-aligned diagonal blocks of op(A): one launch inverts them all, then each
+// base call is a left-looking (lower) / right-looking (upper) chain of MFMA products:
+//   X_s = inv(A_ss) (B_s - sum_{j done} A_sj X_j).
+// op(A)(i,j) = TRA ? A[j + i lda] : A[i + j lda]; no index reversal here.
+constexpr int DB = 32;   // diagonal sub-block
+constexpr int BCW = 16;  // right-hand sides per workgroup
+
+template <typename T, bool TRA> __device__ __forceinline__ T opa(const T *A, i64 lda, int i, int j) {
+    return TRA ? A[j + (i64)i * lda] : A[i + (i64)j * lda];
+}
+
+// inverse of each DB x DB diagonal block of op(A) (lower or upper), column-major into
+// Dinv[b * DB * DB]; entries outside the triangle are zero.  One wave per block, lane = column.
+template <typename T, bool UP, bool TRA>
+__global__ __launch_bounds__(64) void diag_inv_kernel(int n, const T *__restrict__ A, i64 lda, T *__restrict__ Dinv) {
+    __shared__ T L[DB][DB + 1];
+    const int b = blockIdx.x, o = b * DB, bs = min(DB, n - o);
+    const int j = threadIdx.x;
+    for (int idx = j; idx < DB * DB; idx += 64) {
+        const int r = idx % DB, c = idx / DB;
+        L[r][c] = (r < bs && c < bs && (UP ? r <= c : r >= c)) ? opa<T, TRA>(A, lda, o + r, o + c) : T(0);
+    }
+    __syncthreads();
+    if (j < DB) {
+        T x[DB];
+#pragma unroll
+        for (int i = 0; i < DB; ++i) x[i] = 0;
+        if (j < bs) {
+            if (!UP) {
+#pragma unroll
+                for (int i = 0; i < DB; ++i) {
+                    if (i < j || i >= bs) continue;
+                    T s = (i == j) ? T(1) : T(0);
+#pragma unroll
+                    for (int k = 0; k < DB; ++k)
+                        if (k >= j && k < i) s -= L[i][k] * x[k];
+                    x[i] = s / L[i][i];
+                }
+            } else {
+#pragma unroll
+                for (int ii = DB - 1; ii >= 0; --ii) {
+                    if (ii > j) continue;
+                    T s = (ii == j) ? T(1) : T(0);
+#pragma unroll
+                    for (int k = 0; k < DB; ++k)
+                        if (k > ii && k <= j) s -= L[ii][k] * x[k];
+                    x[ii] = s / L[ii][ii];
+                }
+            }
+        }
+        T *D = Dinv + (size_t)b * DB * DB;
+#pragma unroll
+        for (int i = 0; i < DB; ++i) D[i + DB * j] = x[i];
+    }
+}
+
+// One base block (n <= 1024 rows of op(A) starting at the pointer A; B likewise), BCW
+// right-hand sides per workgroup.  Dinv points at this block's first inverted sub-block.
+template <typename T, bool UP, bool TRA, bool TRB>
+__global__ __launch_bounds__(256) void trsm_blk_kernel(int n, int m, const T *__restrict__ A, i64 lda,
+                                                       T *__restrict__ B, i64 ldb, const T *__restrict__ Dinv) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int npad = (n + DB - 1) / DB * DB;
+    T *Xs = reinterpret_cast<T *>(smem_raw);   // [npad][BCW + 1]
+    T *P = Xs + (size_t)npad * (BCW + 1);       // [4 waves][2 tiles][16][17]
+    T *Ys = P + 4 * 2 * 16 * 17;                // [DB][BCW + 1]
+    constexpr int XS = BCW + 1;
+    const int c0 = blockIdx.x * BCW;
+    const int cw = min(BCW, m - c0);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    for (int idx = t; idx < npad * BCW; idx += 256) {
+        int r, c;
+        if (TRB) { c = idx % BCW; r = idx / BCW; } else { r = idx % npad; c = idx / npad; }
+        T v = 0;
+        if (r < n && c < cw) v = TRB ? B[(c0 + c) + (i64)r * ldb] : B[r + (i64)(c0 + c) * ldb];
+        Xs[r * XS + c] = v;
+    }
+    const int nsb = npad / DB;
+    for (int step = 0; step < nsb; ++step) {
+        const int sb = UP ? nsb - 1 - step : step;
+        const int r0 = sb * DB;
+        // K range of already-solved rows
+        const int kb = UP ? r0 + DB : 0, ke = UP ? n : r0;
+        typename Acc<T>::t acc0 = acc_zero<T>(), acc1 = acc_zero<T>();
+        __syncthreads();
+        for (int k0 = kb + 4 * w; k0 < ke; k0 += 16) {
+            const int kc = k0 + lk;
+            T bf = Xs[kc * XS + lr];                       // rows >= n are zero in Xs
+            T a0 = 0, a1 = 0;
+            if (kc < n) {
+                if (r0 + lr < n) a0 = opa<T, TRA>(A, lda, r0 + lr, kc);
+                if (r0 + 16 + lr < n) a1 = opa<T, TRA>(A, lda, r0 + 16 + lr, kc);
+            }
+            mfma16(acc0, a0, bf);
+            mfma16(acc1, a1, bf);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = acc_row<T>(lk, r);
+            P[((w * 2 + 0) * 16 + row) * 17 + lr] = acc0[r];
+            P[((w * 2 + 1) * 16 + row) * 17 + lr] = acc1[r];
+        }
+        __syncthreads();
+        for (int idx = t; idx < DB * BCW; idx += 256) {
+            const int row = idx / BCW, c = idx % BCW, tile = row >> 4, rr = row & 15;
+            T s = Xs[(r0 + row) * XS + c];
+#pragma unroll
+            for (int ww = 0; ww < 4; ++ww) s -= P[((ww * 2 + tile) * 16 + rr) * 17 + c];
+            Ys[row * XS + c] = s;
+        }
+        __syncthreads();
+        if (w < 2) {
+            // X_s rows 16w.. = Dinv_s[16w.., :] * Y
+            const T *D = Dinv + (size_t)sb * DB * DB;
+            typename Acc<T>::t acc = acc_zero<T>();
+#pragma unroll
+            for (int k0 = 0; k0 < DB; k0 += 4) {
+                const T a = D[(16 * w + lr) + DB * (k0 + lk)];
+                const T bf = Ys[(k0 + lk) * XS + lr];
+                mfma16(acc, a, bf);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + 16 * w + acc_row<T>(lk, r);
+                Xs[row * XS + lr] = row < n ? acc[r] : T(0);
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < n * BCW; idx += 256) {
+        int r, c;
+        if (TRB) { c = idx % BCW; r = idx / BCW; } else { r = idx % n; c = idx / n; }
+        if (c < cw) {
+            if (TRB) B[(c0 + c) + (i64)r * ldb] = Xs[r * XS + c];
+            else B[r + (i64)(c0 + c) * ldb] = Xs[r * XS + c];
+        }
+    }
+}
+
 template <typename T>
 __global__ void scale_kernel(int rows, int cols, T alpha, T *__restrict__ B, i64 ldb) {
     const i64 total = (i64)rows * cols;
@@ -166,13 +311,67 @@ static hipError_t base_launch(char func, bool up, bool trA, bool trB, int n, int
     return hipGetLastError();
 }
 
-// Base case of unified_rec for (side, uplo) of op(A), where op(A)(i,j) = tr ? A(j,i) : A(i,j).
+template <typename T> static size_t blk_lds_bytes(int n) {
+    const int npad = (n + DB - 1) / DB * DB;
+    return ((size_t)npad * (BCW + 1) + 4 * 2 * 16 * 17 + DB * (BCW + 1)) * sizeof(T);
+}
+// The blocked solve holds a base block of n rows in LDS (<= 160 KiB).
+template <typename T> static bool blk_ok(int n) { return blk_lds_bytes<T>(n) <= 160 * 1024; }
+
+template <typename T>
+static hipError_t diag_inv_launch(bool up, bool trA, int n, const T *A, i64 lda, T *Dinv, hipStream_t s) {
+    const int nb = (n + DB - 1) / DB;
+    if (!up && !trA) diag_inv_kernel<T, false, false><<<nb, 64, 0, s>>>(n, A, lda, Dinv);
+    else if (!up && trA) diag_inv_kernel<T, false, true><<<nb, 64, 0, s>>>(n, A, lda, Dinv);
+    else if (up && !trA) diag_inv_kernel<T, true, false><<<nb, 64, 0, s>>>(n, A, lda, Dinv);
+    else diag_inv_kernel<T, true, true><<<nb, 64, 0, s>>>(n, A, lda, Dinv);
+    return hipGetLastError();
+}
+
+template <typename T, bool U, bool TA, bool TB>
+static hipError_t blk_launch_one(int n, int m, const T *A, i64 lda, T *B, i64 ldb, const T *Dinv, hipStream_t s) {
+    const size_t sh = blk_lds_bytes<T>(n);
+    if (sh > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&trsm_blk_kernel<T, U, TA, TB>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        if (e != hipSuccess) return e;
+    }
+    trsm_blk_kernel<T, U, TA, TB><<<(m + BCW - 1) / BCW, 256, sh, s>>>(n, m, A, lda, B, ldb, Dinv);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t blk_launch(bool up, bool trA, bool trB, int n, int m, const T *A, i64 lda, T *B, i64 ldb,
+                             const T *Dinv, hipStream_t s) {
+    if (n <= 0 || m <= 0) return hipSuccess;
+    ProfScope prof(PK_TRSM_BASE, s);
+    if (!up && !trA && !trB) return blk_launch_one<T, false, false, false>(n, m, A, lda, B, ldb, Dinv, s);
+    if (!up && !trA && trB) return blk_launch_one<T, false, false, true>(n, m, A, lda, B, ldb, Dinv, s);
+    if (!up && trA && !trB) return blk_launch_one<T, false, true, false>(n, m, A, lda, B, ldb, Dinv, s);
+    if (!up && trA && trB) return blk_launch_one<T, false, true, true>(n, m, A, lda, B, ldb, Dinv, s);
+    if (up && !trA && !trB) return blk_launch_one<T, true, false, false>(n, m, A, lda, B, ldb, Dinv, s);
+    if (up && !trA && trB) return blk_launch_one<T, true, false, true>(n, m, A, lda, B, ldb, Dinv, s);
+    if (up && trA && !trB) return blk_launch_one<T, true, true, false>(n, m, A, lda, B, ldb, Dinv, s);
+    return blk_launch_one<T, true, true, true>(n, m, A, lda, B, ldb, Dinv, s);
+}
+
+// Left-form flags of the base call for (side, uplo) of op(A), op(A)(i,j) = tr ? A(j,i) : A(i,j):
+// X op(A) = B  <=>  op(A)^T X^T = B^T.
+static inline void left_form(char side, char uplo, bool tr, bool *up, bool *trA, bool *trB) {
+    if (side == 'L') { *up = uplo == 'U'; *trA = tr; *trB = false; }
+    else { *up = uplo != 'U'; *trA = !tr; *trB = true; }
+}
+
+// Base case of unified_rec.  Dinv (or nullptr) holds the inverted 32x32 diagonal blocks of the
+// whole left-form matrix; diag_off is this block's offset along the diagonal.
 template <typename T>
 static hipError_t base_case(char func, char side, char uplo, bool tr, int n, int m, const T *A, i64 lda, T *B, i64 ldb,
-                            hipStream_t s) {
-    if (side == 'L') return base_launch<T>(func, uplo == 'U', tr, false, n, m, A, lda, B, ldb, s);
-    // X op(A) = B  <=>  op(A)^T X^T = B^T ; op(A)^T(i,j) = op(A)(j,i)
-    return base_launch<T>(func, uplo != 'U', !tr, true, n, m, A, lda, B, ldb, s);
+                            const T *Dinv, i64 diag_off, hipStream_t s) {
+    bool up, trA, trB;
+    left_form(side, uplo, tr, &up, &trA, &trB);
+    if (func == 'S' && Dinv && (diag_off % DB) == 0 && blk_ok<T>(n))
+        return blk_launch<T>(up, trA, trB, n, m, A, lda, B, ldb, Dinv + (diag_off / DB) * DB * DB, s);
+    return base_launch<T>(func, up, trA, trB, n, m, A, lda, B, ldb, s);
 }
 
 // Pointer to block (i0, j0) of op(A).
@@ -183,8 +382,8 @@ template <typename T> static inline const T *opA_block(const T *A, i64 lda, bool
 // unified_rec: src/rectrxm.jl:101-198 (same split rule, same forward/backward orders).
 template <typename T>
 static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, const T *A, i64 lda, T *B, i64 ldb,
-                      i64 threshold, hipStream_t s) {
-    if (n <= threshold) return base_case<T>(func, side, uplo, tr, (int)n, (int)m, A, lda, B, ldb, s);
+                      i64 threshold, const T *Dinv, i64 off, hipStream_t s) {
+    if (n <= threshold) return base_case<T>(func, side, uplo, tr, (int)n, (int)m, A, lda, B, ldb, Dinv, off, s);
     i64 mid;
     if ((n & (n - 1)) == 0) mid = n / 2;
     else { mid = 1; while (mid * 2 <= n) mid *= 2; }
@@ -200,7 +399,7 @@ static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, co
                          (side == 'L' && uplo == 'U' && func == 'M') || (side == 'R' && uplo == 'L' && func == 'M');
     hipError_t e;
     if (forward) {
-        if ((e = rec<T>(func, side, uplo, tr, mid, m, A11, lda, B1, ldb, threshold, s)) != hipSuccess) return e;
+        if ((e = rec<T>(func, side, uplo, tr, mid, m, A11, lda, B1, ldb, threshold, Dinv, off, s)) != hipSuccess) return e;
         if (side == 'L') {
             if (func == 'S') e = gemm<T>(ta, 'N', rem, m, mid, T(-1), A21, lda, B1, ldb, T(1), B2, ldb, s);  // B2 -= A21 B1
             else e = gemm<T>(ta, 'N', mid, m, rem, T(1), A12, lda, B2, ldb, T(1), B1, ldb, s);              // B1 += A12 B2
@@ -209,9 +408,10 @@ static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, co
             else e = gemm<T>('N', ta, m, mid, rem, T(1), B2, ldb, A21, lda, T(1), B1, ldb, s);              // B1 += B2 A21
         }
         if (e != hipSuccess) return e;
-        return rec<T>(func, side, uplo, tr, rem, m, A22, lda, B2, ldb, threshold, s);
+        return rec<T>(func, side, uplo, tr, rem, m, A22, lda, B2, ldb, threshold, Dinv, off + mid, s);
     }
-    if ((e = rec<T>(func, side, uplo, tr, rem, m, A22, lda, B2, ldb, threshold, s)) != hipSuccess) return e;
+    if ((e = rec<T>(func, side, uplo, tr, rem, m, A22, lda, B2, ldb, threshold, Dinv, off + mid, s)) != hipSuccess)
+        return e;
     if (side == 'L') {
         if (func == 'S') e = gemm<T>(ta, 'N', mid, m, rem, T(-1), A12, lda, B2, ldb, T(1), B1, ldb, s);  // B1 -= A12 B2
         else e = gemm<T>(ta, 'N', rem, m, mid, T(1), A21, lda, B1, ldb, T(1), B2, ldb, s);              // B2 += A21 B1
@@ -220,7 +420,7 @@ static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, co
         else e = gemm<T>('N', ta, m, rem, mid, T(1), B1, ldb, A12, lda, T(1), B2, ldb, s);              // B2 += B1 A12
     }
     if (e != hipSuccess) return e;
-    return rec<T>(func, side, uplo, tr, mid, m, A11, lda, B1, ldb, threshold, s);
+    return rec<T>(func, side, uplo, tr, mid, m, A11, lda, B1, ldb, threshold, Dinv, off, s);
 }
 
 template <typename T>
@@ -256,7 +456,18 @@ static int rectrxm_entry(char side, char uplo, char trans, T alpha, char func, i
     hipStream_t s = as_stream(stream);
     const i64 rows = side == 'L' ? n : m, cols = side == 'L' ? m : n;
     if (func == 'S') NLA_HIP(scale<T>(rows, cols, alpha, B, ldb, s));
-    NLA_HIP(rec<T>(func, side, uplo, tr, n, m, A, lda, B, ldb, thr, s));
+    const T *Dinv = nullptr;
+    if (func == 'S' && thr >= DB) {
+        // every base block starts on the 32-grid (see diag_inv_kernel): invert all 32x32 blocks once
+        int st = 0;
+        T *D = (T *)scratch(SLOT_TRSM, sizeof(T) * (size_t)((n + DB - 1) / DB) * DB * DB, s, &st);
+        if (!D) return st;
+        bool upf, trA, trB;
+        left_form(side, uplo, tr, &upf, &trA, &trB);
+        NLA_HIP(diag_inv_launch<T>(upf, trA, (int)n, A, lda, D, s));
+        Dinv = D;
+    }
+    NLA_HIP(rec<T>(func, side, uplo, tr, n, m, A, lda, B, ldb, thr, Dinv, 0, s));
     if (func == 'M') NLA_HIP(scale<T>(rows, cols, alpha, B, ldb, s));
     return 0;
 }
@@ -272,7 +483,18 @@ static int base_entry(char func, char side, char uplo, i64 n, i64 m, const T *A,
     if (n == 0 || m == 0) return 0;
     if (lda < n) return arg_error(6, "lda must be >= n");
     if (ldb < (side == 'L' ? n : m)) return arg_error(8, "ldb too small");
-    NLA_HIP(base_case<T>(func, side, uplo, false, (int)n, (int)m, A, lda, B, ldb, as_stream(stream)));
+    hipStream_t s = as_stream(stream);
+    const T *Dinv = nullptr;
+    if (func == 'S' && blk_ok<T>((int)n)) {
+        int st = 0;
+        T *D = (T *)scratch(SLOT_TRSM, sizeof(T) * (size_t)((n + DB - 1) / DB) * DB * DB, s, &st);
+        if (!D) return st;
+        bool upf, trA, trB;
+        left_form(side, uplo, false, &upf, &trA, &trB);
+        NLA_HIP(diag_inv_launch<T>(upf, trA, (int)n, A, lda, D, s));
+        Dinv = D;
+    }
+    NLA_HIP(base_case<T>(func, side, uplo, false, (int)n, (int)m, A, lda, B, ldb, Dinv, 0, s));
     return 0;
 }
 
