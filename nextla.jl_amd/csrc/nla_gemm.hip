@@ -18,17 +18,18 @@ namespace nla {
 
 template <typename T> struct GemmPad { static constexpr int v = 2; };
 
-template <typename T, int BM, int BN, int BK, bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T *__restrict__ A,
-                                                   i64 lda, const T *__restrict__ B, i64 ldb, T beta,
-                                                   T *__restrict__ C, i64 ldc, int tiles_m) {
+// EDGE = false: every tile is interior (M % BM == N % BN == K % BK == 0): no bounds checks.
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, bool EDGE>
+__global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alpha, const T *__restrict__ A,
+                                                      i64 lda, const T *__restrict__ B, i64 ldb, T beta,
+                                                      T *__restrict__ C, i64 ldc, int tiles_m) {
     constexpr int PAD = GemmPad<T>::v;
     constexpr int LA = (BM * BK) / 256;  // elements of the A slab per thread
     constexpr int LB = (BK * BN) / 256;
     constexpr int FM = BM / 32;          // 16-row blocks per wave (2 waves along M)
     constexpr int FN = BN / 32;
-    __shared__ T As[BK][BM + PAD];
-    __shared__ T Bs[BK][BN + PAD];
+    __shared__ T As[2][BK][BM + PAD];
+    __shared__ T Bs[2][BK][BN + PAD];
 
     // XCD-aware tile numbering: hardware deals workgroups round-robin over 8 XCDs.
     int bid = blockIdx.x;
@@ -36,44 +37,50 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
     if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
     const int tm = bid % tiles_m, tn = bid / tiles_m;
     const int m0 = tm * BM, n0 = tn * BN;
-
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
+
+    // Per-thread slab coordinates: element i of this thread is (am + i*ASTEP_M, ak + i*ASTEP_K).
+    // !TA: m = t % BM fixed, k = t / BM + i * (256 / BM);   TA: k = t % BK fixed, m = t / BK + i * (256 / BK)
+    const int am = TA ? t / BK : t % BM, ak = TA ? t % BK : t / BM;
+    const int bk = TB ? t / BN : t % BK, bn = TB ? t % BN : t / BK;
+    constexpr int ASM = TA ? 256 / BK : 0, ASK = TA ? 0 : 256 / BM;
+    constexpr int BSK = TB ? 256 / BN : 0, BSN = TB ? 0 : 256 / BK;
+    const T *pa = TA ? A + (ak) + (i64)(m0 + am) * lda : A + (m0 + am) + (i64)ak * lda;
+    const T *pb = TB ? B + (n0 + bn) + (i64)bk * ldb : B + bk + (i64)(n0 + bn) * ldb;
+    const i64 a_step_i = TA ? (i64)ASM * lda : (i64)ASK * lda;   // advance per element i
+    const i64 b_step_i = TB ? (i64)BSK * ldb : (i64)BSN * ldb;
+    const i64 a_step_k = TA ? BK : (i64)BK * lda;               // advance per K slab
+    const i64 b_step_k = TB ? (i64)BK * ldb : BK;
 
     T ra[LA], rb[LB];
     auto load = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
-            int idx = t + 256 * i, m, k;
-            if (!TA) { m = idx % BM; k = idx / BM; } else { k = idx % BK; m = idx / BK; }
-            int gm = m0 + m, gk = k0 + k;
-            T v = 0;
-            if (gm < M && gk < K) v = TA ? A[gk + (i64)gm * lda] : A[gm + (i64)gk * lda];
-            ra[i] = v;
+            if (EDGE) {
+                const int gm = m0 + am + i * ASM, gk = k0 + ak + i * ASK;
+                ra[i] = (gm < M && gk < K) ? pa[i * a_step_i] : T(0);
+            } else {
+                ra[i] = pa[i * a_step_i];
+            }
         }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
-            int idx = t + 256 * i, n, k;
-            if (!TB) { k = idx % BK; n = idx / BK; } else { n = idx % BN; k = idx / BN; }
-            int gn = n0 + n, gk = k0 + k;
-            T v = 0;
-            if (gn < N && gk < K) v = TB ? B[gn + (i64)gk * ldb] : B[gk + (i64)gn * ldb];
-            rb[i] = v;
+            if (EDGE) {
+                const int gn = n0 + bn + i * BSN, gk = k0 + bk + i * BSK;
+                rb[i] = (gn < N && gk < K) ? pb[i * b_step_i] : T(0);
+            } else {
+                rb[i] = pb[i * b_step_i];
+            }
         }
+        pa += a_step_k;
+        pb += b_step_k;
     };
-    auto store = [&]() {
+    auto store = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < LA; ++i) {
-            int idx = t + 256 * i, m, k;
-            if (!TA) { m = idx % BM; k = idx / BM; } else { k = idx % BK; m = idx / BK; }
-            As[k][m] = ra[i];
-        }
+        for (int i = 0; i < LA; ++i) As[buf][ak + i * ASK][am + i * ASM] = ra[i];
 #pragma unroll
-        for (int i = 0; i < LB; ++i) {
-            int idx = t + 256 * i, n, k;
-            if (!TB) { k = idx % BK; n = idx / BK; } else { n = idx % BN; k = idx / BN; }
-            Bs[k][n] = rb[i];
-        }
+        for (int i = 0; i < LB; ++i) Bs[buf][bk + i * BSK][bn + i * BSN] = rb[i];
     };
 
     typename Acc<T>::t acc[FM][FN];
@@ -84,36 +91,40 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
 
     const int lr = lane & 15, lk = lane >> 4;
     load(0);
+    store(0);
+    __syncthreads();
+    int cur = 0;
     for (int k0 = 0; k0 < K; k0 += BK) {
-        __syncthreads();
-        store();
-        __syncthreads();
-        if (k0 + BK < K) load(k0 + BK);
+        const bool more = k0 + BK < K;
+        if (more) load(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             T a[FM], b[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) a[i] = As[kk + lk][wm * (BM / 2) + i * 16 + lr];
+            for (int i = 0; i < FM; ++i) a[i] = As[cur][kk + lk][wm * (BM / 2) + i * 16 + lr];
 #pragma unroll
-            for (int j = 0; j < FN; ++j) b[j] = Bs[kk + lk][wn * (BN / 2) + j * 16 + lr];
+            for (int j = 0; j < FN; ++j) b[j] = Bs[cur][kk + lk][wn * (BN / 2) + j * 16 + lr];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
                 for (int j = 0; j < FN; ++j) mfma16(acc[i][j], b[j], a[i]);
         }
+        if (more) store(cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
     }
 
     // acc[i][j] element r of lane l is C[m0 + wm*BM/2 + 16 i + (l%16), n0 + wn*BN/2 + 16 j + acc_row(l/16, r)]
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int gm = m0 + wm * (BM / 2) + i * 16 + lr;
-        if (gm >= M) continue;
+        if (EDGE && gm >= M) continue;
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gn = n0 + wn * (BN / 2) + j * 16 + acc_row<T>(lk, r);
-                if (gn < N) {
+                if (!EDGE || gn < N) {
                     T *c = &C[gm + (i64)gn * ldc];
                     T v = alpha * acc[i][j][r];
                     if (beta != T(0)) v += beta * *c;
@@ -124,16 +135,24 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
     }
 }
 
-template <typename T, int BM, int BN, int BK>
+template <typename T, int BM, int BN, int BK, bool EDGE>
 static hipError_t launch_tiles(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, i64 lda, const T *B,
                                i64 ldb, T beta, T *C, i64 ldc, hipStream_t s) {
     const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
     dim3 grid(tm * tn), block(256);
-    if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
-    else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
-    else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
-    else gemm_kernel<T, BM, BN, BK, true, true><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false, EDGE><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false, EDGE><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true, EDGE><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else gemm_kernel<T, BM, BN, BK, true, true, EDGE><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
     return hipGetLastError();
+}
+
+template <typename T, int BM, int BN, int BK>
+static hipError_t launch_sized(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, i64 lda, const T *B,
+                               i64 ldb, T beta, T *C, i64 ldc, hipStream_t s) {
+    if (M % BM == 0 && N % BN == 0 && K % BK == 0 && K > 0)
+        return launch_tiles<T, BM, BN, BK, false>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch_tiles<T, BM, BN, BK, true>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 template <typename T>
@@ -148,8 +167,8 @@ hipError_t gemm(char ta, char tb, i64 M, i64 N, i64 K, T alpha, const T *A, i64 
     ProfScope prof(PK_GEMM, s);
     const i64 big_tiles = ((M + 127) / 128) * ((N + 127) / 128);
     if (big_tiles >= 512)
-        return launch_tiles<T, 128, 128, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-    return launch_tiles<T, 64, 64, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+        return launch_sized<T, 128, 128, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch_sized<T, 64, 64, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 template hipError_t gemm<double>(char, char, i64, i64, i64, double, const double *, i64, const double *, i64, double,

@@ -210,16 +210,30 @@ __global__ __launch_bounds__(256) void trsm_blk_kernel(int n, int m, const T *__
         const int kb = UP ? r0 + DB : 0, ke = UP ? n : r0;
         typename Acc<T>::t acc0 = acc_zero<T>(), acc1 = acc_zero<T>();
         __syncthreads();
-        for (int k0 = kb + 4 * w; k0 < ke; k0 += 16) {
-            const int kc = k0 + lk;
-            T bf = Xs[kc * XS + lr];                       // rows >= n are zero in Xs
-            T a0 = 0, a1 = 0;
-            if (kc < n) {
-                if (r0 + lr < n) a0 = opa<T, TRA>(A, lda, r0 + lr, kc);
-                if (r0 + 16 + lr < n) a1 = opa<T, TRA>(A, lda, r0 + 16 + lr, kc);
+        // all A fragments of a chunk are requested before the first MFMA: one memory latency
+        // per chunk of MAXIT k-steps instead of one per k-step
+        constexpr int MAXIT = 16;
+        for (int kc0 = kb + 4 * w; kc0 < ke; kc0 += 16 * MAXIT) {
+            T ab0[MAXIT], ab1[MAXIT];
+#pragma unroll
+            for (int it = 0; it < MAXIT; ++it) {
+                const int kc = kc0 + 16 * it + lk;
+                T a0 = 0, a1 = 0;
+                if (kc0 + 16 * it < ke && kc < n) {
+                    if (r0 + lr < n) a0 = opa<T, TRA>(A, lda, r0 + lr, kc);
+                    if (r0 + 16 + lr < n) a1 = opa<T, TRA>(A, lda, r0 + 16 + lr, kc);
+                }
+                ab0[it] = a0;
+                ab1[it] = a1;
             }
-            mfma16(acc0, a0, bf);
-            mfma16(acc1, a1, bf);
+#pragma unroll
+            for (int it = 0; it < MAXIT; ++it) {
+                if (kc0 + 16 * it < ke) {
+                    const T bf = Xs[(kc0 + 16 * it + lk) * XS + lr];  // rows >= n are zero in Xs
+                    mfma16(acc0, ab0[it], bf);
+                    mfma16(acc1, ab1[it], bf);
+                }
+            }
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {

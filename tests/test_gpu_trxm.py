@@ -121,3 +121,21 @@ def test_nextlamatrix_trsm():
     nl.NextLAMatrix.trsm("L", "L", "N", "N", 1.0, nl.NextLAMatrix(A), B)
     assert relerr(np.tril(A) @ B.data, B0) < 1e-13
     assert B.size() == (n, 7) and len(B) == n * 7
+
+
+@pytest.mark.parametrize("side", "LR")
+@pytest.mark.parametrize("uplo", "LU")
+@pytest.mark.parametrize("trans", "NT")
+@pytest.mark.parametrize("func", "SM")
+def test_rectrxm_interior_tiles_vs_oracle(side, uplo, trans, func):
+    # n = 1024 with cut-off 128: every GEMM update takes the unchecked interior-tile path,
+    # in all four transpose combinations of its operands
+    rng = np.random.default_rng(abs(hash((side, uplo, trans, func))) % 997)
+    n, m = 1024, 128
+    A = tri_matrix(rng, n, uplo, np.float64)
+    B0 = F(rng.standard_normal((n, m) if side == "L" else (m, n)))
+    dB = nl.to_device(B0)
+    nl.unified_rectrxm_(side, uplo, trans, 1.0, func, nl.to_device(A), dB, cutoff=128)
+    B = B0.copy(order="F")
+    O.rectrxm(side, uplo, trans, 1.0, func, A, B, threshold=128)
+    assert relerr(host(dB), B) < 1e-12
