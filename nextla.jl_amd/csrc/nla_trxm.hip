@@ -203,38 +203,65 @@ __global__ __launch_bounds__(256) void trsm_blk_kernel(int n, int m, const T *__
         Xs[r * XS + c] = v;
     }
     const int nsb = npad / DB;
+    // Software pipeline over steps: the A fragments (first chunk of the K range) and the
+    // inverted diagonal block of step s+1 are requested while step s reduces and applies
+    // its own inverse, so global-memory latency is paid once per call, not once per step.
+    constexpr int MAXIT = 16;
+    T ab0[MAXIT], ab1[MAXIT], dv[DB / 4];
+    auto issue = [&](int step) {
+        const int sb = UP ? nsb - 1 - step : step;
+        const int r0 = sb * DB;
+        const int kb = UP ? r0 + DB : 0, ke = UP ? n : r0;
+        const int kc0 = kb + 4 * w;
+#pragma unroll
+        for (int it = 0; it < MAXIT; ++it) {
+            const int kc = kc0 + 16 * it + lk;
+            T a0 = 0, a1 = 0;
+            if (kc0 + 16 * it < ke && kc < n) {
+                if (r0 + lr < n) a0 = opa<T, TRA>(A, lda, r0 + lr, kc);
+                if (r0 + 16 + lr < n) a1 = opa<T, TRA>(A, lda, r0 + 16 + lr, kc);
+            }
+            ab0[it] = a0;
+            ab1[it] = a1;
+        }
+        if (w < 2) {
+            const T *D = Dinv + (size_t)sb * DB * DB;
+#pragma unroll
+            for (int q = 0; q < DB / 4; ++q) dv[q] = D[(16 * w + lr) + DB * (4 * q + lk)];
+        }
+    };
+    issue(0);
     for (int step = 0; step < nsb; ++step) {
         const int sb = UP ? nsb - 1 - step : step;
         const int r0 = sb * DB;
-        // K range of already-solved rows
         const int kb = UP ? r0 + DB : 0, ke = UP ? n : r0;
         typename Acc<T>::t acc0 = acc_zero<T>(), acc1 = acc_zero<T>();
         __syncthreads();
-        // all A fragments of a chunk are requested before the first MFMA: one memory latency
-        // per chunk of MAXIT k-steps instead of one per k-step
-        constexpr int MAXIT = 16;
-        for (int kc0 = kb + 4 * w; kc0 < ke; kc0 += 16 * MAXIT) {
-            T ab0[MAXIT], ab1[MAXIT];
+        const int kc0 = kb + 4 * w;
 #pragma unroll
-            for (int it = 0; it < MAXIT; ++it) {
-                const int kc = kc0 + 16 * it + lk;
-                T a0 = 0, a1 = 0;
-                if (kc0 + 16 * it < ke && kc < n) {
-                    if (r0 + lr < n) a0 = opa<T, TRA>(A, lda, r0 + lr, kc);
-                    if (r0 + 16 + lr < n) a1 = opa<T, TRA>(A, lda, r0 + 16 + lr, kc);
-                }
-                ab0[it] = a0;
-                ab1[it] = a1;
-            }
-#pragma unroll
-            for (int it = 0; it < MAXIT; ++it) {
-                if (kc0 + 16 * it < ke) {
-                    const T bf = Xs[(kc0 + 16 * it + lk) * XS + lr];  // rows >= n are zero in Xs
-                    mfma16(acc0, ab0[it], bf);
-                    mfma16(acc1, ab1[it], bf);
-                }
+        for (int it = 0; it < MAXIT; ++it) {
+            if (kc0 + 16 * it < ke) {
+                const T bf = Xs[(kc0 + 16 * it + lk) * XS + lr];  // rows >= n are zero in Xs
+                mfma16(acc0, ab0[it], bf);
+                mfma16(acc1, ab1[it], bf);
             }
         }
+        // K ranges longer than one chunk (base blocks over 256 rows): remaining chunks load in place
+        for (int kc1 = kc0 + 16 * MAXIT; kc1 < ke; kc1 += 16) {
+            const int kc = kc1 + lk;
+            T a0 = 0, a1 = 0;
+            if (kc < n) {
+                if (r0 + lr < n) a0 = opa<T, TRA>(A, lda, r0 + lr, kc);
+                if (r0 + 16 + lr < n) a1 = opa<T, TRA>(A, lda, r0 + 16 + lr, kc);
+            }
+            const T bf = Xs[kc * XS + lr];
+            mfma16(acc0, a0, bf);
+            mfma16(acc1, a1, bf);
+        }
+        T dcur[DB / 4];
+#pragma unroll
+        for (int q = 0; q < DB / 4; ++q) dcur[q] = dv[q];
+        if (step + 1 < nsb) issue(step + 1);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = acc_row<T>(lk, r);
@@ -252,14 +279,9 @@ __global__ __launch_bounds__(256) void trsm_blk_kernel(int n, int m, const T *__
         __syncthreads();
         if (w < 2) {
             // X_s rows 16w.. = Dinv_s[16w.., :] * Y
-            const T *D = Dinv + (size_t)sb * DB * DB;
             typename Acc<T>::t acc = acc_zero<T>();
 #pragma unroll
-            for (int k0 = 0; k0 < DB; k0 += 4) {
-                const T a = D[(16 * w + lr) + DB * (k0 + lk)];
-                const T bf = Ys[(k0 + lk) * XS + lr];
-                mfma16(acc, a, bf);
-            }
+            for (int q = 0; q < DB / 4; ++q) mfma16(acc, dcur[q], Ys[(4 * q + lk) * XS + lr]);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = r0 + 16 * w + acc_row<T>(lk, r);
