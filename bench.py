@@ -168,9 +168,9 @@ def main():
         tm = time_steps(torch, trmm, max(3, args.steps // 2), 1)
         extras["trmm_f64_tflops"] = round(flops / tm / 1e12, 3)
 
-        # tile QR, Float64, nb = 256, ib = 32
+        # tile QR, Float64, nb = 128, ib = 32
         qn = args.qr_n
-        nb, ib = (256, 32) if qn >= 4096 else (128, 32)
+        nb, ib = 128, 32
         Q0 = nl.colmajor(torch.randn((qn, qn), generator=gen, device="cuda", dtype=torch.float64))
         Qw = nl.colmajor(torch.empty_like(Q0))
         holder = {}

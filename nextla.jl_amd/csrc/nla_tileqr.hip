@@ -9,6 +9,9 @@
 namespace nla {
 
 template <typename T>
+int tile_geqrf_dag(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s, bool *handled);
+
+template <typename T>
 static int tile_geqrf_entry(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, void *stream) {
     if (m < 0) return arg_error(1, "m must be non-negative");
     if (n < 0) return arg_error(2, "n must be non-negative");
@@ -21,6 +24,12 @@ static int tile_geqrf_entry(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, 
     if (ldt < mt * ib) return arg_error(8, "ldt must be >= (m/nb)*ib");
     if (ib > 256) return arg_error(4, "ib must be <= 256");
     hipStream_t s = as_stream(stream);
+    {
+        bool handled = false;
+        const int r = tile_geqrf_dag<T>(m, n, nb, ib, A, lda, Tm, ldt, s, &handled);
+        if (handled) return r < 0 ? 1 : r;
+    }
+    // other (nb, ib): host loop over per-step launches
     int st = 0;
     T *tau = (T *)scratch(SLOT_AUX, sizeof(T) * (size_t)nb, s, &st);
     if (!tau) return st;

@@ -291,3 +291,30 @@ def test_python_errors_match_reference():
         nl.geqrt_(4, 4, 0, A, dev(np.zeros((1, 4))), torch.zeros(4, dtype=torch.float64, device="cuda"))
     with pytest.raises(nl.ArgumentError, match="For side='L', A1 and A2 must have same number of columns"):
         nl.tsmqr("L", "N", A, dev(np.zeros((4, 3))), A, dev(np.zeros((2, 4))))
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("m,n,nb", [(128, 128, 64), (256, 512, 64), (512, 256, 64), (640, 640, 128), (1024, 384, 128)])
+def test_tile_geqrf_dataflow_shapes_vs_oracle(dt, m, n, nb):
+    # square, wide and tall tile grids through the persistent dataflow kernel (ib = 32)
+    rng = np.random.default_rng(m + 3 * n + nb)
+    A0 = F(rng.standard_normal((m, n)).astype(dt))
+    dA = dev(A0)
+    dT = nl.tile_geqrf(dA, nb, 32)
+    A = A0.copy(order="F")
+    T = O.tile_geqrf(A, nb, 32)
+    assert relerr(host(dA), A) < 10 * tol(dt)
+    assert relerr(host(dT), T) < 100 * tol(dt)
+
+
+def test_tile_geqrf_repeatable():
+    # the dataflow schedule varies run to run; the arithmetic of every tile does not
+    rng = np.random.default_rng(77)
+    A0 = F(rng.standard_normal((512, 512)))
+    outs = []
+    for _ in range(3):
+        dA = dev(A0)
+        dT = nl.tile_geqrf(dA, 128, 32)
+        outs.append((host(dA), host(dT)))
+    for a, t in outs[1:]:
+        assert np.array_equal(a, outs[0][0]) and np.array_equal(t, outs[0][1])
