@@ -80,449 +80,476 @@ __device__ void publish(int *p, int v) {
 }
 
 // ---------------------------------------------------------------------------------
-// LDS layout (elements of T) shared by all task types.
-template <typename T, int NB> struct Smem {
-    static constexpr int LDX = NB + 1;                // strip buffers: [SW][LDX] (column c, row r)
-    static constexpr int A1 = 0;                      // walker's A_kj strip / panel's R rows
-    static constexpr int ST = A1 + SW * LDX;          // staging for global <-> accumulator moves
-    static constexpr int P = ST + SW * LDX;           // W0 partials [4][IB/16][16][17]
-    static constexpr int W0 = P + 4 * (IB / 16) * 16 * 17;  // [IB][SW+1]
-    static constexpr int WF = W0 + IB * (SW + 1);           // [IB][SW+1]
-    static constexpr int APPLY_END = WF + IB * (SW + 1);
-    // panel block factorization (overlays the apply area; both are never live together)
-    static constexpr int F = 0;                       // [IB][LDX] block columns
-    static constexpr int RB = F + IB * LDX;           // [IB][IB+1] R block (tsqrt)
-    static constexpr int TB = RB + IB * (IB + 1);     // [IB][IB+1] T block
-    static constexpr int DS = TB + IB * (IB + 1);     // [IB] dots
-    static constexpr int WS = DS + IB;                // [IB] w / z
-    static constexpr int PANEL_END = WS + IB;
-    static constexpr int TOTAL = APPLY_END > PANEL_END ? APPLY_END : PANEL_END;
+// Data layout.  Every wave owns a strip of SW = 16 columns and NB rows of the matrix in MFMA
+// accumulator registers (NB/16 tiles of 16x16; element (tile rt, lane l, reg r) is row
+// 16 rt + acc_row(l/16, r), column l % 16), so a block reflector is applied to a strip by
+// one wave with no cross-wave reduction.  The block's reflectors V (NB x IB) and T (IB x IB)
+// are staged once in LDS and shared by the four waves (four strips = TW columns).
+constexpr int WAVES = NTH / 64;
+constexpr int TW = SW * WAVES;  // matrix columns per workgroup task
+
+template <typename T, int NB> struct Lds {
+    static constexpr int LDV = NB + 2;                 // Vs[j * LDV + p] = V[p][j]
+    static constexpr int V = 0;
+    static constexpr int TT = V + IB * LDV;            // Ts[q * (IB+1) + j] = T[q][j] (zero below diag)
+    static constexpr int RS = TT + IB * (IB + 1);      // Rs[i * (IB+1) + j] = R block (tsqrt)
+    static constexpr int TOTAL = RS + IB * (IB + 1);
 };
 
-// ---------------------------------------------------------------------------------
-// Strip register file: wave w owns rows [w*NB/4, (w+1)*NB/4) of an NB x SW strip as RT
-// 16x16 accumulator tiles; element (lane l, reg r) of tile rt is row
-// w*NB/4 + 16 rt + acc_row(l/16, r), column l % 16.
-template <typename T, int NB> struct Strip {
-    static constexpr int RT = NB / 64;
-    typename Acc<T>::t a[RT];
+// LDS base with its address space visible at every use (a pointer passed through a call
+// would degrade to FLAT accesses).
+template <typename T> __device__ __forceinline__ T *lds_base() {
+    extern __shared__ __align__(16) unsigned char dyn_raw[];
+    return reinterpret_cast<T *>(dyn_raw);
+}
+
+template <typename T, int NB> struct WStrip {
+    typename Acc<T>::t a[NB / 16];
 };
 
-template <typename T, int NB>
-__device__ void strip_load(Strip<T, NB> &S, const T *G, i64 ld, int ncols, T *sm) {
-    constexpr int LDX = Smem<T, NB>::LDX;
-    T *st = sm + Smem<T, NB>::ST;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    __syncthreads();
-    for (int idx = t; idx < NB * SW; idx += NTH) {
-        const int r = idx % NB, c = idx / NB;
-        st[c * LDX + r] = c < ncols ? cld(&G[r + (i64)c * ld]) : T(0);
-    }
-    __syncthreads();
+// rows [r0, r0 + 16*NT) of a 16-column strip starting at G (column-major, ld)
+template <typename T, int NT>
+__device__ __forceinline__ void tiles_load(typename Acc<T>::t *a, const T *G, i64 ld, int r0) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int rt = 0; rt < Strip<T, NB>::RT; ++rt)
+    for (int rt = 0; rt < NT; ++rt)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            S.a[rt][r] = st[(lane & 15) * LDX + w * (NB / 4) + 16 * rt + acc_row<T>(lane >> 4, r)];
+        for (int r = 0; r < 4; ++r) a[rt][r] = cld(&G[(r0 + 16 * rt + acc_row<T>(lk, r)) + (i64)lr * ld]);
+}
+template <typename T, int NT>
+__device__ __forceinline__ void tiles_store(const typename Acc<T>::t *a, T *G, i64 ld, int r0) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cst(&G[(r0 + 16 * rt + acc_row<T>(lk, r)) + (i64)lr * ld], a[rt][r]);
 }
 
-template <typename T, int NB>
-__device__ void strip_store(const Strip<T, NB> &S, T *G, i64 ld, int ncols, T *sm) {
-    constexpr int LDX = Smem<T, NB>::LDX;
-    T *st = sm + Smem<T, NB>::ST;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    __syncthreads();
-#pragma unroll
-    for (int rt = 0; rt < Strip<T, NB>::RT; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            st[(lane & 15) * LDX + w * (NB / 4) + 16 * rt + acc_row<T>(lane >> 4, r)] = S.a[rt][r];
-    __syncthreads();
-    for (int idx = t; idx < NB * SW; idx += NTH) {
-        const int r = idx % NB, c = idx / NB;
-        if (c < ncols) cst(&G[r + (i64)c * ld], st[c * LDX + r]);
-    }
-}
-
-// Reflector element of block column jrel (absolute tile column jabs) at tile row p.  Vb
-// points at the block's first column.  TS: full column; GE: unit lower part of a geqrt
-// tile (1 on the diagonal, 0 above, where the tile holds R).
-template <typename T, bool GE>
-__device__ __forceinline__ T vel(const T *Vb, i64 ldv, int p, int jrel, int jabs) {
-    if (GE) {
-        if (p < jabs) return T(0);
-        if (p == jabs) return T(1);
-    }
-    return cld(&Vb[p + (i64)jrel * ldv]);
-}
-
-// Apply op(H_b) of one inner block (reflectors at tile columns j0.. j0+kb-1, kb <= IB;
-// V points at column j0) from the left to
-//   [A1s rows j0..j0+kb-1 ; strip]   (TS: V = full NB x kb, A1s = LDS strip)
-//   [strip]                          (GE: V unit lower inside the strip rows)
-// W0 = V^T strip (+ A1 rows), W = op(T_b) W0, A1 rows -= W, strip -= V W.
-// notrans: op(T) = T (apply H); otherwise T^T (apply H^T, the reference's trans 'C').
+// Stage block reflectors (columns j0.. j0+IB-1 of the tile, Vg = first of them) and T block.
+// GE: unit lower part of a geqrt tile (1 on the diagonal, 0 above); TS: full columns.
 template <typename T, int NB, bool GE>
-__device__ void apply_block(Strip<T, NB> &S, const T *V, i64 ldv, const T *Tb, i64 ldt, int j0, int kb, bool notrans,
-                            T *sm) {
-    constexpr int LDX = Smem<T, NB>::LDX;
-    constexpr int RT = Strip<T, NB>::RT;
-    T *P = sm + Smem<T, NB>::P;
-    T *W0 = sm + Smem<T, NB>::W0;
-    T *WF = sm + Smem<T, NB>::WF;
-    T *A1s = sm + Smem<T, NB>::A1;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const int rbase = w * (NB / 4);
+__device__ void stage_block(const T *Vg, i64 ldv, int j0, const T *Tg, i64 ldt) {
+    T *sm = lds_base<T>();
+    constexpr int LDV = Lds<T, NB>::LDV;
+    T *Vs = sm + Lds<T, NB>::V;
+    T *Ts = sm + Lds<T, NB>::TT;
+    for (int idx = threadIdx.x; idx < NB * IB; idx += NTH) {
+        const int p = idx % NB, j = idx / NB;
+        T v;
+        v = cld(&Vg[p + (i64)j * ldv]);
+        if (GE) {
+            const int ja = j0 + j;
+            v = p < ja ? T(0) : (p == ja ? T(1) : v);
+        }
+        Vs[j * LDV + p] = v;
+    }
+    for (int idx = threadIdx.x; idx < IB * IB; idx += NTH) {
+        const int q = idx % IB, j = idx / IB;
+        Ts[q * (IB + 1) + j] = q <= j ? cld(&Tg[q + (i64)j * ldt]) : T(0);
+    }
+}
 
-    // W0 partial over this wave's rows: output tile ot = reflectors j0 + 16 ot + (0..15)
+// op(H) strip with H = I - V T V^T of the staged block (one wave, registers only):
+//   W0 = V^T S (+ A1 rows),  W = op(T) W0,  A1 rows -= W,  S -= V W.
+// notrans: op(T) = T (apply H); otherwise T^T (apply H^T: the reference's trans 'C').
+template <typename T, int NB, bool GE>
+__device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t *a1, bool notrans) {
+    const T *sm = lds_base<T>();
+    constexpr int LDV = Lds<T, NB>::LDV;
+    constexpr int RT = NB / 16;
+    const T *Vs = sm + Lds<T, NB>::V;
+    const T *Ts = sm + Lds<T, NB>::TT;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    typename Acc<T>::t w0[2] = {acc_zero<T>(), acc_zero<T>()};
 #pragma unroll
-    for (int ot = 0; ot < IB / 16; ++ot) {
-        typename Acc<T>::t acc = acc_zero<T>();
-        const int jrel = 16 * ot + lr;
-        const bool jok = jrel < kb;
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
+        for (int r = 0; r < 4; ++r) {
+            const int p = 16 * rt + acc_row<T>(lk, r);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int p = rbase + 16 * rt + acc_row<T>(lk, r);
-                const T a = jok ? vel<T, GE>(V, ldv, p, jrel, j0 + jrel) : T(0);
-                mfma16(acc, a, S.a[rt][r]);
+            for (int ot = 0; ot < 2; ++ot) mfma16(w0[ot], Vs[(16 * ot + lr) * LDV + p], S.a[rt][r]);
+        }
+    if (!GE) {
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) w0[ot][r] += a1[ot][r];
+    }
+    typename Acc<T>::t wv[2] = {acc_zero<T>(), acc_zero<T>()};
+#pragma unroll
+    for (int tq = 0; tq < 2; ++tq)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int q = 16 * tq + acc_row<T>(lk, r);
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) {
+                const int j = 16 * ot + lr;
+                const T a = notrans ? Ts[j * (IB + 1) + q] : Ts[q * (IB + 1) + j];
+                mfma16(wv[ot], a, w0[tq][r]);
             }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) P[((w * (IB / 16) + ot) * 16 + acc_row<T>(lk, r)) * 17 + lr] = acc[r];
-    }
-    __syncthreads();
-    for (int idx = t; idx < IB * SW; idx += NTH) {
-        const int j = idx / SW, c = idx % SW, ot = j >> 4, jr = j & 15;
-        T s = 0;
-#pragma unroll
-        for (int ww = 0; ww < 4; ++ww) s += P[((ww * (IB / 16) + ot) * 16 + jr) * 17 + c];
-        if (!GE && j < kb) s += A1s[c * LDX + j0 + j];
-        W0[j * (SW + 1) + c] = j < kb ? s : T(0);
-    }
-    __syncthreads();
-    // W = op(T_b) W0 (upper triangle of T_b only)
-    if (w < IB / 16) {
-        typename Acc<T>::t acc = acc_zero<T>();
-        const int jj = 16 * w + lr;
-#pragma unroll
-        for (int k0 = 0; k0 < IB; k0 += 4) {
-            const int q = k0 + lk;
-            T a = 0;
-            if (jj < kb && q < kb) {
-                if (notrans) { if (jj <= q) a = cld(&Tb[jj + (i64)q * ldt]); }
-                else         { if (q <= jj) a = cld(&Tb[q + (i64)jj * ldt]); }
-            }
-            mfma16(acc, a, W0[q * (SW + 1) + lr]);
         }
+    if (!GE) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) WF[(16 * w + acc_row<T>(lk, r)) * (SW + 1) + lr] = acc[r];
+        for (int ot = 0; ot < 2; ++ot)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a1[ot][r] -= wv[ot][r];
     }
-    __syncthreads();
-    if (!GE)
-        for (int idx = t; idx < kb * SW; idx += NTH) {
-            const int j = idx % kb, c = idx / kb;
-            A1s[c * LDX + j0 + j] -= WF[j * (SW + 1) + c];
-        }
-    // strip -= V W
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-        const int p = rbase + 16 * rt + lr;
+    for (int tq = 0; tq < 2; ++tq)
 #pragma unroll
-        for (int k0 = 0; k0 < IB; k0 += 4) {
-            const int q = k0 + lk;
-            const T a = q < kb ? -vel<T, GE>(V, ldv, p, q, j0 + q) : T(0);
-            mfma16(S.a[rt], a, WF[q * (SW + 1) + lr]);
+        for (int r = 0; r < 4; ++r) {
+            const int q = 16 * tq + acc_row<T>(lk, r);
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) mfma16(S.a[rt], -Vs[q * LDV + 16 * rt + lr], wv[tq][r]);
         }
-    }
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------
-// Panel block factorizations (LDS resident, one 256-thread workgroup).
-// 8 threads per block column compute all column dots with the pivot column in one pass
-// (norm included), reduced by shuffles inside the 8-lane group; then every thread derives
-// beta/tau/scale itself.  Three barriers per column.
+// Single-wave block factorization.  The wave holds the IB block columns of the tile in
+// registers (lane l: rows l, l+64, ...); per column, the dots of the pivot column with all
+// block columns (its norm included) are summed by a butterfly reduce-scatter (32 shuffles)
+// and gathered with readlane, so a column costs no LDS round trip and no workgroup barrier.
+template <typename T> __device__ __forceinline__ T readlane_t(T v, int l);
+template <> __device__ __forceinline__ double readlane_t<double>(double v, int l) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+template <> __device__ __forceinline__ float readlane_t<float>(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 
-// geqr2! + larft! on block columns [c0, c0+kb) of an NB x NB tile A (rows c0..NB-1).
-template <typename T, int NB>
-__device__ void ge_block_factor(T *A, i64 lda, T *tau, T *Tm, i64 ldt, int c0, int kb, T *sm) {
-    constexpr int LDX = Smem<T, NB>::LDX;
-    T *F = sm + Smem<T, NB>::F;
-    T *TBs = sm + Smem<T, NB>::TB;
-    T *Ds = sm + Smem<T, NB>::DS;
-    T *Ws = sm + Smem<T, NB>::WS;
-    const int t = threadIdx.x;
-    __syncthreads();
-    for (int idx = t; idx < kb * NB; idx += NTH) {
-        const int r = idx % NB, c = idx / NB;
-        F[c * LDX + r] = cld(&A[r + (i64)(c0 + c) * lda]);
+// v[0..15] per lane -> g[j] = sum over the wave of v[j] (wave-uniform), butterfly
+// reduce-scatter (8+4+2+1+1+1 shuffles) then readlane gather.
+template <typename T> __device__ __forceinline__ T wave_allreduce16(const T (&v)[16], T *g) {
+    const int lane = threadIdx.x & 63;
+    T a8[8], a4[4], a2[2];
+    bool h = (lane >> 5) & 1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const T keep = h ? v[k + 8] : v[k], send = h ? v[k] : v[k + 8];
+        a8[k] = keep + __shfl_xor(send, 32, 64);
     }
-    __syncthreads();
-    const int gj = t >> 3, part = t & 7;  // 32 columns x 8 threads
-    for (int c = 0; c < kb; ++c) {
-        const int cc = c0 + c;  // pivot row
-        if (gj < kb) {
-            T s = 0;
-            for (int p = cc + 1 + part; p < NB; p += 8) s += F[gj * LDX + p] * F[c * LDX + p];
-            s += __shfl_xor(s, 1, 8);
-            s += __shfl_xor(s, 2, 8);
-            s += __shfl_xor(s, 4, 8);
-            if (part == 0) Ds[gj] = s;
+    h = (lane >> 4) & 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const T keep = h ? a8[k + 4] : a8[k], send = h ? a8[k] : a8[k + 4];
+        a4[k] = keep + __shfl_xor(send, 16, 64);
+    }
+    h = (lane >> 3) & 1;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const T keep = h ? a4[k + 2] : a4[k], send = h ? a4[k] : a4[k + 2];
+        a2[k] = keep + __shfl_xor(send, 8, 64);
+    }
+    h = (lane >> 2) & 1;
+    T a1 = (h ? a2[1] : a2[0]) + __shfl_xor(h ? a2[0] : a2[1], 4, 64);
+    a1 += __shfl_xor(a1, 2, 64);
+    a1 += __shfl_xor(a1, 1, 64);
+    // lane l now holds the sum of value (l >> 2)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) g[j] = readlane_t<T>(a1, 4 * j);
+    return __shfl(a1, 4 * (lane & 15), 64);  // lane l: sum of value l % 16
+}
+
+// GE: geqr2!+larft! of the IB block columns held in Xs (LDS, Xs[j*LDV + p], tile rows p),
+//     pivot rows c0 + c (rows < pivot untouched);
+// TS: tsqrt block of [R block (Rs, LDS) ; A2 block (Xs)].
+// One wave; writes the T block (upper) into Ts.  The column loop stays rolled (its body is
+// large); every loop inside it is over a compile-time range, so no register array is ever
+// indexed at run time.
+template <typename T, int NB, bool GE>
+__device__ void wave_factor(T *Xs, int c0, T *Rs, T *Ts) {
+    constexpr int RPL = NB / 64, LDV = Lds<T, NB>::LDV;
+    const int lane = threadIdx.x & 63;
+    const int li = lane < IB ? lane : IB - 1;  // clamped lane for per-reflector LDS rows
+#pragma unroll 1
+    for (int c = 0; c < IB; ++c) {
+        const int cc = c0 + c;
+        // pivot column, masked to the rows below the pivot (GE) -- selects, no divergent branches
+        T xc[RPL], xm[RPL];
+#pragma unroll
+        for (int h = 0; h < RPL; ++h) {
+            const int p = lane + 64 * h;
+            xc[h] = Xs[c * LDV + p];
+            xm[h] = (!GE || p > cc) ? xc[h] : T(0);
         }
-        __syncthreads();
-        const T alpha = F[c * LDX + cc], xs = Ds[c];
+        T g[IB], gl = 0;
+#pragma unroll
+        for (int half = 0; half < IB / 16; ++half) {
+            T v[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = 16 * half + jj;
+                T sacc = 0;
+#pragma unroll
+                for (int h = 0; h < RPL; ++h) sacc += Xs[j * LDV + lane + 64 * h] * xm[h];
+                v[jj] = sacc;
+            }
+            const T gh = wave_allreduce16<T>(v, &g[16 * half]);
+            if ((lane >> 4) == half) gl = gh;  // lane j < IB: g[j]
+        }
+        T xs = 0;
+#pragma unroll
+        for (int j = 0; j < IB; ++j) xs = (j == c) ? g[j] : xs;
+        const T alpha = GE ? Xs[c * LDV + cc] : Rs[c * (IB + 1) + c];
         T beta = alpha, tv = 0, scale = 0;
         if (xs != T(0)) {
             beta = -copysign(sqrt(alpha * alpha + xs), alpha);
             tv = (beta - alpha) / beta;
             scale = T(1) / (alpha - beta);
         }
-        if (t < kb && t != c) Ws[t] = F[t * LDX + cc] + Ds[t] * scale;  // w_j (j > c), z_j (j < c)
-        __syncthreads();
         if (tv != T(0)) {
-            for (int p = cc + 1 + t; p < NB; p += NTH) {
-                const T v = F[c * LDX + p] * scale;
-                F[c * LDX + p] = v;
-                for (int j = c + 1; j < kb; ++j) F[j * LDX + p] -= tv * v * Ws[j];
+            // row c of the block: GE = tile row cc, TS = R block row c (read before any update)
+            T rowc[IB];
+#pragma unroll
+            for (int j = 0; j < IB; ++j) rowc[j] = GE ? Xs[j * LDV + cc] : Rs[c * (IB + 1) + j];
+            const T rl = GE ? T(0) : Rs[c * (IB + 1) + li];
+            __builtin_amdgcn_wave_barrier();
+            // x_p <- v_p (rows below the pivot), beta on the pivot row; rank-1 update of the rest:
+            //   A[p][j] -= tau v_p w_j,  v_cc = 1
+            T coef[RPL];
+#pragma unroll
+            for (int h = 0; h < RPL; ++h) {
+                const int p = lane + 64 * h;
+                const bool below = !GE || p > cc, piv = GE && p == cc;
+                const T vp = xc[h] * scale;
+                coef[h] = below ? -tv * vp : (piv ? -tv : T(0));
+                Xs[c * LDV + p] = below ? vp : (piv ? beta : xc[h]);
             }
-            if (t > c && t < kb) F[t * LDX + cc] -= tv * Ws[t];
-        }
-        if (t < c) {
-            T s = 0;
-            if (tv != T(0))
-                for (int q = t; q < c; ++q) s += TBs[t * (IB + 1) + q] * (-tv * Ws[q]);
-            TBs[t * (IB + 1) + c] = s;
-        }
-        if (t == 0) {
-            F[c * LDX + cc] = beta;
-            TBs[c * (IB + 1) + c] = tv;
-            tau[cc] = tv;
-        }
-        __syncthreads();
-    }
-    for (int idx = t; idx < kb * NB; idx += NTH) {
-        const int r = idx % NB, c = idx / NB;
-        if (r >= c0) cst(&A[r + (i64)(c0 + c) * lda], F[c * LDX + r]);
-    }
-    for (int idx = t; idx < kb * kb; idx += NTH) {
-        const int i = idx % kb, j = idx / kb;
-        if (i <= j) cst(&Tm[i + (i64)(c0 + j) * ldt], TBs[i * (IB + 1) + j]);
-    }
-    __syncthreads();
-}
-
-// tsqrt block: [R (NB x NB upper, rows/cols c0..c0+kb-1 used) ; A2 (NB x NB)], block
-// columns [c0, c0+kb).  V = A2 columns, T block into Tm[:, c0..].
-template <typename T, int NB>
-__device__ void ts_block_factor(T *R, i64 ldr, T *A2, i64 lda2, T *tau, T *Tm, i64 ldt, int c0, int kb, T *sm) {
-    constexpr int LDX = Smem<T, NB>::LDX;
-    T *F = sm + Smem<T, NB>::F;
-    T *RBs = sm + Smem<T, NB>::RB;
-    T *TBs = sm + Smem<T, NB>::TB;
-    T *Ds = sm + Smem<T, NB>::DS;
-    T *Ws = sm + Smem<T, NB>::WS;
-    const int t = threadIdx.x;
-    __syncthreads();
-    for (int idx = t; idx < kb * NB; idx += NTH) {
-        const int r = idx % NB, c = idx / NB;
-        F[c * LDX + r] = cld(&A2[r + (i64)(c0 + c) * lda2]);
-    }
-    for (int idx = t; idx < kb * kb; idx += NTH) {
-        const int i = idx % kb, j = idx / kb;
-        RBs[i * (IB + 1) + j] = cld(&R[(c0 + i) + (i64)(c0 + j) * ldr]);
-    }
-    __syncthreads();
-    const int gj = t >> 3, part = t & 7;
-    for (int c = 0; c < kb; ++c) {
-        if (gj < kb) {
-            T s = 0;
-            for (int p = part; p < NB; p += 8) s += F[gj * LDX + p] * F[c * LDX + p];
-            s += __shfl_xor(s, 1, 8);
-            s += __shfl_xor(s, 2, 8);
-            s += __shfl_xor(s, 4, 8);
-            if (part == 0) Ds[gj] = s;
-        }
-        __syncthreads();
-        const T alpha = RBs[c * (IB + 1) + c], xs = Ds[c];
-        T beta = alpha, tv = 0, scale = 0;
-        if (xs != T(0)) {
-            beta = -copysign(sqrt(alpha * alpha + xs), alpha);
-            tv = (beta - alpha) / beta;
-            scale = T(1) / (alpha - beta);
-        }
-        if (t < kb && t != c) Ws[t] = (t > c ? RBs[c * (IB + 1) + t] : T(0)) + Ds[t] * scale;
-        __syncthreads();
-        if (tv != T(0)) {
-            for (int p = t; p < NB; p += NTH) {
-                const T v = F[c * LDX + p] * scale;
-                F[c * LDX + p] = v;
-                for (int j = c + 1; j < kb; ++j) F[j * LDX + p] -= tv * v * Ws[j];
+#pragma unroll
+            for (int j = 0; j < IB; ++j) {
+                if (j > c) {
+                    const T wj = rowc[j] + g[j] * scale;
+#pragma unroll
+                    for (int h = 0; h < RPL; ++h) Xs[j * LDV + lane + 64 * h] += coef[h] * wj;
+                }
             }
-            if (t > c && t < kb) RBs[c * (IB + 1) + t] -= tv * Ws[t];
+            if (!GE && lane > c && lane < IB) Rs[c * (IB + 1) + lane] = rl - tv * (rl + gl * scale);
+            // T column c: T[j][c] = -tau * sum_{q=j}^{c-1} T[j][q] z_q with z_q = (V^T v)_q
+            T sacc = 0;
+#pragma unroll
+            for (int q = 0; q < IB; ++q) {
+                if (q < c) {
+                    const T zq = (GE ? rowc[q] : T(0)) + g[q] * scale;
+                    sacc += (q >= lane ? Ts[li * (IB + 1) + q] : T(0)) * zq;
+                }
+            }
+            if (lane < c) Ts[lane * (IB + 1) + c] = -tv * sacc;
+        } else if (lane < c) {
+            Ts[lane * (IB + 1) + c] = T(0);
         }
-        if (t < c) {
-            T s = 0;
-            if (tv != T(0))
-                for (int q = t; q < c; ++q) s += TBs[t * (IB + 1) + q] * (-tv * Ws[q]);
-            TBs[t * (IB + 1) + c] = s;
-        }
-        if (t == 0) {
-            RBs[c * (IB + 1) + c] = beta;
-            TBs[c * (IB + 1) + c] = tv;
-            tau[c0 + c] = tv;
-        }
-        __syncthreads();
+        if (!GE && lane == c) Rs[c * (IB + 1) + c] = beta;
+        if (lane == c) Ts[c * (IB + 1) + c] = tv;
+        __builtin_amdgcn_wave_barrier();
     }
-    for (int idx = t; idx < kb * NB; idx += NTH) {
-        const int r = idx % NB, c = idx / NB;
-        cst(&A2[r + (i64)(c0 + c) * lda2], F[c * LDX + r]);
-    }
-    for (int idx = t; idx < kb * kb; idx += NTH) {
-        const int i = idx % kb, j = idx / kb;
-        if (i <= j) {
-            cst(&R[(c0 + i) + (i64)(c0 + j) * ldr], RBs[i * (IB + 1) + j]);
-            cst(&Tm[i + (i64)(c0 + j) * ldt], TBs[i * (IB + 1) + j]);
-        }
-    }
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------
-// Tile-level operations built from the block routines.
+// Tile operations (one workgroup).
 
-// geqrt!(NB, NB, IB, A, T, tau): src/geqrt.jl:44-67
-template <typename T, int NB> __device__ void tile_geqrt(T *A, i64 lda, T *Tm, i64 ldt, T *tau, T *sm) {
+// geqrt!(NB, NB, IB, A, T): src/geqrt.jl:44-67
+template <typename T, int NB> __device__ void tile_geqrt(T *A, i64 lda, T *Tm, i64 ldt) {
+    T *sm = lds_base<T>();
+    constexpr int LDV = Lds<T, NB>::LDV;
+    T *Vs = sm + Lds<T, NB>::V;
+    T *Ts = sm + Lds<T, NB>::TT;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     for (int c0 = 0; c0 < NB; c0 += IB) {
-        ge_block_factor<T, NB>(A, lda, tau, Tm, ldt, c0, IB, sm);
-        // larfb!('L','C') of this block on the tile's trailing columns, strip by strip
-        for (int s0 = c0 + IB; s0 < NB; s0 += SW) {
-            Strip<T, NB> S;
-            strip_load<T, NB>(S, A + (i64)s0 * lda, lda, SW, sm);
-            apply_block<T, NB, true>(S, A + (i64)c0 * lda, lda, Tm + (i64)c0 * ldt, ldt, c0, IB, false, sm);
-            strip_store<T, NB>(S, A + (i64)s0 * lda, lda, SW, sm);
+        __syncthreads();
+        for (int idx = t; idx < IB * (IB + 1); idx += NTH) Ts[idx] = T(0);
+        for (int idx = t; idx < NB * IB; idx += NTH) {
+            const int p = idx % NB, j = idx / NB;
+            Vs[j * LDV + p] = cld(&A[p + (i64)(c0 + j) * lda]);
         }
+        __syncthreads();
+        if (w == 0) {
+            wave_factor<T, NB, true>(Vs, c0, nullptr, Ts);
+            // write the factored block back, then leave the unit-lower V in Vs
+            for (int idx = lane; idx < NB * IB; idx += 64) {
+                const int p = idx % NB, j = idx / NB, ja = c0 + j;
+                const T v = Vs[j * LDV + p];
+                if (p >= c0) cst(&A[p + (i64)ja * lda], v);
+                Vs[j * LDV + p] = p < ja ? T(0) : (p == ja ? T(1) : v);
+            }
+            for (int idx = lane; idx < IB * IB; idx += 64) {
+                const int q = idx % IB, j = idx / IB;
+                if (q <= j) cst(&Tm[q + (i64)(c0 + j) * ldt], Ts[q * (IB + 1) + j]);
+            }
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        // larfb!('L','C') of this block on the trailing columns of the tile
+        const int nst = (NB - c0 - IB) / SW;
+        for (int st = w; st < nst; st += WAVES) {
+            T *G = A + (i64)(c0 + IB + st * SW) * lda;
+            WStrip<T, NB> S;
+            tiles_load<T, NB / 16>(S.a, G, lda, 0);
+            apply_wave<T, NB, true>(S, nullptr, false);
+            tiles_store<T, NB / 16>(S.a, G, lda, 0);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
+    __syncthreads();
+}
+
+__device__ unsigned long long *g_phase_trace = nullptr;  // debug: per-block phase stamps of one tile
+__device__ __forceinline__ void phase_stamp(int slot) {
+    if (g_phase_trace && threadIdx.x == 0) {
+        unsigned long long *p = g_phase_trace;
+        if (p[slot] == 0) p[slot] = __builtin_amdgcn_s_memrealtime();
     }
 }
 
-// Load A1 rows [r0, r0+nr) of a strip (columns s0..s0+SW-1 of R) into A1s.
+// tsqrt!(NB, NB, IB, R, A2, T): src/tsqrt.jl:146-155
 template <typename T, int NB>
-__device__ void a1_load(const T *R, i64 ldr, int r0, int nr, int ncols, T *sm) {
-    constexpr int LDX = Smem<T, NB>::LDX;
-    T *A1s = sm + Smem<T, NB>::A1;
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nr * SW; idx += NTH) {
-        const int r = idx % nr, c = idx / nr;
-        A1s[c * LDX + r0 + r] = c < ncols ? cld(&R[(r0 + r) + (i64)c * ldr]) : T(0);
-    }
-    __syncthreads();
-}
-template <typename T, int NB>
-__device__ void a1_store(T *R, i64 ldr, int r0, int nr, int ncols, T *sm) {
-    constexpr int LDX = Smem<T, NB>::LDX;
-    T *A1s = sm + Smem<T, NB>::A1;
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nr * SW; idx += NTH) {
-        const int r = idx % nr, c = idx / nr;
-        if (c < ncols) cst(&R[(r0 + r) + (i64)c * ldr], A1s[c * LDX + r0 + r]);
-    }
-    __syncthreads();
-}
-
-// tsqrt!(NB, NB, IB, R, A2, T, tau): src/tsqrt.jl:146-155
-template <typename T, int NB>
-__device__ void tile_tsqrt(T *R, i64 ldr, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau, T *sm) {
+__device__ void tile_tsqrt(T *R, i64 ldr, T *A2, i64 lda2, T *Tm, i64 ldt) {
+    T *sm = lds_base<T>();
+    constexpr int LDV = Lds<T, NB>::LDV;
+    T *Vs = sm + Lds<T, NB>::V;
+    T *Ts = sm + Lds<T, NB>::TT;
+    T *Rs = sm + Lds<T, NB>::RS;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     for (int c0 = 0; c0 < NB; c0 += IB) {
-        ts_block_factor<T, NB>(R, ldr, A2, lda2, tau, Tm, ldt, c0, IB, sm);
-        // tsmqr!('L','C', ...) of this block on the trailing columns
-        for (int s0 = c0 + IB; s0 < NB; s0 += SW) {
-            a1_load<T, NB>(R + (i64)s0 * ldr, ldr, c0, IB, SW, sm);
-            Strip<T, NB> S;
-            strip_load<T, NB>(S, A2 + (i64)s0 * lda2, lda2, SW, sm);
-            apply_block<T, NB, false>(S, A2 + (i64)c0 * lda2, lda2, Tm + (i64)c0 * ldt, ldt, c0, IB, false, sm);
-            strip_store<T, NB>(S, A2 + (i64)s0 * lda2, lda2, SW, sm);
-            a1_store<T, NB>(R + (i64)s0 * ldr, ldr, c0, IB, SW, sm);
+        __syncthreads();
+        for (int idx = t; idx < IB * (IB + 1); idx += NTH) Ts[idx] = T(0);
+        for (int idx = t; idx < IB * IB; idx += NTH) {
+            const int i = idx % IB, j = idx / IB;
+            Rs[i * (IB + 1) + j] = cld(&R[(c0 + i) + (i64)(c0 + j) * ldr]);
         }
+        for (int idx = t; idx < NB * IB; idx += NTH) {
+            const int p = idx % NB, j = idx / NB;
+            Vs[j * LDV + p] = cld(&A2[p + (i64)(c0 + j) * lda2]);
+        }
+        __syncthreads();
+        phase_stamp(4 * (c0 / IB) + 0);
+        if (w == 0) {
+            wave_factor<T, NB, false>(Vs, c0, Rs, Ts);
+            phase_stamp(4 * (c0 / IB) + 1);
+            for (int idx = lane; idx < NB * IB; idx += 64) {
+                const int p = idx % NB, j = idx / NB;
+                cst(&A2[p + (i64)(c0 + j) * lda2], Vs[j * LDV + p]);
+            }
+            for (int idx = lane; idx < IB * IB; idx += 64) {
+                const int i = idx % IB, j = idx / IB;
+                if (i <= j) {
+                    cst(&R[(c0 + i) + (i64)(c0 + j) * ldr], Rs[i * (IB + 1) + j]);
+                    cst(&Tm[i + (i64)(c0 + j) * ldt], Ts[i * (IB + 1) + j]);
+                }
+            }
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        phase_stamp(4 * (c0 / IB) + 2);
+        // tsmqr!('L','C') of this block on the trailing columns: A1 = R rows c0.., A2 strip
+        const int nst = (NB - c0 - IB) / SW;
+        for (int st = w; st < nst; st += WAVES) {
+            const i64 col = c0 + IB + st * SW;
+            typename Acc<T>::t a1[2];
+            tiles_load<T, 2>(a1, R + col * ldr, ldr, c0);
+            WStrip<T, NB> S;
+            tiles_load<T, NB / 16>(S.a, A2 + col * lda2, lda2, 0);
+            apply_wave<T, NB, false>(S, a1, false);
+            tiles_store<T, NB / 16>(S.a, A2 + col * lda2, lda2, 0);
+            tiles_store<T, 2>(a1, R + col * ldr, ldr, c0);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        phase_stamp(4 * (c0 / IB) + 3);
     }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------
 struct DagArgs {
-    int mt, nt, nstrips;      // tile rows/cols, SW-strips of the whole matrix
+    int mt, nt, nwalk;        // tile rows/cols, TW-column walker groups of the matrix
     int ntasks;
     const int *tasks;
     int *counter;             // next task
     int *panel_prog;          // [kt]
-    int *strip_prog;          // [kt][nstrips]
+    int *strip_prog;          // [kt][nwalk]
     int *err;
-    unsigned long long *trace;  // optional timeline: [ntasks][4] = claim, ready, end, workgroup
+    unsigned long long *trace;  // optional timeline: [ntasks][4] = claim, -, end, workgroup
     unsigned long long *ptrace; // optional: [kt][mt] panel tile completion times
 };
 
 template <typename T, int NB>
-__device__ __noinline__ void run_panel(int k, T *A, i64 lda, T *Tm, i64 ldt, T *tau, const DagArgs &g, T *sm,
-                                       int *sh_flag) {
-    constexpr int SPT = NB / SW;  // strips per tile column
+__device__ __noinline__ void run_panel(int k, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+    constexpr int WPT = NB / TW;  // walker groups per tile column
     T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
     T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
     if (k > 0)
-        for (int s = k * SPT; s < (k + 1) * SPT; ++s) wait_ge(&g.strip_prog[(k - 1) * g.nstrips + s], k + 1, g.err, sh_flag);
-    tile_geqrt<T, NB>(Akk, lda, Tkk, ldt, tau, sm);
+        for (int s = k * WPT; s < (k + 1) * WPT; ++s) wait_ge(&g.strip_prog[(k - 1) * g.nwalk + s], k + 1, g.err, sh_flag);
+    tile_geqrt<T, NB>(Akk, lda, Tkk, ldt);
     publish(&g.panel_prog[k], k + 1);
     if (g.ptrace && threadIdx.x == 0) g.ptrace[(size_t)k * g.mt + k] = __builtin_amdgcn_s_memrealtime();
     for (int i = k + 1; i < g.mt; ++i) {
         if (k > 0)
-            for (int s = k * SPT; s < (k + 1) * SPT; ++s)
-                wait_ge(&g.strip_prog[(k - 1) * g.nstrips + s], i + 1, g.err, sh_flag);
+            for (int s = k * WPT; s < (k + 1) * WPT; ++s)
+                wait_ge(&g.strip_prog[(k - 1) * g.nwalk + s], i + 1, g.err, sh_flag);
         T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
-        tile_tsqrt<T, NB>(Akk, lda, Aik, lda, Tik, ldt, tau, sm);
+        tile_tsqrt<T, NB>(Akk, lda, Aik, lda, Tik, ldt);
         publish(&g.panel_prog[k], i + 1);
         if (g.ptrace && threadIdx.x == 0) g.ptrace[(size_t)k * g.mt + i] = __builtin_amdgcn_s_memrealtime();
     }
 }
 
+// tsmqr! blocks b = B.. of tile pair (A1 rows 32b.., S) with compile-time A1 tile indices
+template <typename T, int NB, int B>
+__device__ __forceinline__ void walk_blocks(WStrip<T, NB> &S, WStrip<T, NB> &A1, const T *Aik, i64 lda, const T *Tik,
+                                            i64 ldt) {
+    if constexpr (B < NB / IB) {
+        __syncthreads();
+        stage_block<T, NB, false>(Aik + (i64)(B * IB) * lda, lda, B * IB, Tik + (i64)(B * IB) * ldt, ldt);
+        __syncthreads();
+        apply_wave<T, NB, false>(S, &A1.a[2 * B], false);
+        walk_blocks<T, NB, B + 1>(S, A1, Aik, lda, Tik, ldt);
+    }
+}
+
 template <typename T, int NB>
-__device__ __noinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, const DagArgs &g, T *sm,
-                                      int *sh_flag) {
-    const int col0 = s * SW;  // first matrix column of the strip
-    int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nstrips + s] : nullptr;
-    int *mine = &g.strip_prog[k * g.nstrips + s];
+__device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+    const int w = threadIdx.x >> 6;
+    const i64 col = (i64)s * TW + w * SW;  // this wave's first matrix column
+    int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
+    int *mine = &g.strip_prog[k * g.nwalk + s];
     const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
     const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
-    T *Akj = A + (i64)k * NB + (i64)col0 * lda;
+    T *Akj = A + (i64)k * NB + col * lda;
     wait_ge(&g.panel_prog[k], k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
-    // unmqr!('L','C') with geqrt(A_kk)'s reflectors, forward over blocks
-    Strip<T, NB> S;
-    strip_load<T, NB>(S, Akj, lda, SW, sm);
-    for (int c0 = 0; c0 < NB; c0 += IB)
-        apply_block<T, NB, true>(S, Akk + (i64)c0 * lda, lda, Tkk + (i64)c0 * ldt, ldt, c0, IB, false, sm);
-    strip_store<T, NB>(S, Akj, lda, SW, sm);
-    publish(mine, k + 1);
-    a1_load<T, NB>(Akj, lda, 0, NB, SW, sm);
+    // unmqr!('L','C') with geqrt(A_kk)'s reflectors, blocks in forward order
+    WStrip<T, NB> A1;
+    tiles_load<T, NB / 16>(A1.a, Akj, lda, 0);
+#pragma unroll
+    for (int c0 = 0; c0 < NB; c0 += IB) {
+        __syncthreads();
+        stage_block<T, NB, true>(Akk + (i64)c0 * lda, lda, c0, Tkk + (i64)c0 * ldt, ldt);
+        __syncthreads();
+        apply_wave<T, NB, true>(A1, nullptr, false);
+    }
+    // tsmqr!('L','C') against every tile below, A_kj's strip (A1) kept in registers
     for (int i = k + 1; i < g.mt; ++i) {
         wait_ge(&g.panel_prog[k], i + 1, g.err, sh_flag);
         if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
-        T *Aij = A + (i64)i * NB + (i64)col0 * lda;
-        strip_load<T, NB>(S, Aij, lda, SW, sm);
-        for (int c0 = 0; c0 < NB; c0 += IB)
-            apply_block<T, NB, false>(S, Aik + (i64)c0 * lda, lda, Tik + (i64)c0 * ldt, ldt, c0, IB, false, sm);
-        strip_store<T, NB>(S, Aij, lda, SW, sm);
-        if (i + 1 == g.mt) a1_store<T, NB>(Akj, lda, 0, NB, SW, sm);
+        T *Aij = A + (i64)i * NB + col * lda;
+        WStrip<T, NB> S;
+        tiles_load<T, NB / 16>(S.a, Aij, lda, 0);
+        walk_blocks<T, NB, 0>(S, A1, Aik, lda, Tik, ldt);
+        tiles_store<T, NB / 16>(S.a, Aij, lda, 0);
         publish(mine, i + 1);
     }
+    tiles_store<T, NB / 16>(A1.a, Akj, lda, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 template <typename T, int NB>
-__global__ __launch_bounds__(NTH, 2) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i64 ldt, T *tau_ws, DagArgs g) {
-    extern __shared__ __align__(16) unsigned char dyn_raw[];
-    T *sm = reinterpret_cast<T *>(dyn_raw);
+__global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i64 ldt, DagArgs g) {
     __shared__ int sh_task, sh_flag;
-    T *tau = tau_ws + (size_t)blockIdx.x * NB;  // per-workgroup tau scratch (NB entries)
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) sh_task = atomicAdd(g.counter, 1);
@@ -534,8 +561,8 @@ __global__ __launch_bounds__(NTH, 2) void tileqr_dag_kernel(T *A, i64 lda, T *Tm
             g.trace[(size_t)task * 4 + 0] = __builtin_amdgcn_s_memrealtime();
             g.trace[(size_t)task * 4 + 3] = blockIdx.x;
         }
-        if (task_type(code) == TASK_PANEL) run_panel<T, NB>(task_k(code), A, lda, Tm, ldt, tau, g, sm, &sh_flag);
-        else run_walk<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, sm, &sh_flag);
+        if (task_type(code) == TASK_PANEL) run_panel<T, NB>(task_k(code), A, lda, Tm, ldt, g, &sh_flag);
+        else run_walk<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
         if (g.trace && threadIdx.x == 0) g.trace[(size_t)task * 4 + 2] = __builtin_amdgcn_s_memrealtime();
     }
 }
@@ -545,9 +572,9 @@ __global__ void fill_int_kernel(int *p, int n, int v) {
 }
 
 // Task list in a topological order with one step of lookahead:
-// PANEL(0), then for each k: WALK(k, strips of column k+1), PANEL(k+1), WALK(k, other strips).
+// PANEL(0), then for each k: WALK(k, groups of column k+1), PANEL(k+1), WALK(k, other groups).
 static std::vector<int> build_tasks(int mt, int nt, int nb) {
-    const int kt = mt < nt ? mt : nt, spt = nb / SW;
+    const int kt = mt < nt ? mt : nt, spt = nb / TW;
     std::vector<int> v;
     v.push_back(enc(TASK_PANEL, 0, 0));
     for (int k = 0; k < kt; ++k) {
@@ -567,13 +594,13 @@ template <typename T, int NB>
 static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
     using namespace dag;
     const int mt = (int)(m / NB), nt = (int)(n / NB), kt = mt < nt ? mt : nt;
-    const int nstrips = (int)(n / SW);
+    const int nwalk = (int)(n / TW);
     std::vector<int> tasks = build_tasks(mt, nt, NB);
     const int ntasks = (int)tasks.size();
     int dev = 0, ncu = 0;
     NLA_HIP(hipGetDevice(&dev));
     NLA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t sh = sizeof(T) * (size_t)Smem<T, NB>::TOTAL;
+    const size_t sh = sizeof(T) * (size_t)Lds<T, NB>::TOTAL;
     if (sh > 64 * 1024)
         NLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&tileqr_dag_kernel<T, NB>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
@@ -582,16 +609,14 @@ static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
     if (per_cu < 1) per_cu = 1;
     int grid = ncu * per_cu;
     if (grid > ntasks) grid = ntasks;
-    // scheduler memory: tasks | counter | err | panel_prog[kt] | strip_prog[kt*nstrips]
-    const size_t nint = (size_t)ntasks + 2 + kt + (size_t)kt * nstrips;
+    // scheduler memory: tasks | counter | err | panel_prog[kt] | strip_prog[kt*nwalk]
+    const size_t nint = (size_t)ntasks + 2 + kt + (size_t)kt * nwalk;
     int st = 0;
     int *mem = (int *)scratch(SLOT_SCHED, nint * sizeof(int), s, &st);
     if (!mem) return st;
-    T *tau_ws = (T *)scratch(SLOT_AUX, sizeof(T) * (size_t)grid * NB, s, &st);
-    if (!tau_ws) return st;
     int *d_tasks = mem, *counter = mem + ntasks, *err = counter + 1, *pp = err + 1, *sp = pp + kt;
     NLA_HIP(hipMemcpyAsync(d_tasks, tasks.data(), (size_t)ntasks * sizeof(int), hipMemcpyHostToDevice, s));
-    const int nzero = 2 + kt + kt * nstrips;
+    const int nzero = 2 + kt + kt * nwalk;
     fill_int_kernel<<<(nzero + 255) / 256 > 1024 ? 1024 : (nzero + 255) / 256, 256, 0, s>>>(counter, nzero, 0);
     NLA_LAUNCH_CHECK();
     static const bool want_trace = getenv("NLA_TILEQR_TRACE") != nullptr;
@@ -603,10 +628,17 @@ static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
         ptr = tr + (size_t)ntasks * 4;
         NLA_HIP(hipMemsetAsync(tr, 0, sizeof(unsigned long long) * ((size_t)ntasks * 4 + (size_t)kt * mt), s));
     }
-    DagArgs g{mt, nt, nstrips, ntasks, d_tasks, counter, pp, sp, err, tr, ptr};
+    DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr};
+    unsigned long long *phase = nullptr;
+    if (want_trace) {
+        phase = (unsigned long long *)scratch(SLOT_W, 64 * sizeof(unsigned long long), s, &st);
+        if (!phase) return st;
+        NLA_HIP(hipMemsetAsync(phase, 0, 64 * sizeof(unsigned long long), s));
+    }
+    NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_phase_trace), &phase, sizeof(phase), 0, hipMemcpyHostToDevice, s));
     {
         ProfScope prof(PK_TILE_QR, s);
-        tileqr_dag_kernel<T, NB><<<grid, NTH, sh, s>>>(A, lda, Tm, ldt, tau_ws, g);
+        tileqr_dag_kernel<T, NB><<<grid, NTH, sh, s>>>(A, lda, Tm, ldt, g);
         NLA_LAUNCH_CHECK();
     }
     int herr = 0;
@@ -623,6 +655,9 @@ static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
                         h[i * 4 + 0], h[i * 4 + 2], h[i * 4 + 3]);
             for (int k = 0; k < kt; ++k)
                 for (int i = k; i < mt; ++i) fprintf(f, "panel %d %d %llu\n", k, i, h[(size_t)ntasks * 4 + (size_t)k * mt + i]);
+            unsigned long long ph[64];
+            if (hipMemcpy(ph, phase, sizeof ph, hipMemcpyDeviceToHost) == hipSuccess)
+                for (int i = 0; i < 16; ++i) fprintf(f, "phase %d %llu\n", i, ph[i]);
             fclose(f);
         }
     }
@@ -637,7 +672,7 @@ template <typename T>
 int tile_geqrf_dag(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s, bool *handled) {
     *handled = false;
     if (ib != dag::IB) return 0;
-    if (m / nb >= 4096 || n / dag::SW >= (1 << 18)) return 0;
+    if (m / nb >= 4096 || n / dag::TW >= (1 << 18)) return 0;
     *handled = true;
     if (nb == 64) return dag_run<T, 64>(m, n, A, lda, Tm, ldt, s);
     if (nb == 128) return dag_run<T, 128>(m, n, A, lda, Tm, ldt, s);
