@@ -93,7 +93,11 @@ template <typename T, int NB> struct Lds {
     static constexpr int V = 0;
     static constexpr int TT = V + IB * LDV;            // Ts[q * (IB+1) + j] = T[q][j] (zero below diag)
     static constexpr int RS = TT + IB * (IB + 1);      // Rs[i * (IB+1) + j] = R block (tsqrt)
-    static constexpr int TOTAL = RS + IB * (IB + 1);
+    static constexpr int ZB = RS + IB * (IB + 1);      // block factorization scratch
+    static constexpr int TAU = ZB + IB * IB;
+    static constexpr int RED = TAU + IB;
+    static constexpr int DOT = RED + 16;
+    static constexpr int TOTAL = DOT + 8 * IB;
 };
 
 // LDS base with its address space visible at every use (a pointer passed through a call
@@ -205,144 +209,140 @@ __device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t 
 }
 
 // ---------------------------------------------------------------------------------
-// Single-wave block factorization.  The wave holds the IB block columns of the tile in
-// registers (lane l: rows l, l+64, ...); per column, the dots of the pivot column with all
-// block columns (its norm included) are summed by a butterfly reduce-scatter (32 shuffles)
-// and gathered with readlane, so a column costs no LDS round trip and no workgroup barrier.
-template <typename T> __device__ __forceinline__ T readlane_t(T v, int l);
-template <> __device__ __forceinline__ double readlane_t<double>(double v, int l) {
-    const unsigned long long u = __double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-    return __longlong_as_double(((unsigned long long)hi << 32) | lo);
-}
-template <> __device__ __forceinline__ float readlane_t<float>(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-// v[0..15] per lane -> g[j] = sum over the wave of v[j] (wave-uniform), butterfly
-// reduce-scatter (8+4+2+1+1+1 shuffles) then readlane gather.
-template <typename T> __device__ __forceinline__ T wave_allreduce16(const T (&v)[16], T *g) {
-    const int lane = threadIdx.x & 63;
-    T a8[8], a4[4], a2[2];
-    bool h = (lane >> 5) & 1;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const T keep = h ? v[k + 8] : v[k], send = h ? v[k] : v[k + 8];
-        a8[k] = keep + __shfl_xor(send, 32, 64);
-    }
-    h = (lane >> 4) & 1;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const T keep = h ? a8[k + 4] : a8[k], send = h ? a8[k] : a8[k + 4];
-        a4[k] = keep + __shfl_xor(send, 16, 64);
-    }
-    h = (lane >> 3) & 1;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const T keep = h ? a4[k + 2] : a4[k], send = h ? a4[k] : a4[k + 2];
-        a2[k] = keep + __shfl_xor(send, 8, 64);
-    }
-    h = (lane >> 2) & 1;
-    T a1 = (h ? a2[1] : a2[0]) + __shfl_xor(h ? a2[0] : a2[1], 4, 64);
-    a1 += __shfl_xor(a1, 2, 64);
-    a1 += __shfl_xor(a1, 1, 64);
-    // lane l now holds the sum of value (l >> 2)
-#pragma unroll
-    for (int j = 0; j < 16; ++j) g[j] = readlane_t<T>(a1, 4 * j);
-    return __shfl(a1, 4 * (lane & 15), 64);  // lane l: sum of value l % 16
+// Block factorization by the whole workgroup.  Thread (wave w, lane l) owns column
+// j = l % 32 of the IB-column block and the RPT = NB/8 consecutive rows starting at
+// row0 = RPT * (2w + l/32), in registers.  Per column c two workgroup rounds:
+//   norm round: the eight owners of column c write their partial sums of squares (and, for
+//     geqrt, the owner of the pivot row its entry) to LDS; after the barrier every thread
+//     forms beta, tau and the scale (identical in all threads), and the owners of column c
+//     scale their rows and store the reflector (unit pivot, zeros above) in column c of Vs;
+//   dot round: every thread reads v on its own rows (stored by a lane of its own wave),
+//     forms its partial dot with its column and writes it to LDS; after the barrier each
+//     column j > c is updated with w_j = v^T x_j (+ R[c][j] for tsqrt) and each j < c
+//     records z_j = v_j^T v for the T factor.
+// The T factor (larft forward/columnwise: T[0:c, c] = -tau_c T[0:c, 0:c] z) is formed after
+// the loop by wave 0, one row per lane, while the other waves return to the caller; the
+// caller synchronises before it reads Ts.
+template <typename T>
+__device__ __forceinline__ T sum8(const T *p) {
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
 }
 
-// GE: geqr2!+larft! of the IB block columns held in Xs (LDS, Xs[j*LDV + p], tile rows p),
-//     pivot rows c0 + c (rows < pivot untouched);
-// TS: tsqrt block of [R block (Rs, LDS) ; A2 block (Xs)].
-// One wave; writes the T block (upper) into Ts.  The column loop stays rolled (its body is
-// large); every loop inside it is over a compile-time range, so no register array is ever
-// indexed at run time.
 template <typename T, int NB, bool GE>
-__device__ void wave_factor(T *Xs, int c0, T *Rs, T *Ts) {
-    constexpr int RPL = NB / 64, LDV = Lds<T, NB>::LDV;
-    const int lane = threadIdx.x & 63;
-    const int li = lane < IB ? lane : IB - 1;  // clamped lane for per-reflector LDS rows
+__device__ __forceinline__ void block_factor(int c0, T *Ablk, i64 lda) {
+    T *sm = lds_base<T>();
+    constexpr int LDV = Lds<T, NB>::LDV, RPT = NB / 8;
+    static_assert(RPT % 4 == 0, "rows per thread must be a multiple of 4");
+    T *Vs = sm + Lds<T, NB>::V;
+    T *Ts = sm + Lds<T, NB>::TT;
+    T *Rs = sm + Lds<T, NB>::RS;
+    T *Zs = sm + Lds<T, NB>::ZB;      // Zs[c * IB + q] = z_q of column c (q < c)
+    T *taus = sm + Lds<T, NB>::TAU;   // [IB]
+    T *red = sm + Lds<T, NB>::RED;    // [8] partial sums of squares, [8] the pivot entry
+    T *dots = sm + Lds<T, NB>::DOT;   // [IB][8] partial dots, group index xor (column % 8)
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int j = lane & 31, grp = 2 * w + (lane >> 5), row0 = grp * RPT;
+    T x[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) x[r] = Vs[j * LDV + row0 + r];
+    __syncthreads();  // every column is in registers before Vs is overwritten
 #pragma unroll 1
     for (int c = 0; c < IB; ++c) {
-        const int cc = c0 + c;
-        // pivot column, masked to the rows below the pivot (GE) -- selects, no divergent branches
-        T xc[RPL], xm[RPL];
+        const int piv = c0 + c;
+        // ---- norm round
+        if (j == c) {
+            T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
 #pragma unroll
-        for (int h = 0; h < RPL; ++h) {
-            const int p = lane + 64 * h;
-            xc[h] = Xs[c * LDV + p];
-            xm[h] = (!GE || p > cc) ? xc[h] : T(0);
-        }
-        T g[IB], gl = 0;
-#pragma unroll
-        for (int half = 0; half < IB / 16; ++half) {
-            T v[16];
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const int j = 16 * half + jj;
-                T sacc = 0;
-#pragma unroll
-                for (int h = 0; h < RPL; ++h) sacc += Xs[j * LDV + lane + 64 * h] * xm[h];
-                v[jj] = sacc;
+            for (int r = 0; r < RPT; r += 4) {
+                const T y0 = (!GE || row0 + r > piv) ? x[r] : T(0);
+                const T y1 = (!GE || row0 + r + 1 > piv) ? x[r + 1] : T(0);
+                const T y2 = (!GE || row0 + r + 2 > piv) ? x[r + 2] : T(0);
+                const T y3 = (!GE || row0 + r + 3 > piv) ? x[r + 3] : T(0);
+                q0 = fma(y0, y0, q0);
+                q1 = fma(y1, y1, q1);
+                q2 = fma(y2, y2, q2);
+                q3 = fma(y3, y3, q3);
             }
-            const T gh = wave_allreduce16<T>(v, &g[16 * half]);
-            if ((lane >> 4) == half) gl = gh;  // lane j < IB: g[j]
-        }
-        T xs = 0;
+            red[grp] = (q0 + q1) + (q2 + q3);
+            if (GE && piv >= row0 && piv < row0 + RPT) {
+                T ap = 0;
 #pragma unroll
-        for (int j = 0; j < IB; ++j) xs = (j == c) ? g[j] : xs;
-        const T alpha = GE ? Xs[c * LDV + cc] : Rs[c * (IB + 1) + c];
-        T beta = alpha, tv = 0, scale = 0;
-        if (xs != T(0)) {
-            beta = -copysign(sqrt(alpha * alpha + xs), alpha);
-            tv = (beta - alpha) / beta;
+                for (int r = 0; r < RPT; ++r) ap = (row0 + r == piv) ? x[r] : ap;
+                red[8] = ap;
+            }
+        }
+        __syncthreads();
+        const T ssq = sum8(red);
+        const T alpha = GE ? red[8] : Rs[c * (IB + 1) + c];
+        T beta = alpha, tau = 0, scale = 1;
+        if (ssq != T(0)) {
+            beta = -copysign(sqrt(alpha * alpha + ssq), alpha);
+            tau = (beta - alpha) / beta;
             scale = T(1) / (alpha - beta);
         }
-        if (tv != T(0)) {
-            // row c of the block: GE = tile row cc, TS = R block row c (read before any update)
-            T rowc[IB];
+        if (j == c) {
 #pragma unroll
-            for (int j = 0; j < IB; ++j) rowc[j] = GE ? Xs[j * LDV + cc] : Rs[c * (IB + 1) + j];
-            const T rl = GE ? T(0) : Rs[c * (IB + 1) + li];
-            __builtin_amdgcn_wave_barrier();
-            // x_p <- v_p (rows below the pivot), beta on the pivot row; rank-1 update of the rest:
-            //   A[p][j] -= tau v_p w_j,  v_cc = 1
-            T coef[RPL];
-#pragma unroll
-            for (int h = 0; h < RPL; ++h) {
-                const int p = lane + 64 * h;
-                const bool below = !GE || p > cc, piv = GE && p == cc;
-                const T vp = xc[h] * scale;
-                coef[h] = below ? -tv * vp : (piv ? -tv : T(0));
-                Xs[c * LDV + p] = below ? vp : (piv ? beta : xc[h]);
+            for (int r = 0; r < RPT; ++r) {
+                const int p = row0 + r;
+                const bool below = !GE || p > piv, pv = GE && p == piv;
+                const T v = x[r] * scale;
+                Vs[c * LDV + p] = below ? v : (pv ? T(1) : T(0));
+                x[r] = below ? v : (pv ? beta : x[r]);
             }
-#pragma unroll
-            for (int j = 0; j < IB; ++j) {
-                if (j > c) {
-                    const T wj = rowc[j] + g[j] * scale;
-#pragma unroll
-                    for (int h = 0; h < RPL; ++h) Xs[j * LDV + lane + 64 * h] += coef[h] * wj;
-                }
-            }
-            if (!GE && lane > c && lane < IB) Rs[c * (IB + 1) + lane] = rl - tv * (rl + gl * scale);
-            // T column c: T[j][c] = -tau * sum_{q=j}^{c-1} T[j][q] z_q with z_q = (V^T v)_q
-            T sacc = 0;
-#pragma unroll
-            for (int q = 0; q < IB; ++q) {
-                if (q < c) {
-                    const T zq = (GE ? rowc[q] : T(0)) + g[q] * scale;
-                    sacc += (q >= lane ? Ts[li * (IB + 1) + q] : T(0)) * zq;
-                }
-            }
-            if (lane < c) Ts[lane * (IB + 1) + c] = -tv * sacc;
-        } else if (lane < c) {
-            Ts[lane * (IB + 1) + c] = T(0);
+            if (grp == 0) taus[c] = tau;
         }
-        if (!GE && lane == c) Rs[c * (IB + 1) + c] = beta;
-        if (lane == c) Ts[c * (IB + 1) + c] = tv;
         __builtin_amdgcn_wave_barrier();
+        // ---- dot round
+        T v[RPT];
+        T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+#pragma unroll
+        for (int r = 0; r < RPT; r += 4) {
+            v[r] = Vs[c * LDV + row0 + r];
+            v[r + 1] = Vs[c * LDV + row0 + r + 1];
+            v[r + 2] = Vs[c * LDV + row0 + r + 2];
+            v[r + 3] = Vs[c * LDV + row0 + r + 3];
+            q0 = fma(v[r], x[r], q0);
+            q1 = fma(v[r + 1], x[r + 1], q1);
+            q2 = fma(v[r + 2], x[r + 2], q2);
+            q3 = fma(v[r + 3], x[r + 3], q3);
+        }
+        dots[j * 8 + (grp ^ (j & 7))] = (q0 + q1) + (q2 + q3);
+        const T rc = GE ? T(0) : Rs[c * (IB + 1) + j];
+        __syncthreads();
+        const T gsum = sum8(dots + j * 8);
+        if (j > c) {
+            const T cw = tau * (gsum + rc);
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) x[r] = fma(-cw, v[r], x[r]);
+            if (!GE && grp == 0) Rs[c * (IB + 1) + j] = rc - cw;
+        } else if (grp == 0) {
+            if (j < c) Zs[c * IB + j] = gsum;
+            else if (!GE) Rs[c * (IB + 1) + c] = beta;
+        }
+    }
+    if (GE) {
+        // Vs holds the unit-lower reflectors for the trailing update; the tile gets R, beta
+        // and V (rows >= c0; the rows above are untouched) from the registers.
+#pragma unroll
+        for (int r = 0; r < RPT; ++r)
+            if (row0 + r >= c0) cst(&Ablk[(row0 + r) + (i64)j * lda], x[r]);
+    }
+    __syncthreads();
+    if (w == 0) {
+        const int i = lane;
+        T trow[IB];
+#pragma unroll
+        for (int c = 0; c < IB; ++c) {
+            T acc = 0;
+#pragma unroll
+            for (int q = 0; q < c; ++q) acc += trow[q] * Zs[c * IB + q];
+            const T tc = taus[c];
+            trow[c] = (i < c) ? -tc * acc : (i == c ? tc : T(0));
+        }
+        if (i < IB) {
+#pragma unroll
+            for (int c = 0; c < IB; ++c) Ts[i * (IB + 1) + c] = trow[c];
+        }
     }
 }
 
@@ -355,31 +355,20 @@ template <typename T, int NB> __device__ void tile_geqrt(T *A, i64 lda, T *Tm, i
     constexpr int LDV = Lds<T, NB>::LDV;
     T *Vs = sm + Lds<T, NB>::V;
     T *Ts = sm + Lds<T, NB>::TT;
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int t = threadIdx.x, w = t >> 6;
     for (int c0 = 0; c0 < NB; c0 += IB) {
         __syncthreads();
-        for (int idx = t; idx < IB * (IB + 1); idx += NTH) Ts[idx] = T(0);
         for (int idx = t; idx < NB * IB; idx += NTH) {
             const int p = idx % NB, j = idx / NB;
             Vs[j * LDV + p] = cld(&A[p + (i64)(c0 + j) * lda]);
         }
         __syncthreads();
-        if (w == 0) {
-            wave_factor<T, NB, true>(Vs, c0, nullptr, Ts);
-            // write the factored block back, then leave the unit-lower V in Vs
-            for (int idx = lane; idx < NB * IB; idx += 64) {
-                const int p = idx % NB, j = idx / NB, ja = c0 + j;
-                const T v = Vs[j * LDV + p];
-                if (p >= c0) cst(&A[p + (i64)ja * lda], v);
-                Vs[j * LDV + p] = p < ja ? T(0) : (p == ja ? T(1) : v);
-            }
-            for (int idx = lane; idx < IB * IB; idx += 64) {
-                const int q = idx % IB, j = idx / IB;
-                if (q <= j) cst(&Tm[q + (i64)(c0 + j) * ldt], Ts[q * (IB + 1) + j]);
-            }
-        }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        block_factor<T, NB, true>(c0, A + (i64)c0 * lda, lda);
         __syncthreads();
+        for (int idx = t; idx < IB * IB; idx += NTH) {
+            const int q = idx % IB, j = idx / IB;
+            if (q <= j) cst(&Tm[q + (i64)(c0 + j) * ldt], Ts[q * (IB + 1) + j]);
+        }
         // larfb!('L','C') of this block on the trailing columns of the tile
         const int nst = (NB - c0 - IB) / SW;
         for (int st = w; st < nst; st += WAVES) {
@@ -410,10 +399,9 @@ __device__ void tile_tsqrt(T *R, i64 ldr, T *A2, i64 lda2, T *Tm, i64 ldt) {
     T *Vs = sm + Lds<T, NB>::V;
     T *Ts = sm + Lds<T, NB>::TT;
     T *Rs = sm + Lds<T, NB>::RS;
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int t = threadIdx.x, w = t >> 6;
     for (int c0 = 0; c0 < NB; c0 += IB) {
         __syncthreads();
-        for (int idx = t; idx < IB * (IB + 1); idx += NTH) Ts[idx] = T(0);
         for (int idx = t; idx < IB * IB; idx += NTH) {
             const int i = idx % IB, j = idx / IB;
             Rs[i * (IB + 1) + j] = cld(&R[(c0 + i) + (i64)(c0 + j) * ldr]);
@@ -424,23 +412,21 @@ __device__ void tile_tsqrt(T *R, i64 ldr, T *A2, i64 lda2, T *Tm, i64 ldt) {
         }
         __syncthreads();
         phase_stamp(4 * (c0 / IB) + 0);
-        if (w == 0) {
-            wave_factor<T, NB, false>(Vs, c0, Rs, Ts);
-            phase_stamp(4 * (c0 / IB) + 1);
-            for (int idx = lane; idx < NB * IB; idx += 64) {
-                const int p = idx % NB, j = idx / NB;
-                cst(&A2[p + (i64)(c0 + j) * lda2], Vs[j * LDV + p]);
-            }
-            for (int idx = lane; idx < IB * IB; idx += 64) {
-                const int i = idx % IB, j = idx / IB;
-                if (i <= j) {
-                    cst(&R[(c0 + i) + (i64)(c0 + j) * ldr], Rs[i * (IB + 1) + j]);
-                    cst(&Tm[i + (i64)(c0 + j) * ldt], Ts[i * (IB + 1) + j]);
-                }
-            }
+        block_factor<T, NB, false>(c0, nullptr, 0);
+        phase_stamp(4 * (c0 / IB) + 1);
+        for (int idx = t; idx < NB * IB; idx += NTH) {
+            const int p = idx % NB, j = idx / NB;
+            cst(&A2[p + (i64)(c0 + j) * lda2], Vs[j * LDV + p]);
         }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        for (int idx = t; idx < IB * IB; idx += NTH) {
+            const int i = idx % IB, j = idx / IB;
+            if (i <= j) cst(&R[(c0 + i) + (i64)(c0 + j) * ldr], Rs[i * (IB + 1) + j]);
+        }
         __syncthreads();
+        for (int idx = t; idx < IB * IB; idx += NTH) {
+            const int i = idx % IB, j = idx / IB;
+            if (i <= j) cst(&Tm[i + (i64)(c0 + j) * ldt], Ts[i * (IB + 1) + j]);
+        }
         phase_stamp(4 * (c0 / IB) + 2);
         // tsmqr!('L','C') of this block on the trailing columns: A1 = R rows c0.., A2 strip
         const int nst = (NB - c0 - IB) / SW;
