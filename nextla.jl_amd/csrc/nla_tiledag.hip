@@ -3,17 +3,18 @@
 // The composition geqrt!(A_kk) -> unmqr!(A_kj) -> tsqrt!(A_kk, A_ik) -> tsmqr!(A_kj, A_ij)
 // (src/geqrt.jl, src/unmqr.jl, src/tsqrt.jl, src/tsmqr.jl) is run as one launch of resident
 // workgroups that claim tasks from an ordered list:
-//   PANEL(k)    geqrt on A_kk, then tsqrt(A_kk, A_ik) for i = k+1.. (R_kk stays in place)
-//   WALK(k, s)  a strip of SW columns of tile column j > k: unmqr('L','C') on A_kj's strip,
-//               then tsmqr('L','C') against every A_ij strip, i = k+1.., A_kj's strip held
-//               in LDS for the whole walk and each A_ij strip held in MFMA accumulators
-//               across all ib-blocks of that tile.
+//   PANEL(k, b) ib-block b of geqrt on A_kk, then of tsqrt(A_kk, A_ik) for i = k+1.., tile
+//               by tile (R_kk stays in place); the NB/IB block stages of one panel form a
+//               pipeline over the tiles
+//   WALK(k, s)  TW columns of tile column j > k, one 16-column strip per wave: unmqr('L','C')
+//               on A_kj's strip, then tsmqr('L','C') against every A_ij strip, i = k+1..,
+//               A_kj's strip held in MFMA accumulators for the whole walk.
 // Columns of C are independent under a left block reflector, so strips never interact.
-// Tasks publish progress counters (panel_prog[k] = rows of column k factored,
-// strip_prog[k][s] = rows of strip s updated by step k); a task waits on counters of
-// tasks listed before it, so the lowest unfinished claimed task can always run and the
-// launch cannot deadlock.  Waits are bounded by a wall-clock limit that sets an error
-// flag and lets every wave finish, so no input can hang the device.
+// Tasks publish progress counters (panel_prog[k][b] = tiles of column k done by block
+// stage b, strip_prog[k][s] = rows of strip group s updated by step k); a task waits only
+// on counters of tasks listed before it, so the lowest unfinished claimed task can always
+// run and the launch cannot deadlock.  Waits are bounded by a wall-clock limit that sets an
+// error flag and lets every wave finish, so no input can hang the device.
 //
 // Arithmetic per tile is that of the reference routines (same blocks, same order); MFMA
 // tiles only change the association of the inner sums.
@@ -127,6 +128,19 @@ __device__ __forceinline__ void tiles_store(const typename Acc<T>::t *a, T *G, i
     for (int rt = 0; rt < NT; ++rt)
 #pragma unroll
         for (int r = 0; r < 4; ++r) cst(&G[(r0 + 16 * rt + acc_row<T>(lk, r)) + (i64)lr * ld], a[rt][r]);
+}
+
+// rows >= rmin of the same strip (rows above are left to whoever owns them)
+template <typename T, int NT>
+__device__ __forceinline__ void tiles_store_from(const typename Acc<T>::t *a, T *G, i64 ld, int rmin) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * rt + acc_row<T>(lk, r);
+            if (row >= rmin) cst(&G[row + (i64)lr * ld], a[rt][r]);
+        }
 }
 
 // Stage block reflectors (columns j0.. j0+IB-1 of the tile, Vg = first of them) and T block.
@@ -349,37 +363,37 @@ __device__ __forceinline__ void block_factor(int c0, T *Ablk, i64 lda) {
 // ---------------------------------------------------------------------------------
 // Tile operations (one workgroup).
 
-// geqrt!(NB, NB, IB, A, T): src/geqrt.jl:44-67
-template <typename T, int NB> __device__ void tile_geqrt(T *A, i64 lda, T *Tm, i64 ldt) {
+// One ib-block of geqrt!(NB, NB, IB, A, T) (src/geqrt.jl:44-67): geqr2+larft of columns
+// c0..c0+IB-1, then larfb!('L','C') of that block on the trailing columns of the tile.
+template <typename T, int NB> __device__ void geqrt_block(T *A, i64 lda, T *Tm, i64 ldt, int c0) {
     T *sm = lds_base<T>();
     constexpr int LDV = Lds<T, NB>::LDV;
     T *Vs = sm + Lds<T, NB>::V;
     T *Ts = sm + Lds<T, NB>::TT;
     const int t = threadIdx.x, w = t >> 6;
-    for (int c0 = 0; c0 < NB; c0 += IB) {
-        __syncthreads();
-        for (int idx = t; idx < NB * IB; idx += NTH) {
-            const int p = idx % NB, j = idx / NB;
-            Vs[j * LDV + p] = cld(&A[p + (i64)(c0 + j) * lda]);
-        }
-        __syncthreads();
-        block_factor<T, NB, true>(c0, A + (i64)c0 * lda, lda);
-        __syncthreads();
-        for (int idx = t; idx < IB * IB; idx += NTH) {
-            const int q = idx % IB, j = idx / IB;
-            if (q <= j) cst(&Tm[q + (i64)(c0 + j) * ldt], Ts[q * (IB + 1) + j]);
-        }
-        // larfb!('L','C') of this block on the trailing columns of the tile
-        const int nst = (NB - c0 - IB) / SW;
-        for (int st = w; st < nst; st += WAVES) {
-            T *G = A + (i64)(c0 + IB + st * SW) * lda;
-            WStrip<T, NB> S;
-            tiles_load<T, NB / 16>(S.a, G, lda, 0);
-            apply_wave<T, NB, true>(S, nullptr, false);
-            tiles_store<T, NB / 16>(S.a, G, lda, 0);
-        }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int idx = t; idx < NB * IB; idx += NTH) {
+        const int p = idx % NB, j = idx / NB;
+        Vs[j * LDV + p] = cld(&A[p + (i64)(c0 + j) * lda]);
     }
+    __syncthreads();
+    block_factor<T, NB, true>(c0, A + (i64)c0 * lda, lda);
+    __syncthreads();
+    for (int idx = t; idx < IB * IB; idx += NTH) {
+        const int q = idx % IB, j = idx / IB;
+        if (q <= j) cst(&Tm[q + (i64)(c0 + j) * ldt], Ts[q * (IB + 1) + j]);
+    }
+    const int nst = (NB - c0 - IB) / SW;
+    for (int st = w; st < nst; st += WAVES) {
+        T *G = A + (i64)(c0 + IB + st * SW) * lda;
+        WStrip<T, NB> S;
+        tiles_load<T, NB / 16>(S.a, G, lda, 0);
+        apply_wave<T, NB, true>(S, nullptr, false);
+        // rows above c0 are unchanged by these reflectors and may be in flight in another
+        // block stage (R rows of earlier blocks updated by tsqrt): store only rows >= c0
+        tiles_store_from<T, NB / 16>(S.a, G, lda, c0);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 }
 
@@ -391,60 +405,58 @@ __device__ __forceinline__ void phase_stamp(int slot) {
     }
 }
 
-// tsqrt!(NB, NB, IB, R, A2, T): src/tsqrt.jl:146-155
+// One ib-block of tsqrt!(NB, NB, IB, R, A2, T) (src/tsqrt.jl:146-155): the TS block QR of
+// columns c0..c0+IB-1 of [R; A2], then tsmqr!('L','C') of that block on the trailing columns
+// (A1 = rows c0..c0+IB-1 of R, A2).
 template <typename T, int NB>
-__device__ void tile_tsqrt(T *R, i64 ldr, T *A2, i64 lda2, T *Tm, i64 ldt) {
+__device__ void tsqrt_block(T *R, i64 ldr, T *A2, i64 lda2, T *Tm, i64 ldt, int c0) {
     T *sm = lds_base<T>();
     constexpr int LDV = Lds<T, NB>::LDV;
     T *Vs = sm + Lds<T, NB>::V;
     T *Ts = sm + Lds<T, NB>::TT;
     T *Rs = sm + Lds<T, NB>::RS;
     const int t = threadIdx.x, w = t >> 6;
-    for (int c0 = 0; c0 < NB; c0 += IB) {
-        __syncthreads();
-        for (int idx = t; idx < IB * IB; idx += NTH) {
-            const int i = idx % IB, j = idx / IB;
-            Rs[i * (IB + 1) + j] = cld(&R[(c0 + i) + (i64)(c0 + j) * ldr]);
-        }
-        for (int idx = t; idx < NB * IB; idx += NTH) {
-            const int p = idx % NB, j = idx / NB;
-            Vs[j * LDV + p] = cld(&A2[p + (i64)(c0 + j) * lda2]);
-        }
-        __syncthreads();
-        phase_stamp(4 * (c0 / IB) + 0);
-        block_factor<T, NB, false>(c0, nullptr, 0);
-        phase_stamp(4 * (c0 / IB) + 1);
-        for (int idx = t; idx < NB * IB; idx += NTH) {
-            const int p = idx % NB, j = idx / NB;
-            cst(&A2[p + (i64)(c0 + j) * lda2], Vs[j * LDV + p]);
-        }
-        for (int idx = t; idx < IB * IB; idx += NTH) {
-            const int i = idx % IB, j = idx / IB;
-            if (i <= j) cst(&R[(c0 + i) + (i64)(c0 + j) * ldr], Rs[i * (IB + 1) + j]);
-        }
-        __syncthreads();
-        for (int idx = t; idx < IB * IB; idx += NTH) {
-            const int i = idx % IB, j = idx / IB;
-            if (i <= j) cst(&Tm[i + (i64)(c0 + j) * ldt], Ts[i * (IB + 1) + j]);
-        }
-        phase_stamp(4 * (c0 / IB) + 2);
-        // tsmqr!('L','C') of this block on the trailing columns: A1 = R rows c0.., A2 strip
-        const int nst = (NB - c0 - IB) / SW;
-        for (int st = w; st < nst; st += WAVES) {
-            const i64 col = c0 + IB + st * SW;
-            typename Acc<T>::t a1[2];
-            tiles_load<T, 2>(a1, R + col * ldr, ldr, c0);
-            WStrip<T, NB> S;
-            tiles_load<T, NB / 16>(S.a, A2 + col * lda2, lda2, 0);
-            apply_wave<T, NB, false>(S, a1, false);
-            tiles_store<T, NB / 16>(S.a, A2 + col * lda2, lda2, 0);
-            tiles_store<T, 2>(a1, R + col * ldr, ldr, c0);
-        }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();
-        phase_stamp(4 * (c0 / IB) + 3);
+    __syncthreads();
+    for (int idx = t; idx < IB * IB; idx += NTH) {
+        const int i = idx % IB, j = idx / IB;
+        Rs[i * (IB + 1) + j] = cld(&R[(c0 + i) + (i64)(c0 + j) * ldr]);
+    }
+    for (int idx = t; idx < NB * IB; idx += NTH) {
+        const int p = idx % NB, j = idx / NB;
+        Vs[j * LDV + p] = cld(&A2[p + (i64)(c0 + j) * lda2]);
     }
     __syncthreads();
+    phase_stamp(4 * (c0 / IB) + 0);
+    block_factor<T, NB, false>(c0, nullptr, 0);
+    phase_stamp(4 * (c0 / IB) + 1);
+    for (int idx = t; idx < NB * IB; idx += NTH) {
+        const int p = idx % NB, j = idx / NB;
+        cst(&A2[p + (i64)(c0 + j) * lda2], Vs[j * LDV + p]);
+    }
+    for (int idx = t; idx < IB * IB; idx += NTH) {
+        const int i = idx % IB, j = idx / IB;
+        if (i <= j) cst(&R[(c0 + i) + (i64)(c0 + j) * ldr], Rs[i * (IB + 1) + j]);
+    }
+    __syncthreads();
+    for (int idx = t; idx < IB * IB; idx += NTH) {
+        const int i = idx % IB, j = idx / IB;
+        if (i <= j) cst(&Tm[i + (i64)(c0 + j) * ldt], Ts[i * (IB + 1) + j]);
+    }
+    phase_stamp(4 * (c0 / IB) + 2);
+    const int nst = (NB - c0 - IB) / SW;
+    for (int st = w; st < nst; st += WAVES) {
+        const i64 col = c0 + IB + st * SW;
+        typename Acc<T>::t a1[2];
+        tiles_load<T, 2>(a1, R + col * ldr, ldr, c0);
+        WStrip<T, NB> S;
+        tiles_load<T, NB / 16>(S.a, A2 + col * lda2, lda2, 0);
+        apply_wave<T, NB, false>(S, a1, false);
+        tiles_store<T, NB / 16>(S.a, A2 + col * lda2, lda2, 0);
+        tiles_store<T, 2>(a1, R + col * ldr, ldr, c0);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    phase_stamp(4 * (c0 / IB) + 3);
 }
 
 // ---------------------------------------------------------------------------------
@@ -453,32 +465,38 @@ struct DagArgs {
     int ntasks;
     const int *tasks;
     int *counter;             // next task
-    int *panel_prog;          // [kt]
+    int *panel_prog;          // [kt][NB/IB]: tiles of column k done by block stage b
     int *strip_prog;          // [kt][nwalk]
     int *err;
     unsigned long long *trace;  // optional timeline: [ntasks][4] = claim, -, end, workgroup
     unsigned long long *ptrace; // optional: [kt][mt] panel tile completion times
 };
 
+// Block stage b of panel k: ib-block b of geqrt(A_kk), then of tsqrt(A_kk, A_ik) for
+// i = k+1.., tile by tile.  Block b of tile i needs block b of tile i-1 (rows c0.. of R_kk,
+// this task's own previous step) and block b-1 of tile i (the trailing update of A_ik's
+// columns c0.. and the factorization of R_kk's rows above), so the stages of one panel
+// run as a pipeline over the tiles.  Stage 0 waits for the previous step's walkers instead.
 template <typename T, int NB>
-__device__ __noinline__ void run_panel(int k, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
-    constexpr int WPT = NB / TW;  // walker groups per tile column
+__device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+    constexpr int WPT = NB / TW, NBLK = NB / IB;  // walker groups per tile column, stages
+    const int c0 = b * IB;
+    int *mine = &g.panel_prog[k * NBLK + b];
+    const int *prev = b > 0 ? &g.panel_prog[k * NBLK + b - 1] : nullptr;
     T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
-    T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
-    if (k > 0)
-        for (int s = k * WPT; s < (k + 1) * WPT; ++s) wait_ge(&g.strip_prog[(k - 1) * g.nwalk + s], k + 1, g.err, sh_flag);
-    tile_geqrt<T, NB>(Akk, lda, Tkk, ldt);
-    publish(&g.panel_prog[k], k + 1);
-    if (g.ptrace && threadIdx.x == 0) g.ptrace[(size_t)k * g.mt + k] = __builtin_amdgcn_s_memrealtime();
-    for (int i = k + 1; i < g.mt; ++i) {
-        if (k > 0)
+    for (int i = k; i < g.mt; ++i) {
+        if (prev) {
+            wait_ge(prev, i + 1, g.err, sh_flag);
+        } else if (k > 0) {
             for (int s = k * WPT; s < (k + 1) * WPT; ++s)
                 wait_ge(&g.strip_prog[(k - 1) * g.nwalk + s], i + 1, g.err, sh_flag);
-        T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
+        }
         T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
-        tile_tsqrt<T, NB>(Akk, lda, Aik, lda, Tik, ldt);
-        publish(&g.panel_prog[k], i + 1);
-        if (g.ptrace && threadIdx.x == 0) g.ptrace[(size_t)k * g.mt + i] = __builtin_amdgcn_s_memrealtime();
+        if (i == k) geqrt_block<T, NB>(Akk, lda, Tik, ldt, c0);
+        else tsqrt_block<T, NB>(Akk, lda, A + (i64)i * NB + (i64)k * NB * lda, lda, Tik, ldt, c0);
+        publish(mine, i + 1);
+        if (b == NBLK - 1 && g.ptrace && threadIdx.x == 0)
+            g.ptrace[(size_t)k * g.mt + i] = __builtin_amdgcn_s_memrealtime();
     }
 }
 
@@ -504,7 +522,8 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
     const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
     const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
     T *Akj = A + (i64)k * NB + col * lda;
-    wait_ge(&g.panel_prog[k], k + 1, g.err, sh_flag);
+    const int *pdone = &g.panel_prog[k * (NB / IB) + NB / IB - 1];  // last block stage of panel k
+    wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     // unmqr!('L','C') with geqrt(A_kk)'s reflectors, blocks in forward order
     WStrip<T, NB> A1;
@@ -518,7 +537,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
     }
     // tsmqr!('L','C') against every tile below, A_kj's strip (A1) kept in registers
     for (int i = k + 1; i < g.mt; ++i) {
-        wait_ge(&g.panel_prog[k], i + 1, g.err, sh_flag);
+        wait_ge(pdone, i + 1, g.err, sh_flag);
         if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
@@ -547,7 +566,7 @@ __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i
             g.trace[(size_t)task * 4 + 0] = __builtin_amdgcn_s_memrealtime();
             g.trace[(size_t)task * 4 + 3] = blockIdx.x;
         }
-        if (task_type(code) == TASK_PANEL) run_panel<T, NB>(task_k(code), A, lda, Tm, ldt, g, &sh_flag);
+        if (task_type(code) == TASK_PANEL) run_panel<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
         else run_walk<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
         if (g.trace && threadIdx.x == 0) g.trace[(size_t)task * 4 + 2] = __builtin_amdgcn_s_memrealtime();
     }
@@ -558,15 +577,18 @@ __global__ void fill_int_kernel(int *p, int n, int v) {
 }
 
 // Task list in a topological order with one step of lookahead:
-// PANEL(0), then for each k: WALK(k, groups of column k+1), PANEL(k+1), WALK(k, other groups).
+// PANEL(0, all block stages), then for each k: WALK(k, groups of column k+1),
+// PANEL(k+1, all block stages), WALK(k, other groups).
 static std::vector<int> build_tasks(int mt, int nt, int nb) {
     const int kt = mt < nt ? mt : nt, spt = nb / TW;
     std::vector<int> v;
-    v.push_back(enc(TASK_PANEL, 0, 0));
+    const int nblk = nb / IB;
+    for (int b = 0; b < nblk; ++b) v.push_back(enc(TASK_PANEL, 0, b));
     for (int k = 0; k < kt; ++k) {
         if (k + 1 < nt)
             for (int s = (k + 1) * spt; s < (k + 2) * spt; ++s) v.push_back(enc(TASK_WALK, k, s));
-        if (k + 1 < kt) v.push_back(enc(TASK_PANEL, k + 1, 0));
+        if (k + 1 < kt)
+            for (int b = 0; b < nblk; ++b) v.push_back(enc(TASK_PANEL, k + 1, b));
         for (int s = (k + 2) * spt; s < nt * spt; ++s) v.push_back(enc(TASK_WALK, k, s));
     }
     return v;
@@ -595,14 +617,15 @@ static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
     if (per_cu < 1) per_cu = 1;
     int grid = ncu * per_cu;
     if (grid > ntasks) grid = ntasks;
-    // scheduler memory: tasks | counter | err | panel_prog[kt] | strip_prog[kt*nwalk]
-    const size_t nint = (size_t)ntasks + 2 + kt + (size_t)kt * nwalk;
+    // scheduler memory: tasks | counter | err | panel_prog[kt*NB/IB] | strip_prog[kt*nwalk]
+    const int npp = kt * (NB / IB);
+    const size_t nint = (size_t)ntasks + 2 + npp + (size_t)kt * nwalk;
     int st = 0;
     int *mem = (int *)scratch(SLOT_SCHED, nint * sizeof(int), s, &st);
     if (!mem) return st;
-    int *d_tasks = mem, *counter = mem + ntasks, *err = counter + 1, *pp = err + 1, *sp = pp + kt;
+    int *d_tasks = mem, *counter = mem + ntasks, *err = counter + 1, *pp = err + 1, *sp = pp + npp;
     NLA_HIP(hipMemcpyAsync(d_tasks, tasks.data(), (size_t)ntasks * sizeof(int), hipMemcpyHostToDevice, s));
-    const int nzero = 2 + kt + kt * nwalk;
+    const int nzero = 2 + npp + kt * nwalk;
     fill_int_kernel<<<(nzero + 255) / 256 > 1024 ? 1024 : (nzero + 255) / 256, 256, 0, s>>>(counter, nzero, 0);
     NLA_LAUNCH_CHECK();
     static const bool want_trace = getenv("NLA_TILEQR_TRACE") != nullptr;
