@@ -21,6 +21,7 @@
 #include "nla_common.hpp"
 #include "nla_qr.hpp"
 
+#include <cstdint>
 #include <cstdlib>
 #include <vector>
 
@@ -93,7 +94,8 @@ template <typename T, int NB> struct Lds {
     static constexpr int LDV = NB + 2;                 // Vs[j * LDV + p] = V[p][j]
     static constexpr int V = 0;
     static constexpr int TT = V + IB * LDV;            // Ts[q * (IB+1) + j] = T[q][j] (zero below diag)
-    static constexpr int RS = TT + IB * (IB + 1);      // Rs[i * (IB+1) + j] = R block (tsqrt)
+    static constexpr int BUF = TT + IB * (IB + 1);     // second V / T pair (walkers double-buffer)
+    static constexpr int RS = BUF + IB * LDV + IB * (IB + 1);  // Rs[i * (IB+1) + j] = R block (tsqrt)
     static constexpr int ZB = RS + IB * (IB + 1);      // block factorization scratch
     static constexpr int TAU = ZB + IB * IB;
     static constexpr int RED = TAU + IB;
@@ -143,27 +145,92 @@ __device__ __forceinline__ void tiles_store_from(const typename Acc<T>::t *a, T 
         }
 }
 
-// Stage block reflectors (columns j0.. j0+IB-1 of the tile, Vg = first of them) and T block.
+// Reflector block prefetch: columns j0..j0+IB-1 of a panel tile (Vg = first of them) and
+// its T block go global -> registers (vload, issued one block ahead) -> LDS buffer (vstore).
 // GE: unit lower part of a geqrt tile (1 on the diagonal, 0 above); TS: full columns.
+template <typename T, int NB> struct VPre {
+    static constexpr int NV = NB * IB / NTH, NT = IB * IB / NTH;
+    T v[NV], t[NT];
+};
+template <typename T, int NB>
+__device__ __forceinline__ void vload(VPre<T, NB> &P, const T *Vg, i64 ldv, const T *Tg, i64 ldt) {
+#pragma unroll
+    for (int e = 0; e < VPre<T, NB>::NV; ++e) {
+        const int idx = threadIdx.x + e * NTH, p = idx % NB, j = idx / NB;
+        P.v[e] = cld(&Vg[p + (i64)j * ldv]);
+    }
+#pragma unroll
+    for (int e = 0; e < VPre<T, NB>::NT; ++e) {
+        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
+        P.t[e] = cld(&Tg[q + (i64)j * ldt]);
+    }
+}
 template <typename T, int NB, bool GE>
-__device__ void stage_block(const T *Vg, i64 ldv, int j0, const T *Tg, i64 ldt) {
-    T *sm = lds_base<T>();
+__device__ __forceinline__ void vstore(const VPre<T, NB> &P, int buf, int j0) {
+    T *sm = lds_base<T>() + buf * Lds<T, NB>::BUF;
     constexpr int LDV = Lds<T, NB>::LDV;
     T *Vs = sm + Lds<T, NB>::V;
     T *Ts = sm + Lds<T, NB>::TT;
-    for (int idx = threadIdx.x; idx < NB * IB; idx += NTH) {
-        const int p = idx % NB, j = idx / NB;
-        T v;
-        v = cld(&Vg[p + (i64)j * ldv]);
+#pragma unroll
+    for (int e = 0; e < VPre<T, NB>::NV; ++e) {
+        const int idx = threadIdx.x + e * NTH, p = idx % NB, j = idx / NB;
+        T v = P.v[e];
         if (GE) {
             const int ja = j0 + j;
             v = p < ja ? T(0) : (p == ja ? T(1) : v);
         }
         Vs[j * LDV + p] = v;
     }
-    for (int idx = threadIdx.x; idx < IB * IB; idx += NTH) {
-        const int q = idx % IB, j = idx / IB;
-        Ts[q * (IB + 1) + j] = q <= j ? cld(&Tg[q + (i64)j * ldt]) : T(0);
+#pragma unroll
+    for (int e = 0; e < VPre<T, NB>::NT; ++e) {
+        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
+        Ts[q * (IB + 1) + j] = q <= j ? P.t[e] : T(0);
+    }
+}
+
+// tsqrt V block (full columns j = 0..IB-1 of Vg) straight into LDS buffer `buf` with
+// global_load_lds (no registers; completion is tracked by vmcnt like any vector load).  Each
+// instruction moves 64 lanes x SZ bytes, a contiguous run of one column of Vs.  Column bases
+// must be 16-byte aligned when SZ is 16 (the caller checks the leading dimension).
+template <typename T, int NB>
+__device__ __forceinline__ void vload_lds(const T *Vg, i64 ldv, int buf) {
+    constexpr int LDV = Lds<T, NB>::LDV;
+    constexpr int CB = NB * (int)sizeof(T);
+    constexpr int SZ = (CB % 1024 == 0) ? 16 : 4;
+    constexpr int CHUNK = 64 * SZ, NCH = CB / CHUNK;
+    static_assert(CB % CHUNK == 0, "column bytes must be a multiple of the chunk");
+    T *Vs = lds_base<T>() + buf * Lds<T, NB>::BUF + Lds<T, NB>::V;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int jj = 0; jj < IB / WAVES; ++jj) {
+        const int j = w + WAVES * jj;
+        const char *src = reinterpret_cast<const char *>(Vg + (i64)j * ldv) + lane * SZ;
+        char *dst = reinterpret_cast<char *>(Vs + j * LDV);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            auto *d = (__attribute__((address_space(3))) void *)(dst + c * CHUNK);
+            if constexpr (SZ == 16) __builtin_amdgcn_global_load_lds(src + c * CHUNK, d, 16, 0, /*sc1*/ 16);
+            else __builtin_amdgcn_global_load_lds(src + c * CHUNK, d, 4, 0, /*sc1*/ 16);
+        }
+    }
+}
+template <typename T, int NB> struct TPre {
+    static constexpr int NT = IB * IB / NTH;
+    T t[NT];
+};
+template <typename T, int NB> __device__ __forceinline__ void tload(TPre<T, NB> &P, const T *Tg, i64 ldt) {
+#pragma unroll
+    for (int e = 0; e < TPre<T, NB>::NT; ++e) {
+        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
+        P.t[e] = cld(&Tg[q + (i64)j * ldt]);
+    }
+}
+template <typename T, int NB> __device__ __forceinline__ void tstore(const TPre<T, NB> &P, int buf) {
+    T *Ts = lds_base<T>() + buf * Lds<T, NB>::BUF + Lds<T, NB>::TT;
+#pragma unroll
+    for (int e = 0; e < TPre<T, NB>::NT; ++e) {
+        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
+        Ts[q * (IB + 1) + j] = q <= j ? P.t[e] : T(0);
     }
 }
 
@@ -171,8 +238,8 @@ __device__ void stage_block(const T *Vg, i64 ldv, int j0, const T *Tg, i64 ldt) 
 //   W0 = V^T S (+ A1 rows),  W = op(T) W0,  A1 rows -= W,  S -= V W.
 // notrans: op(T) = T (apply H); otherwise T^T (apply H^T: the reference's trans 'C').
 template <typename T, int NB, bool GE>
-__device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t *a1, bool notrans) {
-    const T *sm = lds_base<T>();
+__device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t *a1, bool notrans, int buf = 0) {
+    const T *sm = lds_base<T>() + buf * Lds<T, NB>::BUF;
     constexpr int LDV = Lds<T, NB>::LDV;
     constexpr int RT = NB / 16;
     const T *Vs = sm + Lds<T, NB>::V;
@@ -500,51 +567,84 @@ __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 l
     }
 }
 
-// tsmqr! blocks b = B.. of tile pair (A1 rows 32b.., S) with compile-time A1 tile indices
-template <typename T, int NB, int B>
-__device__ __forceinline__ void walk_blocks(WStrip<T, NB> &S, WStrip<T, NB> &A1, const T *Aik, i64 lda, const T *Tik,
-                                            i64 ldt) {
-    if constexpr (B < NB / IB) {
-        __syncthreads();
-        stage_block<T, NB, false>(Aik + (i64)(B * IB) * lda, lda, B * IB, Tik + (i64)(B * IB) * ldt, ldt);
-        __syncthreads();
-        apply_wave<T, NB, false>(S, &A1.a[2 * B], false);
-        walk_blocks<T, NB, B + 1>(S, A1, Aik, lda, Tik, ldt);
-    }
+// Non-blocking: has panel stage counter *p reached v?  (workgroup-uniform result)
+__device__ __forceinline__ bool ready_ge(const int *p, int v, int *sh) {
+    if (threadIdx.x == 0) *sh = ld_flag(p) >= v;
+    __syncthreads();
+    return *sh != 0;
 }
 
+// The walker: unmqr('L','C') of A_kj's strip with geqrt(A_kk)'s blocks, then tsmqr('L','C')
+// of (A_kj, A_ij) for every tile row i > k, A_kj's strip (A1) in MFMA accumulators
+// throughout.  The reflector blocks are a pipeline over two LDS buffers: block B+1 is loaded
+// (V straight into LDS, T through registers) while block B is applied; block 0 of the next
+// tile row is fetched the same way when its panel tile is already done (a walker never
+// waits for the next tile before finishing the current one).
 template <typename T, int NB>
 __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+    constexpr int NBLK = NB / IB;
+    static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
+    __shared__ int sh_ready;
     const int w = threadIdx.x >> 6;
     const i64 col = (i64)s * TW + w * SW;  // this wave's first matrix column
-    int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
+    const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
     int *mine = &g.strip_prog[k * g.nwalk + s];
+    const int *pdone = &g.panel_prog[k * NBLK + NBLK - 1];  // last block stage of panel k
     const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
     const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
     T *Akj = A + (i64)k * NB + col * lda;
-    const int *pdone = &g.panel_prog[k * (NB / IB) + NB / IB - 1];  // last block stage of panel k
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
-    // unmqr!('L','C') with geqrt(A_kk)'s reflectors, blocks in forward order
     WStrip<T, NB> A1;
     tiles_load<T, NB / 16>(A1.a, Akj, lda, 0);
+    // unmqr!('L','C'): geqrt(A_kk)'s blocks in forward order (unit-lower V through registers)
+    {
+        VPre<T, NB> P;
+        vload<T, NB>(P, Akk, lda, Tkk, ldt);
+        vstore<T, NB, true>(P, 0, 0);
 #pragma unroll
-    for (int c0 = 0; c0 < NB; c0 += IB) {
-        __syncthreads();
-        stage_block<T, NB, true>(Akk + (i64)c0 * lda, lda, c0, Tkk + (i64)c0 * ldt, ldt);
-        __syncthreads();
-        apply_wave<T, NB, true>(A1, nullptr, false);
+        for (int B = 0; B < NBLK; ++B) {
+            if (B + 1 < NBLK) vload<T, NB>(P, Akk + (i64)(B + 1) * IB * lda, lda, Tkk + (i64)(B + 1) * IB * ldt, ldt);
+            __syncthreads();
+            apply_wave<T, NB, true>(A1, nullptr, false, B & 1);
+            if (B + 1 < NBLK) vstore<T, NB, true>(P, (B + 1) & 1, (B + 1) * IB);
+        }
     }
-    // tsmqr!('L','C') against every tile below, A_kj's strip (A1) kept in registers
+    // tsmqr!('L','C') against every tile below
+    TPre<T, NB> P;
+    bool have0 = false;  // block 0 of the next tile row is already in (or on its way to) buffer 0
     for (int i = k + 1; i < g.mt; ++i) {
-        wait_ge(pdone, i + 1, g.err, sh_flag);
-        if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
         T *Aij = A + (i64)i * NB + col * lda;
+        if (!have0) {
+            wait_ge(pdone, i + 1, g.err, sh_flag);
+            vload_lds<T, NB>(Aik, lda, 0);
+            tload<T, NB>(P, Tik, ldt);
+            tstore<T, NB>(P, 0);
+        }
+        if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
         WStrip<T, NB> S;
         tiles_load<T, NB / 16>(S.a, Aij, lda, 0);
-        walk_blocks<T, NB, 0>(S, A1, Aik, lda, Tik, ldt);
+        bool next0 = false;
+#pragma unroll
+        for (int B = 0; B < NBLK; ++B) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this block's V (and the strip) landed
+            __syncthreads();
+            if (B + 1 < NBLK) {
+                vload_lds<T, NB>(Aik + (i64)(B + 1) * IB * lda, lda, (B + 1) & 1);
+                tload<T, NB>(P, Tik + (i64)(B + 1) * IB * ldt, ldt);
+            } else if (i + 1 < g.mt) {
+                next0 = ready_ge(pdone, i + 2, &sh_ready);
+                if (next0) {
+                    vload_lds<T, NB>(Aik + NB, lda, 0);
+                    tload<T, NB>(P, Tik + IB, ldt);
+                }
+            }
+            apply_wave<T, NB, false>(S, &A1.a[2 * B], false, B & 1);
+            if (B + 1 < NBLK || next0) tstore<T, NB>(P, (B + 1) & 1);
+        }
+        have0 = next0;
         tiles_store<T, NB / 16>(S.a, Aij, lda, 0);
         publish(mine, i + 1);
     }
@@ -681,6 +781,8 @@ template <typename T>
 int tile_geqrf_dag(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s, bool *handled) {
     *handled = false;
     if (ib != dag::IB) return 0;
+    // reflector columns go straight to LDS in 16-byte pieces: 16-byte aligned columns
+    if ((reinterpret_cast<uintptr_t>(A) & 15) || ((lda * (i64)sizeof(T)) & 15)) return 0;
     if (m / nb >= 4096 || n / dag::TW >= (1 << 18)) return 0;
     *handled = true;
     if (nb == 64) return dag_run<T, 64>(m, n, A, lda, Tm, ldt, s);
