@@ -610,27 +610,42 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
             if (B + 1 < NBLK) vstore<T, NB, true>(P, (B + 1) & 1, (B + 1) * IB);
         }
     }
-    // tsmqr!('L','C') against every tile below
+    // tsmqr!('L','C') against every tile below.  Row i's strip is stored at the end of its
+    // last block and published at the first wait of row i+1 (whose strip and reflector loads
+    // are then in flight together with those stores); the next strip is loaded early when the
+    // previous step has already released it.
     TPre<T, NB> P;
+    WStrip<T, NB> S;
     bool have0 = false;  // block 0 of the next tile row is already in (or on its way to) buffer 0
+    bool haveS = false;  // the next strip is already on its way to S
+    int pending = 0;     // a finished row whose stores are in flight, not yet published (row + 1)
     for (int i = k + 1; i < g.mt; ++i) {
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
         T *Aij = A + (i64)i * NB + col * lda;
+        if (pending && (!have0 || !haveS)) {  // about to block: release the finished row first
+            publish(mine, pending);
+            pending = 0;
+        }
         if (!have0) {
             wait_ge(pdone, i + 1, g.err, sh_flag);
             vload_lds<T, NB>(Aik, lda, 0);
             tload<T, NB>(P, Tik, ldt);
             tstore<T, NB>(P, 0);
         }
-        if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-        WStrip<T, NB> S;
-        tiles_load<T, NB / 16>(S.a, Aij, lda, 0);
+        if (!haveS) {
+            if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
+            tiles_load<T, NB / 16>(S.a, Aij, lda, 0);
+        }
         bool next0 = false;
 #pragma unroll
         for (int B = 0; B < NBLK; ++B) {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this block's V (and the strip) landed
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this block's V, the strip, earlier stores
             __syncthreads();
+            if (B == 0 && pending) {
+                if (threadIdx.x == 0) cst(mine, pending);
+                pending = 0;
+            }
             if (B + 1 < NBLK) {
                 vload_lds<T, NB>(Aik + (i64)(B + 1) * IB * lda, lda, (B + 1) & 1);
                 tload<T, NB>(P, Tik + (i64)(B + 1) * IB * ldt, ldt);
@@ -646,8 +661,11 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
         }
         have0 = next0;
         tiles_store<T, NB / 16>(S.a, Aij, lda, 0);
-        publish(mine, i + 1);
+        pending = i + 1;
+        haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
+        if (haveS) tiles_load<T, NB / 16>(S.a, Aij + NB, lda, 0);
     }
+    if (pending) publish(mine, pending);
     tiles_store<T, NB / 16>(A1.a, Akj, lda, 0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
