@@ -165,6 +165,32 @@ int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, 
                     const float *A, int64_t lda, const float *T, int64_t ldt, float *C, int64_t ldc,
                     void *stream);
 
+/* ---- Adjoints (no reference function: the reference has no differentiation code; these
+ * are what a ChainRulesCore.rrule for unified_rectrxm! and for a thin tile QR calls) ---- */
+/* Solve X = op(A)^-1 (alpha B) (side 'L') / (alpha B) op(A)^-1 ('R') given X and Xbar:
+ * Xbar is overwritten with Bbar, Abar (n x n) receives the adjoint of A on its uplo
+ * triangle (zeros elsewhere).  Shapes as nla_?rectrxm; cutoff as there. */
+int nla_dtrsm_rrule(char side, char uplo, char trans, double alpha, int64_t n, int64_t m, const double *A,
+                    int64_t lda, const double *X, int64_t ldx, double *Xbar, int64_t ldxbar, double *Abar,
+                    int64_t ldabar, int64_t cutoff, void *stream);
+int nla_strsm_rrule(char side, char uplo, char trans, float alpha, int64_t n, int64_t m, const float *A,
+                    int64_t lda, const float *X, int64_t ldx, float *Xbar, int64_t ldxbar, float *Abar,
+                    int64_t ldabar, int64_t cutoff, void *stream);
+/* Multiply Y = alpha op(A) B ('L') / alpha B op(A) ('R') given the input B and Ybar:
+ * Ybar is overwritten with Bbar, Abar as above. */
+int nla_dtrmm_rrule(char side, char uplo, char trans, double alpha, int64_t n, int64_t m, const double *A,
+                    int64_t lda, const double *B, int64_t ldb, double *Ybar, int64_t ldybar, double *Abar,
+                    int64_t ldabar, int64_t cutoff, void *stream);
+int nla_strmm_rrule(char side, char uplo, char trans, float alpha, int64_t n, int64_t m, const float *A,
+                    int64_t lda, const float *B, int64_t ldb, float *Ybar, int64_t ldybar, float *Abar,
+                    int64_t ldabar, int64_t cutoff, void *stream);
+/* Thin QR A = Q R (m x n, n <= m, both multiples of nb) from nla_?tile_geqrf's output (AF, T):
+ * given Qbar (m x n, overwritten with Abar) and Rbar (n x n). */
+int nla_dqr_rrule(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *AF, int64_t lda, const double *T,
+                  int64_t ldt, double *Qbar, int64_t ldqbar, const double *Rbar, int64_t ldrbar, void *stream);
+int nla_sqr_rrule(int64_t m, int64_t n, int64_t nb, int64_t ib, const float *AF, int64_t lda, const float *T,
+                  int64_t ldt, float *Qbar, int64_t ldqbar, const float *Rbar, int64_t ldrbar, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,9 @@ _SIGS = {
     "gemm_acc": [i64, i64, i64, "R", vp, i64, vp, i64, vp, i64, vp],
     "tile_geqrf": [i64, i64, i64, i64, vp, i64, vp, i64, vp],
     "tile_ormqr": [ch, i64, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp],
+    "trsm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
+    "trmm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
+    "qr_rrule": [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp],
 }
 
 _lib = None
