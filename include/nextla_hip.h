@@ -165,6 +165,26 @@ int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, 
                     const float *A, int64_t lda, const float *T, int64_t ldt, float *C, int64_t ldc,
                     void *stream);
 
+/* ---- Mixed precision and least squares (no reference function: config "mixed-precision
+ * tile QR + least squares"; the reference stops at the tile kernels) --------------------- */
+/* nla_dtile_geqrf with the trailing updates (unmqr!/tsmqr!) in Float32 MFMA when prec = 'S'
+ * (panels, larfg norms/scaling and the T factors stay Float64); prec = 'D' is nla_dtile_geqrf.
+ * Needs nb in {64, 128}, ib = 32 and 16-byte aligned columns. */
+int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
+                          int64_t ldt, char prec, void *stream);
+/* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
+ * out.  Tile QR of a copy of A (prec 'S': Float32 trailing updates, Float64 only), then
+ * x0 = R^-1 (Q^T b)[0:n] and up to maxit refinement steps x += R^-1 (Q^T (b - A x))[0:n]
+ * with the residual in the working precision, stopping when ||dx|| <= tol ||x||.
+ * iters / last_correction (may be NULL) receive the steps taken and the final ||dx||/||x||.
+ * Synchronises the stream. */
+int nla_dtile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *A, int64_t lda, const double *b,
+                   double *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
+                   void *stream);
+int nla_stile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const float *A, int64_t lda, const float *b,
+                   float *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
+                   void *stream);
+
 /* ---- Adjoints (no reference function: the reference has no differentiation code; these
  * are what a ChainRulesCore.rrule for unified_rectrxm! and for a thin tile QR calls) ---- */
 /* Solve X = op(A)^-1 (alpha B) (side 'L') / (alpha B) op(A)^-1 ('R') given X and Xbar:

@@ -537,6 +537,9 @@ struct DagArgs {
     int *err;
     unsigned long long *trace;  // optional timeline: [ntasks][4] = claim, -, end, workgroup
     unsigned long long *ptrace; // optional: [kt][mt] panel tile completion times
+    void *C;                    // matrix the walkers update (A itself when factoring)
+    i64 ldc;
+    int qmode;                  // walkers apply Q (reverse order) instead of Q^T
 };
 
 // Block stage b of panel k: ib-block b of geqrt(A_kk), then of tsqrt(A_kk, A_ik) for
@@ -592,11 +595,13 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
     const int *pdone = &g.panel_prog[k * NBLK + NBLK - 1];  // last block stage of panel k
     const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
     const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
-    T *Akj = A + (i64)k * NB + col * lda;
+    T *Cm = reinterpret_cast<T *>(g.C);
+    const i64 ldc = g.ldc;
+    T *Akj = Cm + (i64)k * NB + col * ldc;
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     WStrip<T, NB> A1;
-    tiles_load<T, NB / 16>(A1.a, Akj, lda, 0);
+    tiles_load<T, NB / 16>(A1.a, Akj, ldc, 0);
     // unmqr!('L','C'): geqrt(A_kk)'s blocks in forward order (unit-lower V through registers)
     {
         VPre<T, NB> P;
@@ -622,7 +627,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
     for (int i = k + 1; i < g.mt; ++i) {
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
-        T *Aij = A + (i64)i * NB + col * lda;
+        T *Aij = Cm + (i64)i * NB + col * ldc;
         if (pending && (!have0 || !haveS)) {  // about to block: release the finished row first
             publish(mine, pending);
             pending = 0;
@@ -635,7 +640,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            tiles_load<T, NB / 16>(S.a, Aij, lda, 0);
+            tiles_load<T, NB / 16>(S.a, Aij, ldc, 0);
         }
         bool next0 = false;
 #pragma unroll
@@ -660,17 +665,181 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
             if (B + 1 < NBLK || next0) tstore<T, NB>(P, (B + 1) & 1);
         }
         have0 = next0;
-        tiles_store<T, NB / 16>(S.a, Aij, lda, 0);
+        tiles_store<T, NB / 16>(S.a, Aij, ldc, 0);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
-        if (haveS) tiles_load<T, NB / 16>(S.a, Aij + NB, lda, 0);
+        if (haveS) tiles_load<T, NB / 16>(S.a, Aij + NB, ldc, 0);
     }
     if (pending) publish(mine, pending);
-    tiles_store<T, NB / 16>(A1.a, Akj, lda, 0);
+    tiles_store<T, NB / 16>(A1.a, Akj, ldc, 0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
+// Reverse walker: Q C for the strip, step k of Q = Q_0 Q_1 ... (steps run k = kt-1 down to 0).
+// Q_k = G_k TS_{k,k+1} ... TS_{k,mt-1}, so the rows go mt-1 down to k+1 (tsmqr('L','N'),
+// blocks in reverse order) and then row k (unmqr('L','N')).  Row k is final input for step
+// k at once; row i > k must be done by step k+1, whose counter counts rows finished from the
+// bottom (its own row k+1 last).
 template <typename T, int NB>
+__device__ __forceinline__ void run_walk_q(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+    constexpr int NBLK = NB / IB;
+    const int kt = g.mt < g.nt ? g.mt : g.nt;
+    const int w = threadIdx.x >> 6;
+    const i64 col = (i64)s * TW + w * SW;
+    const int *prev = k + 1 < kt ? &g.strip_prog[(k + 1) * g.nwalk + s] : nullptr;
+    int *mine = &g.strip_prog[k * g.nwalk + s];
+    T *Cm = reinterpret_cast<T *>(g.C);
+    const i64 ldc = g.ldc;
+    T *Ckj = Cm + (i64)k * NB + col * ldc;
+    WStrip<T, NB> A1;
+    tiles_load<T, NB / 16>(A1.a, Ckj, ldc, 0);
+    VPre<T, NB> P;
+    for (int i = g.mt - 1; i > k; --i) {
+        if (prev) wait_ge(prev, g.mt - i, g.err, sh_flag);
+        const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
+        const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
+        T *Cij = Cm + (i64)i * NB + col * ldc;
+        WStrip<T, NB> S;
+        tiles_load<T, NB / 16>(S.a, Cij, ldc, 0);
+#pragma unroll
+        for (int B = NBLK - 1; B >= 0; --B) {
+            vload<T, NB>(P, Aik + (i64)B * IB * lda, lda, Tik + (i64)B * IB * ldt, ldt);
+            __syncthreads();
+            vstore<T, NB, false>(P, 0, 0);
+            __syncthreads();
+            apply_wave<T, NB, false>(S, &A1.a[2 * B], true, 0);
+        }
+        tiles_store<T, NB / 16>(S.a, Cij, ldc, 0);
+        publish(mine, g.mt - i);
+    }
+    const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
+    const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
+#pragma unroll
+    for (int B = NBLK - 1; B >= 0; --B) {
+        vload<T, NB>(P, Akk + (i64)B * IB * lda, lda, Tkk + (i64)B * IB * ldt, ldt);
+        __syncthreads();
+        vstore<T, NB, true>(P, 0, B * IB);
+        __syncthreads();
+        apply_wave<T, NB, true>(A1, nullptr, true, 0);
+    }
+    tiles_store<T, NB / 16>(A1.a, Ckj, ldc, 0);
+    publish(mine, g.mt - k);
+}
+
+// ---------------------------------------------------------------------------------
+// Mixed-precision walker: the panel (geqrt/tsqrt blocks and T) stays in T = Float64; the
+// trailing update of the strips (unmqr/tsmqr) runs in Tw = Float32 on v_mfma_f32_16x16x4:
+// strips are rounded to Tw on load and widened back on store, the reflector blocks and T
+// are rounded when staged.  Same pipeline as the Float64 walker, with V through registers.
+template <typename Tw, typename T, int NT>
+__device__ __forceinline__ void tiles_load_cvt(typename Acc<Tw>::t *a, const T *G, i64 ld, int r0) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[rt][r] = (Tw)cld(&G[(r0 + 16 * rt + acc_row<Tw>(lk, r)) + (i64)lr * ld]);
+}
+template <typename Tw, typename T, int NT>
+__device__ __forceinline__ void tiles_store_cvt(const typename Acc<Tw>::t *a, T *G, i64 ld, int r0) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cst(&G[(r0 + 16 * rt + acc_row<Tw>(lk, r)) + (i64)lr * ld], (T)a[rt][r]);
+}
+template <typename T, typename Tw, int NB, bool GE>
+__device__ __forceinline__ void vstore_cvt(const VPre<T, NB> &P, int buf, int j0) {
+    Tw *sm = lds_base<Tw>() + buf * Lds<Tw, NB>::BUF;
+    constexpr int LDV = Lds<Tw, NB>::LDV;
+    Tw *Vs = sm + Lds<Tw, NB>::V;
+    Tw *Ts = sm + Lds<Tw, NB>::TT;
+#pragma unroll
+    for (int e = 0; e < VPre<T, NB>::NV; ++e) {
+        const int idx = threadIdx.x + e * NTH, p = idx % NB, j = idx / NB;
+        T v = P.v[e];
+        if (GE) {
+            const int ja = j0 + j;
+            v = p < ja ? T(0) : (p == ja ? T(1) : v);
+        }
+        Vs[j * LDV + p] = (Tw)v;
+    }
+#pragma unroll
+    for (int e = 0; e < VPre<T, NB>::NT; ++e) {
+        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
+        Ts[q * (IB + 1) + j] = q <= j ? (Tw)P.t[e] : Tw(0);
+    }
+}
+
+template <typename T, typename Tw, int NB>
+__device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+    constexpr int NBLK = NB / IB;
+    static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
+    __shared__ int sh_ready;
+    const int w = threadIdx.x >> 6;
+    const i64 col = (i64)s * TW + w * SW;
+    const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
+    int *mine = &g.strip_prog[k * g.nwalk + s];
+    const int *pdone = &g.panel_prog[k * NBLK + NBLK - 1];
+    const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
+    const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
+    T *Cm = reinterpret_cast<T *>(g.C);
+    const i64 ldc = g.ldc;
+    T *Akj = Cm + (i64)k * NB + col * ldc;
+    wait_ge(pdone, k + 1, g.err, sh_flag);
+    if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
+    WStrip<Tw, NB> A1;
+    tiles_load_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
+    VPre<T, NB> P;
+    vload<T, NB>(P, Akk, lda, Tkk, ldt);
+    vstore_cvt<T, Tw, NB, true>(P, 0, 0);
+    bool have0 = false;
+#pragma unroll
+    for (int B = 0; B < NBLK; ++B) {
+        if (B + 1 < NBLK) {
+            vload<T, NB>(P, Akk + (i64)(B + 1) * IB * lda, lda, Tkk + (i64)(B + 1) * IB * ldt, ldt);
+        } else if (k + 1 < g.mt) {
+            have0 = ready_ge(pdone, k + 2, &sh_ready);
+            if (have0) vload<T, NB>(P, Akk + NB, lda, Tkk + IB, ldt);
+        }
+        __syncthreads();
+        apply_wave<Tw, NB, true>(A1, nullptr, false, B & 1);
+        if (B + 1 < NBLK) vstore_cvt<T, Tw, NB, true>(P, (B + 1) & 1, (B + 1) * IB);
+        else if (have0) vstore_cvt<T, Tw, NB, false>(P, 0, 0);
+    }
+    for (int i = k + 1; i < g.mt; ++i) {
+        const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
+        const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
+        T *Aij = Cm + (i64)i * NB + col * ldc;
+        if (!have0) {
+            wait_ge(pdone, i + 1, g.err, sh_flag);
+            vload<T, NB>(P, Aik, lda, Tik, ldt);
+            vstore_cvt<T, Tw, NB, false>(P, 0, 0);
+        }
+        if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
+        WStrip<Tw, NB> S;
+        tiles_load_cvt<Tw, T, NB / 16>(S.a, Aij, ldc, 0);
+        bool next0 = false;
+#pragma unroll
+        for (int B = 0; B < NBLK; ++B) {
+            if (B + 1 < NBLK) {
+                vload<T, NB>(P, Aik + (i64)(B + 1) * IB * lda, lda, Tik + (i64)(B + 1) * IB * ldt, ldt);
+            } else if (i + 1 < g.mt) {
+                next0 = ready_ge(pdone, i + 2, &sh_ready);
+                if (next0) vload<T, NB>(P, Aik + NB, lda, Tik + IB, ldt);
+            }
+            __syncthreads();
+            apply_wave<Tw, NB, false>(S, &A1.a[2 * B], false, B & 1);
+            if (B + 1 < NBLK || next0) vstore_cvt<T, Tw, NB, false>(P, (B + 1) & 1, 0);
+        }
+        have0 = next0;
+        tiles_store_cvt<Tw, T, NB / 16>(S.a, Aij, ldc, 0);
+        publish(mine, i + 1);
+    }
+    tiles_store_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <typename T, int NB, typename Tw>
 __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i64 ldt, DagArgs g) {
     __shared__ int sh_task, sh_flag;
     for (;;) {
@@ -685,7 +854,12 @@ __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i
             g.trace[(size_t)task * 4 + 3] = blockIdx.x;
         }
         if (task_type(code) == TASK_PANEL) run_panel<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
-        else run_walk<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
+        else if constexpr (sizeof(Tw) == sizeof(T)) {
+            if (g.qmode) run_walk_q<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
+            else run_walk<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
+        } else {
+            run_walk_mixed<T, Tw, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
+        }
         if (g.trace && threadIdx.x == 0) g.trace[(size_t)task * 4 + 2] = __builtin_amdgcn_s_memrealtime();
     }
 }
@@ -714,29 +888,45 @@ static std::vector<int> build_tasks(int mt, int nt, int nb) {
 
 }  // namespace dag
 
-// Host driver.  Returns 0, a HIP error code, or -1 (reported as a HIP-side failure by the
-// caller) when a dependency wait timed out.
-template <typename T, int NB>
-static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
+// Host driver.  FACTOR: the whole tile QR of A (walkers update A itself).  APPLY: Q^T of
+// an already factored A applied to C (m x ncols, ncols a multiple of TW) by the walkers
+// alone, every panel marked done up front.  Returns 0, a HIP error code, or -1 (reported as
+// a HIP-side failure by the caller) when a dependency wait timed out.
+enum DagMode { DAG_FACTOR = 0, DAG_APPLY = 1, DAG_APPLY_Q = 2 };
+
+template <typename T, int NB, typename Tw>
+static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T *C, i64 ldc, i64 ncols,
+                   hipStream_t s) {
     using namespace dag;
     const int mt = (int)(m / NB), nt = (int)(n / NB), kt = mt < nt ? mt : nt;
-    const int nwalk = (int)(n / TW);
-    std::vector<int> tasks = build_tasks(mt, nt, NB);
+    const int nblk = NB / IB;
+    const int nwalk = (int)((mode == DAG_FACTOR ? n : ncols) / TW);
+    std::vector<int> tasks;
+    if (mode == DAG_FACTOR) {
+        tasks = build_tasks(mt, nt, NB);
+    } else if (mode == DAG_APPLY) {
+        for (int k = 0; k < kt; ++k)
+            for (int w = 0; w < nwalk; ++w) tasks.push_back(enc(TASK_WALK, k, w));
+    } else {
+        for (int k = kt - 1; k >= 0; --k)
+            for (int w = 0; w < nwalk; ++w) tasks.push_back(enc(TASK_WALK, k, w));
+    }
     const int ntasks = (int)tasks.size();
     int dev = 0, ncu = 0;
     NLA_HIP(hipGetDevice(&dev));
     NLA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t sh = sizeof(T) * (size_t)Lds<T, NB>::TOTAL;
+    const size_t sh_t = sizeof(T) * (size_t)Lds<T, NB>::TOTAL, sh_w = sizeof(Tw) * (size_t)Lds<Tw, NB>::TOTAL;
+    const size_t sh = sh_t > sh_w ? sh_t : sh_w;
     if (sh > 64 * 1024)
-        NLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&tileqr_dag_kernel<T, NB>),
+        NLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&tileqr_dag_kernel<T, NB, Tw>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     int per_cu = 0;
-    NLA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tileqr_dag_kernel<T, NB>, NTH, sh));
+    NLA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tileqr_dag_kernel<T, NB, Tw>, NTH, sh));
     if (per_cu < 1) per_cu = 1;
     int grid = ncu * per_cu;
     if (grid > ntasks) grid = ntasks;
     // scheduler memory: tasks | counter | err | panel_prog[kt*NB/IB] | strip_prog[kt*nwalk]
-    const int npp = kt * (NB / IB);
+    const int npp = kt * nblk;
     const size_t nint = (size_t)ntasks + 2 + npp + (size_t)kt * nwalk;
     int st = 0;
     int *mem = (int *)scratch(SLOT_SCHED, nint * sizeof(int), s, &st);
@@ -746,18 +936,23 @@ static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
     const int nzero = 2 + npp + kt * nwalk;
     fill_int_kernel<<<(nzero + 255) / 256 > 1024 ? 1024 : (nzero + 255) / 256, 256, 0, s>>>(counter, nzero, 0);
     NLA_LAUNCH_CHECK();
+    if (mode != DAG_FACTOR) {  // every panel tile is factored already
+        fill_int_kernel<<<(npp + 255) / 256, 256, 0, s>>>(pp, npp, mt);
+        NLA_LAUNCH_CHECK();
+    }
     static const bool want_trace = getenv("NLA_TILEQR_TRACE") != nullptr;
+    const bool trace = want_trace && mode == DAG_FACTOR;
     unsigned long long *tr = nullptr, *ptr = nullptr;
-    if (want_trace) {
+    if (trace) {
         tr = (unsigned long long *)scratch(SLOT_MIXED, sizeof(unsigned long long) * ((size_t)ntasks * 4 + (size_t)kt * mt),
                                           s, &st);
         if (!tr) return st;
         ptr = tr + (size_t)ntasks * 4;
         NLA_HIP(hipMemsetAsync(tr, 0, sizeof(unsigned long long) * ((size_t)ntasks * 4 + (size_t)kt * mt), s));
     }
-    DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr};
+    DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr, (void *)C, ldc, mode == DAG_APPLY_Q ? 1 : 0};
     unsigned long long *phase = nullptr;
-    if (want_trace) {
+    if (trace) {
         phase = (unsigned long long *)scratch(SLOT_W, 64 * sizeof(unsigned long long), s, &st);
         if (!phase) return st;
         NLA_HIP(hipMemsetAsync(phase, 0, 64 * sizeof(unsigned long long), s));
@@ -765,14 +960,14 @@ static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
     NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_phase_trace), &phase, sizeof(phase), 0, hipMemcpyHostToDevice, s));
     {
         ProfScope prof(PK_TILE_QR, s);
-        tileqr_dag_kernel<T, NB><<<grid, NTH, sh, s>>>(A, lda, Tm, ldt, g);
+        tileqr_dag_kernel<T, NB, Tw><<<grid, NTH, sh, s>>>(A, lda, Tm, ldt, g);
         NLA_LAUNCH_CHECK();
     }
     int herr = 0;
     NLA_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
     NLA_HIP(hipStreamSynchronize(s));
-    if (want_trace) {
-        // "task type k s claim ready end wg" lines, then "panel k i time" lines
+    if (trace) {
+        // "task type k s claim end wg" lines, then "panel k i time" lines
         const char *path = getenv("NLA_TILEQR_TRACE");
         std::vector<unsigned long long> h((size_t)ntasks * 4 + (size_t)kt * mt);
         NLA_HIP(hipMemcpy(h.data(), tr, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
@@ -795,21 +990,48 @@ static int dag_run(i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s) {
     return 0;
 }
 
+// Shapes the dataflow kernel handles: ib = 32, nb in {64, 128}, 16-byte aligned columns.
+template <typename T> static bool dag_shape_ok(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda) {
+    if (ib != dag::IB || (nb != 64 && nb != 128)) return false;
+    if (m / nb >= 4096 || n / dag::TW >= (1 << 18)) return false;
+    return !((reinterpret_cast<uintptr_t>(A) & 15) || ((lda * (i64)sizeof(T)) & 15));
+}
+
 template <typename T>
 int tile_geqrf_dag(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s, bool *handled) {
-    *handled = false;
-    if (ib != dag::IB) return 0;
-    // reflector columns go straight to LDS in 16-byte pieces: 16-byte aligned columns
-    if ((reinterpret_cast<uintptr_t>(A) & 15) || ((lda * (i64)sizeof(T)) & 15)) return 0;
-    if (m / nb >= 4096 || n / dag::TW >= (1 << 18)) return 0;
-    *handled = true;
-    if (nb == 64) return dag_run<T, 64>(m, n, A, lda, Tm, ldt, s);
-    if (nb == 128) return dag_run<T, 128>(m, n, A, lda, Tm, ldt, s);
-    *handled = false;
-    return 0;
+    *handled = dag_shape_ok<T>(m, n, nb, ib, A, lda);
+    if (!*handled) return 0;
+    if (nb == 64) return dag_run<T, 64, T>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+    return dag_run<T, 128, T>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+}
+
+// Float64 panels, Float32 trailing updates.
+int tile_geqrf_dag_mixed(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 lda, double *Tm, i64 ldt, hipStream_t s,
+                         bool *handled) {
+    *handled = dag_shape_ok<double>(m, n, nb, ib, A, lda);
+    if (!*handled) return 0;
+    if (nb == 64) return dag_run<double, 64, float>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+    return dag_run<double, 128, float>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+}
+
+// C (m x ncols, ncols a multiple of 64) := Q^T C (qt) or Q C with the factors of tile_geqrf.
+template <typename T>
+int tile_apply_qt_dag(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, const T *Tm, i64 ldt, T *C, i64 ldc,
+                      i64 ncols, hipStream_t s, bool *handled, bool qt) {
+    *handled = dag_shape_ok<T>(m, n, nb, ib, A, lda) && ncols % dag::TW == 0 &&
+               !((reinterpret_cast<uintptr_t>(C) & 15) || ((ldc * (i64)sizeof(T)) & 15));
+    if (!*handled) return 0;
+    T *Am = const_cast<T *>(A), *Tmm = const_cast<T *>(Tm);
+    const DagMode md = qt ? DAG_APPLY : DAG_APPLY_Q;
+    if (nb == 64) return dag_run<T, 64, T>(md, m, n, Am, lda, Tmm, ldt, C, ldc, ncols, s);
+    return dag_run<T, 128, T>(md, m, n, Am, lda, Tmm, ldt, C, ldc, ncols, s);
 }
 
 template int tile_geqrf_dag<double>(i64, i64, i64, i64, double *, i64, double *, i64, hipStream_t, bool *);
 template int tile_geqrf_dag<float>(i64, i64, i64, i64, float *, i64, float *, i64, hipStream_t, bool *);
+template int tile_apply_qt_dag<double>(i64, i64, i64, i64, const double *, i64, const double *, i64, double *, i64, i64,
+                                       hipStream_t, bool *, bool);
+template int tile_apply_qt_dag<float>(i64, i64, i64, i64, const float *, i64, const float *, i64, float *, i64, i64,
+                                      hipStream_t, bool *, bool);
 
 }  // namespace nla

@@ -37,6 +37,11 @@ _SIGS = {
     "trsm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
     "trmm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
     "qr_rrule": [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp],
+    "tile_gels": [i64, i64, i64, i64, vp, i64, vp, vp, ch, i64, dbl, vp, vp, vp],
+}
+# Float64-only entry points
+_SIGS_D = {
+    "tile_geqrf_mixed": [i64, i64, i64, i64, vp, i64, vp, i64, ch, vp],
 }
 
 _lib = None
@@ -70,6 +75,10 @@ def load():
             f = getattr(lib, f"nla_{p}{name}")
             f.argtypes = [real if a == "R" else a for a in sig]
             f.restype = ctypes.c_int
+    for name, sig in _SIGS_D.items():
+        f = getattr(lib, f"nla_d{name}")
+        f.argtypes = sig
+        f.restype = ctypes.c_int
     _lib = lib
     return lib
 

@@ -18,7 +18,8 @@ __all__ = [
     "unmqr_", "unmqr", "tsqrt_", "tsqrt", "tsmqr_", "tsmqr", "parfb_",
     "unified_rectrxm_", "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_", "trsm", "trmm",
-    "GEMM_ADD_", "GEMM_SUB_", "tile_geqrf", "tile_ormqr", "NextLAMatrix", "synchronize",
+    "GEMM_ADD_", "GEMM_SUB_", "tile_geqrf", "tile_ormqr", "tile_geqrf_mixed", "tile_gels", "NextLAMatrix",
+    "synchronize",
 ]
 
 
@@ -391,6 +392,37 @@ def tile_ormqr(trans, A, T, C, nb, ib):
     pc, ldc = _m(C)
     call(p, "tile_ormqr", _c(trans), m, n, C.shape[1], nb, ib, pa, lda, pt, ldt, pc, ldc, _stream())
     return C
+
+
+def tile_geqrf_mixed(A, nb, ib, prec="S"):
+    """tile_geqrf of a Float64 A with the trailing updates in Float32 MFMA (prec 'S'); the
+    panels, Householder norms/scaling and T stay Float64.  Returns T."""
+    if A.dtype != torch.float64:
+        raise ArgumentError("tile_geqrf_mixed needs Float64 data")
+    m, n = A.shape
+    T = zeros_cm((m // nb) * ib, n, dtype=A.dtype, device=A.device)
+    pa, lda = _m(A)
+    pt, ldt = _m(T)
+    call("d", "tile_geqrf_mixed", m, n, nb, ib, pa, lda, pt, ldt, _c(prec), _stream())
+    return T
+
+
+def tile_gels(A, b, nb=128, ib=32, prec="D", maxit=10, tol=1e-15):
+    """Least-squares solution of min ||A x - b|| through the tile QR with iterative
+    refinement (residuals in the working precision).  prec 'S' (Float64 only) factors with
+    Float32 trailing updates.  Returns (x, iterations, last ||dx||/||x||)."""
+    import ctypes
+
+    p = _p(A, b)
+    m, n = A.shape
+    x = torch.zeros(n, dtype=A.dtype, device=A.device)
+    pa, lda = _m(A)
+    it = ctypes.c_int64(0)
+    last = ctypes.c_double(0)
+    call(p, "tile_gels", m, n, nb, ib, pa, lda, _ptr(b.contiguous()), _ptr(x), _c(prec), int(maxit), float(tol),
+         ctypes.cast(ctypes.pointer(it), ctypes.c_void_p), ctypes.cast(ctypes.pointer(last), ctypes.c_void_p),
+         _stream())
+    return x, it.value, last.value
 
 
 # ---------------------------------------------------------------------------- NextLAMatrix

@@ -1,0 +1,117 @@
+"""Q^T application by the dataflow walkers, mixed-precision tile QR and least squares with
+iterative refinement (the mixed-precision least-squares workload, at reduced sizes).
+
+Pins: the oracle's tile_ormqr (restatement of the reference's unmqr!/tsmqr! composition),
+LAPACK's FP64 least-squares solution (numpy.linalg.lstsq), and the closed-form properties
+of a least-squares solution (A^T r = 0, ||A - Q R|| at the working precision)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _nl():
+    import nextla_hip as nl
+
+    nl.load()
+    return nl
+
+
+def ls_problem(rng, m, n, kappa, noise=1e-8):
+    """A = U diag(s) V^T with s geometric from 1 to 1/kappa; b = A x0 + noise."""
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = kappa ** (-np.arange(n) / (n - 1))
+    A = (U * s) @ V.T
+    x0 = rng.standard_normal(n)
+    b = A @ x0 + noise * rng.standard_normal(m)
+    return np.asfortranarray(A), b
+
+
+@pytest.mark.parametrize("trans", ["T", "N"])
+@pytest.mark.parametrize("nrhs", [5, 64, 130])
+@pytest.mark.parametrize("m,n,nb", [(512, 256, 64), (768, 512, 128), (512, 512, 128)])
+def test_ormqr_walkers_vs_oracle(m, n, nb, nrhs, trans):
+    nl = _nl()
+    import oracle as O
+
+    rng = np.random.default_rng(3)
+    A0 = np.asfortranarray(rng.standard_normal((m, n)))
+    C0 = np.asfortranarray(rng.standard_normal((m, nrhs)))
+    dA = nl.to_device(A0)
+    dT = nl.tile_geqrf(dA, nb, 32)
+    dC = nl.to_device(C0)
+    nl.tile_ormqr(trans, dA, dT, dC, nb, 32)
+    torch.cuda.synchronize()
+    Ah, Th = nl.to_host(dA), nl.to_host(dT)
+    C = C0.copy(order="F")
+    O.tile_ormqr(trans, Ah, Th, C, nb, 32)
+    assert relerr(nl.to_host(dC), C) < 1e-13
+
+
+@pytest.mark.parametrize("m,n,nb", [(1024, 512, 128), (512, 512, 64), (2048, 256, 128)])
+def test_tile_geqrf_mixed_backward_error(m, n, nb):
+    nl = _nl()
+    rng = np.random.default_rng(9)
+    A0 = np.asfortranarray(rng.standard_normal((m, n)))
+    dA = nl.to_device(A0)
+    dT = nl.tile_geqrf_mixed(dA, nb, 32, "S")
+    # Q R from the factors: R (upper part, zero below) then Q applied from (V, T)
+    QR = nl.zeros_cm(m, n)
+    QR[:n, :] = torch.triu(dA[:n, :])
+    nl.tile_ormqr("N", dA, dT, QR, nb, 32)
+    torch.cuda.synchronize()
+    berr = relerr(nl.to_host(QR), A0)
+    assert berr < 2e-5, berr          # Float32 trailing updates
+    # and the factorization is not the plain Float64 one (the updates really are Float32)
+    dA2 = nl.to_device(A0)
+    nl.tile_geqrf(dA2, nb, 32)
+    d = relerr(nl.to_host(dA), nl.to_host(dA2))
+    assert 1e-9 < d < 1e-3, d
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e6])
+@pytest.mark.parametrize("prec", ["D", "S"])
+def test_gels_refinement_vs_lapack(prec, kappa):
+    nl = _nl()
+    rng = np.random.default_rng(int(kappa) % 1000 + (prec == "S"))
+    m, n, nb = 2048, 512, 128
+    A, b = ls_problem(rng, m, n, kappa)
+    xref = np.linalg.lstsq(A, b, rcond=None)[0]
+    dA, db = nl.to_device(A), torch.tensor(b, device="cuda")
+    x, iters, last = nl.tile_gels(dA, db, nb, 32, prec, maxit=8, tol=1e-15)
+    xh = x.cpu().numpy()
+    r = b - A @ xh
+    # error of a backward-stable LS solver: ~ u (kappa + kappa^2 ||r|| / (||A|| ||x||))
+    assert relerr(xh, xref) < 50 * 2.2e-16 * kappa * (1 + kappa * 1e-8), (relerr(xh, xref), iters, last)
+    # normwise LS backward-error indicator ||A^T r|| / (||A|| ||r||), against LAPACK's own
+    rr = b - A @ xref
+    eta = np.linalg.norm(A.T @ r) / (np.linalg.norm(A, 2) * np.linalg.norm(r))
+    eta_ref = np.linalg.norm(A.T @ rr) / (np.linalg.norm(A, 2) * np.linalg.norm(rr))
+    assert eta < 10 * max(eta_ref, 1e-14), (eta, eta_ref)
+    if prec == "S":
+        assert iters >= 1
+
+
+def test_gels_mixed_without_refinement_is_float32_accurate():
+    nl = _nl()
+    rng = np.random.default_rng(4)
+    m, n = 1024, 256
+    A, b = ls_problem(rng, m, n, 1e2)
+    xref = np.linalg.lstsq(A, b, rcond=None)[0]
+    x, iters, _ = nl.tile_gels(nl.to_device(A), torch.tensor(b, device="cuda"), 128, 32, "S", maxit=0)
+    e = relerr(x.cpu().numpy(), xref)
+    assert iters == 0 and 1e-12 < e < 1e-3, e
+
+
+def test_gels_argument_errors():
+    nl = _nl()
+    A = nl.zeros_cm(256, 128)
+    b = torch.zeros(256, dtype=torch.float64, device="cuda")
+    with pytest.raises(nl.ArgumentError, match="prec must be"):
+        nl.tile_gels(A, b, 128, 32, "X")
+    with pytest.raises(nl.ArgumentError, match="Float64"):
+        nl.tile_geqrf_mixed(nl.zeros_cm(256, 128, dtype=torch.float32), 128, 32)
