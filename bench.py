@@ -85,6 +85,40 @@ def cpu_baseline(n, cut, target_s=12.0):
                       f"cut-off {cut}, {dt:.1f} s"}
 
 
+def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
+    """Mixed-precision tile QR + least squares: A = U diag(s) V^T (U, V from QR of Gaussian,
+    s geometric from 1 to 1/kappa), b = A x0 + 1e-8 N(0,1); Float64 vs Float32-update
+    factorization, both refined on the augmented system to the Float64 solution."""
+    out = {"m": m, "n": n}
+    for kappa in (1e2, 1e6):
+        U = torch.linalg.qr(torch.randn((m, n), generator=gen, device="cuda", dtype=torch.float64))[0]
+        V = torch.linalg.qr(torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64))[0]
+        sv = kappa ** (-torch.arange(n, device="cuda", dtype=torch.float64) / (n - 1))
+        A = nl.colmajor((U * sv) @ V.T)
+        del U, V
+        x0 = torch.randn(n, generator=gen, device="cuda", dtype=torch.float64)
+        b = A @ x0 + 1e-8 * torch.randn(m, generator=gen, device="cuda", dtype=torch.float64)
+        row = {}
+        for prec in ("D", "S"):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, it, last = nl.tile_gels(A, b, nb, ib, prec, maxit=10, tol=1e-15)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            r = b - A @ x
+            # ||A^T r|| / (||A||_2 ||r||) with ||A||_2 = s_1 = 1 by construction
+            eta = (torch.linalg.norm(A.T @ r) / torch.linalg.norm(r)).item()
+            row[prec] = {"s": round(dt, 3), "iterations": it, "eta": eta, "x": x}
+            log(f"[bench] LS kappa={kappa:g} prec {prec}: {dt:.3f} s, {it} steps, eta {eta}")
+        d = (torch.linalg.norm(row["S"]["x"] - row["D"]["x"]) / torch.linalg.norm(row["D"]["x"])).item()
+        for v in row.values():
+            v.pop("x")
+        out[f"kappa_{kappa:g}"] = {"f64": row["D"], "mixed_f32": row["S"], "mixed_vs_f64_time": round(row["D"]["s"] / row["S"]["s"], 3),
+                                  "rel_diff_mixed_vs_f64_solution": d}
+        del A, b
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +127,8 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--m", type=int, default=4096)
     ap.add_argument("--cutoff", type=int, default=256)
-    ap.add_argument("--qr-n", type=int, default=4096)
+    ap.add_argument("--qr-n", type=int, default=16384)
+    ap.add_argument("--ls", action="store_true", help="also run the 32768 x 8192 least-squares workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     args = ap.parse_args()
@@ -168,22 +203,52 @@ def main():
         tm = time_steps(torch, trmm, max(3, args.steps // 2), 1)
         extras["trmm_f64_tflops"] = round(flops / tm / 1e12, 3)
 
-        # tile QR, Float64, nb = 128, ib = 32
+        # mixed-precision TRSM (Float32 solve + Float64 refinement), same problem
+        Bm = nl.colmajor(torch.empty_like(B0))
+        info = {}
+
+        def smix():
+            Bm.copy_(B0)
+            _, info["it"], info["last"] = nl.rectrsm_mixed_("L", "L", "N", 1.0, A, Bm, cutoff=cut, maxit=6, tol=1e-15)
+        tmx = time_steps(torch, smix, 3, 1)
+        Rm = torch.tril(A) @ Bm - B0
+        berr_m = (torch.linalg.norm(Rm) / (torch.linalg.norm(torch.tril(A)) * torch.linalg.norm(Bm))).item()
+        del Rm, Bm
+        extras["trsm_mixed"] = {"scheme": "Float32 recursive solve + Float64 residual refinement",
+                                "ms": round(tmx * 1e3, 3), "tflops": round(flops / tmx / 1e12, 3),
+                                "vs_f64": round(sec / tmx, 3), "iterations": info["it"],
+                                "backward_error": berr_m, "backward_error_f64": berr}
+        log(f"[bench] TRSM mixed: {tmx*1e3:.2f} ms, {info['it']} refinement steps, berr {berr_m:.2e}")
+
+        # tile QR at the north-star size: Float64 and Float32-update mixed, nb = 128, ib = 32
         qn = args.qr_n
         nb, ib = 128, 32
         Q0 = nl.colmajor(torch.randn((qn, qn), generator=gen, device="cuda", dtype=torch.float64))
         Qw = nl.colmajor(torch.empty_like(Q0))
-        holder = {}
-
-        def qr():
-            Qw.copy_(Q0)
-            holder["T"] = nl.tile_geqrf(Qw, nb, ib)
-        tq = time_steps(torch, qr, 2, 1)
         qflops = 2.0 * qn ** 3 - 2.0 / 3.0 * qn ** 3
-        extras["tile_qr"] = {"n": qn, "nb": nb, "ib": ib, "dtype": "f64", "ms": round(tq * 1e3, 2),
-                             "tflops": round(qflops / tq / 1e12, 3),
-                             "frac_of_peak": round(qflops / tq / 1e12 / PEAK["f64"], 4)}
-        del Q0, Qw, holder
+        res = {}
+        for tag, fn in (("f64", lambda: nl.tile_geqrf(Qw, nb, ib)), ("mixed_f32", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "S"))):
+            holder = {}
+
+            def qr():
+                Qw.copy_(Q0)
+                holder["T"] = fn()
+            tq = time_steps(torch, qr, 2, 1)
+            # backward error ||A - Q R|| / ||A|| with Q applied from the factors
+            QR = nl.zeros_cm(qn, qn)
+            QR.copy_(torch.triu(Qw))
+            nl.tile_ormqr("N", Qw, holder["T"], QR, nb, ib)
+            be = (torch.linalg.norm(QR - Q0) / torch.linalg.norm(Q0)).item()
+            del QR
+            res[tag] = {"ms": round(tq * 1e3, 2), "tflops": round(qflops / tq / 1e12, 3),
+                        "frac_of_peak": round(qflops / tq / 1e12 / PEAK["f64"], 4), "backward_error": be}
+            log(f"[bench] tile QR {tag} n={qn}: {tq*1e3:.1f} ms, {qflops/tq/1e12:.2f} TFLOP/s, berr {be:.2e}")
+        extras["tile_qr"] = {"n": qn, "nb": nb, "ib": ib, **res,
+                             "mixed_vs_f64": round(res["f64"]["ms"] / res["mixed_f32"]["ms"], 3)}
+        del Q0, Qw
+
+    if args.ls and not args.no_extras:
+        extras["least_squares"] = least_squares(torch, nl, gen)
 
     traffic = None
     tpath = os.path.join(REPO, "profiles", "trsm_f64_gemm_traffic.json")

@@ -175,7 +175,8 @@ int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *
 /* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
  * out.  Tile QR of a copy of A (prec 'S': Float32 trailing updates, Float64 only), then
  * x0 = R^-1 (Q^T b)[0:n] and up to maxit refinement steps x += R^-1 (Q^T (b - A x))[0:n]
- * with the residual in the working precision, stopping when ||dx|| <= tol ||x||.
+ * (augmented-system form, residuals in the working precision), stopping when
+ * ||dx|| <= tol ||x|| or when a step shrinks ||dx|| by less than half (stagnation).
  * iters / last_correction (may be NULL) receive the steps taken and the final ||dx||/||x||.
  * Synchronises the stream. */
 int nla_dtile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *A, int64_t lda, const double *b,
@@ -184,6 +185,14 @@ int nla_dtile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *A
 int nla_stile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const float *A, int64_t lda, const float *b,
                    float *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
                    void *stream);
+/* Triangular solve op(A) X = alpha B / X op(A) = alpha B (arguments as nla_drectrxm with
+ * func 'S'; X overwrites B) in Float32 with Float64 iterative refinement: X0 from the
+ * Float32 recursion (cutoff as there), then up to maxit steps X += op(A)^-1 (alpha B - op(A) X)
+ * with the residual in Float64 and the correction in Float32, stopping when
+ * ||dX|| <= tol ||X|| or when a step shrinks ||dX|| by less than half.  iters / last_correction may be NULL.  Synchronises the stream. */
+int nla_drectrsm_mixed(char side, char uplo, char trans, double alpha, int64_t n, int64_t m, const double *A,
+                       int64_t lda, double *B, int64_t ldb, int64_t cutoff, int64_t maxit, double tol,
+                       int64_t *iters, double *last_correction, void *stream);
 
 /* ---- Adjoints (no reference function: the reference has no differentiation code; these
  * are what a ChainRulesCore.rrule for unified_rectrxm! and for a thin tile QR calls) ---- */

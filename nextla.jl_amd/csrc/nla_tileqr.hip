@@ -205,6 +205,7 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     NLA_HIP(hipMemcpyAsync(r, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
     NLA_HIP(gemm<T>('N', 'N', m, 1, n, T(-1), A, lda, x, n, T(1), r, m, s));
     std::vector<T> hx(n), hd(n);
+    double prev = 0;
     for (i64 it = 0; it < maxit; ++it) {
         // f = b - r - A x (in Cb's first column, the other columns zero); g = -A^T r (in h)
         NLA_HIP(hipMemsetAsync(Cb, 0, sizeof(T) * (size_t)m * W, s));
@@ -234,6 +235,8 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
         if (iters) *iters = it + 1;
         if (last) *last = ratio;
         if (ratio <= tol) break;
+        if (it > 0 && ratio > 0.5 * prev) break;  // stagnation: the working precision's floor
+        prev = ratio;
     }
     NLA_HIP(hipStreamSynchronize(s));
     return 0;

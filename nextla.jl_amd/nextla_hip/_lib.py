@@ -42,6 +42,7 @@ _SIGS = {
 # Float64-only entry points
 _SIGS_D = {
     "tile_geqrf_mixed": [i64, i64, i64, i64, vp, i64, vp, i64, ch, vp],
+    "rectrsm_mixed": [ch, ch, ch, dbl, i64, i64, vp, i64, vp, i64, i64, i64, dbl, vp, vp, vp],
 }
 
 _lib = None

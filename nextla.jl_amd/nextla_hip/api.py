@@ -18,7 +18,8 @@ __all__ = [
     "unmqr_", "unmqr", "tsqrt_", "tsqrt", "tsmqr_", "tsmqr", "parfb_",
     "unified_rectrxm_", "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_", "trsm", "trmm",
-    "GEMM_ADD_", "GEMM_SUB_", "tile_geqrf", "tile_ormqr", "tile_geqrf_mixed", "tile_gels", "NextLAMatrix",
+    "GEMM_ADD_", "GEMM_SUB_", "tile_geqrf", "tile_ormqr", "tile_geqrf_mixed", "tile_gels", "rectrsm_mixed_",
+    "NextLAMatrix",
     "synchronize",
 ]
 
@@ -392,6 +393,25 @@ def tile_ormqr(trans, A, T, C, nb, ib):
     pc, ldc = _m(C)
     call(p, "tile_ormqr", _c(trans), m, n, C.shape[1], nb, ib, pa, lda, pt, ldt, pc, ldc, _stream())
     return C
+
+
+def rectrsm_mixed_(side, uplo, transpose, alpha, A, B, cutoff=0, maxit=5, tol=1e-15):
+    """unified_rectrxm!(side, uplo, transpose, alpha, 'S', A, B) for Float64 data solved in
+    Float32 with Float64 iterative refinement.  Returns (B, iterations, last ||dX||/||X||)."""
+    import ctypes
+
+    if A.dtype != torch.float64 or B.dtype != torch.float64:
+        raise ArgumentError("rectrsm_mixed_ needs Float64 data")
+    pa, lda = _m(A)
+    pb, ldb = _m(B)
+    n = A.shape[0]
+    m = B.shape[1] if side == "L" else B.shape[0]
+    it = ctypes.c_int64(0)
+    last = ctypes.c_double(0)
+    call("d", "rectrsm_mixed", _c(side), _c(uplo), _c(transpose), float(alpha), n, m, pa, lda, pb, ldb, int(cutoff),
+         int(maxit), float(tol), ctypes.cast(ctypes.pointer(it), ctypes.c_void_p),
+         ctypes.cast(ctypes.pointer(last), ctypes.c_void_p), _stream())
+    return B, it.value, last.value
 
 
 def tile_geqrf_mixed(A, nb, ib, prec="S"):

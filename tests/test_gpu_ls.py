@@ -115,3 +115,32 @@ def test_gels_argument_errors():
         nl.tile_gels(A, b, 128, 32, "X")
     with pytest.raises(nl.ArgumentError, match="Float64"):
         nl.tile_geqrf_mixed(nl.zeros_cm(256, 128, dtype=torch.float32), 128, 32)
+
+
+@pytest.mark.parametrize("side,uplo,trans", [("L", "L", "N"), ("L", "U", "T"), ("R", "L", "N"), ("R", "U", "N")])
+def test_rectrsm_mixed_refinement(side, uplo, trans):
+    """Float32 solve + Float64 refinement reaches the Float64 solve's backward error."""
+    nl = _nl()
+    rng = np.random.default_rng(21)
+    n, m = 1024, 96
+    A = rng.standard_normal((n, n)) / np.sqrt(n)
+    A = np.tril(A, -1) if uplo == "L" else np.triu(A, 1)
+    A[np.diag_indices(n)] = rng.uniform(1, 2, n)
+    B = rng.standard_normal((n, m) if side == "L" else (m, n))
+    dA = nl.to_device(np.asfortranarray(A))
+    X64 = nl.to_device(B)
+    nl.unified_rectrxm_(side, uplo, trans, 0.5, "S", dA, X64)
+    X0 = nl.to_device(B)
+    _, it0, _ = nl.rectrsm_mixed_(side, uplo, trans, 0.5, dA, X0, maxit=0)
+    Xr = nl.to_device(B)
+    _, it, last = nl.rectrsm_mixed_(side, uplo, trans, 0.5, dA, Xr, maxit=6, tol=1e-15)
+    torch.cuda.synchronize()
+    opA = A if trans == "N" else A.T
+
+    def berr(X):
+        R = (opA @ X if side == "L" else X @ opA) - 0.5 * B
+        return np.linalg.norm(R) / (np.linalg.norm(A) * np.linalg.norm(X))
+
+    e64, e0, er = berr(nl.to_host(X64)), berr(nl.to_host(X0)), berr(nl.to_host(Xr))
+    assert it0 == 0 and 1e-9 < e0 < 1e-5, e0           # Float32-level before refinement
+    assert it >= 1 and er < max(4 * e64, 1e-16), (er, e64, it, last)
