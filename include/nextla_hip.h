@@ -194,6 +194,51 @@ int nla_drectrsm_mixed(char side, char uplo, char trans, double alpha, int64_t n
                        int64_t lda, double *B, int64_t ldb, int64_t cutoff, int64_t maxit, double tol,
                        int64_t *iters, double *last_correction, void *stream);
 
+/* ---- Batched uneven problems (no reference function: the batched workload) ------------- */
+/* One work-list entry.  QR part: geqrt!(m, n, ib, A, T) (src/geqrt.jl:22-68) of the m x n
+ * matrix A (n <= m, 1 <= ib <= 64; A -> R and V, T ib x n).  Solve part: X = L^-1 B with L
+ * nt x nt lower triangular (non-unit) and B nt x nrhs overwritten with X (nt <= 512;
+ * nt = 0 or nrhs = 0: no solve).  All pointers are device pointers. */
+typedef struct nla_dbatch_problem {
+    int64_t m, n, ib;
+    double *A;
+    int64_t lda;
+    double *T;
+    int64_t ldt;
+    int64_t nt, nrhs;
+    const double *L;
+    int64_t ldl;
+    double *B;
+    int64_t ldb;
+} nla_dbatch_problem;
+typedef struct nla_sbatch_problem {
+    int64_t m, n, ib;
+    float *A;
+    int64_t lda;
+    float *T;
+    int64_t ldt;
+    int64_t nt, nrhs;
+    const float *L;
+    int64_t ldl;
+    float *B;
+    int64_t ldb;
+} nla_sbatch_problem;
+/* Every QR and every solve of the `count` entries of the DEVICE array `problems` in one
+ * persistent launch (a workgroup per task, tasks claimed from the device int `counter`, which
+ * the call clears with a kernel).  validate != 0 copies the list to the host and checks it
+ * first (synchronising); validate = 0 only enqueues two kernels, allocates nothing and issues
+ * no memset, so the call can be captured into a hipGraph and replayed. */
+int nla_dbatched_qr_trsm(int64_t count, const void *problems, int *counter, int validate, void *stream);
+int nla_sbatched_qr_trsm(int64_t count, const void *problems, int *counter, int validate, void *stream);
+/* Autotuner of the batched QR's ib: for each class c (m_values[c] x n_values[c], host
+ * arrays) times the batched launch on `samples` random problems for each of the ncand
+ * candidates and writes the fastest to ib_out[c] (host) and its time in ms to ms_out[c]
+ * (host, may be NULL).  Allocates and frees its own buffers; synchronises. */
+int nla_dbatched_autotune(int64_t nclass, const int64_t *m_values, const int64_t *n_values, int64_t samples,
+                          const int64_t *candidates, int64_t ncand, int64_t *ib_out, double *ms_out, void *stream);
+int nla_sbatched_autotune(int64_t nclass, const int64_t *m_values, const int64_t *n_values, int64_t samples,
+                          const int64_t *candidates, int64_t ncand, int64_t *ib_out, double *ms_out, void *stream);
+
 /* ---- Adjoints (no reference function: the reference has no differentiation code; these
  * are what a ChainRulesCore.rrule for unified_rectrxm! and for a thin tile QR calls) ---- */
 /* Solve X = op(A)^-1 (alpha B) (side 'L') / (alpha B) op(A)^-1 ('R') given X and Xbar:

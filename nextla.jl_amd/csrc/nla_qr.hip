@@ -18,6 +18,8 @@
 #include "nla_common.hpp"
 #include "nla_qr.hpp"
 
+#include <vector>
+
 namespace nla {
 
 // ------------------------------------------------------------------------------------
@@ -100,13 +102,9 @@ __global__ __launch_bounds__(256) void larfg_kernel(int n, T *alpha, T *x, i64 i
 // T[0:i, i] = -tau_i * T[0:i,0:i] * (V^T v_i), T[i,i] = tau_i.  LAPACK's lastv trimming
 // only skips exact-zero products, so the dense sums here give the same T.
 template <typename T>
-__global__ __launch_bounds__(256) void panel_qr_kernel(int m, int n, T *__restrict__ A, i64 lda, T *__restrict__ tau,
-                                                       T *__restrict__ Tm, i64 ldt) {
-    __shared__ T red[16];
-    __shared__ T sh[4];
-    extern __shared__ __align__(16) unsigned char dyn_raw[];
-    T *z = reinterpret_cast<T *>(dyn_raw);  // z[n]: v^T A[:, j] for every panel column j
-    T *tcol = z + n;                         // tcol[n] (only with Tm; n <= 256 then)
+__device__ void panel_qr_dev(int m, int n, T *__restrict__ A, i64 lda, T *__restrict__ tau, T *__restrict__ Tm,
+                             i64 ldt, T *z, T *tcol, T *red, T *sh) {
+    // z[n]: v^T A[:, j] for every panel column j; tcol[n] (only with Tm)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int k = min(m, n);
     for (int i = 0; i < k; ++i) {
@@ -168,6 +166,16 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int n, T *__restri
         __syncthreads();
     }
     // rows/columns beyond k: geqr2 leaves tau[k..] untouched (k = min(m, n)).
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void panel_qr_kernel(int m, int n, T *__restrict__ A, i64 lda, T *__restrict__ tau,
+                                                       T *__restrict__ Tm, i64 ldt) {
+    __shared__ T red[16];
+    __shared__ T sh[4];
+    extern __shared__ __align__(16) unsigned char dyn_raw[];
+    T *z = reinterpret_cast<T *>(dyn_raw);
+    panel_qr_dev<T>(m, n, A, lda, tau, Tm, ldt, z, z + n, red, sh);
 }
 
 // ------------------------------------------------------------------------------------
@@ -526,6 +534,139 @@ template hipError_t tsqrt_async<float>(i64, i64, i64, i64, float *, i64, float *
                                        hipStream_t);
 
 // ------------------------------------------------------------------------------------
+// Batched uneven problems: one persistent launch over a device work list.  Task t < count is
+// geqrt!(m, n, ib, A, T) of problem t (src/geqrt.jl:22-68: per ib-block the fused
+// geqr2+larft panel, then larfb!('L','C') of the block on the trailing columns, 16-column
+// strips); task count + t is the lower-triangular solve X = L^-1 B of problem t (the
+// LeftLowerTRSM! semantics of src/trsm.jl:5-33 with any number of right-hand sides).  A
+// workgroup owns a whole task, so problems of any shape share the launch; the caller orders
+// the work list largest first.  The launch allocates nothing and uses no memset (the task
+// counter is cleared by a fill kernel), so it can be recorded in a hipGraph and replayed.
+template <typename T> struct BatchProblem {
+    i64 m, n, ib;
+    T *A;
+    i64 lda;
+    T *Tm;
+    i64 ldt;
+    i64 nt, nrhs;
+    const T *L;
+    i64 ldl;
+    T *B;
+    i64 ldb;
+};
+
+constexpr int BATCH_RHS = 32;    // right-hand sides staged per LDS pass
+constexpr int BATCH_MAXN = 512;  // largest triangular order / panel width handled per problem
+
+template <typename T> __device__ void batched_geqrt(const BatchProblem<T> &P, T *smem) {
+    T *sV = smem, *sC = sV + KBM * RC, *sW0 = sC + CW * RC, *sW = sW0 + KBM * CW;
+    T *z = sW + KBM * CW, *tcol = z + 64, *red = tcol + 64, *sh = red + 16, *tau = sh + 4;
+    const int m = (int)P.m, n = (int)P.n, ib = (int)P.ib;
+    const int k = m < n ? m : n;
+    for (int i = 0; i < k; i += ib) {
+        const int sb = ib < k - i ? ib : k - i;
+        T *panel = P.A + i + (i64)i * P.lda;
+        panel_qr_dev<T>(m - i, sb, panel, P.lda, tau, P.Tm + (i64)i * P.ldt, P.ldt, z, tcol, red, sh);
+        __syncthreads();
+        const int ncols = n - i - sb;
+        for (int c0 = 0; c0 < ncols; c0 += CW) {
+            const int cw = ncols - c0 < CW ? ncols - c0 : CW;
+            T *C = P.A + i + (i64)(i + sb) * P.lda;
+            apply_strip<T, true, false>(sb, m - i - sb, panel, panel + sb, P.lda, P.Tm + (i64)i * P.ldt, P.ldt, false,
+                                        C, P.lda, C + sb, P.lda, c0, cw, sV, sC, sW0, sW);
+        }
+        __syncthreads();
+    }
+}
+
+// X = L^-1 B (L nt x nt lower, B nt x nrhs): column-oriented forward substitution on LDS-
+// staged blocks of BATCH_RHS right-hand sides, L read straight from memory.
+template <typename T> __device__ void batched_trsm(const BatchProblem<T> &P, T *smem) {
+    const int n = (int)P.nt, t = threadIdx.x;
+    for (i64 r0 = 0; r0 < P.nrhs; r0 += BATCH_RHS) {
+        const int nr = P.nrhs - r0 < BATCH_RHS ? (int)(P.nrhs - r0) : BATCH_RHS;
+        T *Bb = P.B + r0 * P.ldb;
+        __syncthreads();
+        for (int idx = t; idx < n * nr; idx += 256) smem[idx] = Bb[(idx % n) + (i64)(idx / n) * P.ldb];
+        __syncthreads();
+        for (int i = 0; i < n; ++i) {
+            const T d = P.L[i + (i64)i * P.ldl];
+            if (t < nr) smem[i + t * n] /= d;
+            __syncthreads();
+            const int rows = n - i - 1;
+            for (int idx = t; idx < rows * nr; idx += 256) {
+                const int p = i + 1 + idx % rows, c = idx / rows;
+                smem[p + c * n] -= P.L[p + (i64)i * P.ldl] * smem[i + c * n];
+            }
+            __syncthreads();
+        }
+        for (int idx = t; idx < n * nr; idx += 256) Bb[(idx % n) + (i64)(idx / n) * P.ldb] = smem[idx];
+    }
+}
+
+template <typename T> static size_t batched_lds_bytes() {
+    const size_t geq = (size_t)KBM * RC + (size_t)CW * RC + 2 * (size_t)KBM * CW + 64 + 64 + 16 + 4 + 64;
+    const size_t trs = (size_t)BATCH_MAXN * BATCH_RHS;
+    return sizeof(T) * (geq > trs ? geq : trs);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void batched_kernel(int count, const BatchProblem<T> *__restrict__ probs, int *counter) {
+    extern __shared__ __align__(16) unsigned char dyn_raw[];
+    T *smem = reinterpret_cast<T *>(dyn_raw);
+    __shared__ int sh_task;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh_task = atomicAdd(counter, 1);
+        __syncthreads();
+        const int task = sh_task;
+        if (task >= 2 * count) break;
+        const BatchProblem<T> P = probs[task < count ? task : task - count];
+        if (task < count) {
+            if (P.m > 0 && P.n > 0) batched_geqrt<T>(P, smem);
+        } else if (P.nt > 0 && P.nrhs > 0) {
+            batched_trsm<T>(P, smem);
+        }
+    }
+}
+
+__global__ void batched_reset_kernel(int *counter) { *counter = 0; }
+
+template <typename T>
+static hipError_t batched_launch(int count, const BatchProblem<T> *probs, int *counter, hipStream_t s) {
+    // device queries and the LDS attribute once per device (nothing but the two launches runs
+    // when the call is being recorded into a graph after a first eager call)
+    static std::mutex mu;
+    static int cached_dev = -1, cached_grid = 0;
+    const size_t sh = batched_lds_bytes<T>();
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    int gridmax;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (cached_dev != dev) {
+            int ncu = 0, per = 0;
+            if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+            if (sh > 64 * 1024 &&
+                (e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batched_kernel<T>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh)) != hipSuccess)
+                return e;
+            if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batched_kernel<T>, 256, sh)) != hipSuccess)
+                return e;
+            cached_grid = ncu * (per > 0 ? per : 1);
+            cached_dev = dev;
+        }
+        gridmax = cached_grid;
+    }
+    int grid = gridmax;
+    if (grid > 2 * count) grid = 2 * count;
+    batched_reset_kernel<<<1, 1, 0, s>>>(counter);
+    batched_kernel<T><<<grid, 256, sh, s>>>(count, probs, counter);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
 // Entry points: validation mirrors the reference's ArgumentError checks (file:line on each).
 static bool valid_trans(char t) { return t == 'N' || t == 'C' || t == 'T'; }
 
@@ -819,4 +960,139 @@ NLA_API int nla_sparfb(char side, char trans, char direct, char storev, int64_t 
     (void)work; (void)ldwork;
     return parfb_entry<float>(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, lda1, A2, lda2, V, ldv, T, ldt,
                               stream);
+}
+
+namespace nla {
+// Host-side validation of a work list (copied to the host once; a device list is the
+// interface so that a graph replay reads the list the caller may have refreshed).
+template <typename T>
+static int batched_entry(i64 count, const BatchProblem<T> *probs, int *counter, int validate, void *stream) {
+    if (count < 0) return arg_error(1, "count must be non-negative");
+    if (count == 0) return 0;
+    if (count > (1 << 29)) return arg_error(1, "count too large");
+    if (!probs || !counter) return arg_error(2, "problems and counter must be device pointers");
+    hipStream_t s = as_stream(stream);
+    if (validate) {
+        std::vector<BatchProblem<T>> h((size_t)count);
+        NLA_HIP(hipMemcpyAsync(h.data(), probs, sizeof(BatchProblem<T>) * (size_t)count, hipMemcpyDeviceToHost, s));
+        NLA_HIP(hipStreamSynchronize(s));
+        for (i64 i = 0; i < count; ++i) {
+            const BatchProblem<T> &P = h[i];
+            if (P.m < 0 || P.n < 0 || P.n > P.m) return arg_error(2, "problem %lld: need 0 <= n <= m", (long long)i);
+            if (P.m > 0 && P.n > 0) {
+                if (P.ib <= 0 || P.ib > 64) return arg_error(2, "problem %lld: ib must be in 1..64", (long long)i);
+                if (P.lda < P.m) return arg_error(2, "problem %lld: lda < m", (long long)i);
+                if (P.ldt < P.ib) return arg_error(2, "problem %lld: ldt < ib", (long long)i);
+                if (!P.A || !P.Tm) return arg_error(2, "problem %lld: null A or T", (long long)i);
+            }
+            if (P.nt < 0 || P.nrhs < 0) return arg_error(2, "problem %lld: negative solve size", (long long)i);
+            if (P.nt > 0 && P.nrhs > 0) {
+                if (P.nt > BATCH_MAXN) return arg_error(2, "problem %lld: triangular order must be <= %d", (long long)i, BATCH_MAXN);
+                if (P.ldl < P.nt || P.ldb < P.nt) return arg_error(2, "problem %lld: ldl/ldb < nt", (long long)i);
+                if (!P.L || !P.B) return arg_error(2, "problem %lld: null L or B", (long long)i);
+            }
+        }
+    }
+    NLA_HIP(batched_launch<T>((int)count, probs, counter, s));
+    return 0;
+}
+}  // namespace nla
+
+NLA_API int nla_dbatched_qr_trsm(int64_t count, const void *problems, int *counter, int validate, void *stream) {
+    return nla::batched_entry<double>(count, (const nla::BatchProblem<double> *)problems, counter, validate, stream);
+}
+NLA_API int nla_sbatched_qr_trsm(int64_t count, const void *problems, int *counter, int validate, void *stream) {
+    return nla::batched_entry<float>(count, (const nla::BatchProblem<float> *)problems, counter, validate, stream);
+}
+
+namespace nla {
+// fill with a deterministic pseudo-random N(0,1)-like sequence (sum of 4 uniforms)
+template <typename T> __global__ void fill_rand_kernel(T *p, i64 n, unsigned seed) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        unsigned h = (unsigned)i * 2654435761u ^ seed;
+        T acc = 0;
+        for (int k = 0; k < 4; ++k) {
+            h ^= h << 13; h ^= h >> 17; h ^= h << 5;
+            acc += T(h & 0xffffff) * T(1.0 / 16777216.0);
+        }
+        p[i] = (acc - T(2)) * T(1.7320508);
+    }
+}
+
+// Autotuner of the batched QR's inner block ib: for each (m, n) class, time the batched
+// launch on `samples` problems of that shape for every candidate and keep the fastest.
+template <typename T>
+static int batched_autotune_entry(i64 nclass, const i64 *m_values, const i64 *n_values, i64 samples,
+                                  const i64 *cand, i64 ncand, i64 *ib_out, double *ms_out, void *stream) {
+    if (nclass < 0 || samples <= 0 || ncand <= 0) return arg_error(1, "need nclass >= 0, samples > 0 and candidates");
+    hipStream_t s = as_stream(stream);
+    int *counter = nullptr;
+    BatchProblem<T> *dprob = nullptr;
+    NLA_HIP(hipMalloc(&counter, sizeof(int)));
+    NLA_HIP(hipMalloc(&dprob, sizeof(BatchProblem<T>) * (size_t)samples));
+    hipEvent_t e0, e1;
+    NLA_HIP(hipEventCreate(&e0));
+    NLA_HIP(hipEventCreate(&e1));
+    int rc = 0;
+    for (i64 c = 0; c < nclass && rc == 0; ++c) {
+        const i64 m = m_values[c], n = n_values[c];
+        if (m <= 0 || n <= 0 || n > m) { rc = arg_error(2, "class %lld: need 0 < n <= m", (long long)c); break; }
+        T *A = nullptr, *A0 = nullptr, *Tm = nullptr;
+        const size_t na = (size_t)m * n * samples;
+        if (hipMalloc(&A, sizeof(T) * na) != hipSuccess || hipMalloc(&A0, sizeof(T) * na) != hipSuccess ||
+            hipMalloc(&Tm, sizeof(T) * 64 * (size_t)n * samples) != hipSuccess) {
+            rc = hip_error(hipErrorOutOfMemory, "autotune buffers");
+        } else {
+            fill_rand_kernel<T><<<1024, 256, 0, s>>>(A0, (i64)na, 0x9e3779b9u + (unsigned)c);
+            double best = 1e300;
+            i64 bestib = cand[0];
+            for (i64 q = 0; q < ncand && rc == 0; ++q) {
+                const i64 ib = cand[q] < n ? cand[q] : n;
+                std::vector<BatchProblem<T>> h((size_t)samples);
+                for (i64 i = 0; i < samples; ++i)
+                    h[i] = BatchProblem<T>{m, n, ib, A + (size_t)i * m * n, m, Tm + (size_t)i * 64 * n, 64, 0, 0,
+                                           nullptr, 1, nullptr, 1};
+                if (hipMemcpyAsync(dprob, h.data(), sizeof(BatchProblem<T>) * samples, hipMemcpyHostToDevice, s) ||
+                    hipMemcpyAsync(A, A0, sizeof(T) * na, hipMemcpyDeviceToDevice, s)) {
+                    rc = hip_error(hipErrorUnknown, "autotune copy");
+                    break;
+                }
+                float ms = 0, tot = 0;
+                for (int rep = 0; rep < 3 && rc == 0; ++rep) {
+                    (void)hipMemcpyAsync(A, A0, sizeof(T) * na, hipMemcpyDeviceToDevice, s);
+                    (void)hipEventRecord(e0, s);
+                    if (batched_launch<T>((int)samples, dprob, counter, s) != hipSuccess) rc = hip_error(hipGetLastError(), "autotune launch");
+                    (void)hipEventRecord(e1, s);
+                    (void)hipEventSynchronize(e1);
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (rep > 0) tot += ms;  // first run warms up
+                }
+                if (tot / 2 < best) best = tot / 2, bestib = ib;
+            }
+            ib_out[c] = bestib;
+            if (ms_out) ms_out[c] = best;
+        }
+        (void)hipFree(A);
+        (void)hipFree(A0);
+        (void)hipFree(Tm);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(dprob);
+    (void)hipFree(counter);
+    return rc;
+}
+}  // namespace nla
+
+NLA_API int nla_sbatched_autotune(int64_t nclass, const int64_t *m_values, const int64_t *n_values, int64_t samples,
+                                  const int64_t *candidates, int64_t ncand, int64_t *ib_out, double *ms_out,
+                                  void *stream) {
+    return nla::batched_autotune_entry<float>(nclass, m_values, n_values, samples, candidates, ncand, ib_out, ms_out,
+                                              stream);
+}
+NLA_API int nla_dbatched_autotune(int64_t nclass, const int64_t *m_values, const int64_t *n_values, int64_t samples,
+                                  const int64_t *candidates, int64_t ncand, int64_t *ib_out, double *ms_out,
+                                  void *stream) {
+    return nla::batched_autotune_entry<double>(nclass, m_values, n_values, samples, candidates, ncand, ib_out, ms_out,
+                                               stream);
 }

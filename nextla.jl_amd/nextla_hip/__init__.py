@@ -2,4 +2,4 @@
 recursive TRSM/TRMM behind NextLA's interface (see api.py)."""
 from ._lib import ArgumentError, HipError, LIB_PATH, header_symbols, load, prof_enable, prof_read, prof_reset  # noqa: F401
 from .api import *  # noqa: F401,F403
-from . import autodiff  # noqa: F401
+from . import autodiff, batched  # noqa: F401

@@ -38,6 +38,8 @@ _SIGS = {
     "trmm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
     "qr_rrule": [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp],
     "tile_gels": [i64, i64, i64, i64, vp, i64, vp, vp, ch, i64, dbl, vp, vp, vp],
+    "batched_qr_trsm": [i64, vp, vp, ctypes.c_int, vp],
+    "batched_autotune": [i64, vp, vp, i64, vp, i64, vp, vp, vp],
 }
 # Float64-only entry points
 _SIGS_D = {
