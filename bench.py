@@ -119,6 +119,105 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
     return out
 
 
+def batched(torch, nl, gen, count=4096, nrhs=32):
+    """Batched uneven workload: `count` Float32 problems, m uniform over multiples of 32 in
+    [64, 1024], n uniform over multiples of 16 in [16, min(128, m)]; geqrt! of each A and a
+    lower solve with 32 right-hand sides, all in one persistent launch, replayed from a hipGraph.
+    ib per problem width from the autotuner."""
+    import numpy as np
+
+    from nextla_hip import batched as bt
+
+    rng = np.random.default_rng(4096)
+    shapes = []
+    for _ in range(count):
+        m = 32 * int(rng.integers(2, 33))
+        n = 16 * int(rng.integers(1, min(128, m) // 16 + 1))
+        shapes.append((m, n))
+    As, Ls, Bs = [], [], []
+    for m, n in shapes:
+        As.append(nl.colmajor(torch.randn((n, m), generator=gen, device="cuda").t()))
+        L = torch.tril(torch.randn((n, n), generator=gen, device="cuda") / n ** 0.5, -1)
+        L.diagonal().copy_(torch.rand(n, generator=gen, device="cuda") + 1)
+        Ls.append(nl.colmajor(L))
+        Bs.append(nl.colmajor(torch.randn((nrhs, n), generator=gen, device="cuda").t()))
+    A0 = [a.clone() for a in As]
+    B0 = [b.clone() for b in Bs]
+    t0 = time.perf_counter()
+    tune = bt.autotune(torch.float32, [(512, n) for n in range(16, 129, 16)], samples=32, candidates=(8, 16, 32))
+    t_tune = time.perf_counter() - t0
+    ibs = {n: ib for (m, n), (ib, ms) in tune.items()}
+    plan = bt.BatchedQRTRSM(As, Ls, Bs, ib=ibs)
+    qflops = sum(2.0 * m * n * n - 2.0 / 3.0 * n ** 3 for m, n in shapes)
+    sflops = sum(float(n) * n * nrhs for m, n in shapes)
+    nbytes = 4.0 * sum(2 * m * n + n * n + 2 * n * nrhs for m, n in shapes)
+
+    def restore():
+        for a, a0 in zip(As, A0):
+            a.copy_(a0)
+        for b, b0 in zip(Bs, B0):
+            b.copy_(b0)
+
+    plan.launch()  # validates the work list
+    torch.cuda.synchronize()
+    restore()
+    plan.capture()
+    times = []
+    for _ in range(5):
+        restore()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        plan.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    t = sorted(times)[len(times) // 2]
+    log(f"[bench] batched {count} problems: {t*1e3:.2f} ms per graph replay, autotune {t_tune:.1f} s, ib {ibs}")
+    return {"problems": count, "dtype": "f32", "ms": round(t * 1e3, 3),
+            "tflops": round((qflops + sflops) / t / 1e12, 3), "qr_gflop": round(qflops / 1e9, 2),
+            "trsm_gflop": round(sflops / 1e9, 3), "effective_gbps": round(nbytes / t / 1e9, 1),
+            "frac_of_f32_peak": round((qflops + sflops) / t / 1e12 / PEAK["f32"], 4), "ib_by_n": ibs,
+            "launch": "one persistent kernel replayed from a hipGraph"}
+
+
+def adjoints(torch, nl, gen):
+    """Adjoint kernels at the differentiation workload's sizes: TRSM A 4096, B 4096 x 512;
+    thin QR of 8192 x 1024 (Float64); time of the backward pass alone."""
+    from nextla_hip import autodiff as ad
+
+    out = {}
+    n, m = 4096, 512
+    A = make_tri(torch, nl, n, torch.float64, gen)
+    B = torch.randn((n, m), generator=gen, device="cuda", dtype=torch.float64)
+    W = torch.randn((n, m), generator=gen, device="cuda", dtype=torch.float64)
+    X = nl.colmajor(B.clone())
+    nl.unified_rectrxm_("L", "L", "N", 1.0, "S", A, X)
+    Xb = nl.colmajor(W.clone())
+
+    def trsm_bw():
+        Xb.copy_(W)
+        ad.trsm_rrule_("L", "L", "N", 1.0, A, X, Xb)
+    t = time_steps(torch, trsm_bw, 5, 2)
+    out["trsm_rrule_ms"] = round(t * 1e3, 3)
+    out["trsm_rrule_tflops"] = round(2.0 * n * n * m / t / 1e12, 3)  # solve n^2 m + masked product n^2 m
+    mq, nq = 8192, 1024
+    Aq = torch.randn((mq, nq), generator=gen, device="cuda", dtype=torch.float64, requires_grad=True)
+    GQ = torch.randn((mq, nq), generator=gen, device="cuda", dtype=torch.float64)
+    GR = torch.randn((nq, nq), generator=gen, device="cuda", dtype=torch.float64)
+    for _ in range(2):
+        Q, R = ad.thin_qr(Aq, 128, 32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ((Q * GQ).sum() + (R * GR).sum()).backward()
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        Aq.grad = None
+    out["qr_rrule_ms"] = round(tb * 1e3, 3)
+    log(f"[bench] adjoints: TRSM rrule {t*1e3:.2f} ms, thin-QR rrule {tb*1e3:.1f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +346,9 @@ def main():
                              "mixed_vs_f64": round(res["f64"]["ms"] / res["mixed_f32"]["ms"], 3)}
         del Q0, Qw
 
+    if not args.no_extras:
+        extras["batched"] = batched(torch, nl, gen)
+        extras["adjoints"] = adjoints(torch, nl, gen)
     if args.ls and not args.no_extras:
         extras["least_squares"] = least_squares(torch, nl, gen)
 
