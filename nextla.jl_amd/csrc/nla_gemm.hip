@@ -31,11 +31,19 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
     __shared__ T As[2][BK][BM + PAD];
     __shared__ T Bs[2][BK][BN + PAD];
 
-    // XCD-aware tile numbering: hardware deals workgroups round-robin over 8 XCDs.
+    // XCD-aware tile numbering: hardware deals workgroups round-robin over 8 XCDs, so the
+    // ids are first made contiguous per XCD; within that order tiles go in groups of GM tile
+    // rows (column by column inside a group), so the ~64 tiles an XCD runs at once form an
+    // 8 x 8 block that shares 8 A row panels and 8 B column panels through the XCD's L2.
     int bid = blockIdx.x;
     const int nb = gridDim.x;
     if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
-    const int tm = bid % tiles_m, tn = bid / tiles_m;
+    constexpr int GM = 8;
+    const int tiles_n = nb / tiles_m;
+    const int group = bid / (GM * tiles_n), first_m = group * GM;
+    const int gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
+    const int within = bid - group * GM * tiles_n;
+    const int tm = first_m + within % gsize, tn = within / gsize;
     const int m0 = tm * BM, n0 = tn * BN;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
