@@ -174,9 +174,11 @@ int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *
                           int64_t ldt, char prec, void *stream);
 /* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
  * out.  Tile QR of a copy of A (prec 'S': Float32 trailing updates, Float64 only), then
- * x0 = R^-1 (Q^T b)[0:n] and up to maxit refinement steps x += R^-1 (Q^T (b - A x))[0:n]
- * (augmented-system form, residuals in the working precision), stopping when
- * ||dx|| <= tol ||x|| or when a step shrinks ||dx|| by less than half (stagnation).
+ * x0 = R^-1 (Q^T b)[0:n], r0 = b - A x0, and up to maxit refinement steps on the augmented
+ * system [I A; A^T 0][r; x] = [b; 0] with residuals in the working precision:
+ * f = b - r - A x, g = -A^T r, h = R^-T g, d = Q^T f, dx = R^-1 (d[0:n] - h),
+ * dr = Q [h; d[n:m]], x += dx, r += dr; stopping when ||dx|| <= tol ||x|| or when a step
+ * shrinks ||dx|| by less than half (stagnation).
  * iters / last_correction (may be NULL) receive the steps taken and the final ||dx||/||x||.
  * Synchronises the stream. */
 int nla_dtile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *A, int64_t lda, const double *b,
