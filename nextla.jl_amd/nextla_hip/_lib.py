@@ -73,6 +73,8 @@ def load():
     lib.nla_prof_enable.argtypes = [ctypes.c_int]
     lib.nla_prof_reset.restype = None
     lib.nla_prof_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(dbl), ctypes.POINTER(i64)]
+    lib.nla_tile_qr_schedule.argtypes = [i64, i64, i64, ctypes.POINTER(ctypes.c_int32), i64]
+    lib.nla_tile_qr_schedule.restype = i64
     for name, sig in _SIGS.items():
         for p, real in (("d", dbl), ("s", flt)):
             f = getattr(lib, f"nla_{p}{name}")
@@ -124,3 +126,15 @@ def prof_read(kernel):
     n = i64(0)
     check(load().nla_prof_read(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)), "nla_prof_read")
     return ms.value, n.value
+
+
+def tile_qr_schedule(m, n, nb):
+    """The claim order of the tile-QR persistent launch for an m x n matrix: a list of
+    ("P", k, stage) and ("W", k, strip_group) tuples (host-only, no GPU call)."""
+    lib = load()
+    cnt = lib.nla_tile_qr_schedule(m, n, nb, None, 0)
+    if cnt < 0:
+        raise ArgumentError(lib.nla_last_error().decode())
+    buf = (ctypes.c_int32 * cnt)()
+    lib.nla_tile_qr_schedule(m, n, nb, buf, cnt)
+    return [("P" if (c >> 30) == 0 else "W", (c >> 18) & 0xFFF, c & 0x3FFFF) for c in buf]
