@@ -96,12 +96,19 @@ template <typename T, int NB> struct Lds {
     static constexpr int V = 0;
     static constexpr int TT = V + IB * LDV;            // Ts[q * (IB+1) + j] = T[q][j] (zero below diag)
     static constexpr int BUF = TT + IB * (IB + 1);     // second V / T pair (walkers double-buffer)
+    // panel tasks only:
     static constexpr int RS = BUF + IB * LDV + IB * (IB + 1);  // Rs[i * (IB+1) + j] = R block (tsqrt)
     static constexpr int ZB = RS + IB * (IB + 1);      // block factorization scratch
     static constexpr int TAU = ZB + IB * IB;
     static constexpr int RED = TAU + IB;
     static constexpr int DOT = RED + 16;
-    static constexpr int TOTAL = DOT + 8 * IB;
+    static constexpr int PEND = DOT + 8 * IB;
+    // walker tasks only (same space as the panel scratch): the strip staging columns,
+    // LDS_S = NB + one 16-byte vector per column, so every column starts 16-byte aligned
+    static constexpr int LDS_S = NB + 16 / (int)sizeof(T);
+    static constexpr int SSTG = RS;
+    static constexpr int WEND = SSTG + TW * LDS_S;
+    static constexpr int TOTAL = PEND > WEND ? PEND : WEND;
 };
 
 // LDS base with its address space visible at every use (a pointer passed through a call
@@ -144,6 +151,81 @@ __device__ __forceinline__ void tiles_store_from(const typename Acc<T>::t *a, T 
             const int row = 16 * rt + acc_row<T>(lk, r);
             if (row >= rmin) cst(&G[row + (i64)lr * ld], a[rt][r]);
         }
+}
+
+// Strip staging.  A wave's 16-column strip moves between HBM and its MFMA registers through
+// its own SW columns of LDS (no workgroup synchronisation): loads go straight to LDS with
+// global_load_lds, one whole column (NB elements, 16 bytes per lane) per instruction, and
+// stores leave as column-contiguous 16-byte device-coherent buffer stores.  In the register
+// layout a lane holds 4 rows of one column per 16x16 tile, so direct register loads/stores
+// touch 16 partial cache lines per instruction; staged, each instruction moves whole lines.
+template <typename T, int NB> struct Stage {
+    static constexpr int VEC = 16 / (int)sizeof(T);  // elements per 16-byte access
+    static constexpr int LPC = NB / VEC;             // lanes per column
+    static constexpr int CPI = 64 / LPC;             // columns per store instruction
+    static_assert(LPC <= 64 && 64 % LPC == 0, "a column must be a whole number of lane groups");
+};
+template <typename T, int NB> __device__ __forceinline__ T *stage_ptr() {
+    return lds_base<T>() + Lds<T, NB>::SSTG + (threadIdx.x >> 6) * SW * Lds<T, NB>::LDS_S;
+}
+// Issue the loads of the wave's strip (columns G, G + ld, ...) into its staging columns.
+// Completion is tracked by vmcnt; the caller waits before stage_to_regs.
+template <typename T, int NB> __device__ __forceinline__ void stage_load(const T *G, i64 ld) {
+    constexpr int LDS_S = Lds<T, NB>::LDS_S;
+    T *stg = stage_ptr<T, NB>();
+    const int lane = threadIdx.x & 63;
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // earlier reads of the staging columns are done
+    if (lane < Stage<T, NB>::LPC) {
+#pragma unroll
+        for (int c = 0; c < SW; ++c) {
+            const char *src = reinterpret_cast<const char *>(G + (i64)c * ld) + lane * 16;
+            auto *d = (__attribute__((address_space(3))) void *)(stg + c * LDS_S);
+            __builtin_amdgcn_global_load_lds(src, d, 16, 0, /*sc1*/ 16);
+        }
+    }
+}
+template <typename T, int NB> __device__ __forceinline__ void stage_to_regs(WStrip<T, NB> &S) {
+    constexpr int LDS_S = Lds<T, NB>::LDS_S;
+    const T *stg = stage_ptr<T, NB>();
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NB / 16; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) S.a[rt][r] = stg[lr * LDS_S + 16 * rt + acc_row<T>(lk, r)];
+}
+// Registers -> staging columns -> HBM (rows 0..NB-1 of the wave's 16 columns at G).
+template <typename T, int NB> __device__ __forceinline__ void stage_store(const WStrip<T, NB> &S, T *G, i64 ld) {
+    using St = Stage<T, NB>;
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef T tvec __attribute__((ext_vector_type(St::VEC)));
+    constexpr int LDS_S = Lds<T, NB>::LDS_S;
+    T *stg = stage_ptr<T, NB>();
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NB / 16; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) stg[lr * LDS_S + 16 * rt + acc_row<T>(lk, r)] = S.a[rt][r];
+    // the wave's LDS accesses execute in order; this keeps the compiler from moving the
+    // column reads above the writes of other lanes' elements
+    asm volatile("" ::: "memory");
+    const int cl = lane / St::LPC, e = (lane % St::LPC) * St::VEC;
+    const unsigned bytes = (unsigned)((((i64)SW - 1) * ld + NB) * (i64)sizeof(T));
+    // G is the same in every lane of the wave; say so, so the descriptor is built in SGPRs
+    const unsigned long long gb = reinterpret_cast<unsigned long long>(G);
+    const unsigned glo = __builtin_amdgcn_readfirstlane((unsigned)gb), ghi = __builtin_amdgcn_readfirstlane((unsigned)(gb >> 32));
+    T *Gu = reinterpret_cast<T *>(((unsigned long long)ghi << 32) | glo);
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(Gu, 0, __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+#pragma unroll
+    for (int c0 = 0; c0 < SW; c0 += St::CPI) {
+        const int c = c0 + cl;
+        const T *p = stg + c * LDS_S + e;
+        tvec v;
+#pragma unroll
+        for (int q = 0; q < St::VEC; ++q) v[q] = p[q];
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rsrc,
+                                               (int)(((i64)c * ld + e) * (i64)sizeof(T)), 0, /*sc1*/ 16);
+    }
 }
 
 // Reflector block prefetch: columns j0..j0+IB-1 of a panel tile (Vg = first of them) and
@@ -641,16 +723,19 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            tiles_load<T, NB / 16>(S.a, Aij, ldc, 0);
+            stage_load<T, NB>(Aij, ldc);
         }
         bool next0 = false;
 #pragma unroll
         for (int B = 0; B < NBLK; ++B) {
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this block's V, the strip, earlier stores
             __syncthreads();
-            if (B == 0 && pending) {
-                if (threadIdx.x == 0) cst(mine, pending);
-                pending = 0;
+            if (B == 0) {
+                if (pending) {
+                    if (threadIdx.x == 0) cst(mine, pending);
+                    pending = 0;
+                }
+                stage_to_regs<T, NB>(S);
             }
             if (B + 1 < NBLK) {
                 vload_lds<T, NB>(Aik + (i64)(B + 1) * IB * lda, lda, (B + 1) & 1);
@@ -666,10 +751,10 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
             if (B + 1 < NBLK || next0) tstore<T, NB>(P, (B + 1) & 1);
         }
         have0 = next0;
-        tiles_store<T, NB / 16>(S.a, Aij, ldc, 0);
+        stage_store<T, NB>(S, Aij, ldc);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
-        if (haveS) tiles_load<T, NB / 16>(S.a, Aij + NB, ldc, 0);
+        if (haveS) stage_load<T, NB>(Aij + NB, ldc);
     }
     if (pending) publish(mine, pending);
     tiles_store<T, NB / 16>(A1.a, Akj, ldc, 0);
