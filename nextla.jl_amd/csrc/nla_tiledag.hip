@@ -165,17 +165,19 @@ template <typename T, int NB> struct Stage {
     static constexpr int CPI = 64 / LPC;             // columns per store instruction
     static_assert(LPC <= 64 && 64 % LPC == 0, "a column must be a whole number of lane groups");
 };
-template <typename T, int NB> __device__ __forceinline__ T *stage_ptr() {
-    return lds_base<T>() + Lds<T, NB>::SSTG + (threadIdx.x >> 6) * SW * Lds<T, NB>::LDS_S;
+// The staging columns hold the storage type Ts (the matrix's); they start at the walker
+// region of the kernel's LDS layout type Tl (Ts itself, or the Float32 of the mixed walker).
+template <typename Ts, typename Tl, int NB> __device__ __forceinline__ Ts *stage_ptr() {
+    return reinterpret_cast<Ts *>(lds_base<Tl>() + Lds<Tl, NB>::SSTG) + (threadIdx.x >> 6) * SW * Lds<Ts, NB>::LDS_S;
 }
 // Issue the loads of the wave's strip (columns G, G + ld, ...) into its staging columns.
 // Completion is tracked by vmcnt; the caller waits before stage_to_regs.
-template <typename T, int NB> __device__ __forceinline__ void stage_load(const T *G, i64 ld) {
-    constexpr int LDS_S = Lds<T, NB>::LDS_S;
-    T *stg = stage_ptr<T, NB>();
+template <typename Ts, typename Tl, int NB> __device__ __forceinline__ void stage_load(const Ts *G, i64 ld) {
+    constexpr int LDS_S = Lds<Ts, NB>::LDS_S;
+    Ts *stg = stage_ptr<Ts, Tl, NB>();
     const int lane = threadIdx.x & 63;
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // earlier reads of the staging columns are done
-    if (lane < Stage<T, NB>::LPC) {
+    if (lane < Stage<Ts, NB>::LPC) {
 #pragma unroll
         for (int c = 0; c < SW; ++c) {
             const char *src = reinterpret_cast<const char *>(G + (i64)c * ld) + lane * 16;
@@ -184,47 +186,50 @@ template <typename T, int NB> __device__ __forceinline__ void stage_load(const T
         }
     }
 }
-template <typename T, int NB> __device__ __forceinline__ void stage_to_regs(WStrip<T, NB> &S) {
-    constexpr int LDS_S = Lds<T, NB>::LDS_S;
-    const T *stg = stage_ptr<T, NB>();
+// staging columns (Ts) -> accumulator registers (Tr)
+template <typename Ts, typename Tr, typename Tl, int NB> __device__ __forceinline__ void stage_to_regs(WStrip<Tr, NB> &S) {
+    constexpr int LDS_S = Lds<Ts, NB>::LDS_S;
+    const Ts *stg = stage_ptr<Ts, Tl, NB>();
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int rt = 0; rt < NB / 16; ++rt)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) S.a[rt][r] = stg[lr * LDS_S + 16 * rt + acc_row<T>(lk, r)];
+        for (int r = 0; r < 4; ++r) S.a[rt][r] = (Tr)stg[lr * LDS_S + 16 * rt + acc_row<Tr>(lk, r)];
 }
-// Registers -> staging columns -> HBM (rows 0..NB-1 of the wave's 16 columns at G).
-template <typename T, int NB> __device__ __forceinline__ void stage_store(const WStrip<T, NB> &S, T *G, i64 ld) {
-    using St = Stage<T, NB>;
+// accumulator registers (Tr) -> staging columns (Ts) -> HBM (rows 0..NB-1 of the wave's 16
+// columns at G)
+template <typename Ts, typename Tr, typename Tl, int NB>
+__device__ __forceinline__ void stage_store(const WStrip<Tr, NB> &S, Ts *G, i64 ld) {
+    using St = Stage<Ts, NB>;
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    typedef T tvec __attribute__((ext_vector_type(St::VEC)));
-    constexpr int LDS_S = Lds<T, NB>::LDS_S;
-    T *stg = stage_ptr<T, NB>();
+    typedef Ts tvec __attribute__((ext_vector_type(St::VEC)));
+    constexpr int LDS_S = Lds<Ts, NB>::LDS_S;
+    Ts *stg = stage_ptr<Ts, Tl, NB>();
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int rt = 0; rt < NB / 16; ++rt)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) stg[lr * LDS_S + 16 * rt + acc_row<T>(lk, r)] = S.a[rt][r];
+        for (int r = 0; r < 4; ++r) stg[lr * LDS_S + 16 * rt + acc_row<Tr>(lk, r)] = (Ts)S.a[rt][r];
     // the wave's LDS accesses execute in order; this keeps the compiler from moving the
     // column reads above the writes of other lanes' elements
     asm volatile("" ::: "memory");
     const int cl = lane / St::LPC, e = (lane % St::LPC) * St::VEC;
-    const unsigned bytes = (unsigned)((((i64)SW - 1) * ld + NB) * (i64)sizeof(T));
+    const unsigned bytes = (unsigned)((((i64)SW - 1) * ld + NB) * (i64)sizeof(Ts));
     // G is the same in every lane of the wave; say so, so the descriptor is built in SGPRs
     const unsigned long long gb = reinterpret_cast<unsigned long long>(G);
     const unsigned glo = __builtin_amdgcn_readfirstlane((unsigned)gb), ghi = __builtin_amdgcn_readfirstlane((unsigned)(gb >> 32));
-    T *Gu = reinterpret_cast<T *>(((unsigned long long)ghi << 32) | glo);
+    Ts *Gu = reinterpret_cast<Ts *>(((unsigned long long)ghi << 32) | glo);
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(Gu, 0, __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
 #pragma unroll
     for (int c0 = 0; c0 < SW; c0 += St::CPI) {
         const int c = c0 + cl;
-        const T *p = stg + c * LDS_S + e;
+        const Ts *p = stg + c * LDS_S + e;
         tvec v;
 #pragma unroll
         for (int q = 0; q < St::VEC; ++q) v[q] = p[q];
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rsrc,
-                                               (int)(((i64)c * ld + e) * (i64)sizeof(T)), 0, /*sc1*/ 16);
+                                               (int)(((i64)c * ld + e) * (i64)sizeof(Ts)), 0, /*sc1*/ 16);
     }
 }
 
@@ -723,7 +728,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            stage_load<T, NB>(Aij, ldc);
+            stage_load<T, T, NB>(Aij, ldc);
         }
         bool next0 = false;
 #pragma unroll
@@ -735,7 +740,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-                stage_to_regs<T, NB>(S);
+                stage_to_regs<T, T, T, NB>(S);
             }
             if (B + 1 < NBLK) {
                 vload_lds<T, NB>(Aik + (i64)(B + 1) * IB * lda, lda, (B + 1) & 1);
@@ -751,10 +756,10 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
             if (B + 1 < NBLK || next0) tstore<T, NB>(P, (B + 1) & 1);
         }
         have0 = next0;
-        stage_store<T, NB>(S, Aij, ldc);
+        stage_store<T, T, T, NB>(S, Aij, ldc);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
-        if (haveS) stage_load<T, NB>(Aij + NB, ldc);
+        if (haveS) stage_load<T, T, NB>(Aij + NB, ldc);
     }
     if (pending) publish(mine, pending);
     tiles_store<T, NB / 16>(A1.a, Akj, ldc, 0);
@@ -892,21 +897,41 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
         if (B + 1 < NBLK) vstore_cvt<T, Tw, NB, true>(P, (B + 1) & 1, (B + 1) * IB);
         else if (have0) vstore_cvt<T, Tw, NB, false>(P, 0, 0);
     }
+    // Same row pipeline as the Float64 walker: strips staged (Float64) in LDS and converted
+    // on the way to and from the registers, the next strip prefetched when released, a
+    // finished row published at the next row's first block.
+    WStrip<Tw, NB> S;
+    bool haveS = false;
+    int pending = 0;
     for (int i = k + 1; i < g.mt; ++i) {
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
         T *Aij = Cm + (i64)i * NB + col * ldc;
+        if (pending && (!have0 || !haveS)) {
+            publish(mine, pending);
+            pending = 0;
+        }
         if (!have0) {
             wait_ge(pdone, i + 1, g.err, sh_flag);
             vload<T, NB>(P, Aik, lda, Tik, ldt);
             vstore_cvt<T, Tw, NB, false>(P, 0, 0);
         }
-        if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-        WStrip<Tw, NB> S;
-        tiles_load_cvt<Tw, T, NB / 16>(S.a, Aij, ldc, 0);
+        if (!haveS) {
+            if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
+            stage_load<T, Tw, NB>(Aij, ldc);
+        }
         bool next0 = false;
 #pragma unroll
         for (int B = 0; B < NBLK; ++B) {
+            if (B == 0) {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the strip, the last row's stores
+                __syncthreads();
+                if (pending) {
+                    if (threadIdx.x == 0) cst(mine, pending);
+                    pending = 0;
+                }
+                stage_to_regs<T, Tw, Tw, NB>(S);
+            }
             if (B + 1 < NBLK) {
                 vload<T, NB>(P, Aik + (i64)(B + 1) * IB * lda, lda, Tik + (i64)(B + 1) * IB * ldt, ldt);
             } else if (i + 1 < g.mt) {
@@ -918,15 +943,22 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
             if (B + 1 < NBLK || next0) vstore_cvt<T, Tw, NB, false>(P, (B + 1) & 1, 0);
         }
         have0 = next0;
-        tiles_store_cvt<Tw, T, NB / 16>(S.a, Aij, ldc, 0);
-        publish(mine, i + 1);
+        stage_store<T, Tw, Tw, NB>(S, Aij, ldc);
+        pending = i + 1;
+        haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
+        if (haveS) stage_load<T, Tw, NB>(Aij + NB, ldc);
     }
+    if (pending) publish(mine, pending);
     tiles_store_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 template <typename T, int NB, typename Tw>
 __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i64 ldt, DagArgs g) {
+    static_assert(Lds<Tw, NB>::SSTG * sizeof(Tw) + TW * Lds<T, NB>::LDS_S * sizeof(T) <=
+                      (Lds<T, NB>::TOTAL * sizeof(T) > Lds<Tw, NB>::TOTAL * sizeof(Tw) ? Lds<T, NB>::TOTAL * sizeof(T)
+                                                                                       : Lds<Tw, NB>::TOTAL * sizeof(Tw)),
+                  "the walker's staging columns must fit the launch's LDS");
     __shared__ int sh_task, sh_flag;
     for (;;) {
         __syncthreads();
