@@ -106,7 +106,7 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
         for prec in ("D", "S"):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            x, it, last = nl.tile_gels(A, b, nb, ib, prec, maxit=10, tol=1e-15)
+            x, it, last = nl.tile_gels(A, b, nb, ib, prec, maxit=200, tol=1e-15)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             r = b - A @ x

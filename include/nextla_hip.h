@@ -179,14 +179,13 @@ int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, 
 int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
                           int64_t ldt, char prec, void *stream);
 /* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
- * out.  Tile QR of a copy of A (prec 'S': Float32 trailing updates, Float64 only), then
- * x0 = R^-1 (Q^T b)[0:n], r0 = b - A x0, and up to maxit refinement steps on the augmented
- * system [I A; A^T 0][r; x] = [b; 0] with residuals in the working precision:
- * f = b - r - A x, g = -A^T r, h = R^-T g, d = Q^T f, dx = R^-1 (d[0:n] - h),
- * dr = Q [h; d[n:m]], x += dx, r += dr; stopping when ||dx|| <= tol ||x|| or when a step
- * shrinks ||dx|| by less than half (stagnation).
- * iters / last_correction (may be NULL) receive the steps taken and the final ||dx||/||x||.
- * Synchronises the stream. */
+ * out.  Tile QR of a copy of A (prec 'S': Float32 trailing updates, Float64 only), the direct
+ * solution x0 = R^-1 (Q^T b)[0:n], then up to maxit iterations of LSQR (Paige & Saunders) on
+ * the correction, right-preconditioned by R: min ||(A R^-1) y - (b - A x0)||, x = x0 + R^-1 y,
+ * with every product by A in the working precision.  Stops when LSQR's estimate of
+ * ||M^T r|| / (||M||_F ||r||) (M = A R^-1) reaches tol or the residual estimate falls to
+ * tol ||b - A x0||.  maxit = 0 returns x0.  iters / last_correction (may be NULL) receive the
+ * iterations taken and the final estimate.  Synchronises the stream. */
 int nla_dtile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *A, int64_t lda, const double *b,
                    double *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
                    void *stream);

@@ -7,6 +7,7 @@
 #include "nla_gemm.hpp"
 #include "nla_qr.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -138,32 +139,105 @@ static int tile_geqrf_mixed_entry(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 l
     return r < 0 ? 1 : r;
 }
 
+// ---------------------------------------------------------------------------------
+// Least squares.  Vector kernels of the refinement (Float64 work for Float64 data).
+
+// part[s][i] = sum over column slice s of A[i, j] p[j]  (two rows per thread, coalesced
+// 16-byte column reads; the slices make the grid large enough to stream A at HBM rate)
+template <typename T>
+__global__ void gemv_n_part_kernel(i64 m, i64 n, const T *A, i64 lda, const T *p, T *part, int nsplit) {
+    const i64 i = 2 * ((i64)blockIdx.x * blockDim.x + threadIdx.x);
+    const i64 js = (n * blockIdx.y) / nsplit, je = (n * (blockIdx.y + 1)) / nsplit;
+    T a0 = 0, a1 = 0;
+    if (i + 1 < m) {
+        for (i64 j = js; j < je; ++j) {
+            const T pj = p[j];
+            a0 += A[i + j * lda] * pj;
+            a1 += A[i + 1 + j * lda] * pj;
+        }
+    } else if (i < m) {
+        for (i64 j = js; j < je; ++j) a0 += A[i + j * lda] * p[j];
+    }
+    if (i < m) part[(i64)blockIdx.y * m + i] = a0;
+    if (i + 1 < m) part[(i64)blockIdx.y * m + i + 1] = a1;
+}
+// y[i] = sum_s part[s][i] - beta * y[i];  bsum[block] = sum of y[i]^2 over the block
+template <typename T>
+__global__ void gemv_n_finish_kernel(i64 m, int nsplit, const T *part, T beta, T *y, double *bsum) {
+    __shared__ double red[16];
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    double q = 0;
+    if (i < m) {
+        T acc = 0;
+        for (int k = 0; k < nsplit; ++k) acc += part[(i64)k * m + i];
+        const T v = acc - beta * y[i];
+        y[i] = v;
+        q = (double)v * v;
+    }
+    q = block_sum(q, red);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = q;
+}
+// z[j] = sum_i A[i, j] u[i]  (one wave per column)
+template <typename T> __global__ void gemv_t_kernel(i64 m, i64 n, const T *A, i64 lda, const T *u, T *z) {
+    const i64 j = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= n) return;
+    const T *a = A + j * lda;
+    T acc = 0;
+    for (i64 i = lane; i < m; i += 64) acc += a[i] * u[i];
+    acc = wave_sum(acc);
+    if (lane == 0) z[j] = acc;
+}
+// v[i] = z[i] - beta v[i];  bsum[block] = sum of v[i]^2 over the block
+template <typename T> __global__ void lincomb_kernel(i64 n, const T *z, T beta, T *v, double *bsum) {
+    __shared__ double red[16];
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    double q = 0;
+    if (i < n) {
+        const T t = z[i] - beta * v[i];
+        v[i] = t;
+        q = (double)t * t;
+    }
+    q = block_sum(q, red);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = q;
+}
+// out = sqrt(sum of bsum[0..nb))  (one block; fixed order, so the result is reproducible)
+__global__ void norm_finish_kernel(int nb, const double *bsum, double *out) {
+    __shared__ double red[16];
+    double q = 0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) q += bsum[k];
+    q = block_sum(q, red);
+    if (threadIdx.x == 0) *out = sqrt(q);
+}
+template <typename T> __global__ void scale_kernel(i64 n, T s, T *v) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] *= s;
+}
+// y += a w;  w = v - b w
+template <typename T> __global__ void lsqr_update_kernel(i64 n, T a, T b, const T *v, T *w, T *y) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const T wi = w[i];
+        y[i] += a * wi;
+        w[i] = v[i] - b * wi;
+    }
+}
 // y := y + x (n)
 template <typename T> __global__ void axpy1_kernel(i64 n, const T *x, T *y) {
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] += x[i];
 }
 
-// f := b - r (m)
-template <typename T> __global__ void bmr_kernel(i64 m, const T *b, const T *r, T *f) {
-    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) f[i] = b[i] - r[i];
-}
-// u := d - h (n)
-template <typename T> __global__ void dmh_kernel(i64 n, const T *d, const T *h, T *u) {
-    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) u[i] = d[i] - h[i];
-}
-
-// Least squares min ||A x - b|| (A m x n, m >= n, full rank) through the tile QR A = Q R,
-// refined on the augmented system [I A; A^T 0] [r; x] = [b; 0] (Bjorck): with the residuals
-//   f = b - r - A x,  g = -A^T r   (working precision, unmodified A),
-// the correction is  h = R^-T g,  d = Q^T f,  dx = R^-1 (d[0:n] - h),  dr = Q [h; d[n:m]].
-// Classical refinement (x += R^-1 (Q^T (b - A x))[0:n]) would stall at the Q error times the
-// least-squares residual; the augmented form converges to the least-squares solution while
-// kappa(A) times the factorization's precision stays below one.  prec 'S' (Float64 data
-// only) factors with Float32 trailing updates.  Stops after maxit steps or when
-// ||dx|| <= tol ||x||.
+// Least squares min ||A x - b|| (A m x n, m >= n, full rank).
+// x0 = R^-1 (Q^T b)[0:n] from the tile QR A = Q R (prec 'D': Float64; 'S': Float32 trailing
+// updates, Float64 data only), then LSQR (Paige & Saunders, ACM TOMS 8 (1982), algorithm
+// LSQR) on the correction, right-preconditioned by R: min ||(A R^-1) y - r0||, x = x0 + R^-1 y,
+// every product with A in the working precision on the unmodified A.  A R^-1 has condition
+// number about 1 + kappa(A) times the factorization's backward error, so the iteration
+// converges quickly whenever R is a fair preconditioner, also where plain refinement
+// (x += R^-1 (Q^T r)[0:n]) diverges: kappa(A) times the factorization error near or above one.
+// Stops after maxit iterations or when LSQR's estimate of ||M^T r|| / (||M|| ||r||)
+// (M = A R^-1) falls to tol, or the residual estimate to tol ||b - A x0||.
 template <typename T>
 static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, const T *b, T *x, char prec,
                            i64 maxit, double tol, i64 *iters, double *last, void *stream) {
@@ -182,62 +256,118 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     if (m == 0 || n == 0) return 0;
     hipStream_t s = as_stream(stream);
     const i64 mt = m / nb, ldt = mt * ib;
-    constexpr i64 W = 64;  // right-hand side block width of the Q / Q^T walkers
+    constexpr i64 W = 64;  // right-hand side block width of the Q^T walkers
+    const int rb = (int)((m + 511) / 512);
+    const int nsplit = std::max(1, std::min(32, 2048 / rb));
+    const int gm = (int)((m + 255) / 256), gn = (int)((n + 255) / 256);
     int st = 0;
-    const size_t ne = (size_t)m * n + (size_t)ldt * n + (size_t)m * W + (size_t)m + 2 * (size_t)n;
-    T *AF = (T *)scratch(SLOT_LS, sizeof(T) * ne, s, &st);
-    if (!AF) return st;
-    T *Tm = AF + (size_t)m * n, *Cb = Tm + (size_t)ldt * n, *r = Cb + (size_t)m * W;
-    T *h = r + m, *u = h + n;
-    const unsigned gm = (unsigned)((m + 255) / 256), gn = (unsigned)((n + 255) / 256);
+    const size_t ne = (size_t)m * n + (size_t)ldt * n + (size_t)m * W + (size_t)m * (1 + nsplit) + 5 * (size_t)n;
+    const size_t nd = (size_t)std::max(gm, gn) + 8;
+    char *base = (char *)scratch(SLOT_LS, sizeof(T) * ne + sizeof(double) * nd, s, &st);
+    if (!base) return st;
+    T *AF = (T *)base, *Tm = AF + (size_t)m * n, *Cb = Tm + (size_t)ldt * n, *u = Cb + (size_t)m * W;
+    T *part = u + m, *v = part + (size_t)m * nsplit, *w = v + n, *y = w + n, *t = y + n, *q = t + n;
+    double *bsum = (double *)(q + n), *dnorm = bsum + nd - 8;
     NLA_HIP(hipMemcpy2DAsync(AF, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, s));
     NLA_HIP(hipMemsetAsync(Tm, 0, sizeof(T) * (size_t)ldt * n, s));
     int rc;
     if (prec == 'S') rc = tile_geqrf_mixed_entry(m, n, nb, ib, (double *)AF, m, (double *)Tm, ldt, 'S', s);
     else rc = tile_geqrf_entry<T>(m, n, nb, ib, AF, m, Tm, ldt, s);
     if (rc) return rc;
+    // x0 = R^-1 (Q^T b)[0:n]
     NLA_HIP(hipMemsetAsync(Cb, 0, sizeof(T) * (size_t)m * W, s));
-    // x0 = R^-1 (Q^T b)[0:n];  r0 = b - A x0
     NLA_HIP(hipMemcpyAsync(Cb, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
     if ((rc = tile_ormqr_entry<T>('T', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s))) return rc;
     NLA_HIP(rectrxm_async<T>('L', 'U', 'N', T(1), 'S', n, 1, AF, m, Cb, m, 0, s));
     NLA_HIP(hipMemcpyAsync(x, Cb, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-    NLA_HIP(hipMemcpyAsync(r, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
-    NLA_HIP(gemm<T>('N', 'N', m, 1, n, T(-1), A, lda, x, n, T(1), r, m, s));
-    std::vector<T> hx(n), hd(n);
-    double prev = 0;
-    for (i64 it = 0; it < maxit; ++it) {
-        // f = b - r - A x (in Cb's first column, the other columns zero); g = -A^T r (in h)
-        NLA_HIP(hipMemsetAsync(Cb, 0, sizeof(T) * (size_t)m * W, s));
-        bmr_kernel<T><<<gm, 256, 0, s>>>(m, b, r, Cb);
-        NLA_LAUNCH_CHECK();
-        NLA_HIP(gemm<T>('N', 'N', m, 1, n, T(-1), A, lda, x, n, T(1), Cb, m, s));
-        NLA_HIP(gemm<T>('T', 'N', n, 1, m, T(-1), A, lda, r, m, T(0), h, n, s));
-        // h = R^-T g;  d = Q^T f (in Cb);  dx = R^-1 (d[0:n] - h) (in u)
-        NLA_HIP(rectrxm_async<T>('L', 'U', 'T', T(1), 'S', n, 1, AF, m, h, n, 0, s));
-        if ((rc = tile_ormqr_entry<T>('T', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s))) return rc;
-        dmh_kernel<T><<<gn, 256, 0, s>>>(n, Cb, h, u);
-        NLA_LAUNCH_CHECK();
-        NLA_HIP(rectrxm_async<T>('L', 'U', 'N', T(1), 'S', n, 1, AF, m, u, n, 0, s));
-        // dr = Q [h; d[n:m]] (in Cb);  x += dx;  r += dr
-        NLA_HIP(hipMemcpyAsync(Cb, h, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-        if ((rc = tile_ormqr_entry<T>('N', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s))) return rc;
-        axpy1_kernel<T><<<gn, 256, 0, s>>>(n, u, x);
-        NLA_LAUNCH_CHECK();
-        axpy1_kernel<T><<<gm, 256, 0, s>>>(m, Cb, r);
-        NLA_LAUNCH_CHECK();
-        NLA_HIP(hipMemcpyAsync(hd.data(), u, sizeof(T) * n, hipMemcpyDeviceToHost, s));
-        NLA_HIP(hipMemcpyAsync(hx.data(), x, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    if (maxit == 0) {
         NLA_HIP(hipStreamSynchronize(s));
-        double nd = 0, nx = 0;
-        for (i64 i = 0; i < n; ++i) nd += (double)hd[i] * hd[i], nx += (double)hx[i] * hx[i];
-        const double ratio = nx > 0 ? std::sqrt(nd / nx) : std::sqrt(nd);
-        if (iters) *iters = it + 1;
-        if (last) *last = ratio;
-        if (ratio <= tol) break;
-        if (it > 0 && ratio > 0.5 * prev) break;  // stagnation: the working precision's floor
-        prev = ratio;
+        return 0;
     }
+    const dim3 gpart((unsigned)rb, (unsigned)nsplit);
+    auto norm_read = [&](int nblocks, double *out) -> hipError_t {
+        norm_finish_kernel<<<1, 256, 0, s>>>(nblocks, bsum, dnorm);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(out, dnorm, sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        return hipStreamSynchronize(s);
+    };
+    // M = A R^-1.  beta1 u1 = r0 = b - A x0;  alpha1 v1 = M^T u1 = R^-T (A^T u1)
+    NLA_HIP(hipMemcpyAsync(u, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
+    gemv_n_part_kernel<T><<<gpart, 256, 0, s>>>(m, n, A, lda, x, part, nsplit);
+    NLA_LAUNCH_CHECK();
+    gemv_n_finish_kernel<T><<<gm, 256, 0, s>>>(m, nsplit, part, T(1), u, bsum);  // u = A x0 - b
+    NLA_LAUNCH_CHECK();
+    double beta = 0, alpha = 0;
+    NLA_HIP(norm_read(gm, &beta));
+    const double r0norm = beta;
+    if (beta == 0) {
+        NLA_HIP(hipStreamSynchronize(s));
+        return 0;
+    }
+    scale_kernel<T><<<gm, 256, 0, s>>>(m, T(-1.0 / beta), u);  // u1 = (b - A x0) / beta1
+    NLA_LAUNCH_CHECK();
+    gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, u, q);
+    NLA_LAUNCH_CHECK();
+    NLA_HIP(rectrxm_async<T>('L', 'U', 'T', T(1), 'S', n, 1, AF, m, q, n, 0, s));
+    NLA_HIP(hipMemsetAsync(v, 0, sizeof(T) * n, s));
+    lincomb_kernel<T><<<gn, 256, 0, s>>>(n, q, T(0), v, bsum);
+    NLA_LAUNCH_CHECK();
+    NLA_HIP(norm_read(gn, &alpha));
+    if (alpha == 0) {
+        NLA_HIP(hipStreamSynchronize(s));
+        return 0;
+    }
+    scale_kernel<T><<<gn, 256, 0, s>>>(n, T(1.0 / alpha), v);
+    NLA_LAUNCH_CHECK();
+    NLA_HIP(hipMemcpyAsync(w, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    NLA_HIP(hipMemsetAsync(y, 0, sizeof(T) * n, s));
+    double phibar = beta, rhobar = alpha, anorm2 = 0, test2 = 1;
+    for (i64 it = 0; it < maxit; ++it) {
+        // beta u = M v - alpha u = A (R^-1 v) - alpha u
+        NLA_HIP(hipMemcpyAsync(t, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+        NLA_HIP(rectrxm_async<T>('L', 'U', 'N', T(1), 'S', n, 1, AF, m, t, n, 0, s));
+        gemv_n_part_kernel<T><<<gpart, 256, 0, s>>>(m, n, A, lda, t, part, nsplit);
+        NLA_LAUNCH_CHECK();
+        gemv_n_finish_kernel<T><<<gm, 256, 0, s>>>(m, nsplit, part, T(alpha), u, bsum);
+        NLA_LAUNCH_CHECK();
+        NLA_HIP(norm_read(gm, &beta));
+        anorm2 += alpha * alpha + beta * beta;
+        if (beta > 0) {
+            scale_kernel<T><<<gm, 256, 0, s>>>(m, T(1.0 / beta), u);
+            NLA_LAUNCH_CHECK();
+            // alpha v = M^T u - beta v = R^-T (A^T u) - beta v
+            gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, u, q);
+            NLA_LAUNCH_CHECK();
+            NLA_HIP(rectrxm_async<T>('L', 'U', 'T', T(1), 'S', n, 1, AF, m, q, n, 0, s));
+            lincomb_kernel<T><<<gn, 256, 0, s>>>(n, q, T(beta), v, bsum);
+            NLA_LAUNCH_CHECK();
+            NLA_HIP(norm_read(gn, &alpha));
+            if (alpha > 0) {
+                scale_kernel<T><<<gn, 256, 0, s>>>(n, T(1.0 / alpha), v);
+                NLA_LAUNCH_CHECK();
+            }
+        } else {
+            alpha = 0;
+        }
+        // plane rotation eliminating beta from the lower bidiagonal
+        const double rho = std::hypot(rhobar, beta), c = rhobar / rho, sn = beta / rho;
+        const double theta = sn * alpha;
+        rhobar = -c * alpha;
+        const double phi = c * phibar;
+        phibar = sn * phibar;
+        lsqr_update_kernel<T><<<gn, 256, 0, s>>>(n, T(phi / rho), T(theta / rho), v, w, y);
+        NLA_LAUNCH_CHECK();
+        // ||M^T r|| / (||M||_F ||r||) = alpha |c| / ||B_k||_F;  ||r|| = phibar
+        test2 = anorm2 > 0 ? alpha * std::fabs(c) / std::sqrt(anorm2) : 0;
+        if (iters) *iters = it + 1;
+        if (last) *last = test2;
+        if (test2 <= tol || phibar <= tol * r0norm || alpha == 0 || beta == 0) break;
+    }
+    // x = x0 + R^-1 y
+    NLA_HIP(rectrxm_async<T>('L', 'U', 'N', T(1), 'S', n, 1, AF, m, y, n, 0, s));
+    axpy1_kernel<T><<<gn, 256, 0, s>>>(n, y, x);
+    NLA_LAUNCH_CHECK();
     NLA_HIP(hipStreamSynchronize(s));
     return 0;
 }

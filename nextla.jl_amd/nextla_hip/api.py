@@ -427,10 +427,11 @@ def tile_geqrf_mixed(A, nb, ib, prec="S"):
     return T
 
 
-def tile_gels(A, b, nb=128, ib=32, prec="D", maxit=10, tol=1e-15):
-    """Least-squares solution of min ||A x - b|| through the tile QR with iterative
-    refinement (residuals in the working precision).  prec 'S' (Float64 only) factors with
-    Float32 trailing updates.  Returns (x, iterations, last ||dx||/||x||)."""
+def tile_gels(A, b, nb=128, ib=32, prec="D", maxit=100, tol=1e-15):
+    """Least-squares solution of min ||A x - b||: tile QR (prec 'S', Float64 only: Float32
+    trailing updates), direct solution x0, then R-preconditioned LSQR on the correction with
+    products by A in the working precision.  Returns (x, iterations, final estimate of
+    ||M^T r|| / (||M|| ||r||) for M = A R^-1)."""
     import ctypes
 
     p = _p(A, b)

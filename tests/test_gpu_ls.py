@@ -82,7 +82,7 @@ def test_gels_refinement_vs_lapack(prec, kappa):
     A, b = ls_problem(rng, m, n, kappa)
     xref = np.linalg.lstsq(A, b, rcond=None)[0]
     dA, db = nl.to_device(A), torch.tensor(b, device="cuda")
-    x, iters, last = nl.tile_gels(dA, db, nb, 32, prec, maxit=8, tol=1e-15)
+    x, iters, last = nl.tile_gels(dA, db, nb, 32, prec, maxit=200, tol=1e-15)
     xh = x.cpu().numpy()
     r = b - A @ xh
     # error of a backward-stable LS solver: ~ u (kappa + kappa^2 ||r|| / (||A|| ||x||))
