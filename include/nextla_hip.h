@@ -173,13 +173,17 @@ int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, 
 
 /* ---- Mixed precision and least squares (no reference function: config "mixed-precision
  * tile QR + least squares"; the reference stops at the tile kernels) --------------------- */
-/* nla_dtile_geqrf with the trailing updates (unmqr!/tsmqr!) in Float32 MFMA when prec = 'S'
- * (panels, larfg norms/scaling and the T factors stay Float64); prec = 'D' is nla_dtile_geqrf.
- * Needs nb in {64, 128}, ib = 32 and 16-byte aligned columns. */
+/* nla_dtile_geqrf with the trailing updates (unmqr!/tsmqr!) on MFMA with Float32
+ * accumulation: prec 'S' Float32 operands, 'H' FP16 operands, 'B' BF16 operands (the strips
+ * stay Float32 between products; V, T and each product's operands are rounded); the panels,
+ * larfg norms/scaling and the T factors stay Float64.  prec 'D' is nla_dtile_geqrf.  'H'
+ * assumes the matrix's entries are within FP16 range.  Needs nb in {64, 128}, ib = 32 and
+ * 16-byte aligned columns. */
 int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
                           int64_t ldt, char prec, void *stream);
 /* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
- * out.  Tile QR of a copy of A (prec 'S': Float32 trailing updates, Float64 only), the direct
+ * out.  Tile QR of a copy of A (prec 'D' plain; 'S' / 'H' / 'B' as nla_dtile_geqrf_mixed,
+ * Float64 data only), the direct
  * solution x0 = R^-1 (Q^T b)[0:n], then up to maxit iterations of LSQR (Paige & Saunders) on
  * the correction, right-preconditioned by R: min ||(A R^-1) y - (b - A x0)||, x = x0 + R^-1 y,
  * with every product by A in the working precision.  Stops when LSQR's estimate of

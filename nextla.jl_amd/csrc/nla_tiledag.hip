@@ -172,9 +172,8 @@ template <typename Ts, typename Tl, int NB> __device__ __forceinline__ Ts *stage
 }
 // Issue the loads of the wave's strip (columns G, G + ld, ...) into its staging columns.
 // Completion is tracked by vmcnt; the caller waits before stage_to_regs.
-template <typename Ts, typename Tl, int NB> __device__ __forceinline__ void stage_load(const Ts *G, i64 ld) {
+template <typename Ts, int NB> __device__ __forceinline__ void stage_load(Ts *stg, const Ts *G, i64 ld) {
     constexpr int LDS_S = Lds<Ts, NB>::LDS_S;
-    Ts *stg = stage_ptr<Ts, Tl, NB>();
     const int lane = threadIdx.x & 63;
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // earlier reads of the staging columns are done
     if (lane < Stage<Ts, NB>::LPC) {
@@ -187,9 +186,9 @@ template <typename Ts, typename Tl, int NB> __device__ __forceinline__ void stag
     }
 }
 // staging columns (Ts) -> accumulator registers (Tr)
-template <typename Ts, typename Tr, typename Tl, int NB> __device__ __forceinline__ void stage_to_regs(WStrip<Tr, NB> &S) {
+template <typename Ts, typename Tr, int NB>
+__device__ __forceinline__ void stage_to_regs(const Ts *stg, WStrip<Tr, NB> &S) {
     constexpr int LDS_S = Lds<Ts, NB>::LDS_S;
-    const Ts *stg = stage_ptr<Ts, Tl, NB>();
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int rt = 0; rt < NB / 16; ++rt)
@@ -198,13 +197,12 @@ template <typename Ts, typename Tr, typename Tl, int NB> __device__ __forceinlin
 }
 // accumulator registers (Tr) -> staging columns (Ts) -> HBM (rows 0..NB-1 of the wave's 16
 // columns at G)
-template <typename Ts, typename Tr, typename Tl, int NB>
-__device__ __forceinline__ void stage_store(const WStrip<Tr, NB> &S, Ts *G, i64 ld) {
+template <typename Ts, typename Tr, int NB>
+__device__ __forceinline__ void stage_store(Ts *stg, const WStrip<Tr, NB> &S, Ts *G, i64 ld) {
     using St = Stage<Ts, NB>;
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
     typedef Ts tvec __attribute__((ext_vector_type(St::VEC)));
     constexpr int LDS_S = Lds<Ts, NB>::LDS_S;
-    Ts *stg = stage_ptr<Ts, Tl, NB>();
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int rt = 0; rt < NB / 16; ++rt)
@@ -686,6 +684,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
     T *Cm = reinterpret_cast<T *>(g.C);
     const i64 ldc = g.ldc;
     T *Akj = Cm + (i64)k * NB + col * ldc;
+    T *stg = stage_ptr<T, T, NB>();
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     WStrip<T, NB> A1;
@@ -728,7 +727,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            stage_load<T, T, NB>(Aij, ldc);
+            stage_load<T, NB>(stg, Aij, ldc);
         }
         bool next0 = false;
 #pragma unroll
@@ -740,7 +739,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-                stage_to_regs<T, T, T, NB>(S);
+                stage_to_regs<T, T, NB>(stg, S);
             }
             if (B + 1 < NBLK) {
                 vload_lds<T, NB>(Aik + (i64)(B + 1) * IB * lda, lda, (B + 1) & 1);
@@ -756,10 +755,10 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
             if (B + 1 < NBLK || next0) tstore<T, NB>(P, (B + 1) & 1);
         }
         have0 = next0;
-        stage_store<T, T, T, NB>(S, Aij, ldc);
+        stage_store<T, T, NB>(stg, S, Aij, ldc);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
-        if (haveS) stage_load<T, T, NB>(Aij + NB, ldc);
+        if (haveS) stage_load<T, NB>(stg, Aij + NB, ldc);
     }
     if (pending) publish(mine, pending);
     tiles_store<T, NB / 16>(A1.a, Akj, ldc, 0);
@@ -876,6 +875,7 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     T *Cm = reinterpret_cast<T *>(g.C);
     const i64 ldc = g.ldc;
     T *Akj = Cm + (i64)k * NB + col * ldc;
+    T *stg = stage_ptr<T, Tw, NB>();
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     WStrip<Tw, NB> A1;
@@ -918,7 +918,7 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            stage_load<T, Tw, NB>(Aij, ldc);
+            stage_load<T, NB>(stg, Aij, ldc);
         }
         bool next0 = false;
 #pragma unroll
@@ -930,7 +930,7 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-                stage_to_regs<T, Tw, Tw, NB>(S);
+                stage_to_regs<T, Tw, NB>(stg, S);
             }
             if (B + 1 < NBLK) {
                 vload<T, NB>(P, Aik + (i64)(B + 1) * IB * lda, lda, Tik + (i64)(B + 1) * IB * ldt, ldt);
@@ -943,21 +943,216 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
             if (B + 1 < NBLK || next0) vstore_cvt<T, Tw, NB, false>(P, (B + 1) & 1, 0);
         }
         have0 = next0;
-        stage_store<T, Tw, Tw, NB>(S, Aij, ldc);
+        stage_store<T, Tw, NB>(stg, S, Aij, ldc);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
-        if (haveS) stage_load<T, Tw, NB>(Aij + NB, ldc);
+        if (haveS) stage_load<T, NB>(stg, Aij + NB, ldc);
     }
     if (pending) publish(mine, pending);
     tiles_store_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
+// ---------------------------------------------------------------------------------
+// 16-bit walker (Tw = _Float16 or __bf16, Float32 accumulation).  The strips stay Float32 in
+// MFMA accumulators; every product rounds its operands to Tw: V and T when they are staged,
+// the strip and the intermediate W from the accumulators right before each
+// v_mfma_f32_16x16x16.  The Float32 accumulator layout of a 16 x 16 tile (lane l holds rows
+// 4 (l/16) .. 4 (l/16) + 3 of column l % 16) is exactly the B operand of the 16x16x16 MFMA for
+// a 16-deep slice of K, so the strip and W feed the products without data movement.  V is
+// staged twice, column-major for V^T S and row-major for V W, so each lane's four A-operand
+// elements are one 8-byte LDS read in both products.
+template <typename Tw> struct LpMfma;
+template <> struct LpMfma<_Float16> {
+    typedef _Float16 v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ void mma(fx4 &c, v4 a, v4 b) {
+        c = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct LpMfma<__bf16> {
+    typedef __bf16 v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ void mma(fx4 &c, v4 a, v4 b) {
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+};
+template <typename Tw> __device__ __forceinline__ typename LpMfma<Tw>::v4 lp_cvt(const fx4 &x, float sign) {
+    typename LpMfma<Tw>::v4 r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[q] = (Tw)(sign * x[q]);
+    return r;
+}
+// LDS of the 16-bit walker, in Tw elements: per buffer V column-major (Vc[j * LDC + p]),
+// V row-major (Vr[p * LDR + j]) and T column-major (Tc[j * LDR + q] = T[q][j]); the Float64
+// strip staging columns follow the two buffers.
+template <typename Tw, int NB> struct LdsLp {
+    static constexpr int LDC = NB + 8, LDR = IB + 8;
+    static constexpr int VC = 0, VR = VC + IB * LDC, TC = VR + NB * LDR, BUF = TC + IB * LDR;
+    static constexpr int SSTG_BYTES = 2 * BUF * (int)sizeof(Tw);
+    static_assert(SSTG_BYTES % 16 == 0, "staging columns must start 16-byte aligned");
+};
+template <typename T, typename Tw, int NB> __device__ __forceinline__ T *stage_ptr_lp() {
+    return reinterpret_cast<T *>(lds_base<unsigned char>() + LdsLp<Tw, NB>::SSTG_BYTES) +
+           (threadIdx.x >> 6) * SW * Lds<T, NB>::LDS_S;
+}
+template <typename T, typename Tw, int NB, bool GE>
+__device__ __forceinline__ void vstore_lp(const VPre<T, NB> &P, int buf, int j0) {
+    using L = LdsLp<Tw, NB>;
+    Tw *sm = lds_base<Tw>() + buf * L::BUF;
+#pragma unroll
+    for (int e = 0; e < VPre<T, NB>::NV; ++e) {
+        const int idx = threadIdx.x + e * NTH, p = idx % NB, j = idx / NB;
+        T v = P.v[e];
+        if (GE) {
+            const int ja = j0 + j;
+            v = p < ja ? T(0) : (p == ja ? T(1) : v);
+        }
+        const Tw h = (Tw)v;
+        sm[L::VC + j * L::LDC + p] = h;
+        sm[L::VR + p * L::LDR + j] = h;
+    }
+#pragma unroll
+    for (int e = 0; e < VPre<T, NB>::NT; ++e) {
+        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
+        sm[L::TC + j * L::LDR + q] = q <= j ? (Tw)P.t[e] : (Tw)0.0f;
+    }
+}
+// H^T (the reference's trans 'C') of the staged block applied to the wave's strip:
+//   W0 = V^T S (+ A1 rows),  W = T^T W0,  A1 rows -= W,  S -= V W.
+template <typename Tw, int NB, bool GE>
+__device__ __forceinline__ void apply_lp(WStrip<float, NB> &S, fx4 *a1, int buf) {
+    using L = LdsLp<Tw, NB>;
+    using M = LpMfma<Tw>;
+    typedef typename M::v4 v4;
+    const Tw *sm = lds_base<Tw>() + buf * L::BUF;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    fx4 w0[2] = {acc_zero<float>(), acc_zero<float>()};
+#pragma unroll
+    for (int rt = 0; rt < NB / 16; ++rt) {
+        const v4 b = lp_cvt<Tw>(S.a[rt], 1.0f);
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot)
+            M::mma(w0[ot], *reinterpret_cast<const v4 *>(sm + L::VC + (16 * ot + lr) * L::LDC + 16 * rt + 4 * lk), b);
+    }
+    if (!GE) {
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) w0[ot] += a1[ot];
+    }
+    fx4 wv[2] = {acc_zero<float>(), acc_zero<float>()};
+#pragma unroll
+    for (int tq = 0; tq < 2; ++tq) {
+        const v4 b = lp_cvt<Tw>(w0[tq], 1.0f);
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot)
+            M::mma(wv[ot], *reinterpret_cast<const v4 *>(sm + L::TC + (16 * ot + lr) * L::LDR + 16 * tq + 4 * lk), b);
+    }
+    if (!GE) {
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) a1[ot] -= wv[ot];
+    }
+#pragma unroll
+    for (int tq = 0; tq < 2; ++tq) {
+        const v4 b = lp_cvt<Tw>(wv[tq], -1.0f);
+#pragma unroll
+        for (int rt = 0; rt < NB / 16; ++rt)
+            M::mma(S.a[rt], *reinterpret_cast<const v4 *>(sm + L::VR + (16 * rt + lr) * L::LDR + 16 * tq + 4 * lk), b);
+    }
+}
+
+template <typename T, typename Tw, int NB>
+__device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+    constexpr int NBLK = NB / IB;
+    static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
+    __shared__ int sh_ready;
+    const int w = threadIdx.x >> 6;
+    const i64 col = (i64)s * TW + w * SW;
+    const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
+    int *mine = &g.strip_prog[k * g.nwalk + s];
+    const int *pdone = &g.panel_prog[k * NBLK + NBLK - 1];
+    const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
+    const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
+    T *Cm = reinterpret_cast<T *>(g.C);
+    const i64 ldc = g.ldc;
+    T *Akj = Cm + (i64)k * NB + col * ldc;
+    T *stg = stage_ptr_lp<T, Tw, NB>();
+    wait_ge(pdone, k + 1, g.err, sh_flag);
+    if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
+    WStrip<float, NB> A1;
+    tiles_load_cvt<float, T, NB / 16>(A1.a, Akj, ldc, 0);
+    VPre<T, NB> P;
+    vload<T, NB>(P, Akk, lda, Tkk, ldt);
+    vstore_lp<T, Tw, NB, true>(P, 0, 0);
+    bool have0 = false;
+#pragma unroll
+    for (int B = 0; B < NBLK; ++B) {
+        if (B + 1 < NBLK) {
+            vload<T, NB>(P, Akk + (i64)(B + 1) * IB * lda, lda, Tkk + (i64)(B + 1) * IB * ldt, ldt);
+        } else if (k + 1 < g.mt) {
+            have0 = ready_ge(pdone, k + 2, &sh_ready);
+            if (have0) vload<T, NB>(P, Akk + NB, lda, Tkk + IB, ldt);
+        }
+        __syncthreads();
+        apply_lp<Tw, NB, true>(A1, nullptr, B & 1);
+        if (B + 1 < NBLK) vstore_lp<T, Tw, NB, true>(P, (B + 1) & 1, (B + 1) * IB);
+        else if (have0) vstore_lp<T, Tw, NB, false>(P, 0, 0);
+    }
+    WStrip<float, NB> S;
+    bool haveS = false;
+    int pending = 0;
+    for (int i = k + 1; i < g.mt; ++i) {
+        const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
+        const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
+        T *Aij = Cm + (i64)i * NB + col * ldc;
+        if (pending && (!have0 || !haveS)) {
+            publish(mine, pending);
+            pending = 0;
+        }
+        if (!have0) {
+            wait_ge(pdone, i + 1, g.err, sh_flag);
+            vload<T, NB>(P, Aik, lda, Tik, ldt);
+            vstore_lp<T, Tw, NB, false>(P, 0, 0);
+        }
+        if (!haveS) {
+            if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
+            stage_load<T, NB>(stg, Aij, ldc);
+        }
+        bool next0 = false;
+#pragma unroll
+        for (int B = 0; B < NBLK; ++B) {
+            if (B == 0) {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the strip, the last row's stores
+                __syncthreads();
+                if (pending) {
+                    if (threadIdx.x == 0) cst(mine, pending);
+                    pending = 0;
+                }
+                stage_to_regs<T, float, NB>(stg, S);
+            }
+            if (B + 1 < NBLK) {
+                vload<T, NB>(P, Aik + (i64)(B + 1) * IB * lda, lda, Tik + (i64)(B + 1) * IB * ldt, ldt);
+            } else if (i + 1 < g.mt) {
+                next0 = ready_ge(pdone, i + 2, &sh_ready);
+                if (next0) vload<T, NB>(P, Aik + NB, lda, Tik + IB, ldt);
+            }
+            __syncthreads();
+            apply_lp<Tw, NB, false>(S, &A1.a[2 * B], B & 1);
+            if (B + 1 < NBLK || next0) vstore_lp<T, Tw, NB, false>(P, (B + 1) & 1, 0);
+        }
+        have0 = next0;
+        stage_store<T, float, NB>(stg, S, Aij, ldc);
+        pending = i + 1;
+        haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
+        if (haveS) stage_load<T, NB>(stg, Aij + NB, ldc);
+    }
+    if (pending) publish(mine, pending);
+    tiles_store_cvt<float, T, NB / 16>(A1.a, Akj, ldc, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
 template <typename T, int NB, typename Tw>
 __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i64 ldt, DagArgs g) {
-    static_assert(Lds<Tw, NB>::SSTG * sizeof(Tw) + TW * Lds<T, NB>::LDS_S * sizeof(T) <=
-                      (Lds<T, NB>::TOTAL * sizeof(T) > Lds<Tw, NB>::TOTAL * sizeof(Tw) ? Lds<T, NB>::TOTAL * sizeof(T)
-                                                                                       : Lds<Tw, NB>::TOTAL * sizeof(Tw)),
+    static_assert((sizeof(Tw) == 2 ? (size_t)LdsLp<Tw, NB>::SSTG_BYTES : Lds<Tw, NB>::SSTG * sizeof(Tw)) +
+                          TW * Lds<T, NB>::LDS_S * sizeof(T) <=
+                      Lds<T, NB>::TOTAL * sizeof(T),
                   "the walker's staging columns must fit the launch's LDS");
     __shared__ int sh_task, sh_flag;
     for (;;) {
@@ -975,6 +1170,8 @@ __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i
         else if constexpr (sizeof(Tw) == sizeof(T)) {
             if (g.qmode) run_walk_q<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
             else run_walk<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
+        } else if constexpr (sizeof(Tw) == 2) {
+            run_walk_lp<T, Tw, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
         } else {
             run_walk_mixed<T, Tw, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
         }
@@ -1141,10 +1338,19 @@ int tile_geqrf_dag(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, 
 }
 
 // Float64 panels, Float32 trailing updates.
-int tile_geqrf_dag_mixed(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 lda, double *Tm, i64 ldt, hipStream_t s,
-                         bool *handled) {
+// Trailing updates in Float32 (prec 'S'), FP16 ('H') or BF16 ('B'); the panels in Float64.
+int tile_geqrf_dag_mixed(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 lda, double *Tm, i64 ldt, char prec,
+                         hipStream_t s, bool *handled) {
     *handled = dag_shape_ok<double>(m, n, nb, ib, A, lda);
     if (!*handled) return 0;
+    if (prec == 'H') {
+        if (nb == 64) return dag_run<double, 64, _Float16>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+        return dag_run<double, 128, _Float16>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+    }
+    if (prec == 'B') {
+        if (nb == 64) return dag_run<double, 64, __bf16>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+        return dag_run<double, 128, __bf16>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+    }
     if (nb == 64) return dag_run<double, 64, float>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
     return dag_run<double, 128, float>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
 }

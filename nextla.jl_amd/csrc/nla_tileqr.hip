@@ -15,7 +15,7 @@ namespace nla {
 
 template <typename T>
 int tile_geqrf_dag(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, hipStream_t s, bool *handled);
-int tile_geqrf_dag_mixed(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 lda, double *Tm, i64 ldt, hipStream_t s,
+int tile_geqrf_dag_mixed(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 lda, double *Tm, i64 ldt, char prec, hipStream_t s,
                          bool *handled);
 template <typename T>
 int tile_apply_qt_dag(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, const T *Tm, i64 ldt, T *C, i64 ldc,
@@ -119,11 +119,11 @@ static int tile_ormqr_entry(char trans, i64 m, i64 n, i64 nrhs, i64 nb, i64 ib, 
     return 0;
 }
 
-// Float64 tile QR with Float32 trailing updates (prec 'S') or plain ('D').
+// Float64 tile QR with Float32 ('S'), FP16 ('H') or BF16 ('B') trailing updates, or plain ('D').
 static int tile_geqrf_mixed_entry(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 lda, double *Tm, i64 ldt, char prec,
                                   void *stream) {
     prec = up(prec);
-    if (prec != 'D' && prec != 'S') return arg_error(9, "prec must be 'D' or 'S'");
+    if (prec != 'D' && prec != 'S' && prec != 'H' && prec != 'B') return arg_error(9, "prec must be 'D', 'S', 'H' or 'B'");
     if (prec == 'D') return tile_geqrf_entry<double>(m, n, nb, ib, A, lda, Tm, ldt, stream);
     if (m < 0) return arg_error(1, "m must be non-negative");
     if (n < 0) return arg_error(2, "n must be non-negative");
@@ -134,7 +134,7 @@ static int tile_geqrf_mixed_entry(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 l
     if (lda < m) return arg_error(6, "lda must be >= m");
     if (ldt < (m / nb) * ib) return arg_error(8, "ldt must be >= (m/nb)*ib");
     bool handled = false;
-    const int r = tile_geqrf_dag_mixed(m, n, nb, ib, A, lda, Tm, ldt, as_stream(stream), &handled);
+    const int r = tile_geqrf_dag_mixed(m, n, nb, ib, A, lda, Tm, ldt, prec, as_stream(stream), &handled);
     if (!handled) return arg_error(3, "mixed precision needs nb in {64, 128}, ib = 32 and 16-byte aligned columns");
     return r < 0 ? 1 : r;
 }
@@ -229,8 +229,8 @@ template <typename T> __global__ void axpy1_kernel(i64 n, const T *x, T *y) {
 }
 
 // Least squares min ||A x - b|| (A m x n, m >= n, full rank).
-// x0 = R^-1 (Q^T b)[0:n] from the tile QR A = Q R (prec 'D': Float64; 'S': Float32 trailing
-// updates, Float64 data only), then LSQR (Paige & Saunders, ACM TOMS 8 (1982), algorithm
+// x0 = R^-1 (Q^T b)[0:n] from the tile QR A = Q R (prec 'D': Float64; 'S', 'H', 'B': Float32,
+// FP16 or BF16 trailing updates, Float64 data only), then LSQR (Paige & Saunders, ACM TOMS 8 (1982), algorithm
 // LSQR) on the correction, right-preconditioned by R: min ||(A R^-1) y - r0||, x = x0 + R^-1 y,
 // every product with A in the working precision on the unmodified A.  A R^-1 has condition
 // number about 1 + kappa(A) times the factorization's backward error, so the iteration
@@ -248,8 +248,8 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     if (ib <= 0 || ib > nb) return arg_error(4, "ib must be in 1..nb");
     if (m % nb || n % nb) return arg_error(1, "m and n must be multiples of nb");
     if (lda < m) return arg_error(6, "lda must be >= m");
-    if (prec != 'D' && prec != 'S') return arg_error(9, "prec must be 'D' or 'S'");
-    if (prec == 'S' && sizeof(T) != 8) return arg_error(9, "prec 'S' needs Float64 data");
+    if (prec != 'D' && prec != 'S' && prec != 'H' && prec != 'B') return arg_error(9, "prec must be 'D', 'S', 'H' or 'B'");
+    if (prec != 'D' && sizeof(T) != 8) return arg_error(9, "prec '%c' needs Float64 data", prec);
     if (maxit < 0) return arg_error(10, "maxit must be non-negative");
     if (iters) *iters = 0;
     if (last) *last = 0;
@@ -271,7 +271,7 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     NLA_HIP(hipMemcpy2DAsync(AF, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, s));
     NLA_HIP(hipMemsetAsync(Tm, 0, sizeof(T) * (size_t)ldt * n, s));
     int rc;
-    if (prec == 'S') rc = tile_geqrf_mixed_entry(m, n, nb, ib, (double *)AF, m, (double *)Tm, ldt, 'S', s);
+    if (prec != 'D') rc = tile_geqrf_mixed_entry(m, n, nb, ib, (double *)AF, m, (double *)Tm, ldt, prec, s);
     else rc = tile_geqrf_entry<T>(m, n, nb, ib, AF, m, Tm, ldt, s);
     if (rc) return rc;
     // x0 = R^-1 (Q^T b)[0:n]

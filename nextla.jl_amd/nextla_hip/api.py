@@ -415,8 +415,9 @@ def rectrsm_mixed_(side, uplo, transpose, alpha, A, B, cutoff=0, maxit=5, tol=1e
 
 
 def tile_geqrf_mixed(A, nb, ib, prec="S"):
-    """tile_geqrf of a Float64 A with the trailing updates in Float32 MFMA (prec 'S'); the
-    panels, Householder norms/scaling and T stay Float64.  Returns T."""
+    """tile_geqrf of a Float64 A with the trailing updates on MFMA with Float32 accumulation
+    and Float32 ('S'), FP16 ('H') or BF16 ('B') operands; the panels, Householder
+    norms/scaling and T stay Float64.  Returns T."""
     if A.dtype != torch.float64:
         raise ArgumentError("tile_geqrf_mixed needs Float64 data")
     m, n = A.shape
@@ -428,8 +429,8 @@ def tile_geqrf_mixed(A, nb, ib, prec="S"):
 
 
 def tile_gels(A, b, nb=128, ib=32, prec="D", maxit=100, tol=1e-15):
-    """Least-squares solution of min ||A x - b||: tile QR (prec 'S', Float64 only: Float32
-    trailing updates), direct solution x0, then R-preconditioned LSQR on the correction with
+    """Least-squares solution of min ||A x - b||: tile QR (prec 'S' / 'H' / 'B', Float64
+    only: Float32 / FP16 / BF16 trailing updates), direct solution x0, then R-preconditioned LSQR on the correction with
     products by A in the working precision.  Returns (x, iterations, final estimate of
     ||M^T r|| / (||M|| ||r||) for M = A R^-1)."""
     import ctypes
