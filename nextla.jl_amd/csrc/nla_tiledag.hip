@@ -994,6 +994,15 @@ template <typename T, typename Tw, int NB> __device__ __forceinline__ T *stage_p
     return reinterpret_cast<T *>(lds_base<unsigned char>() + LdsLp<Tw, NB>::SSTG_BYTES) +
            (threadIdx.x >> 6) * SW * Lds<T, NB>::LDS_S;
 }
+// Float64 -> Tw through Float32: the hardware converts Float32 -> FP16 / BF16 in one
+// instruction, while a direct Float64 -> FP16 rounding is a long integer sequence (and the
+// compiler folds a plain two-step cast back into it; the empty asm keeps the Float32 value).
+// The extra rounding is below 2^-24 relative, far under the 16-bit formats' 2^-11 / 2^-8.
+template <typename Tw> __device__ __forceinline__ Tw to_lp(double v) {
+    float f = (float)v;
+    asm volatile("" : "+v"(f));
+    return (Tw)f;
+}
 template <typename T, typename Tw, int NB, bool GE>
 __device__ __forceinline__ void vstore_lp(const VPre<T, NB> &P, int buf, int j0) {
     using L = LdsLp<Tw, NB>;
@@ -1006,14 +1015,14 @@ __device__ __forceinline__ void vstore_lp(const VPre<T, NB> &P, int buf, int j0)
             const int ja = j0 + j;
             v = p < ja ? T(0) : (p == ja ? T(1) : v);
         }
-        const Tw h = (Tw)v;
+        const Tw h = to_lp<Tw>(v);
         sm[L::VC + j * L::LDC + p] = h;
         sm[L::VR + p * L::LDR + j] = h;
     }
 #pragma unroll
     for (int e = 0; e < VPre<T, NB>::NT; ++e) {
         const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
-        sm[L::TC + j * L::LDR + q] = q <= j ? (Tw)P.t[e] : (Tw)0.0f;
+        sm[L::TC + j * L::LDR + q] = q <= j ? to_lp<Tw>(P.t[e]) : (Tw)0.0f;
     }
 }
 // H^T (the reference's trans 'C') of the staged block applied to the wave's strip:
