@@ -140,6 +140,161 @@ static int tile_geqrf_mixed_entry(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 l
 }
 
 // ---------------------------------------------------------------------------------
+// Triangular solve with one right-hand side: R x = v (trans false) or R^T x = v (trans
+// true), R the n x n upper triangle of a column-major matrix (n a multiple of 64), x
+// overwriting v.  One persistent launch: workgroups claim 64-row blocks from a counter in
+// dependency order (bottom up for R, top down for R^T), accumulate the products with the
+// finished blocks as each block's flag appears, solve their diagonal block from LDS and
+// publish it.  A block waits only on blocks claimed before it, so the launch cannot
+// deadlock; waits are bounded (2 s, then an error flag).  R is read once; the recursive
+// triangular solve would instead run log2(n/64) levels of one-column GEMM launches.
+constexpr int TRSV_BS = 64;
+
+template <typename T> __device__ __forceinline__ T ld_sc1(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ void st_sc1(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Inverses of the n/64 diagonal 64 x 64 blocks of the upper triangle R (one workgroup per
+// block, thread j forms column j by back substitution): Dinv[b][i + 64 j] = inv(R_bb)[i][j].
+template <typename T> __global__ __launch_bounds__(64) void diag_block_inv_kernel(const T *R, i64 lda, T *Dinv) {
+    constexpr int BS = TRSV_BS;
+    __shared__ T U[BS][BS + 1];
+    __shared__ T Y[BS][BS + 1];  // Y[i][j] = inv[i][j]
+    const int b = blockIdx.x, j = threadIdx.x;
+    const i64 r0 = (i64)b * BS;
+    for (int i = 0; i < BS; ++i) U[i][j] = R[r0 + i + (r0 + j) * lda];  // row i, column j (coalesced in i? no: lanes = j)
+    __syncthreads();
+    for (int i = BS - 1; i > j; --i) Y[i][j] = 0;
+    Y[j][j] = T(1) / U[j][j];
+    for (int i = j - 1; i >= 0; --i) {
+        T acc = 0;
+        for (int k = i + 1; k <= j; ++k) acc += U[i][k] * Y[k][j];
+        Y[i][j] = -acc / U[i][i];
+    }
+    for (int i = 0; i < BS; ++i) Dinv[(i64)b * BS * BS + i + (i64)j * BS] = Y[i][j];
+}
+
+template <typename T, bool TRANS>
+__global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 lda, const T *Dinv, T *x, int *counter,
+                                                          int *flags, int *err) {
+    constexpr int BS = TRSV_BS;
+    __shared__ T red[4][BS + 1];
+    __shared__ T ys[BS];
+    __shared__ int sh_b, sh_stop;
+    const int nblk = n / BS, t = threadIdx.x, lane = t & 63, g = t >> 6;
+    for (;;) {
+        __syncthreads();
+        if (t == 0) sh_b = atomicAdd(counter, 1);
+        __syncthreads();
+        const int claim = sh_b;
+        if (claim >= nblk) break;
+        const int b = TRANS ? claim : nblk - 1 - claim;
+        const i64 r0 = (i64)b * BS;
+        T acc[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = 0;
+        const int nc = TRANS ? b : nblk - 1 - b;  // finished blocks this one depends on
+        for (int d = 0; d < nc; ++d) {
+            const int c = TRANS ? d : nblk - 1 - d;
+            const i64 c0 = (i64)c * BS;
+            // this tile of R does not depend on the flag: load it first
+            T rv[16];
+            if (!TRANS) {
+                const T *Rc = R + r0 + lane + (c0 + g * 16) * lda;  // R[r0 + lane][c0 + g*16 + q]
+#pragma unroll
+                for (int q = 0; q < 16; ++q) rv[q] = Rc[q * lda];
+            } else {
+                const T *Rc = R + c0 + lane + (r0 + g * 16) * lda;  // R[c0 + lane][r0 + g*16 + q]
+#pragma unroll
+                for (int q = 0; q < 16; ++q) rv[q] = Rc[q * lda];
+            }
+            if (t == 0) {
+                sh_stop = 0;
+                if (ld_sc1(&flags[c]) == 0) {
+                    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+                    while (ld_sc1(&flags[c]) == 0) {
+                        if (ld_sc1(err)) { sh_stop = 1; break; }
+                        if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                            st_sc1(err, 1);
+                            sh_stop = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+            }
+            __syncthreads();
+            if (sh_stop) break;
+            if (!TRANS) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[0] += rv[q] * ld_sc1(&x[c0 + g * 16 + q]);
+            } else {
+                const T xv = ld_sc1(&x[c0 + lane]);
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[q] += rv[q] * xv;
+            }
+        }
+        // the diagonal block's inverse (also independent of the flags)
+        const T *D = Dinv + (i64)b * BS * BS;
+        T dv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)  // !TRANS: D[lane][g*16 + q];  TRANS: D[g*16 + q][lane] read as (D^T)[lane][..]
+            dv[q] = TRANS ? D[(g * 16 + q) + (i64)lane * BS] : D[lane + (i64)(g * 16 + q) * BS];
+        if (!TRANS) {
+            red[g][lane] = acc[0];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const T v = wave_sum(acc[q]);
+                if (lane == 0) red[0][g * 16 + q] = v;
+            }
+        }
+        __syncthreads();
+        if (g == 0) {
+            // y = v_b - sum of the finished blocks' products (v_b not yet overwritten)
+            const T sum = TRANS ? red[0][lane] : (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+            ys[lane] = x[r0 + lane] - sum;
+        }
+        __syncthreads();
+        // x_b = inv(R_bb) y  (or inv(R_bb)^T y): row lane, columns g*16 .. g*16+15, summed over g
+        T p = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) p += dv[q] * ys[g * 16 + q];
+        red[g][lane] = p;
+        __syncthreads();
+        if (g == 0) {
+            st_sc1(&x[r0 + lane], (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+        __syncthreads();
+        if (t == 0) st_sc1(&flags[b], 1);
+    }
+}
+template <typename T> __global__ void trsv_reset_kernel(int nblk, int *counter, int *flags) {
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) flags[i] = 0;
+    if (threadIdx.x == 0) *counter = 0;
+}
+// x := R^-1 x or R^-T x (R upper n x n at R, lda; Dinv from diag_block_inv_kernel); sync =
+// the counter, an error flag and nblk flags (device ints, n/64 + 2 of them)
+template <typename T>
+static hipError_t trsv_upper(bool trans, int n, const T *R, i64 lda, const T *Dinv, T *x, int *sync, hipStream_t s) {
+    const int nblk = n / TRSV_BS;
+    trsv_reset_kernel<T><<<1, 256, 0, s>>>(nblk, sync, sync + 2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int dev = 0, ncu = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    const int grid = std::min(nblk, ncu);
+    if (trans) trsv_upper_kernel<T, true><<<grid, 256, 0, s>>>(n, R, lda, Dinv, x, sync, sync + 2, sync + 1);
+    else trsv_upper_kernel<T, false><<<grid, 256, 0, s>>>(n, R, lda, Dinv, x, sync, sync + 2, sync + 1);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
 // Least squares.  Vector kernels of the refinement (Float64 work for Float64 data).
 
 // part[s][i] = sum over column slice s of A[i, j] p[j]  (two rows per thread, coalesced
@@ -263,22 +418,36 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     int st = 0;
     const size_t ne = (size_t)m * n + (size_t)ldt * n + (size_t)m * W + (size_t)m * (1 + nsplit) + 5 * (size_t)n;
     const size_t nd = (size_t)std::max(gm, gn) + 8;
-    char *base = (char *)scratch(SLOT_LS, sizeof(T) * ne + sizeof(double) * nd, s, &st);
+    const size_t nsync = (size_t)n / TRSV_BS + 2;
+    const size_t ndinv = n % TRSV_BS == 0 ? (size_t)n * TRSV_BS : 0;
+    char *base = (char *)scratch(SLOT_LS, sizeof(T) * (ne + ndinv) + sizeof(double) * nd + sizeof(int) * (nsync + 2), s, &st);
     if (!base) return st;
     T *AF = (T *)base, *Tm = AF + (size_t)m * n, *Cb = Tm + (size_t)ldt * n, *u = Cb + (size_t)m * W;
     T *part = u + m, *v = part + (size_t)m * nsplit, *w = v + n, *y = w + n, *t = y + n, *q = t + n;
     double *bsum = (double *)(q + n), *dnorm = bsum + nd - 8;
+    int *sync = (int *)(bsum + nd);
+    T *Dinv = (T *)(sync + nsync + (nsync & 1));  // 8-byte aligned
+    NLA_HIP(hipMemsetAsync(sync, 0, sizeof(int) * nsync, s));
+    // x := R^-1 x / R^-T x with R the upper triangle of the factor
+    auto rsolve = [&](bool trans, T *vec) -> hipError_t {
+        if (n % TRSV_BS == 0) return trsv_upper<T>(trans, (int)n, AF, m, Dinv, vec, sync, s);
+        return rectrxm_async<T>('L', 'U', trans ? 'T' : 'N', T(1), 'S', n, 1, AF, m, vec, n, 0, s);
+    };
     NLA_HIP(hipMemcpy2DAsync(AF, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, s));
     NLA_HIP(hipMemsetAsync(Tm, 0, sizeof(T) * (size_t)ldt * n, s));
     int rc;
     if (prec != 'D') rc = tile_geqrf_mixed_entry(m, n, nb, ib, (double *)AF, m, (double *)Tm, ldt, prec, s);
     else rc = tile_geqrf_entry<T>(m, n, nb, ib, AF, m, Tm, ldt, s);
     if (rc) return rc;
+    if (n % TRSV_BS == 0) {
+        diag_block_inv_kernel<T><<<(unsigned)(n / TRSV_BS), TRSV_BS, 0, s>>>(AF, m, Dinv);
+        NLA_LAUNCH_CHECK();
+    }
     // x0 = R^-1 (Q^T b)[0:n]
     NLA_HIP(hipMemsetAsync(Cb, 0, sizeof(T) * (size_t)m * W, s));
     NLA_HIP(hipMemcpyAsync(Cb, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
     if ((rc = tile_ormqr_entry<T>('T', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s))) return rc;
-    NLA_HIP(rectrxm_async<T>('L', 'U', 'N', T(1), 'S', n, 1, AF, m, Cb, m, 0, s));
+    NLA_HIP(rsolve(false, Cb));
     NLA_HIP(hipMemcpyAsync(x, Cb, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
     if (maxit == 0) {
         NLA_HIP(hipStreamSynchronize(s));
@@ -309,7 +478,7 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     NLA_LAUNCH_CHECK();
     gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, u, q);
     NLA_LAUNCH_CHECK();
-    NLA_HIP(rectrxm_async<T>('L', 'U', 'T', T(1), 'S', n, 1, AF, m, q, n, 0, s));
+    NLA_HIP(rsolve(true, q));
     NLA_HIP(hipMemsetAsync(v, 0, sizeof(T) * n, s));
     lincomb_kernel<T><<<gn, 256, 0, s>>>(n, q, T(0), v, bsum);
     NLA_LAUNCH_CHECK();
@@ -326,7 +495,7 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     for (i64 it = 0; it < maxit; ++it) {
         // beta u = M v - alpha u = A (R^-1 v) - alpha u
         NLA_HIP(hipMemcpyAsync(t, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-        NLA_HIP(rectrxm_async<T>('L', 'U', 'N', T(1), 'S', n, 1, AF, m, t, n, 0, s));
+        NLA_HIP(rsolve(false, t));
         gemv_n_part_kernel<T><<<gpart, 256, 0, s>>>(m, n, A, lda, t, part, nsplit);
         NLA_LAUNCH_CHECK();
         gemv_n_finish_kernel<T><<<gm, 256, 0, s>>>(m, nsplit, part, T(alpha), u, bsum);
@@ -339,7 +508,7 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
             // alpha v = M^T u - beta v = R^-T (A^T u) - beta v
             gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, u, q);
             NLA_LAUNCH_CHECK();
-            NLA_HIP(rectrxm_async<T>('L', 'U', 'T', T(1), 'S', n, 1, AF, m, q, n, 0, s));
+            NLA_HIP(rsolve(true, q));
             lincomb_kernel<T><<<gn, 256, 0, s>>>(n, q, T(beta), v, bsum);
             NLA_LAUNCH_CHECK();
             NLA_HIP(norm_read(gn, &alpha));
@@ -365,10 +534,16 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
         if (test2 <= tol || phibar <= tol * r0norm || alpha == 0 || beta == 0) break;
     }
     // x = x0 + R^-1 y
-    NLA_HIP(rectrxm_async<T>('L', 'U', 'N', T(1), 'S', n, 1, AF, m, y, n, 0, s));
+    NLA_HIP(rsolve(false, y));
     axpy1_kernel<T><<<gn, 256, 0, s>>>(n, y, x);
     NLA_LAUNCH_CHECK();
+    int herr = 0;
+    NLA_HIP(hipMemcpyAsync(&herr, sync + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     NLA_HIP(hipStreamSynchronize(s));
+    if (herr) {
+        set_error("tile_gels: a triangular-solve dependency wait exceeded its time limit (results are invalid)");
+        return 1;
+    }
     return 0;
 }
 
