@@ -28,7 +28,7 @@ for p in (os.path.join(REPO, "nextla.jl_amd"), os.path.join(REPO, "oracle")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-PEAK = {"f64": 78.6, "f32": 157.3}   # TFLOP/s, MI355X dense (vector = matrix for these types)
+PEAK = {"f64": 78.6, "f32": 157.3, "f16": 2516.6, "bf16": 2516.6}   # TFLOP/s, MI355X dense MFMA
 
 
 def log(*a):
@@ -91,8 +91,9 @@ def cpu_baseline(n, cut, target_s=12.0):
 
 def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
     """Mixed-precision tile QR + least squares: A = U diag(s) V^T (U, V from QR of Gaussian,
-    s geometric from 1 to 1/kappa), b = A x0 + 1e-8 N(0,1); Float64 vs Float32-update
-    factorization, both refined on the augmented system to the Float64 solution."""
+    s geometric from 1 to 1/kappa), b = A x0 + 1e-8 N(0,1); tile QR with Float64 / Float32 /
+    FP16 / BF16 trailing updates, each refined by
+    R-preconditioned LSQR (products by A in Float64) towards the Float64 solution."""
     out = {"m": m, "n": n}
     for kappa in (1e2, 1e6):
         U = torch.linalg.qr(torch.randn((m, n), generator=gen, device="cuda", dtype=torch.float64))[0]
@@ -103,7 +104,7 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
         x0 = torch.randn(n, generator=gen, device="cuda", dtype=torch.float64)
         b = A @ x0 + 1e-8 * torch.randn(m, generator=gen, device="cuda", dtype=torch.float64)
         row = {}
-        for prec in ("D", "S"):
+        for prec in ("D", "S", "H", "B"):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             x, it, last = nl.tile_gels(A, b, nb, ib, prec, maxit=200, tol=1e-15)
@@ -114,11 +115,13 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
             eta = (torch.linalg.norm(A.T @ r) / torch.linalg.norm(r)).item()
             row[prec] = {"s": round(dt, 3), "iterations": it, "eta": eta, "x": x}
             log(f"[bench] LS kappa={kappa:g} prec {prec}: {dt:.3f} s, {it} steps, eta {eta}")
-        d = (torch.linalg.norm(row["S"]["x"] - row["D"]["x"]) / torch.linalg.norm(row["D"]["x"])).item()
+        xd = row["D"]["x"]
+        for p in ("S", "H", "B"):
+            row[p]["rel_diff_vs_f64_solution"] = (torch.linalg.norm(row[p]["x"] - xd) / torch.linalg.norm(xd)).item()
+            row[p]["vs_f64_time"] = round(row["D"]["s"] / row[p]["s"], 3)
         for v in row.values():
             v.pop("x")
-        out[f"kappa_{kappa:g}"] = {"f64": row["D"], "mixed_f32": row["S"], "mixed_vs_f64_time": round(row["D"]["s"] / row["S"]["s"], 3),
-                                  "rel_diff_mixed_vs_f64_solution": d}
+        out[f"kappa_{kappa:g}"] = {"f64": row["D"], "mixed_f32": row["S"], "mixed_f16": row["H"], "mixed_bf16": row["B"]}
         del A, b
     return out
 
@@ -339,7 +342,13 @@ def main():
         Qw = nl.colmajor(torch.empty_like(Q0))
         qflops = 2.0 * qn ** 3 - 2.0 / 3.0 * qn ** 3
         res = {}
-        for tag, fn in (("f64", lambda: nl.tile_geqrf(Qw, nb, ib)), ("mixed_f32", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "S"))):
+        # the panels (geqrt/tsqrt, Float64 in every variant) are about nb/n of the flops: n^2 nb
+        panel_share = nb / qn
+        variants = (("f64", "f64", lambda: nl.tile_geqrf(Qw, nb, ib)),
+                    ("mixed_f32", "f32", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "S")),
+                    ("mixed_f16", "f16", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "H")),
+                    ("mixed_bf16", "bf16", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "B")))
+        for tag, upd, fn in variants:
             holder = {}
 
             def qr():
@@ -354,11 +363,17 @@ def main():
                 nl.tile_ormqr("N", Qw, holder["T"], QR, nb, ib)
                 be = (torch.linalg.norm(QR - Q0) / torch.linalg.norm(Q0)).item()
                 del QR
-            res[tag] = {"ms": round(tq * 1e3, 2), "tflops": round(qflops / tq / 1e12, 3),
-                        "frac_of_peak": round(qflops / tq / 1e12 / PEAK["f64"], 4), "backward_error": be}
-            log(f"[bench] tile QR {tag} n={qn}: {tq*1e3:.1f} ms, {qflops/tq/1e12:.2f} TFLOP/s, berr {be}")
-        extras["tile_qr"] = {"n": qn, "nb": nb, "ib": ib, **res,
-                             "mixed_vs_f64": round(res["f64"]["ms"] / res["mixed_f32"]["ms"], 3)}
+            tf = qflops / tq / 1e12
+            # flop-weighted peak: trailing updates at the update dtype's peak, panels at FP64's
+            blend = 1.0 / ((1 - panel_share) / PEAK[upd] + panel_share / PEAK["f64"])
+            res[tag] = {"ms": round(tq * 1e3, 2), "tflops": round(tf, 3), "update_dtype": upd,
+                        "frac_of_f64_peak": round(tf / PEAK["f64"], 4),
+                        "frac_of_blended_peak": round(tf / blend, 4), "blended_peak_tflops": round(blend, 1),
+                        "backward_error": be}
+            if tag != "f64":
+                res[tag]["vs_f64"] = round(res["f64"]["ms"] / res[tag]["ms"], 3)
+            log(f"[bench] tile QR {tag} n={qn}: {tq*1e3:.1f} ms, {tf:.2f} TFLOP/s, berr {be}")
+        extras["tile_qr"] = {"n": qn, "nb": nb, "ib": ib, "flops": "2n^3 - 2n^3/3 (LAPACK count)", **res}
         del Q0, Qw
 
     if "batched" in chosen:
