@@ -181,10 +181,15 @@ __global__ __launch_bounds__(64) void diag_inv_kernel(int n, const T *__restrict
 
 // One base block (n <= 1024 rows of op(A) starting at the pointer A; B likewise), BCW
 // right-hand sides per workgroup.  Dinv points at this block's first inverted sub-block.
+// blockIdx.y selects one of several equal base blocks: A, B and Dinv advance by the strides.
 template <typename T, bool UP, bool TRA, bool TRB>
 __global__ __launch_bounds__(256) void trsm_blk_kernel(int n, int m, const T *__restrict__ A, i64 lda,
-                                                       T *__restrict__ B, i64 ldb, const T *__restrict__ Dinv) {
+                                                       T *__restrict__ B, i64 ldb, const T *__restrict__ Dinv,
+                                                       i64 a_step, i64 b_step, i64 d_step) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    A += blockIdx.y * a_step;
+    B += blockIdx.y * b_step;
+    Dinv += blockIdx.y * d_step;
     const int npad = (n + DB - 1) / DB * DB;
     T *Xs = reinterpret_cast<T *>(smem_raw);   // [npad][BCW + 1]
     T *P = Xs + (size_t)npad * (BCW + 1);       // [4 waves][2 tiles][16][17]
@@ -365,14 +370,16 @@ static hipError_t diag_inv_launch(bool up, bool trA, int n, const T *A, i64 lda,
 }
 
 template <typename T, bool U, bool TA, bool TB>
-static hipError_t blk_launch_one(int n, int m, const T *A, i64 lda, T *B, i64 ldb, const T *Dinv, hipStream_t s) {
+static hipError_t blk_launch_one(int n, int m, const T *A, i64 lda, T *B, i64 ldb, const T *Dinv, hipStream_t s,
+                                 int batch = 1, i64 a_step = 0, i64 b_step = 0, i64 d_step = 0) {
     const size_t sh = blk_lds_bytes<T>(n);
     if (sh > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&trsm_blk_kernel<T, U, TA, TB>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
         if (e != hipSuccess) return e;
     }
-    trsm_blk_kernel<T, U, TA, TB><<<(m + BCW - 1) / BCW, 256, sh, s>>>(n, m, A, lda, B, ldb, Dinv);
+    const dim3 grid((unsigned)((m + BCW - 1) / BCW), (unsigned)batch);
+    trsm_blk_kernel<T, U, TA, TB><<<grid, 256, sh, s>>>(n, m, A, lda, B, ldb, Dinv, a_step, b_step, d_step);
     return hipGetLastError();
 }
 
@@ -391,6 +398,146 @@ static hipError_t blk_launch(bool up, bool trA, bool trB, int n, int m, const T 
     return blk_launch_one<T, true, true, true>(n, m, A, lda, B, ldb, Dinv, s);
 }
 
+// ---------------------------------------------------------------------------------
+// Base blocks of IBS rows by an explicit inverse.  The blocked substitution above runs its
+// IBS/32 steps one after another inside each workgroup (barriers, a growing K loop, the
+// next step's loads behind the current one); with the dense inverse Inv of the block's
+// triangle, formed once per call for every aligned IBS-block on the diagonal, the base solve
+// is the triangular product X = Inv B with no sequential dependence.  Inverted diagonal
+// blocks of this size are the usual GPU TRSM scheme; the error of applying Inv instead of
+// substituting is of the order cond(A_bb) u, like the substitution's forward error.
+constexpr int IBS = 256;
+
+// X = Inv B for one base block of IBS rows, in place: a workgroup owns BCW right-hand sides
+// and every row (it reads them all into LDS before writing any).  Inv is dense column-major
+// (ld IBS), zero outside its lower (UP false) or upper triangle.  Wave w computes the output
+// row tiles {w, 15-w, 4+w, 11-w}: 34 K steps of 16 per wave, equal for every wave.
+template <typename T, bool UP, bool TRB>
+__global__ __launch_bounds__(256) void trsm_inv_apply_kernel(int m, const T *__restrict__ Inv, T *__restrict__ B,
+                                                             i64 ldb) {
+    __shared__ T Xs[IBS][BCW + 1];
+    const int c0 = blockIdx.x * BCW, cw = min(BCW, m - c0);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, lr = lane & 15, lk = lane >> 4;
+    for (int idx = t; idx < IBS * BCW; idx += 256) {
+        int r, c;
+        if (TRB) { c = idx % BCW; r = idx / BCW; } else { r = idx % IBS; c = idx / IBS; }
+        T v = 0;
+        if (c < cw) v = TRB ? B[(c0 + c) + (i64)r * ldb] : B[r + (i64)(c0 + c) * ldb];
+        Xs[r][c] = v;
+    }
+    __syncthreads();
+    // Wave w owns the tile pairs {w, 15-w} and {4+w, 11-w}.  A pair's two tiles need 17 K
+    // steps of 16 between them (tile x: first step f(x), count c(x); lower f = 0, c = x + 1,
+    // upper f = x, c = 16 - x; c(x) + c(15 - x) = 17), so each pair is one fixed 17-step
+    // sequence: the first c(a) steps accumulate tile a, the rest tile b, the switch done with
+    // selects.  No branches: the loads of step v + PF are issued while step v multiplies
+    // (PF = 3), with exact wait counts, and the two pairs' accumulators alternate.
+    constexpr int NT = IBS / 16, PF = 3, NS = NT + 1;
+    static_assert(NT == 16, "the pairing assumes 16 row tiles");
+    int ta[2], tb[2], fa[2], ca[2], fb[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        ta[p] = p == 0 ? w : 4 + w;
+        tb[p] = NT - 1 - ta[p];
+        fa[p] = UP ? ta[p] : 0;
+        ca[p] = UP ? NT - ta[p] : ta[p] + 1;
+        fb[p] = UP ? tb[p] : 0;
+    }
+    // row tile and K step of virtual step v of pair p
+    auto tile_of = [&](int p, int v) { return v < ca[p] ? ta[p] : tb[p]; };
+    auto step_of = [&](int p, int v) { return v < ca[p] ? fa[p] + v : fb[p] + v - ca[p]; };
+    typename Acc<T>::t acc[2], done[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) acc[p] = done[p] = acc_zero<T>();
+    T a[PF][2][4];
+    auto load_step = [&](int v, T (&dst)[2][4]) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const T *Ap = Inv + (16 * tile_of(p, v) + lr) + (i64)(16 * step_of(p, v) + lk) * IBS;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) dst[p][u] = Ap[(i64)(4 * u) * IBS];
+        }
+    };
+#pragma unroll
+    for (int v = 0; v < PF; ++v) load_step(v, a[v]);
+#pragma unroll
+    for (int v = 0; v < NS; ++v) {
+        T cur[2][4];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cur[p][u] = a[v % PF][p][u];
+        if (v + PF < NS) load_step(v + PF, a[v % PF]);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            // entering tile b: keep tile a's sum, restart the accumulator
+            const bool sw = v == ca[p];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                done[p][r] = sw ? acc[p][r] : done[p][r];
+                acc[p][r] = sw ? T(0) : acc[p][r];
+            }
+        }
+        int krow[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) krow[p] = 16 * step_of(p, v) + lk;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) mfma16(acc[p], cur[p][u], Xs[krow[p] + 4 * u][lr]);
+    }
+    // tile a's result is in done (the switch always happens: c(a) <= 16 < 17), tile b's in acc
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int it = h == 0 ? ta[p] : tb[p];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * it + acc_row<T>(lk, r);
+                const T val = h == 0 ? done[p][r] : acc[p][r];
+                if (lr < cw) {
+                    if (TRB) B[(c0 + lr) + (i64)row * ldb] = val;
+                    else B[row + (i64)(c0 + lr) * ldb] = val;
+                }
+            }
+        }
+}
+
+template <typename T> __global__ void identity_blocks_kernel(int nblocks, T *Inv) {
+    const i64 total = (i64)nblocks * IBS * IBS;
+    for (i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x; idx < total; idx += (i64)gridDim.x * blockDim.x) {
+        const i64 e = idx % ((i64)IBS * IBS);
+        Inv[idx] = (e % IBS) == (e / IBS) ? T(1) : T(0);
+    }
+}
+
+// Inverses of the nblocks aligned IBS x IBS diagonal blocks of the left-form matrix op'(A)
+// (flags as in base_launch), by the blocked solve of each block against the identity, all
+// blocks in one launch.  Dinv = the 32 x 32 inverses of the whole diagonal.
+template <typename T>
+static hipError_t inv_blocks(bool up, bool trA, int nblocks, const T *A, i64 lda, const T *Dinv, T *Inv, hipStream_t s) {
+    identity_blocks_kernel<T><<<1024, 256, 0, s>>>(nblocks, Inv);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const i64 a_step = (i64)IBS * (1 + lda), b_step = (i64)IBS * IBS, d_step = (i64)(IBS / DB) * DB * DB;
+    if (!up && !trA) return blk_launch_one<T, false, false, false>(IBS, IBS, A, lda, Inv, IBS, Dinv, s, nblocks, a_step, b_step, d_step);
+    if (!up && trA) return blk_launch_one<T, false, true, false>(IBS, IBS, A, lda, Inv, IBS, Dinv, s, nblocks, a_step, b_step, d_step);
+    if (up && !trA) return blk_launch_one<T, true, false, false>(IBS, IBS, A, lda, Inv, IBS, Dinv, s, nblocks, a_step, b_step, d_step);
+    return blk_launch_one<T, true, true, false>(IBS, IBS, A, lda, Inv, IBS, Dinv, s, nblocks, a_step, b_step, d_step);
+}
+
+template <typename T>
+static hipError_t inv_apply_launch(bool up, bool trB, int m, const T *Inv, T *B, i64 ldb, hipStream_t s) {
+    ProfScope prof(PK_TRSM_BASE, s);
+    const dim3 grid((unsigned)((m + BCW - 1) / BCW));
+    if (!up && !trB) trsm_inv_apply_kernel<T, false, false><<<grid, 256, 0, s>>>(m, Inv, B, ldb);
+    else if (!up && trB) trsm_inv_apply_kernel<T, false, true><<<grid, 256, 0, s>>>(m, Inv, B, ldb);
+    else if (up && !trB) trsm_inv_apply_kernel<T, true, false><<<grid, 256, 0, s>>>(m, Inv, B, ldb);
+    else trsm_inv_apply_kernel<T, true, true><<<grid, 256, 0, s>>>(m, Inv, B, ldb);
+    return hipGetLastError();
+}
+
 // Left-form flags of the base call for (side, uplo) of op(A), op(A)(i,j) = tr ? A(j,i) : A(i,j):
 // X op(A) = B  <=>  op(A)^T X^T = B^T.
 static inline void left_form(char side, char uplo, bool tr, bool *up, bool *trA, bool *trB) {
@@ -402,9 +549,11 @@ static inline void left_form(char side, char uplo, bool tr, bool *up, bool *trA,
 // whole left-form matrix; diag_off is this block's offset along the diagonal.
 template <typename T>
 static hipError_t base_case(char func, char side, char uplo, bool tr, int n, int m, const T *A, i64 lda, T *B, i64 ldb,
-                            const T *Dinv, i64 diag_off, hipStream_t s) {
+                            const T *Dinv, const T *Inv, i64 diag_off, hipStream_t s) {
     bool up, trA, trB;
     left_form(side, uplo, tr, &up, &trA, &trB);
+    if (func == 'S' && Inv && n == IBS && (diag_off % IBS) == 0)
+        return inv_apply_launch<T>(up, trB, m, Inv + (diag_off / IBS) * (i64)IBS * IBS, B, ldb, s);
     if (func == 'S' && Dinv && (diag_off % DB) == 0 && blk_ok<T>(n))
         return blk_launch<T>(up, trA, trB, n, m, A, lda, B, ldb, Dinv + (diag_off / DB) * DB * DB, s);
     return base_launch<T>(func, up, trA, trB, n, m, A, lda, B, ldb, s);
@@ -418,8 +567,8 @@ template <typename T> static inline const T *opA_block(const T *A, i64 lda, bool
 // unified_rec: src/rectrxm.jl:101-198 (same split rule, same forward/backward orders).
 template <typename T>
 static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, const T *A, i64 lda, T *B, i64 ldb,
-                      i64 threshold, const T *Dinv, i64 off, hipStream_t s) {
-    if (n <= threshold) return base_case<T>(func, side, uplo, tr, (int)n, (int)m, A, lda, B, ldb, Dinv, off, s);
+                      i64 threshold, const T *Dinv, const T *Inv, i64 off, hipStream_t s) {
+    if (n <= threshold) return base_case<T>(func, side, uplo, tr, (int)n, (int)m, A, lda, B, ldb, Dinv, Inv, off, s);
     i64 mid;
     if ((n & (n - 1)) == 0) mid = n / 2;
     else { mid = 1; while (mid * 2 <= n) mid *= 2; }
@@ -435,7 +584,7 @@ static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, co
                          (side == 'L' && uplo == 'U' && func == 'M') || (side == 'R' && uplo == 'L' && func == 'M');
     hipError_t e;
     if (forward) {
-        if ((e = rec<T>(func, side, uplo, tr, mid, m, A11, lda, B1, ldb, threshold, Dinv, off, s)) != hipSuccess) return e;
+        if ((e = rec<T>(func, side, uplo, tr, mid, m, A11, lda, B1, ldb, threshold, Dinv, Inv, off, s)) != hipSuccess) return e;
         if (side == 'L') {
             if (func == 'S') e = gemm<T>(ta, 'N', rem, m, mid, T(-1), A21, lda, B1, ldb, T(1), B2, ldb, s);  // B2 -= A21 B1
             else e = gemm<T>(ta, 'N', mid, m, rem, T(1), A12, lda, B2, ldb, T(1), B1, ldb, s);              // B1 += A12 B2
@@ -444,9 +593,9 @@ static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, co
             else e = gemm<T>('N', ta, m, mid, rem, T(1), B2, ldb, A21, lda, T(1), B1, ldb, s);              // B1 += B2 A21
         }
         if (e != hipSuccess) return e;
-        return rec<T>(func, side, uplo, tr, rem, m, A22, lda, B2, ldb, threshold, Dinv, off + mid, s);
+        return rec<T>(func, side, uplo, tr, rem, m, A22, lda, B2, ldb, threshold, Dinv, Inv, off + mid, s);
     }
-    if ((e = rec<T>(func, side, uplo, tr, rem, m, A22, lda, B2, ldb, threshold, Dinv, off + mid, s)) != hipSuccess)
+    if ((e = rec<T>(func, side, uplo, tr, rem, m, A22, lda, B2, ldb, threshold, Dinv, Inv, off + mid, s)) != hipSuccess)
         return e;
     if (side == 'L') {
         if (func == 'S') e = gemm<T>(ta, 'N', mid, m, rem, T(-1), A12, lda, B2, ldb, T(1), B1, ldb, s);  // B1 -= A12 B2
@@ -456,7 +605,7 @@ static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, co
         else e = gemm<T>('N', ta, m, rem, mid, T(1), B1, ldb, A12, lda, T(1), B2, ldb, s);              // B2 += B1 A12
     }
     if (e != hipSuccess) return e;
-    return rec<T>(func, side, uplo, tr, mid, m, A11, lda, B1, ldb, threshold, Dinv, off, s);
+    return rec<T>(func, side, uplo, tr, mid, m, A11, lda, B1, ldb, threshold, Dinv, Inv, off, s);
 }
 
 template <typename T>
@@ -492,18 +641,26 @@ static int rectrxm_entry(char side, char uplo, char trans, T alpha, char func, i
     hipStream_t s = as_stream(stream);
     const i64 rows = side == 'L' ? n : m, cols = side == 'L' ? m : n;
     if (func == 'S') NLA_HIP(scale<T>(rows, cols, alpha, B, ldb, s));
-    const T *Dinv = nullptr;
+    const T *Dinv = nullptr, *Inv = nullptr;
     if (func == 'S' && thr >= DB) {
-        // every base block starts on the 32-grid (see diag_inv_kernel): invert all 32x32 blocks once
+        // every base block starts on the 32-grid (see diag_inv_kernel): invert all 32x32 blocks
+        // once; with base blocks of IBS rows (cut-off >= IBS) also form the IBS-block inverses
+        const i64 nd = (n + DB - 1) / DB * DB * DB;
+        const i64 ninv = (thr >= IBS && thr < 2 * IBS) ? n / IBS : 0;
         int st = 0;
-        T *D = (T *)scratch(SLOT_TRSM, sizeof(T) * (size_t)((n + DB - 1) / DB) * DB * DB, s, &st);
+        T *D = (T *)scratch(SLOT_TRSM, sizeof(T) * (size_t)(nd + ninv * IBS * IBS), s, &st);
         if (!D) return st;
         bool upf, trA, trB;
         left_form(side, uplo, tr, &upf, &trA, &trB);
         NLA_HIP(diag_inv_launch<T>(upf, trA, (int)n, A, lda, D, s));
         Dinv = D;
+        if (ninv > 0) {
+            ProfScope prof(PK_TRSM_BASE, s);
+            NLA_HIP(inv_blocks<T>(upf, trA, (int)ninv, A, lda, D, D + nd, s));
+            Inv = D + nd;
+        }
     }
-    NLA_HIP(rec<T>(func, side, uplo, tr, n, m, A, lda, B, ldb, thr, Dinv, 0, s));
+    NLA_HIP(rec<T>(func, side, uplo, tr, n, m, A, lda, B, ldb, thr, Dinv, Inv, 0, s));
     if (func == 'M') NLA_HIP(scale<T>(rows, cols, alpha, B, ldb, s));
     return 0;
 }
@@ -530,7 +687,7 @@ static int base_entry(char func, char side, char uplo, i64 n, i64 m, const T *A,
         NLA_HIP(diag_inv_launch<T>(upf, trA, (int)n, A, lda, D, s));
         Dinv = D;
     }
-    NLA_HIP(base_case<T>(func, side, uplo, false, (int)n, (int)m, A, lda, B, ldb, Dinv, 0, s));
+    NLA_HIP(base_case<T>(func, side, uplo, false, (int)n, (int)m, A, lda, B, ldb, Dinv, nullptr, 0, s));
     return 0;
 }
 
