@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 namespace nla {
@@ -627,6 +628,16 @@ struct DagArgs {
     i64 ldc;
     int qmode;                  // walkers apply Q (reverse order) instead of Q^T
 };
+// 16-bit walkers: per (panel tile, block) the V/T image the panel writes.  A device global,
+// not a DagArgs field: the Float64 kernel sits at its register limit and one more argument
+// makes it spill.  Set by the host before each launch (launches on one device are ordered).
+__device__ unsigned char *g_lpimg = nullptr;
+__device__ int g_lpkind = 0;  // the image format the panels write: 0 none, 1 FP16, 2 BF16
+
+// 16-bit walker support, defined with that walker below
+template <typename Tw, int NB> struct LdsLp;
+template <typename T, typename Tw, int NB> __device__ void write_lp_image(unsigned char *dst);
+__host__ __device__ inline i64 lp_tile_index(int k, int i, int mt);
 
 // Block stage b of panel k: ib-block b of geqrt(A_kk), then of tsqrt(A_kk, A_ik) for
 // i = k+1.., tile by tile.  Block b of tile i needs block b of tile i-1 (rows c0.. of R_kk,
@@ -650,6 +661,12 @@ __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 l
         T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
         if (i == k) geqrt_block<T, NB>(Akk, lda, Tik, ldt, c0);
         else tsqrt_block<T, NB>(Akk, lda, A + (i64)i * NB + (i64)k * NB * lda, lda, Tik, ldt, c0);
+        // the block's V and T for the 16-bit walkers (the two formats' images have one size)
+        if (const int lk = g_lpkind) {
+            unsigned char *dst = g_lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)LdsLp<_Float16, NB>::IMG_BYTES;
+            if (lk == 1) write_lp_image<T, _Float16, NB>(dst);
+            else write_lp_image<T, __bf16, NB>(dst);
+        }
         publish(mine, i + 1);
         if (b == NBLK - 1 && g.ptrace && threadIdx.x == 0)
             g.ptrace[(size_t)k * g.mt + i] = __builtin_amdgcn_s_memrealtime();
@@ -984,12 +1001,23 @@ template <typename Tw> __device__ __forceinline__ typename LpMfma<Tw>::v4 lp_cvt
 // LDS of the 16-bit walker, in Tw elements: per buffer V column-major (Vc[j * LDC + p]),
 // V row-major (Vr[p * LDR + j]) and T column-major (Tc[j * LDR + q] = T[q][j]); the Float64
 // strip staging columns follow the two buffers.
+// Each buffer is padded to whole KB: the panel writes exactly this image (IMG_BYTES) to HBM
+// once per (tile, block) and walkers copy it into a buffer with 1 KB LDS-DMA instructions.
 template <typename Tw, int NB> struct LdsLp {
     static constexpr int LDC = NB + 8, LDR = IB + 8;
     static constexpr int VC = 0, VR = VC + IB * LDC, TC = VR + NB * LDR, BUF = TC + IB * LDR;
-    static constexpr int SSTG_BYTES = 2 * BUF * (int)sizeof(Tw);
+    static constexpr int IMG_BYTES = (BUF * (int)sizeof(Tw) + 1023) / 1024 * 1024;
+    static constexpr int BUFS = IMG_BYTES / (int)sizeof(Tw);  // buffer stride in Tw elements
+    static constexpr int SSTG_BYTES = 2 * IMG_BYTES;
     static_assert(SSTG_BYTES % 16 == 0, "staging columns must start 16-byte aligned");
 };
+static_assert(LdsLp<_Float16, 128>::IMG_BYTES == LdsLp<__bf16, 128>::IMG_BYTES &&
+                  LdsLp<_Float16, 64>::IMG_BYTES == LdsLp<__bf16, 64>::IMG_BYTES,
+              "the FP16 and BF16 block images have one size");
+// position of panel tile (i, k), i >= k, among the factorization's panel tiles
+__host__ __device__ inline i64 lp_tile_index(int k, int i, int mt) {
+    return (i64)k * mt - (i64)k * (k - 1) / 2 + (i - k);
+}
 template <typename T, typename Tw, int NB> __device__ __forceinline__ T *stage_ptr_lp() {
     return reinterpret_cast<T *>(lds_base<unsigned char>() + LdsLp<Tw, NB>::SSTG_BYTES) +
            (threadIdx.x >> 6) * SW * Lds<T, NB>::LDS_S;
@@ -1003,26 +1031,48 @@ template <typename Tw> __device__ __forceinline__ Tw to_lp(double v) {
     asm volatile("" : "+v"(f));
     return (Tw)f;
 }
-template <typename T, typename Tw, int NB, bool GE>
-__device__ __forceinline__ void vstore_lp(const VPre<T, NB> &P, int buf, int j0) {
+// Panel side: the block just factored (V in Vs, T in Ts of the panel's LDS layout, V with the
+// unit diagonal and zeros above for a geqrt block) converted to Tw in the 16-bit walker's
+// buffer layout and written to dst (IMG_BYTES), two elements per coherent 32-bit store.
+template <typename T, typename Tw, int NB> __device__ void write_lp_image(unsigned char *dst) {
     using L = LdsLp<Tw, NB>;
-    Tw *sm = lds_base<Tw>() + buf * L::BUF;
+    constexpr int LDV = Lds<T, NB>::LDV;
+    const T *Vs = lds_base<T>() + Lds<T, NB>::V;
+    const T *Ts = lds_base<T>() + Lds<T, NB>::TT;
+    unsigned *d32 = reinterpret_cast<unsigned *>(dst);
+    for (int wd = threadIdx.x; wd < L::BUFS / 2; wd += NTH) {
+        unsigned short h[2];
 #pragma unroll
-    for (int e = 0; e < VPre<T, NB>::NV; ++e) {
-        const int idx = threadIdx.x + e * NTH, p = idx % NB, j = idx / NB;
-        T v = P.v[e];
-        if (GE) {
-            const int ja = j0 + j;
-            v = p < ja ? T(0) : (p == ja ? T(1) : v);
+        for (int q = 0; q < 2; ++q) {
+            const int e = 2 * wd + q;
+            T v = T(0);
+            if (e < L::VR) {
+                const int j = e / L::LDC, p = e % L::LDC;
+                if (p < NB) v = Vs[j * LDV + p];
+            } else if (e < L::TC) {
+                const int f = e - L::VR, p = f / L::LDR, j = f % L::LDR;
+                if (j < IB) v = Vs[j * LDV + p];
+            } else if (e < L::BUF) {
+                const int f = e - L::TC, j = f / L::LDR, r = f % L::LDR;
+                if (r < IB && r <= j) v = Ts[r * (IB + 1) + j];
+            }
+            h[q] = __builtin_bit_cast(unsigned short, to_lp<Tw>(v));
         }
-        const Tw h = to_lp<Tw>(v);
-        sm[L::VC + j * L::LDC + p] = h;
-        sm[L::VR + p * L::LDR + j] = h;
+        cst(&d32[wd], (unsigned)h[0] | ((unsigned)h[1] << 16));
     }
+}
+// Walker side: one block image into LDS buffer buf with 1 KB LDS-DMA instructions (waves
+// take every fourth KB); completion is tracked by vmcnt like any vector load.
+template <typename Tw, int NB> __device__ __forceinline__ void lp_img_load(const unsigned char *src, int buf) {
+    constexpr int BYTES = LdsLp<Tw, NB>::IMG_BYTES, NI = BYTES / 1024;
+    unsigned char *dst = lds_base<unsigned char>() + buf * BYTES;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int e = 0; e < VPre<T, NB>::NT; ++e) {
-        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
-        sm[L::TC + j * L::LDR + q] = q <= j ? to_lp<Tw>(P.t[e]) : (Tw)0.0f;
+    for (int c = 0; c < NI; c += WAVES) {
+        if (c + w < NI) {
+            auto *d = (__attribute__((address_space(3))) void *)(dst + (c + w) * 1024);
+            __builtin_amdgcn_global_load_lds(src + (c + w) * 1024 + lane * 16, d, 16, 0, /*sc1*/ 16);
+        }
     }
 }
 // H^T (the reference's trans 'C') of the staged block applied to the wave's strip:
@@ -1032,7 +1082,7 @@ __device__ __forceinline__ void apply_lp(WStrip<float, NB> &S, fx4 *a1, int buf)
     using L = LdsLp<Tw, NB>;
     using M = LpMfma<Tw>;
     typedef typename M::v4 v4;
-    const Tw *sm = lds_base<Tw>() + buf * L::BUF;
+    const Tw *sm = lds_base<Tw>() + buf * L::BUFS;
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
     fx4 w0[2] = {acc_zero<float>(), acc_zero<float>()};
 #pragma unroll
@@ -1070,6 +1120,7 @@ __device__ __forceinline__ void apply_lp(WStrip<float, NB> &S, fx4 *a1, int buf)
 template <typename T, typename Tw, int NB>
 __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
     constexpr int NBLK = NB / IB;
+    constexpr i64 IMG = LdsLp<Tw, NB>::IMG_BYTES;
     static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
     __shared__ int sh_ready;
     const int w = threadIdx.x >> 6;
@@ -1077,39 +1128,35 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
     const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
     int *mine = &g.strip_prog[k * g.nwalk + s];
     const int *pdone = &g.panel_prog[k * NBLK + NBLK - 1];
-    const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
-    const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
     T *Cm = reinterpret_cast<T *>(g.C);
     const i64 ldc = g.ldc;
     T *Akj = Cm + (i64)k * NB + col * ldc;
     T *stg = stage_ptr_lp<T, Tw, NB>();
+    unsigned char *const lpimg = g_lpimg;
+    auto img = [&](int i, int B) { return lpimg + (lp_tile_index(k, i, g.mt) * NBLK + B) * IMG; };
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     WStrip<float, NB> A1;
     tiles_load_cvt<float, T, NB / 16>(A1.a, Akj, ldc, 0);
-    VPre<T, NB> P;
-    vload<T, NB>(P, Akk, lda, Tkk, ldt);
-    vstore_lp<T, Tw, NB, true>(P, 0, 0);
+    // unmqr!('L','C') with geqrt(A_kk)'s blocks (their images carry the unit-lower V)
+    lp_img_load<Tw, NB>(img(k, 0), 0);
     bool have0 = false;
 #pragma unroll
     for (int B = 0; B < NBLK; ++B) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
         if (B + 1 < NBLK) {
-            vload<T, NB>(P, Akk + (i64)(B + 1) * IB * lda, lda, Tkk + (i64)(B + 1) * IB * ldt, ldt);
+            lp_img_load<Tw, NB>(img(k, B + 1), (B + 1) & 1);
         } else if (k + 1 < g.mt) {
             have0 = ready_ge(pdone, k + 2, &sh_ready);
-            if (have0) vload<T, NB>(P, Akk + NB, lda, Tkk + IB, ldt);
+            if (have0) lp_img_load<Tw, NB>(img(k + 1, 0), 0);
         }
-        __syncthreads();
         apply_lp<Tw, NB, true>(A1, nullptr, B & 1);
-        if (B + 1 < NBLK) vstore_lp<T, Tw, NB, true>(P, (B + 1) & 1, (B + 1) * IB);
-        else if (have0) vstore_lp<T, Tw, NB, false>(P, 0, 0);
     }
     WStrip<float, NB> S;
     bool haveS = false;
     int pending = 0;
     for (int i = k + 1; i < g.mt; ++i) {
-        const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
-        const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
         T *Aij = Cm + (i64)i * NB + col * ldc;
         if (pending && (!have0 || !haveS)) {
             publish(mine, pending);
@@ -1117,8 +1164,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         }
         if (!have0) {
             wait_ge(pdone, i + 1, g.err, sh_flag);
-            vload<T, NB>(P, Aik, lda, Tik, ldt);
-            vstore_lp<T, Tw, NB, false>(P, 0, 0);
+            lp_img_load<Tw, NB>(img(i, 0), 0);
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
@@ -1127,9 +1173,9 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         bool next0 = false;
 #pragma unroll
         for (int B = 0; B < NBLK; ++B) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this block's image, the strip, earlier stores
+            __syncthreads();
             if (B == 0) {
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the strip, the last row's stores
-                __syncthreads();
                 if (pending) {
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
@@ -1137,14 +1183,12 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
                 stage_to_regs<T, float, NB>(stg, S);
             }
             if (B + 1 < NBLK) {
-                vload<T, NB>(P, Aik + (i64)(B + 1) * IB * lda, lda, Tik + (i64)(B + 1) * IB * ldt, ldt);
+                lp_img_load<Tw, NB>(img(i, B + 1), (B + 1) & 1);
             } else if (i + 1 < g.mt) {
                 next0 = ready_ge(pdone, i + 2, &sh_ready);
-                if (next0) vload<T, NB>(P, Aik + NB, lda, Tik + IB, ldt);
+                if (next0) lp_img_load<Tw, NB>(img(i + 1, 0), 0);
             }
-            __syncthreads();
             apply_lp<Tw, NB, false>(S, &A1.a[2 * B], B & 1);
-            if (B + 1 < NBLK || next0) vstore_lp<T, Tw, NB, false>(P, (B + 1) & 1, 0);
         }
         have0 = next0;
         stage_store<T, float, NB>(stg, S, Aij, ldc);
@@ -1291,7 +1335,19 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
         ptr = tr + (size_t)ntasks * 4;
         NLA_HIP(hipMemsetAsync(tr, 0, sizeof(unsigned long long) * ((size_t)ntasks * 4 + (size_t)kt * mt), s));
     }
+    unsigned char *lpimg = nullptr;
+    if constexpr (sizeof(Tw) == 2) {
+        // every panel tile's NB/IB block images, written by the panels, read by the walkers
+        const size_t ntiles = (size_t)kt * mt - (size_t)kt * (kt - 1) / 2;
+        lpimg = (unsigned char *)scratch(SLOT_LPIMG, ntiles * nblk * (size_t)LdsLp<Tw, NB>::IMG_BYTES, s, &st);
+        if (!lpimg) return st;
+    }
     DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr, (void *)C, ldc, mode == DAG_APPLY_Q ? 1 : 0};
+    {
+        const int lpkind = sizeof(Tw) != 2 ? 0 : (std::is_same<Tw, _Float16>::value ? 1 : 2);
+        NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpimg), &lpimg, sizeof(lpimg), 0, hipMemcpyHostToDevice, s));
+        NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpkind), &lpkind, sizeof(lpkind), 0, hipMemcpyHostToDevice, s));
+    }
     unsigned long long *phase = nullptr;
     if (trace) {
         phase = (unsigned long long *)scratch(SLOT_W, 64 * sizeof(unsigned long long), s, &st);

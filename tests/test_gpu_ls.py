@@ -73,10 +73,15 @@ def test_tile_geqrf_mixed_backward_error(m, n, nb, prec):
     berr = relerr(nl.to_host(QR), A0)
     lo, hi = MIXED_BERR[prec]
     assert lo < berr < hi, berr
-    # and the factorization is not the plain Float64 one (the updates really are reduced)
+    # and the factorization is not the plain Float64 one (the updates really are reduced).
+    # R is compared with its rows' signs matched: a Householder step takes the sign of the
+    # entry it eliminates into, so rounding at the reduced precision may flip a whole row
+    # of R (and its reflector) where that entry is small, an equally valid factorization.
     dA2 = nl.to_device(A0)
     nl.tile_geqrf(dA2, nb, 32)
-    d = relerr(nl.to_host(dA), nl.to_host(dA2))
+    R1, R2 = np.triu(nl.to_host(dA)[:n, :]), np.triu(nl.to_host(dA2)[:n, :])
+    D = np.sign(np.diag(R1)) * np.sign(np.diag(R2))
+    d = relerr(D[:, None] * R1, R2)
     assert lo < d < 10 * hi, d
 
 
