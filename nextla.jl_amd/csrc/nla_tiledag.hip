@@ -24,7 +24,6 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
-#include <type_traits>
 #include <vector>
 
 namespace nla {
@@ -519,7 +518,7 @@ __device__ __forceinline__ void block_factor(int c0, T *Ablk, i64 lda) {
 
 // One ib-block of geqrt!(NB, NB, IB, A, T) (src/geqrt.jl:44-67): geqr2+larft of columns
 // c0..c0+IB-1, then larfb!('L','C') of that block on the trailing columns of the tile.
-template <typename T, int NB> __device__ void geqrt_block(T *A, i64 lda, T *Tm, i64 ldt, int c0) {
+template <typename T, int NB> __device__ __forceinline__ void geqrt_block(T *A, i64 lda, T *Tm, i64 ldt, int c0) {
     T *sm = lds_base<T>();
     constexpr int LDV = Lds<T, NB>::LDV;
     T *Vs = sm + Lds<T, NB>::V;
@@ -563,7 +562,7 @@ __device__ __forceinline__ void phase_stamp(int slot) {
 // columns c0..c0+IB-1 of [R; A2], then tsmqr!('L','C') of that block on the trailing columns
 // (A1 = rows c0..c0+IB-1 of R, A2).
 template <typename T, int NB>
-__device__ void tsqrt_block(T *R, i64 ldr, T *A2, i64 lda2, T *Tm, i64 ldt, int c0) {
+__device__ __forceinline__ void tsqrt_block(T *R, i64 ldr, T *A2, i64 lda2, T *Tm, i64 ldt, int c0) {
     T *sm = lds_base<T>();
     constexpr int LDV = Lds<T, NB>::LDV;
     T *Vs = sm + Lds<T, NB>::V;
@@ -632,7 +631,6 @@ struct DagArgs {
 // not a DagArgs field: the Float64 kernel sits at its register limit and one more argument
 // makes it spill.  Set by the host before each launch (launches on one device are ordered).
 __device__ unsigned char *g_lpimg = nullptr;
-__device__ int g_lpkind = 0;  // the image format the panels write: 0 none, 1 FP16, 2 BF16
 
 // 16-bit walker support, defined with that walker below
 template <typename Tw, int NB> struct LdsLp;
@@ -644,7 +642,14 @@ __host__ __device__ inline i64 lp_tile_index(int k, int i, int mt);
 // this task's own previous step) and block b-1 of tile i (the trailing update of A_ik's
 // columns c0.. and the factorization of R_kk's rows above), so the stages of one panel
 // run as a pipeline over the tiles.  Stage 0 waits for the previous step's walkers instead.
-template <typename T, int NB>
+// Tw: the walkers' type; a 16-bit Tw makes the panel also write each block's image for them.
+// The kernels whose walkers read V and T directly share one instance (PanelTw maps their Tw
+// to T): that one out-of-line panel compiles as it did before the 16-bit variants existed.
+template <typename T, typename Tw> struct PanelTwT { using type = T; };
+template <typename T> struct PanelTwT<T, _Float16> { using type = _Float16; };
+template <typename T> struct PanelTwT<T, __bf16> { using type = __bf16; };
+template <typename T, typename Tw> using PanelTw = typename PanelTwT<T, Tw>::type;
+template <typename T, int NB, typename Tw>
 __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
     constexpr int WPT = NB / TW, NBLK = NB / IB;  // walker groups per tile column, stages
     const int c0 = b * IB;
@@ -661,12 +666,9 @@ __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 l
         T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
         if (i == k) geqrt_block<T, NB>(Akk, lda, Tik, ldt, c0);
         else tsqrt_block<T, NB>(Akk, lda, A + (i64)i * NB + (i64)k * NB * lda, lda, Tik, ldt, c0);
-        // the block's V and T for the 16-bit walkers (the two formats' images have one size)
-        if (const int lk = g_lpkind) {
-            unsigned char *dst = g_lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)LdsLp<_Float16, NB>::IMG_BYTES;
-            if (lk == 1) write_lp_image<T, _Float16, NB>(dst);
-            else write_lp_image<T, __bf16, NB>(dst);
-        }
+        // the block's V and T for the 16-bit walkers
+        if constexpr (sizeof(Tw) == 2)
+            write_lp_image<T, Tw, NB>(g_lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)LdsLp<Tw, NB>::IMG_BYTES);
         publish(mine, i + 1);
         if (b == NBLK - 1 && g.ptrace && threadIdx.x == 0)
             g.ptrace[(size_t)k * g.mt + i] = __builtin_amdgcn_s_memrealtime();
@@ -1219,7 +1221,7 @@ __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i
             g.trace[(size_t)task * 4 + 0] = __builtin_amdgcn_s_memrealtime();
             g.trace[(size_t)task * 4 + 3] = blockIdx.x;
         }
-        if (task_type(code) == TASK_PANEL) run_panel<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
+        if (task_type(code) == TASK_PANEL) run_panel<T, NB, PanelTw<T, Tw>>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
         else if constexpr (sizeof(Tw) == sizeof(T)) {
             if (g.qmode) run_walk_q<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
             else run_walk<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
@@ -1343,11 +1345,8 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
         if (!lpimg) return st;
     }
     DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr, (void *)C, ldc, mode == DAG_APPLY_Q ? 1 : 0};
-    {
-        const int lpkind = sizeof(Tw) != 2 ? 0 : (std::is_same<Tw, _Float16>::value ? 1 : 2);
+    if constexpr (sizeof(Tw) == 2)
         NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpimg), &lpimg, sizeof(lpimg), 0, hipMemcpyHostToDevice, s));
-        NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpkind), &lpkind, sizeof(lpkind), 0, hipMemcpyHostToDevice, s));
-    }
     unsigned long long *phase = nullptr;
     if (trace) {
         phase = (unsigned long long *)scratch(SLOT_W, 64 * sizeof(unsigned long long), s, &st);
