@@ -19,7 +19,10 @@ namespace nla {
 template <typename T> struct GemmPad { static constexpr int v = 2; };
 
 // EDGE = false: every tile is interior (M % BM == N % BN == K % BK == 0): no bounds checks.
-template <typename T, int BM, int BN, int BK, bool TA, bool TB, bool EDGE>
+// NSET register sets for the slabs in flight: 1 fetches slab s+1 while slab s multiplies;
+// 2 fetches slab s+2, so a load has two slabs of products to arrive (small tiles, whose
+// slab is too short to cover a cache miss).
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, bool EDGE, int NSET = 1>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alpha, const T *__restrict__ A,
                                                       i64 lda, const T *__restrict__ B, i64 ldb, T beta,
                                                       T *__restrict__ C, i64 ldc, int tiles_m) {
@@ -61,34 +64,35 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
     const i64 a_step_k = TA ? BK : (i64)BK * lda;               // advance per K slab
     const i64 b_step_k = TB ? (i64)BK * ldb : BK;
 
-    T ra[LA], rb[LB];
-    auto load = [&](int k0) {
+    T ra[NSET][LA], rb[NSET][LB];
+    // slabs are loaded in order: the k0 argument only serves the edge checks
+    auto load = [&](int k0, int set) {
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             if (EDGE) {
                 const int gm = m0 + am + i * ASM, gk = k0 + ak + i * ASK;
-                ra[i] = (gm < M && gk < K) ? pa[i * a_step_i] : T(0);
+                ra[set][i] = (gm < M && gk < K) ? pa[i * a_step_i] : T(0);
             } else {
-                ra[i] = pa[i * a_step_i];
+                ra[set][i] = pa[i * a_step_i];
             }
         }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
             if (EDGE) {
                 const int gn = n0 + bn + i * BSN, gk = k0 + bk + i * BSK;
-                rb[i] = (gn < N && gk < K) ? pb[i * b_step_i] : T(0);
+                rb[set][i] = (gn < N && gk < K) ? pb[i * b_step_i] : T(0);
             } else {
-                rb[i] = pb[i * b_step_i];
+                rb[set][i] = pb[i * b_step_i];
             }
         }
         pa += a_step_k;
         pb += b_step_k;
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf, int set) {
 #pragma unroll
-        for (int i = 0; i < LA; ++i) As[buf][ak + i * ASK][am + i * ASM] = ra[i];
+        for (int i = 0; i < LA; ++i) As[buf][ak + i * ASK][am + i * ASM] = ra[set][i];
 #pragma unroll
-        for (int i = 0; i < LB; ++i) Bs[buf][bk + i * BSK][bn + i * BSN] = rb[i];
+        for (int i = 0; i < LB; ++i) Bs[buf][bk + i * BSK][bn + i * BSN] = rb[set][i];
     };
 
     typename Acc<T>::t acc[FM][FN];
@@ -98,13 +102,7 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_zero<T>();
 
     const int lr = lane & 15, lk = lane >> 4;
-    load(0);
-    store(0);
-    __syncthreads();
-    int cur = 0;
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        const bool more = k0 + BK < K;
-        if (more) load(k0 + BK);
+    auto multiply = [&](int cur) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             T a[FM], b[FN];
@@ -117,9 +115,37 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
 #pragma unroll
                 for (int j = 0; j < FN; ++j) mfma16(acc[i][j], b[j], a[i]);
         }
-        if (more) store(cur ^ 1);
+    };
+    load(0, 0);
+    store(0, 0);
+    if constexpr (NSET == 1) {
         __syncthreads();
-        cur ^= 1;
+        int cur = 0;
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            const bool more = k0 + BK < K;
+            if (more) load(k0 + BK, 0);
+            multiply(cur);
+            if (more) store(cur ^ 1, 0);
+            __syncthreads();
+            cur ^= 1;
+        }
+    } else {
+        // slab s multiplies from LDS buffer s&1; register set s&1 is free then and takes
+        // slab s+2, set (s+1)&1 holds slab s+1 (fetched one slab earlier) for the buffer
+        // the previous slab used
+        if (BK < K) load(BK, 1);
+        __syncthreads();
+        for (int k0 = 0; k0 < K; k0 += 2 * BK) {
+            if (k0 + 2 * BK < K) load(k0 + 2 * BK, 0);
+            multiply(0);
+            if (k0 + BK < K) store(1, 1);
+            __syncthreads();
+            if (k0 + BK >= K) break;
+            if (k0 + 3 * BK < K) load(k0 + 3 * BK, 1);
+            multiply(1);
+            if (k0 + 2 * BK < K) store(0, 0);
+            __syncthreads();
+        }
     }
 
     // acc[i][j] element r of lane l is C[m0 + wm*BM/2 + 16 i + (l%16), n0 + wn*BN/2 + 16 j + acc_row(l/16, r)]
@@ -143,24 +169,24 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
     }
 }
 
-template <typename T, int BM, int BN, int BK, bool EDGE>
+template <typename T, int BM, int BN, int BK, bool EDGE, int NSET>
 static hipError_t launch_tiles(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, i64 lda, const T *B,
                                i64 ldb, T beta, T *C, i64 ldc, hipStream_t s) {
     const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
     dim3 grid(tm * tn), block(256);
-    if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false, EDGE><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
-    else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false, EDGE><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
-    else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true, EDGE><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
-    else gemm_kernel<T, BM, BN, BK, true, true, EDGE><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false, EDGE, NSET><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false, EDGE, NSET><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true, EDGE, NSET><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else gemm_kernel<T, BM, BN, BK, true, true, EDGE, NSET><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
     return hipGetLastError();
 }
 
-template <typename T, int BM, int BN, int BK>
+template <typename T, int BM, int BN, int BK, int NSET = 1>
 static hipError_t launch_sized(bool ta, bool tb, int M, int N, int K, T alpha, const T *A, i64 lda, const T *B,
                                i64 ldb, T beta, T *C, i64 ldc, hipStream_t s) {
     if (M % BM == 0 && N % BN == 0 && K % BK == 0 && K > 0)
-        return launch_tiles<T, BM, BN, BK, false>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
-    return launch_tiles<T, BM, BN, BK, true>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+        return launch_tiles<T, BM, BN, BK, false, NSET>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch_tiles<T, BM, BN, BK, true, NSET>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 template <typename T>

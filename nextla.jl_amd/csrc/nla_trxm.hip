@@ -410,45 +410,29 @@ constexpr int IBS = 256;
 
 // X = Inv B for one base block of IBS rows, in place: a workgroup owns BCW right-hand sides
 // and every row (it reads them all into LDS before writing any).  Inv is dense column-major
-// (ld IBS), zero outside its lower (UP false) or upper triangle.  Wave w computes the output
-// row tiles {w, 15-w, 4+w, 11-w}: 34 K steps of 16 per wave, equal for every wave.
-template <typename T, bool UP, bool TRB>
-__global__ __launch_bounds__(256) void trsm_inv_apply_kernel(int m, const T *__restrict__ Inv, T *__restrict__ B,
-                                                             i64 ldb) {
-    __shared__ T Xs[IBS][BCW + 1];
-    const int c0 = blockIdx.x * BCW, cw = min(BCW, m - c0);
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, lr = lane & 15, lk = lane >> 4;
-    for (int idx = t; idx < IBS * BCW; idx += 256) {
-        int r, c;
-        if (TRB) { c = idx % BCW; r = idx / BCW; } else { r = idx % IBS; c = idx / IBS; }
-        T v = 0;
-        if (c < cw) v = TRB ? B[(c0 + c) + (i64)r * ldb] : B[r + (i64)(c0 + c) * ldb];
-        Xs[r][c] = v;
-    }
-    __syncthreads();
-    // Wave w owns the tile pairs {w, 15-w} and {4+w, 11-w}.  A pair's two tiles need 17 K
-    // steps of 16 between them (tile x: first step f(x), count c(x); lower f = 0, c = x + 1,
-    // upper f = x, c = 16 - x; c(x) + c(15 - x) = 17), so each pair is one fixed 17-step
-    // sequence: the first c(a) steps accumulate tile a, the rest tile b, the switch done with
-    // selects.  No branches: the loads of step v + PF are issued while step v multiplies
-    // (PF = 3), with exact wait counts, and the two pairs' accumulators alternate.
+// (ld IBS), zero outside its lower (UP false) or upper triangle.  Wave W computes the output
+// row tiles {W, 15-W} and {4+W, 11-W}: 34 K steps of 16 per wave, equal for every wave.
+//
+// A pair's two tiles need 17 K steps of 16 between them (tile x: first step f(x), count
+// c(x); lower f = 0, c = x + 1, upper f = x, c = 16 - x; c(x) + c(15 - x) = 17), so each pair
+// is one fixed 17-step sequence: the first c(a) steps accumulate tile a, the rest tile b.
+// W is a template parameter, so every tile and step index is a constant and each tile has
+// its own accumulator: the steps are straight-line loads and MFMAs (the loads of step v + PF
+// issued while step v multiplies), the two pairs' chains interleaved.
+template <typename T, bool UP, bool TRB, int W>
+__device__ __forceinline__ void inv_apply_wave(const T *__restrict__ Inv, T *__restrict__ B, i64 ldb,
+                                               const T (*Xs)[BCW + 1], int c0, int cw) {
     constexpr int NT = IBS / 16, PF = 3, NS = NT + 1;
     static_assert(NT == 16, "the pairing assumes 16 row tiles");
-    int ta[2], tb[2], fa[2], ca[2], fb[2];
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    constexpr int TA[2] = {W, 4 + W}, TB[2] = {NT - 1 - W, NT - 1 - (4 + W)};
+    constexpr int FA[2] = {UP ? TA[0] : 0, UP ? TA[1] : 0}, FB[2] = {UP ? TB[0] : 0, UP ? TB[1] : 0};
+    constexpr int CA[2] = {UP ? NT - TA[0] : TA[0] + 1, UP ? NT - TA[1] : TA[1] + 1};
+    auto tile_of = [&](int p, int v) { return v < CA[p] ? TA[p] : TB[p]; };
+    auto step_of = [&](int p, int v) { return v < CA[p] ? FA[p] + v : FB[p] + v - CA[p]; };
+    typename Acc<T>::t acc[2][2];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-        ta[p] = p == 0 ? w : 4 + w;
-        tb[p] = NT - 1 - ta[p];
-        fa[p] = UP ? ta[p] : 0;
-        ca[p] = UP ? NT - ta[p] : ta[p] + 1;
-        fb[p] = UP ? tb[p] : 0;
-    }
-    // row tile and K step of virtual step v of pair p
-    auto tile_of = [&](int p, int v) { return v < ca[p] ? ta[p] : tb[p]; };
-    auto step_of = [&](int p, int v) { return v < ca[p] ? fa[p] + v : fb[p] + v - ca[p]; };
-    typename Acc<T>::t acc[2], done[2];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) acc[p] = done[p] = acc_zero<T>();
+    for (int p = 0; p < 2; ++p) acc[p][0] = acc[p][1] = acc_zero<T>();
     T a[PF][2][4];
     auto load_step = [&](int v, T (&dst)[2][4]) {
 #pragma unroll
@@ -469,39 +453,48 @@ __global__ __launch_bounds__(256) void trsm_inv_apply_kernel(int m, const T *__r
             for (int u = 0; u < 4; ++u) cur[p][u] = a[v % PF][p][u];
         if (v + PF < NS) load_step(v + PF, a[v % PF]);
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            // entering tile b: keep tile a's sum, restart the accumulator
-            const bool sw = v == ca[p];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                done[p][r] = sw ? acc[p][r] : done[p][r];
-                acc[p][r] = sw ? T(0) : acc[p][r];
-            }
-        }
-        int krow[2];
-#pragma unroll
-        for (int p = 0; p < 2; ++p) krow[p] = 16 * step_of(p, v) + lk;
-#pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int p = 0; p < 2; ++p) mfma16(acc[p], cur[p][u], Xs[krow[p] + 4 * u][lr]);
+            for (int p = 0; p < 2; ++p)
+                mfma16(acc[p][v < CA[p] ? 0 : 1], cur[p][u], Xs[16 * step_of(p, v) + lk + 4 * u][lr]);
     }
-    // tile a's result is in done (the switch always happens: c(a) <= 16 < 17), tile b's in acc
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int it = h == 0 ? ta[p] : tb[p];
+            const int it = h == 0 ? TA[p] : TB[p];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * it + acc_row<T>(lk, r);
-                const T val = h == 0 ? done[p][r] : acc[p][r];
                 if (lr < cw) {
-                    if (TRB) B[(c0 + lr) + (i64)row * ldb] = val;
-                    else B[row + (i64)(c0 + lr) * ldb] = val;
+                    if (TRB) B[(c0 + lr) + (i64)row * ldb] = acc[p][h][r];
+                    else B[row + (i64)(c0 + lr) * ldb] = acc[p][h][r];
                 }
             }
         }
+}
+
+template <typename T, bool UP, bool TRB>
+__global__ __launch_bounds__(256) void trsm_inv_apply_kernel(int m, const T *__restrict__ Inv, T *__restrict__ B,
+                                                             i64 ldb) {
+    __shared__ T Xs[IBS][BCW + 1];
+    const int c0 = blockIdx.x * BCW, cw = min(BCW, m - c0);
+    const int t = threadIdx.x;
+    for (int idx = t; idx < IBS * BCW; idx += 256) {
+        int r, c;
+        if (TRB) { c = idx % BCW; r = idx / BCW; } else { r = idx % IBS; c = idx / IBS; }
+        T v = 0;
+        if (c < cw) v = TRB ? B[(c0 + c) + (i64)r * ldb] : B[r + (i64)(c0 + c) * ldb];
+        Xs[r][c] = v;
+    }
+    __syncthreads();
+    // the wave index is uniform across the wave: one scalar branch picks the wave's body
+    switch (__builtin_amdgcn_readfirstlane(t >> 6)) {
+    case 0: inv_apply_wave<T, UP, TRB, 0>(Inv, B, ldb, Xs, c0, cw); break;
+    case 1: inv_apply_wave<T, UP, TRB, 1>(Inv, B, ldb, Xs, c0, cw); break;
+    case 2: inv_apply_wave<T, UP, TRB, 2>(Inv, B, ldb, Xs, c0, cw); break;
+    default: inv_apply_wave<T, UP, TRB, 3>(Inv, B, ldb, Xs, c0, cw); break;
+    }
 }
 
 template <typename T> __global__ void identity_blocks_kernel(int nblocks, T *Inv) {
