@@ -331,15 +331,30 @@ __device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t 
     const T *Vs = sm + Lds<T, NB>::V;
     const T *Ts = sm + Lds<T, NB>::TT;
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    // Every MFMA's LDS operand is read two products ahead of its use (the reads of one
+    // product then overlap the previous two in the matrix pipe instead of waiting before it).
     typename Acc<T>::t w0[2] = {acc_zero<T>(), acc_zero<T>()};
+    {
+        auto vop = [&](int idx, int ot) {
+            const int rt = idx >> 2, r = idx & 3;
+            return Vs[(16 * ot + lr) * LDV + 16 * rt + acc_row<T>(lk, r)];
+        };
+        constexpr int N = RT * 4;
+        T va[3][2];
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
+        for (int d = 0; d < 2; ++d)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int p = 16 * rt + acc_row<T>(lk, r);
+            for (int ot = 0; ot < 2; ++ot) va[d][ot] = vop(d, ot);
 #pragma unroll
-            for (int ot = 0; ot < 2; ++ot) mfma16(w0[ot], Vs[(16 * ot + lr) * LDV + p], S.a[rt][r]);
+        for (int idx = 0; idx < N; ++idx) {
+            if (idx + 2 < N) {
+#pragma unroll
+                for (int ot = 0; ot < 2; ++ot) va[(idx + 2) % 3][ot] = vop(idx + 2, ot);
+            }
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) mfma16(w0[ot], va[idx % 3][ot], S.a[idx >> 2][idx & 3]);
         }
+    }
     if (!GE) {
 #pragma unroll
         for (int ot = 0; ot < 2; ++ot)
@@ -347,32 +362,50 @@ __device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t 
             for (int r = 0; r < 4; ++r) w0[ot][r] += a1[ot][r];
     }
     typename Acc<T>::t wv[2] = {acc_zero<T>(), acc_zero<T>()};
+    {
+        auto top = [&](int idx, int ot) {
+            const int q = 16 * (idx >> 2) + acc_row<T>(lk, idx & 3), j = 16 * ot + lr;
+            return notrans ? Ts[j * (IB + 1) + q] : Ts[q * (IB + 1) + j];
+        };
+        T ta[3][2];
 #pragma unroll
-    for (int tq = 0; tq < 2; ++tq)
+        for (int d = 0; d < 2; ++d)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int q = 16 * tq + acc_row<T>(lk, r);
+            for (int ot = 0; ot < 2; ++ot) ta[d][ot] = top(d, ot);
 #pragma unroll
-            for (int ot = 0; ot < 2; ++ot) {
-                const int j = 16 * ot + lr;
-                const T a = notrans ? Ts[j * (IB + 1) + q] : Ts[q * (IB + 1) + j];
-                mfma16(wv[ot], a, w0[tq][r]);
+        for (int idx = 0; idx < 8; ++idx) {
+            if (idx + 2 < 8) {
+#pragma unroll
+                for (int ot = 0; ot < 2; ++ot) ta[(idx + 2) % 3][ot] = top(idx + 2, ot);
             }
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) mfma16(wv[ot], ta[idx % 3][ot], w0[idx >> 2][idx & 3]);
         }
-    if (!GE) {
-#pragma unroll
-        for (int ot = 0; ot < 2; ++ot)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) a1[ot][r] -= wv[ot][r];
     }
+    // S -= V W as S += V (-W): W negated once (eight values) rather than every V operand
+    T wn[2][4];
 #pragma unroll
     for (int tq = 0; tq < 2; ++tq)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int q = 16 * tq + acc_row<T>(lk, r);
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) mfma16(S.a[rt], -Vs[q * LDV + 16 * rt + lr], wv[tq][r]);
+            if (!GE) a1[tq][r] -= wv[tq][r];
+            wn[tq][r] = -wv[tq][r];
         }
+    {
+        auto vop = [&](int idx) {
+            const int tq = idx / (4 * RT), r = (idx / RT) & 3, rt = idx % RT;
+            return Vs[(16 * tq + acc_row<T>(lk, r)) * LDV + 16 * rt + lr];
+        };
+        constexpr int N = 2 * 4 * RT;
+        T va[3];
+        va[0] = vop(0);
+        va[1] = vop(1);
+#pragma unroll
+        for (int idx = 0; idx < N; ++idx) {
+            if (idx + 2 < N) va[(idx + 2) % 3] = vop(idx + 2);
+            mfma16(S.a[idx % RT], va[idx % 3], wn[idx / (4 * RT)][(idx / RT) & 3]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------
