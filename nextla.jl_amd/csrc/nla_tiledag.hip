@@ -278,8 +278,9 @@ __device__ __forceinline__ void vstore(const VPre<T, NB> &P, int buf, int j0) {
 // global_load_lds (no registers; completion is tracked by vmcnt like any vector load).  Each
 // instruction moves 64 lanes x SZ bytes, a contiguous run of one column of Vs.  Column bases
 // must be 16-byte aligned when SZ is 16 (the caller checks the leading dimension).
+// Column jj (of this wave's IB / WAVES) of vload_lds below.
 template <typename T, int NB>
-__device__ __forceinline__ void vload_lds(const T *Vg, i64 ldv, int buf) {
+__device__ __forceinline__ void vload_lds_col(const T *Vg, i64 ldv, int buf, int jj) {
     constexpr int LDV = Lds<T, NB>::LDV;
     constexpr int CB = NB * (int)sizeof(T);
     constexpr int SZ = (CB % 1024 == 0) ? 16 : 4;
@@ -287,29 +288,32 @@ __device__ __forceinline__ void vload_lds(const T *Vg, i64 ldv, int buf) {
     static_assert(CB % CHUNK == 0, "column bytes must be a multiple of the chunk");
     T *Vs = lds_base<T>() + buf * Lds<T, NB>::BUF + Lds<T, NB>::V;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = w + WAVES * jj;
+    const char *src = reinterpret_cast<const char *>(Vg + (i64)j * ldv) + lane * SZ;
+    char *dst = reinterpret_cast<char *>(Vs + j * LDV);
 #pragma unroll
-    for (int jj = 0; jj < IB / WAVES; ++jj) {
-        const int j = w + WAVES * jj;
-        const char *src = reinterpret_cast<const char *>(Vg + (i64)j * ldv) + lane * SZ;
-        char *dst = reinterpret_cast<char *>(Vs + j * LDV);
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            auto *d = (__attribute__((address_space(3))) void *)(dst + c * CHUNK);
-            if constexpr (SZ == 16) __builtin_amdgcn_global_load_lds(src + c * CHUNK, d, 16, 0, /*sc1*/ 16);
-            else __builtin_amdgcn_global_load_lds(src + c * CHUNK, d, 4, 0, /*sc1*/ 16);
-        }
+    for (int c = 0; c < NCH; ++c) {
+        auto *d = (__attribute__((address_space(3))) void *)(dst + c * CHUNK);
+        if constexpr (SZ == 16) __builtin_amdgcn_global_load_lds(src + c * CHUNK, d, 16, 0, /*sc1*/ 16);
+        else __builtin_amdgcn_global_load_lds(src + c * CHUNK, d, 4, 0, /*sc1*/ 16);
     }
+}
+template <typename T, int NB>
+__device__ __forceinline__ void vload_lds(const T *Vg, i64 ldv, int buf) {
+#pragma unroll
+    for (int jj = 0; jj < IB / WAVES; ++jj) vload_lds_col<T, NB>(Vg, ldv, buf, jj);
 }
 template <typename T, int NB> struct TPre {
     static constexpr int NT = IB * IB / NTH;
     T t[NT];
 };
+template <typename T, int NB> __device__ __forceinline__ void tload_one(TPre<T, NB> &P, const T *Tg, i64 ldt, int e) {
+    const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
+    P.t[e] = cld(&Tg[q + (i64)j * ldt]);
+}
 template <typename T, int NB> __device__ __forceinline__ void tload(TPre<T, NB> &P, const T *Tg, i64 ldt) {
 #pragma unroll
-    for (int e = 0; e < TPre<T, NB>::NT; ++e) {
-        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
-        P.t[e] = cld(&Tg[q + (i64)j * ldt]);
-    }
+    for (int e = 0; e < TPre<T, NB>::NT; ++e) tload_one<T, NB>(P, Tg, ldt, e);
 }
 template <typename T, int NB> __device__ __forceinline__ void tstore(const TPre<T, NB> &P, int buf) {
     T *Ts = lds_base<T>() + buf * Lds<T, NB>::BUF + Lds<T, NB>::TT;
@@ -323,8 +327,15 @@ template <typename T, int NB> __device__ __forceinline__ void tstore(const TPre<
 // op(H) strip with H = I - V T V^T of the staged block (one wave, registers only):
 //   W0 = V^T S (+ A1 rows),  W = op(T) W0,  A1 rows -= W,  S -= V W.
 // notrans: op(T) = T (apply H); otherwise T^T (apply H^T: the reference's trans 'C').
-template <typename T, int NB, bool GE>
-__device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t *a1, bool notrans, int buf = 0) {
+struct NoIssue {
+    __device__ void operator()(int) const {}
+};
+// issue(part): the caller's loads for the next block, one part per product pair of the first
+// phase (parts 0, 1, ... from its third pair on), so that the load instructions go out spread
+// over the time the matrix pipe is busy instead of in one burst that the memory path throttles.
+template <typename T, int NB, bool GE, typename F = NoIssue>
+__device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t *a1, bool notrans, int buf = 0,
+                                           F &&issue = F()) {
     const T *sm = lds_base<T>() + buf * Lds<T, NB>::BUF;
     constexpr int LDV = Lds<T, NB>::LDV;
     constexpr int RT = NB / 16;
@@ -353,6 +364,7 @@ __device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t 
             }
 #pragma unroll
             for (int ot = 0; ot < 2; ++ot) mfma16(w0[ot], va[idx % 3][ot], S.a[idx >> 2][idx & 3]);
+            if (idx >= 2) issue(idx - 2);
         }
     }
     if (!GE) {
@@ -793,17 +805,18 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
                 }
                 stage_to_regs<T, T, NB>(stg, S);
             }
-            if (B + 1 < NBLK) {
-                vload_lds<T, NB>(Aik + (i64)(B + 1) * IB * lda, lda, (B + 1) & 1);
-                tload<T, NB>(P, Tik + (i64)(B + 1) * IB * ldt, ldt);
-            } else if (i + 1 < g.mt) {
-                next0 = ready_ge(pdone, i + 2, &sh_ready);
-                if (next0) {
-                    vload_lds<T, NB>(Aik + NB, lda, 0);
-                    tload<T, NB>(P, Tik + IB, ldt);
+            // the next block's V (LDS DMA) and T, or block 0 of the next tile row
+            bool load_next = B + 1 < NBLK;
+            if (!load_next && i + 1 < g.mt) load_next = next0 = ready_ge(pdone, i + 2, &sh_ready);
+            const T *Vn = B + 1 < NBLK ? Aik + (i64)(B + 1) * IB * lda : Aik + NB;
+            const T *Tn = B + 1 < NBLK ? Tik + (i64)(B + 1) * IB * ldt : Tik + IB;
+            apply_wave<T, NB, false>(S, &A1.a[2 * B], false, B & 1, [&](int part) {
+                constexpr int NV = IB / WAVES, NTL = TPre<T, NB>::NT;
+                if (load_next) {
+                    if (part < NV) vload_lds_col<T, NB>(Vn, lda, (B + 1) & 1, part);
+                    else if (part < NV + NTL) tload_one<T, NB>(P, Tn, ldt, part - NV);
                 }
-            }
-            apply_wave<T, NB, false>(S, &A1.a[2 * B], false, B & 1);
+            });
             if (B + 1 < NBLK || next0) tstore<T, NB>(P, (B + 1) & 1);
         }
         have0 = next0;
