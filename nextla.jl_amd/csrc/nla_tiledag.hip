@@ -238,18 +238,22 @@ template <typename T, int NB> struct VPre {
     static constexpr int NV = NB * IB / NTH, NT = IB * IB / NTH;
     T v[NV], t[NT];
 };
+// Element e of vload below: e < NV a V element, then the T elements.
+template <typename T, int NB>
+__device__ __forceinline__ void vload_one(VPre<T, NB> &P, const T *Vg, i64 ldv, const T *Tg, i64 ldt, int e) {
+    constexpr int NV = VPre<T, NB>::NV;
+    if (e < NV) {
+        const int idx = threadIdx.x + e * NTH, p = idx % NB, j = idx / NB;
+        P.v[e] = cld(&Vg[p + (i64)j * ldv]);
+    } else {
+        const int idx = threadIdx.x + (e - NV) * NTH, q = idx % IB, j = idx / IB;
+        P.t[e - NV] = cld(&Tg[q + (i64)j * ldt]);
+    }
+}
 template <typename T, int NB>
 __device__ __forceinline__ void vload(VPre<T, NB> &P, const T *Vg, i64 ldv, const T *Tg, i64 ldt) {
 #pragma unroll
-    for (int e = 0; e < VPre<T, NB>::NV; ++e) {
-        const int idx = threadIdx.x + e * NTH, p = idx % NB, j = idx / NB;
-        P.v[e] = cld(&Vg[p + (i64)j * ldv]);
-    }
-#pragma unroll
-    for (int e = 0; e < VPre<T, NB>::NT; ++e) {
-        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
-        P.t[e] = cld(&Tg[q + (i64)j * ldt]);
-    }
+    for (int e = 0; e < VPre<T, NB>::NV + VPre<T, NB>::NT; ++e) vload_one<T, NB>(P, Vg, ldv, Tg, ldt, e);
 }
 template <typename T, int NB, bool GE>
 __device__ __forceinline__ void vstore(const VPre<T, NB> &P, int buf, int j0) {
@@ -810,6 +814,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
             if (!load_next && i + 1 < g.mt) load_next = next0 = ready_ge(pdone, i + 2, &sh_ready);
             const T *Vn = B + 1 < NBLK ? Aik + (i64)(B + 1) * IB * lda : Aik + NB;
             const T *Tn = B + 1 < NBLK ? Tik + (i64)(B + 1) * IB * ldt : Tik + IB;
+            static_assert(IB / WAVES + TPre<T, NB>::NT <= NB / 16 * 4 - 2, "every load part gets a product pair");
             apply_wave<T, NB, false>(S, &A1.a[2 * B], false, B & 1, [&](int part) {
                 constexpr int NV = IB / WAVES, NTL = TPre<T, NB>::NT;
                 if (load_next) {
@@ -997,14 +1002,16 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
                 }
                 stage_to_regs<T, Tw, NB>(stg, S);
             }
-            if (B + 1 < NBLK) {
-                vload<T, NB>(P, Aik + (i64)(B + 1) * IB * lda, lda, Tik + (i64)(B + 1) * IB * ldt, ldt);
-            } else if (i + 1 < g.mt) {
-                next0 = ready_ge(pdone, i + 2, &sh_ready);
-                if (next0) vload<T, NB>(P, Aik + NB, lda, Tik + IB, ldt);
-            }
+            bool load_next = B + 1 < NBLK;
+            if (!load_next && i + 1 < g.mt) load_next = next0 = ready_ge(pdone, i + 2, &sh_ready);
+            const T *Vn = B + 1 < NBLK ? Aik + (i64)(B + 1) * IB * lda : Aik + NB;
+            const T *Tn = B + 1 < NBLK ? Tik + (i64)(B + 1) * IB * ldt : Tik + IB;
             __syncthreads();
-            apply_wave<Tw, NB, false>(S, &A1.a[2 * B], false, B & 1);
+            // the next block's V and T, one element per product pair (into registers)
+            static_assert(VPre<T, NB>::NV + VPre<T, NB>::NT <= NB / 16 * 4 - 2, "every load part gets a product pair");
+            apply_wave<Tw, NB, false>(S, &A1.a[2 * B], false, B & 1, [&](int part) {
+                if (load_next && part < VPre<T, NB>::NV + VPre<T, NB>::NT) vload_one<T, NB>(P, Vn, lda, Tn, ldt, part);
+            });
             if (B + 1 < NBLK || next0) vstore_cvt<T, Tw, NB, false>(P, (B + 1) & 1, 0);
         }
         have0 = next0;
