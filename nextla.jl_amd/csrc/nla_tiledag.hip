@@ -779,6 +779,12 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
     bool have0 = false;  // block 0 of the next tile row is already in (or on its way to) buffer 0
     bool haveS = false;  // the next strip is already on its way to S
     int pending = 0;     // a finished row whose stores are in flight, not yet published (row + 1)
+    // Thread 0 reads the two flags that decide the next row's prefetches (block 0 of the next
+    // panel tile, the next strip) a block before the decisions and shares the outcomes at
+    // the last block's barrier, so their round trips stay off the row's critical path.  An
+    // early read can only say "not yet", and the next row then waits for it.
+    __shared__ int sh_nx[2];
+    int fl_p = 0, fl_s = 0;
     for (int i = k + 1; i < g.mt; ++i) {
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
@@ -800,6 +806,10 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
         bool next0 = false;
 #pragma unroll
         for (int B = 0; B < NBLK; ++B) {
+            if (B == NBLK - 1 && threadIdx.x == 0) {
+                sh_nx[0] = fl_p >= i + 2;
+                sh_nx[1] = fl_s >= i + 2;
+            }
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this block's V, the strip, earlier stores
             __syncthreads();
             if (B == 0) {
@@ -809,9 +819,13 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
                 }
                 stage_to_regs<T, T, NB>(stg, S);
             }
+            if (B == NBLK - 2 && threadIdx.x == 0) {
+                fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
+                fl_s = i + 1 < g.mt && prev ? ld_flag(prev) : 0;
+            }
             // the next block's V (LDS DMA) and T, or block 0 of the next tile row
             bool load_next = B + 1 < NBLK;
-            if (!load_next && i + 1 < g.mt) load_next = next0 = ready_ge(pdone, i + 2, &sh_ready);
+            if (!load_next && i + 1 < g.mt) load_next = next0 = sh_nx[0] != 0;
             const T *Vn = B + 1 < NBLK ? Aik + (i64)(B + 1) * IB * lda : Aik + NB;
             const T *Tn = B + 1 < NBLK ? Tik + (i64)(B + 1) * IB * ldt : Tik + IB;
             static_assert(IB / WAVES + TPre<T, NB>::NT <= NB / 16 * 4 - 2, "every load part gets a product pair");
@@ -827,7 +841,7 @@ __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64
         have0 = next0;
         stage_store<T, T, NB>(stg, S, Aij, ldc);
         pending = i + 1;
-        haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
+        haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
         if (haveS) stage_load<T, NB>(stg, Aij + NB, ldc);
     }
     if (pending) publish(mine, pending);
