@@ -987,6 +987,9 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     WStrip<Tw, NB> S;
     bool haveS = false;
     int pending = 0;
+    // the next-row prefetch flags, read a block early (as in the Float64 walker)
+    __shared__ int sh_nx[2];
+    int fl_p = 0, fl_s = 0;
     for (int i = k + 1; i < g.mt; ++i) {
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
@@ -1016,11 +1019,19 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
                 }
                 stage_to_regs<T, Tw, NB>(stg, S);
             }
-            bool load_next = B + 1 < NBLK;
-            if (!load_next && i + 1 < g.mt) load_next = next0 = ready_ge(pdone, i + 2, &sh_ready);
+            if (B == NBLK - 2 && threadIdx.x == 0) {
+                fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
+                fl_s = i + 1 < g.mt && prev ? ld_flag(prev) : 0;
+            }
+            if (B == NBLK - 1 && threadIdx.x == 0) {
+                sh_nx[0] = fl_p >= i + 2;
+                sh_nx[1] = fl_s >= i + 2;
+            }
             const T *Vn = B + 1 < NBLK ? Aik + (i64)(B + 1) * IB * lda : Aik + NB;
             const T *Tn = B + 1 < NBLK ? Tik + (i64)(B + 1) * IB * ldt : Tik + IB;
             __syncthreads();
+            bool load_next = B + 1 < NBLK;
+            if (!load_next && i + 1 < g.mt) load_next = next0 = sh_nx[0] != 0;
             // the next block's V and T, one element per product pair (into registers)
             static_assert(VPre<T, NB>::NV + VPre<T, NB>::NT <= NB / 16 * 4 - 2, "every load part gets a product pair");
             apply_wave<Tw, NB, false>(S, &A1.a[2 * B], false, B & 1, [&](int part) {
@@ -1031,7 +1042,7 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
         have0 = next0;
         stage_store<T, Tw, NB>(stg, S, Aij, ldc);
         pending = i + 1;
-        haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
+        haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
         if (haveS) stage_load<T, NB>(stg, Aij + NB, ldc);
     }
     if (pending) publish(mine, pending);
@@ -1225,6 +1236,9 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
     WStrip<float, NB> S;
     bool haveS = false;
     int pending = 0;
+    // the next-row prefetch flags, read a block early (as in the Float64 walker)
+    __shared__ int sh_nx[2];
+    int fl_p = 0, fl_s = 0;
     for (int i = k + 1; i < g.mt; ++i) {
         T *Aij = Cm + (i64)i * NB + col * ldc;
         if (pending && (!have0 || !haveS)) {
@@ -1242,6 +1256,10 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         bool next0 = false;
 #pragma unroll
         for (int B = 0; B < NBLK; ++B) {
+            if (B == NBLK - 1 && threadIdx.x == 0) {
+                sh_nx[0] = fl_p >= i + 2;
+                sh_nx[1] = fl_s >= i + 2;
+            }
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this block's image, the strip, earlier stores
             __syncthreads();
             if (B == 0) {
@@ -1251,10 +1269,16 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
                 }
                 stage_to_regs<T, float, NB>(stg, S);
             }
+            if (B == NBLK - 2 && threadIdx.x == 0) {
+                fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
+                fl_s = i + 1 < g.mt && prev ? ld_flag(prev) : 0;
+            }
+            // (the image DMA goes out before the block's products: spread over them, as in the
+            // Float64 walker, it delays the short 16-bit apply more than it hides)
             if (B + 1 < NBLK) {
                 lp_img_load<Tw, NB>(img(i, B + 1), (B + 1) & 1);
             } else if (i + 1 < g.mt) {
-                next0 = ready_ge(pdone, i + 2, &sh_ready);
+                next0 = sh_nx[0] != 0;
                 if (next0) lp_img_load<Tw, NB>(img(i + 1, 0), 0);
             }
             apply_lp<Tw, NB, false>(S, &A1.a[2 * B], B & 1);
@@ -1262,7 +1286,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         have0 = next0;
         stage_store<T, float, NB>(stg, S, Aij, ldc);
         pending = i + 1;
-        haveS = i + 1 < g.mt && (!prev || ready_ge(prev, i + 2, &sh_ready));
+        haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
         if (haveS) stage_load<T, NB>(stg, Aij + NB, ldc);
     }
     if (pending) publish(mine, pending);
