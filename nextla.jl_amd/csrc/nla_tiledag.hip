@@ -645,15 +645,22 @@ __device__ __forceinline__ void tsqrt_block(T *R, i64 ldr, T *A2, i64 lda2, T *T
         if (i <= j) cst(&Tm[i + (i64)(c0 + j) * ldt], Ts[i * (IB + 1) + j]);
     }
     phase_stamp(4 * (c0 / IB) + 2);
+    // The trailing columns of A2 move through the wave's strip staging columns (whole-column
+    // LDS-DMA loads, column-contiguous stores), which overlay the R / scratch regions, free
+    // once the block's R and T are stored.  No other block stage touches these columns of
+    // A2 meanwhile, so whole columns are stored back.
     const int nst = (NB - c0 - IB) / SW;
+    T *stg = stage_ptr<T, T, NB>();
     for (int st = w; st < nst; st += WAVES) {
         const i64 col = c0 + IB + st * SW;
+        stage_load<T, NB>(stg, A2 + col * lda2, lda2);
         typename Acc<T>::t a1[2];
         tiles_load<T, 2>(a1, R + col * ldr, ldr, c0);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         WStrip<T, NB> S;
-        tiles_load<T, NB / 16>(S.a, A2 + col * lda2, lda2, 0);
+        stage_to_regs<T, T, NB>(stg, S);
         apply_wave<T, NB, false>(S, a1, false);
-        tiles_store<T, NB / 16>(S.a, A2 + col * lda2, lda2, 0);
+        stage_store<T, T, NB>(stg, S, A2 + col * lda2, lda2);
         tiles_store<T, 2>(a1, R + col * ldr, ldr, c0);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
