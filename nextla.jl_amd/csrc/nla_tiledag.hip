@@ -195,10 +195,10 @@ __device__ __forceinline__ void stage_to_regs(const Ts *stg, WStrip<Tr, NB> &S) 
 #pragma unroll
         for (int r = 0; r < 4; ++r) S.a[rt][r] = (Tr)stg[lr * LDS_S + 16 * rt + acc_row<Tr>(lk, r)];
 }
-// accumulator registers (Tr) -> staging columns (Ts) -> HBM (rows 0..NB-1 of the wave's 16
-// columns at G)
+// accumulator registers (Tr) -> staging columns (Ts) -> HBM (rows rmin..NB-1 of the wave's 16
+// columns at G; rmin a multiple of the 16-byte vector)
 template <typename Ts, typename Tr, int NB>
-__device__ __forceinline__ void stage_store(Ts *stg, const WStrip<Tr, NB> &S, Ts *G, i64 ld) {
+__device__ __forceinline__ void stage_store(Ts *stg, const WStrip<Tr, NB> &S, Ts *G, i64 ld, int rmin = 0) {
     using St = Stage<Ts, NB>;
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
     typedef Ts tvec __attribute__((ext_vector_type(St::VEC)));
@@ -223,6 +223,7 @@ __device__ __forceinline__ void stage_store(Ts *stg, const WStrip<Tr, NB> &S, Ts
     for (int c0 = 0; c0 < SW; c0 += St::CPI) {
         const int c = c0 + cl;
         const Ts *p = stg + c * LDS_S + e;
+        if (e < rmin) continue;  // rows above rmin are left as they are in memory
         tvec v;
 #pragma unroll
         for (int q = 0; q < St::VEC; ++q) v[q] = p[q];
@@ -585,15 +586,21 @@ template <typename T, int NB> __device__ __forceinline__ void geqrt_block(T *A, 
         const int q = idx % IB, j = idx / IB;
         if (q <= j) cst(&Tm[q + (i64)(c0 + j) * ldt], Ts[q * (IB + 1) + j]);
     }
+    // trailing strips through the wave's staging columns (free once T is formed), as in
+    // tsqrt_block below
     const int nst = (NB - c0 - IB) / SW;
+    T *stg = stage_ptr<T, T, NB>();
     for (int st = w; st < nst; st += WAVES) {
         T *G = A + (i64)(c0 + IB + st * SW) * lda;
+        stage_load<T, NB>(stg, G, lda);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         WStrip<T, NB> S;
-        tiles_load<T, NB / 16>(S.a, G, lda, 0);
+        stage_to_regs<T, T, NB>(stg, S);
         apply_wave<T, NB, true>(S, nullptr, false);
-        // rows above c0 are unchanged by these reflectors and may be in flight in another
-        // block stage (R rows of earlier blocks updated by tsqrt): store only rows >= c0
-        tiles_store_from<T, NB / 16>(S.a, G, lda, c0);
+        // rows above c0 are unchanged by these reflectors (zero there) and may be in flight
+        // in another block stage (R rows of earlier blocks updated by tsqrt): store only rows
+        // >= c0 (the rows above may have been read stale; they do not enter the products)
+        stage_store<T, T, NB>(stg, S, G, lda, c0);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
