@@ -426,23 +426,39 @@ __device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t 
 }
 
 // ---------------------------------------------------------------------------------
-// Block factorization by the whole workgroup.  Thread (wave w, lane l) owns column
-// j = l % 32 of the IB-column block and the RPT = NB/8 consecutive rows starting at
-// row0 = RPT * (2w + l/32), in registers.  Per column c two workgroup rounds:
-//   norm round: the eight owners of column c write their partial sums of squares (and, for
-//     geqrt, the owner of the pivot row its entry) to LDS; after the barrier every thread
-//     forms beta, tau and the scale (identical in all threads), and the owners of column c
-//     scale their rows and store the reflector (unit pivot, zeros above) in column c of Vs;
-//   dot round: every thread reads v on its own rows (stored by a lane of its own wave),
-//     forms its partial dot with its column and writes it to LDS; after the barrier each
-//     column j > c is updated with w_j = v^T x_j (+ R[c][j] for tsqrt) and each j < c
-//     records z_j = v_j^T v for the T factor.
+// Block factorization by the whole workgroup.  Wave w owns the eight columns j = 8w .. 8w+7
+// of the IB-column block: lane l holds column 8w + l / 8 on the RPT = NB/8 consecutive rows
+// from row0 = RPT * (l % 8), in registers, so the eight lanes holding one column are
+// neighbours in one wave and its reductions are three DPP exchanges (every lane of the column
+// gets the same sum).  Per column c one workgroup barrier:
+//   the wave owning column c forms its norm below the pivot, beta, tau and the scale
+//     (identically in its lanes), scales the column into the reflector v (unit pivot, zeros
+//     above for geqrt) and stores v, tau (and beta into R for tsqrt) in LDS;
+//   after the barrier every wave forms the dots of v with its columns (v read on its own
+//     rows), updates each column j > c with w_j = tau (v^T x_j + R[c][j]) and records
+//     z_j = v_j^T v for j < c.
+// The next column's owner may start while other waves still update: what it writes for
+// column c + 1 (the reflector, tau, R's diagonal entry) is not read for column c.
 // The T factor (larft forward/columnwise: T[0:c, c] = -tau_c T[0:c, 0:c] z) is formed after
 // the loop by wave 0, one row per lane, while the other waves return to the caller; the
 // caller synchronises before it reads Ts.
-template <typename T>
-__device__ __forceinline__ T sum8(const T *p) {
-    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+// Sum over each aligned group of eight lanes, the same bits in all eight: quad_perm [1,0,3,2]
+// (xor 1), quad_perm [2,3,0,1] (xor 2), then row_half_mirror (lane i <-> 7 - i of its eight,
+// which pairs each quad with the other); every pairing is of two equal-shaped sums, so the
+// additions are commutative images of each other.
+template <int CTRL, typename T> __device__ __forceinline__ T dpp_mov(T v) {
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    struct W { int d[sizeof(T) / 4]; };
+    W a = __builtin_bit_cast(W, v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) a.d[i] = __builtin_amdgcn_mov_dpp(a.d[i], CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(T, a);
+}
+template <typename T> __device__ __forceinline__ T col_sum(T v) {
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);  // row_half_mirror
+    return v;
 }
 
 template <typename T, int NB, bool GE>
@@ -450,15 +466,15 @@ __device__ __forceinline__ void block_factor(int c0, T *Ablk, i64 lda) {
     T *sm = lds_base<T>();
     constexpr int LDV = Lds<T, NB>::LDV, RPT = NB / 8;
     static_assert(RPT % 4 == 0, "rows per thread must be a multiple of 4");
+    static_assert(IB == 8 * WAVES, "eight columns per wave");
     T *Vs = sm + Lds<T, NB>::V;
     T *Ts = sm + Lds<T, NB>::TT;
     T *Rs = sm + Lds<T, NB>::RS;
     T *Zs = sm + Lds<T, NB>::ZB;      // Zs[c * IB + q] = z_q of column c (q < c)
     T *taus = sm + Lds<T, NB>::TAU;   // [IB]
-    T *red = sm + Lds<T, NB>::RED;    // [8] partial sums of squares, [8] the pivot entry
-    T *dots = sm + Lds<T, NB>::DOT;   // [IB][8] partial dots, group index xor (column % 8)
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    const int j = lane & 31, grp = 2 * w + (lane >> 5), row0 = grp * RPT;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int j = 8 * w + (lane >> 3), grp = lane & 7, row0 = grp * RPT;
     T x[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) x[r] = Vs[j * LDV + row0 + r];
@@ -466,50 +482,51 @@ __device__ __forceinline__ void block_factor(int c0, T *Ablk, i64 lda) {
 #pragma unroll 1
     for (int c = 0; c < IB; ++c) {
         const int piv = c0 + c;
-        // ---- norm round
-        if (j == c) {
-            T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+        if (w == (c >> 3)) {  // the wave owning column c (uniform branch)
+            const bool own = j == c;
+            T q0 = 0, q1 = 0, q2 = 0, q3 = 0, ap = 0;
+            if (own) {
 #pragma unroll
-            for (int r = 0; r < RPT; r += 4) {
-                const T y0 = (!GE || row0 + r > piv) ? x[r] : T(0);
-                const T y1 = (!GE || row0 + r + 1 > piv) ? x[r + 1] : T(0);
-                const T y2 = (!GE || row0 + r + 2 > piv) ? x[r + 2] : T(0);
-                const T y3 = (!GE || row0 + r + 3 > piv) ? x[r + 3] : T(0);
-                q0 = fma(y0, y0, q0);
-                q1 = fma(y1, y1, q1);
-                q2 = fma(y2, y2, q2);
-                q3 = fma(y3, y3, q3);
+                for (int r = 0; r < RPT; r += 4) {
+                    const T y0 = (!GE || row0 + r > piv) ? x[r] : T(0);
+                    const T y1 = (!GE || row0 + r + 1 > piv) ? x[r + 1] : T(0);
+                    const T y2 = (!GE || row0 + r + 2 > piv) ? x[r + 2] : T(0);
+                    const T y3 = (!GE || row0 + r + 3 > piv) ? x[r + 3] : T(0);
+                    q0 = fma(y0, y0, q0);
+                    q1 = fma(y1, y1, q1);
+                    q2 = fma(y2, y2, q2);
+                    q3 = fma(y3, y3, q3);
+                }
+                if (GE && piv >= row0 && piv < row0 + RPT) {
+#pragma unroll
+                    for (int r = 0; r < RPT; ++r) ap = (row0 + r == piv) ? x[r] : ap;
+                }
             }
-            red[grp] = (q0 + q1) + (q2 + q3);
-            if (GE && piv >= row0 && piv < row0 + RPT) {
-                T ap = 0;
+            const T ssq = col_sum((q0 + q1) + (q2 + q3));
+            const T alpha = GE ? col_sum(ap) : Rs[c * (IB + 1) + c];
+            T beta = alpha, tau = 0, scale = 1;
+            if (ssq != T(0)) {
+                beta = -copysign(sqrt(alpha * alpha + ssq), alpha);
+                tau = (beta - alpha) / beta;
+                scale = T(1) / (alpha - beta);
+            }
+            if (own) {
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) ap = (row0 + r == piv) ? x[r] : ap;
-                red[8] = ap;
+                for (int r = 0; r < RPT; ++r) {
+                    const int p = row0 + r;
+                    const bool below = !GE || p > piv, pv = GE && p == piv;
+                    const T v = x[r] * scale;
+                    Vs[c * LDV + p] = below ? v : (pv ? T(1) : T(0));
+                    x[r] = below ? v : (pv ? beta : x[r]);
+                }
+                if (grp == 0) {
+                    taus[c] = tau;
+                    if (!GE) Rs[c * (IB + 1) + c] = beta;
+                }
             }
         }
         __syncthreads();
-        const T ssq = sum8(red);
-        const T alpha = GE ? red[8] : Rs[c * (IB + 1) + c];
-        T beta = alpha, tau = 0, scale = 1;
-        if (ssq != T(0)) {
-            beta = -copysign(sqrt(alpha * alpha + ssq), alpha);
-            tau = (beta - alpha) / beta;
-            scale = T(1) / (alpha - beta);
-        }
-        if (j == c) {
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int p = row0 + r;
-                const bool below = !GE || p > piv, pv = GE && p == piv;
-                const T v = x[r] * scale;
-                Vs[c * LDV + p] = below ? v : (pv ? T(1) : T(0));
-                x[r] = below ? v : (pv ? beta : x[r]);
-            }
-            if (grp == 0) taus[c] = tau;
-        }
-        __builtin_amdgcn_wave_barrier();
-        // ---- dot round
+        // ---- dots with every column, updates
         T v[RPT];
         T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
 #pragma unroll
@@ -523,18 +540,15 @@ __device__ __forceinline__ void block_factor(int c0, T *Ablk, i64 lda) {
             q2 = fma(v[r + 2], x[r + 2], q2);
             q3 = fma(v[r + 3], x[r + 3], q3);
         }
-        dots[j * 8 + (grp ^ (j & 7))] = (q0 + q1) + (q2 + q3);
-        const T rc = GE ? T(0) : Rs[c * (IB + 1) + j];
-        __syncthreads();
-        const T gsum = sum8(dots + j * 8);
+        const T gsum = col_sum((q0 + q1) + (q2 + q3));
         if (j > c) {
-            const T cw = tau * (gsum + rc);
+            const T rc = GE ? T(0) : Rs[c * (IB + 1) + j];
+            const T cw = taus[c] * (gsum + rc);
 #pragma unroll
             for (int r = 0; r < RPT; ++r) x[r] = fma(-cw, v[r], x[r]);
             if (!GE && grp == 0) Rs[c * (IB + 1) + j] = rc - cw;
-        } else if (grp == 0) {
-            if (j < c) Zs[c * IB + j] = gsum;
-            else if (!GE) Rs[c * (IB + 1) + c] = beta;
+        } else if (j < c && grp == 0) {
+            Zs[c * IB + j] = gsum;
         }
     }
     if (GE) {
