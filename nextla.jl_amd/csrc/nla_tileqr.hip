@@ -177,13 +177,19 @@ template <typename T> __global__ __launch_bounds__(64) void diag_block_inv_kerne
     for (int i = 0; i < BS; ++i) Dinv[(i64)b * BS * BS + i + (i64)j * BS] = Y[i][j];
 }
 
+// Blocks finish in claim order (each depends on every block claimed before it), so one
+// counter `fin` (blocks finished) replaces per-block flags: block d of a claim is ready when
+// fin > d.  Thread 0 keeps the last value read; only when the loop reaches a block beyond it
+// does the workgroup wait (thread 0 polls, a barrier shares the new value), so the products
+// with long-finished blocks run without flag reads or barriers.
 template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 lda, const T *Dinv, T *x, int *counter,
-                                                          int *flags, int *err) {
+                                                          int *fin, int *err) {
     constexpr int BS = TRSV_BS;
     __shared__ T red[4][BS + 1];
     __shared__ T ys[BS];
-    __shared__ int sh_b, sh_stop;
+    __shared__ T tr[TRANS ? 4 * 16 * 64 : 1];  // TRANS: the lanes' partial sums, transposed
+    __shared__ int sh_b, sh_stop, sh_known[2];
     const int nblk = n / BS, t = threadIdx.x, lane = t & 63, g = t >> 6;
     for (;;) {
         __syncthreads();
@@ -196,11 +202,13 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
         T acc[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[q] = 0;
-        const int nc = TRANS ? b : nblk - 1 - b;  // finished blocks this one depends on
+        const int nc = claim;  // finished blocks this one depends on (claims 0 .. claim-1)
+        int known = 0, par = 0;
+        bool stop = false;
         for (int d = 0; d < nc; ++d) {
             const int c = TRANS ? d : nblk - 1 - d;
             const i64 c0 = (i64)c * BS;
-            // this tile of R does not depend on the flag: load it first
+            // this tile of R does not depend on the counter: load it first
             T rv[16];
             if (!TRANS) {
                 const T *Rc = R + r0 + lane + (c0 + g * 16) * lda;  // R[r0 + lane][c0 + g*16 + q]
@@ -211,23 +219,29 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
 #pragma unroll
                 for (int q = 0; q < 16; ++q) rv[q] = Rc[q * lda];
             }
-            if (t == 0) {
-                sh_stop = 0;
-                if (ld_sc1(&flags[c]) == 0) {
-                    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-                    while (ld_sc1(&flags[c]) == 0) {
-                        if (ld_sc1(err)) { sh_stop = 1; break; }
-                        if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
-                            st_sc1(err, 1);
-                            sh_stop = 1;
-                            break;
+            if (d >= known) {  // workgroup-uniform
+                if (t == 0) {
+                    int f = ld_sc1(fin);
+                    int st = 0;
+                    if (f <= d) {
+                        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+                        while ((f = ld_sc1(fin)) <= d) {
+                            if (ld_sc1(err)) { st = 1; break; }
+                            if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                                st_sc1(err, 1);
+                                st = 1;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
                         }
-                        __builtin_amdgcn_s_sleep(1);
                     }
+                    sh_known[par] = st ? -1 : f;
                 }
+                __syncthreads();
+                known = sh_known[par];  // alternate slots: a wave may still read the last one
+                par ^= 1;
+                if (known < 0) { stop = true; break; }
             }
-            __syncthreads();
-            if (sh_stop) break;
             if (!TRANS) {
 #pragma unroll
                 for (int q = 0; q < 16; ++q) acc[0] += rv[q] * ld_sc1(&x[c0 + g * 16 + q]);
@@ -237,7 +251,8 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
                 for (int q = 0; q < 16; ++q) acc[q] += rv[q] * xv;
             }
         }
-        // the diagonal block's inverse (also independent of the flags)
+        if (stop) break;
+        // the diagonal block's inverse (also independent of the counter)
         const T *D = Dinv + (i64)b * BS * BS;
         T dv[16];
 #pragma unroll
@@ -246,13 +261,29 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
         if (!TRANS) {
             red[g][lane] = acc[0];
         } else {
+            // output g*16 + q is the sum over the wave's 64 lanes of acc[q]: transpose through
+            // LDS, then thread t sums 16 lanes of output t/4 and the quad of threads adds up
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const T v = wave_sum(acc[q]);
-                if (lane == 0) red[0][g * 16 + q] = v;
-            }
+            for (int q = 0; q < 16; ++q) tr[(g * 16 + q) * 64 + lane] = acc[q];
         }
         __syncthreads();
+        if (TRANS) {
+            const int i = t >> 2, sq = t & 3;
+            const T *src = &tr[i * 64 + sq * 16];
+            T p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+#pragma unroll
+            for (int u = 0; u < 16; u += 4) {
+                p0 += src[u];
+                p1 += src[u + 1];
+                p2 += src[u + 2];
+                p3 += src[u + 3];
+            }
+            T p = (p0 + p1) + (p2 + p3);
+            p += __shfl_xor(p, 1);
+            p += __shfl_xor(p, 2);
+            if (sq == 0) red[0][i] = p;
+            __syncthreads();
+        }
         if (g == 0) {
             // y = v_b - sum of the finished blocks' products (v_b not yet overwritten)
             const T sum = TRANS ? red[0][lane] : (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
@@ -270,19 +301,21 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         __syncthreads();
-        if (t == 0) st_sc1(&flags[b], 1);
+        if (t == 0) st_sc1(fin, claim + 1);
     }
 }
-template <typename T> __global__ void trsv_reset_kernel(int nblk, int *counter, int *flags) {
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) flags[i] = 0;
-    if (threadIdx.x == 0) *counter = 0;
+template <typename T> __global__ void trsv_reset_kernel(int *counter, int *fin) {
+    if (threadIdx.x == 0) {
+        *counter = 0;
+        *fin = 0;
+    }
 }
 // x := R^-1 x or R^-T x (R upper n x n at R, lda; Dinv from diag_block_inv_kernel); sync =
 // the counter, an error flag and nblk flags (device ints, n/64 + 2 of them)
 template <typename T>
 static hipError_t trsv_upper(bool trans, int n, const T *R, i64 lda, const T *Dinv, T *x, int *sync, hipStream_t s) {
     const int nblk = n / TRSV_BS;
-    trsv_reset_kernel<T><<<1, 256, 0, s>>>(nblk, sync, sync + 2);
+    trsv_reset_kernel<T><<<1, 64, 0, s>>>(sync, sync + 2);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     int dev = 0, ncu = 0;
