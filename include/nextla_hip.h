@@ -183,13 +183,13 @@ int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *
                           int64_t ldt, char prec, void *stream);
 /* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
  * out.  Tile QR of a copy of A (prec 'D' plain; 'S' / 'H' / 'B' as nla_dtile_geqrf_mixed,
- * Float64 data only), the direct
- * solution x0 = R^-1 (Q^T b)[0:n], then up to maxit iterations of LSQR (Paige & Saunders) on
- * the correction, right-preconditioned by R: min ||(A R^-1) y - (b - A x0)||, x = x0 + R^-1 y,
- * with every product by A in the working precision.  Stops when LSQR's estimate of
- * ||M^T r|| / (||M||_F ||r||) (M = A R^-1) reaches tol or the residual estimate falls to
- * tol ||b - A x0||.  maxit = 0 returns x0.  iters / last_correction (may be NULL) receive the
- * iterations taken and the final estimate.  Synchronises the stream. */
+ * Float64 data only), the direct solution x0 = R^-1 (Q^T b)[0:n] (returned for maxit = 0; with
+ * prec 'D' and maxit > 0, x0 = R^-1 R^-T (A^T b), the seminormal equations), then up to maxit
+ * iterations of LSQR (Paige & Saunders) on the correction, right-preconditioned by R: min ||(A R^-1) y - (b - A x0)||,
+ * x = x0 + R^-1 y, with every product by A in the working precision.  Stops when LSQR's
+ * estimate of ||M^T r|| / (||M||_F ||r||) (M = A R^-1) reaches tol or the residual estimate
+ * falls to tol ||b - A x0||.  iters / last_correction (may be NULL) receive the iterations
+ * taken and the final estimate.  Synchronises the stream. */
 int nla_dtile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *A, int64_t lda, const double *b,
                    double *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
                    void *stream);

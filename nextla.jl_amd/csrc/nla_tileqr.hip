@@ -417,10 +417,17 @@ template <typename T> __global__ void axpy1_kernel(i64 n, const T *x, T *y) {
 }
 
 // Least squares min ||A x - b|| (A m x n, m >= n, full rank).
-// x0 = R^-1 (Q^T b)[0:n] from the tile QR A = Q R (prec 'D': Float64; 'S', 'H', 'B': Float32,
-// FP16 or BF16 trailing updates, Float64 data only), then LSQR (Paige & Saunders, ACM TOMS 8 (1982), algorithm
-// LSQR) on the correction, right-preconditioned by R: min ||(A R^-1) y - r0||, x = x0 + R^-1 y,
-// every product with A in the working precision on the unmodified A.  A R^-1 has condition
+// Tile QR A = Q R (prec 'D': Float64; 'S', 'H', 'B': Float32, FP16 or BF16 trailing updates,
+// Float64 data only).  x0 = R^-1 (Q^T b)[0:n], the direct solution (returned for maxit = 0);
+// with the Float64 factor and maxit > 0, x0 = R^-1 R^-T (A^T b) instead (the seminormal
+// equations: Q^T b = R^-T A^T b for A = QR; one product by A^T and two triangular solves,
+// ~1.2 ms at 32768 x 8192, where Q^T b takes 7.4 ms because its walker launch has a single
+// strip of work for one right-hand side).  Then LSQR (Paige & Saunders, ACM TOMS 8 (1982),
+// algorithm LSQR) on the correction, right-preconditioned by R: min ||(A R^-1) y - (b - A x0)||,
+// x = x0 + R^-1 y, every product with A in the working precision on the unmodified A (with the
+// seminormal x0: Bjorck's corrected seminormal equations).  With a reduced-precision factor the
+// seminormal start (or a zero start) left LSQR short of the direct start's accuracy at kappa
+// 1e6 (relative error 1.4e-8 / 1.2e-8 against 1.1e-8 allowed), so those keep Q^T b.  A R^-1 has condition
 // number about 1 + kappa(A) times the factorization's backward error, so the iteration
 // converges quickly whenever R is a fair preconditioner, also where plain refinement
 // (x += R^-1 (Q^T r)[0:n]) diverges: kappa(A) times the factorization error near or above one.
@@ -476,15 +483,23 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
         diag_block_inv_kernel<T><<<(unsigned)(n / TRSV_BS), TRSV_BS, 0, s>>>(AF, m, Dinv);
         NLA_LAUNCH_CHECK();
     }
-    // x0 = R^-1 (Q^T b)[0:n]
-    NLA_HIP(hipMemsetAsync(Cb, 0, sizeof(T) * (size_t)m * W, s));
-    NLA_HIP(hipMemcpyAsync(Cb, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
-    if ((rc = tile_ormqr_entry<T>('T', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s))) return rc;
-    NLA_HIP(rsolve(false, Cb));
-    NLA_HIP(hipMemcpyAsync(x, Cb, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-    if (maxit == 0) {
-        NLA_HIP(hipStreamSynchronize(s));
-        return 0;
+    if (maxit == 0 || prec != 'D') {
+        // x0 = R^-1 (Q^T b)[0:n]
+        NLA_HIP(hipMemsetAsync(Cb, 0, sizeof(T) * (size_t)m * W, s));
+        NLA_HIP(hipMemcpyAsync(Cb, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
+        if ((rc = tile_ormqr_entry<T>('T', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s))) return rc;
+        NLA_HIP(rsolve(false, Cb));
+        NLA_HIP(hipMemcpyAsync(x, Cb, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+        if (maxit == 0) {
+            NLA_HIP(hipStreamSynchronize(s));
+            return 0;
+        }
+    } else {
+        // x0 = R^-1 R^-T (A^T b)
+        gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, b, x);
+        NLA_LAUNCH_CHECK();
+        NLA_HIP(rsolve(true, x));
+        NLA_HIP(rsolve(false, x));
     }
     const dim3 gpart((unsigned)rb, (unsigned)nsplit);
     auto norm_read = [&](int nblocks, double *out) -> hipError_t {

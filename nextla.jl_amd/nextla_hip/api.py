@@ -430,9 +430,10 @@ def tile_geqrf_mixed(A, nb, ib, prec="S"):
 
 def tile_gels(A, b, nb=128, ib=32, prec="D", maxit=100, tol=1e-15):
     """Least-squares solution of min ||A x - b||: tile QR (prec 'S' / 'H' / 'B', Float64
-    only: Float32 / FP16 / BF16 trailing updates), direct solution x0, then R-preconditioned LSQR on the correction with
-    products by A in the working precision.  Returns (x, iterations, final estimate of
-    ||M^T r|| / (||M|| ||r||) for M = A R^-1)."""
+    only: Float32 / FP16 / BF16 trailing updates); the direct solution x0 = R^-1 (Q^T b)[0:n]
+    (returned for maxit = 0; prec 'D' with maxit > 0 starts from R^-1 R^-T A^T b instead),
+    corrected by R-preconditioned LSQR with products by A in the working precision.  Returns (x, iterations, final estimate of ||M^T r|| / (||M|| ||r||)
+    for M = A R^-1)."""
     import ctypes
 
     p = _p(A, b)
