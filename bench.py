@@ -37,13 +37,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gemm_flops_of_recursion(n, m, cut):
-    """Flops of the GEMM updates unified_rec issues (src/rectrxm.jl:101-198)."""
+def gemm_flops_of_recursion(n, m, cut, off=0):
+    """Flops of the GEMM launches unified_rec issues (src/rectrxm.jl:101-198) for the lower,
+    left, untransposed solve: with 256-row base blocks (cut-off in [256, 512)) an aligned
+    512-row block runs as one fused launch (base, update, base) and issues no GEMM."""
     if n <= cut:
+        return 0
+    if 256 <= cut < 512 and n == 512 and off % 256 == 0:
         return 0
     mid = n // 2 if (n & (n - 1)) == 0 else 1 << (n.bit_length() - 1)
     rem = n - mid
-    return 2 * mid * rem * m + gemm_flops_of_recursion(mid, m, cut) + gemm_flops_of_recursion(rem, m, cut)
+    return (2 * mid * rem * m + gemm_flops_of_recursion(mid, m, cut, off)
+            + gemm_flops_of_recursion(rem, m, cut, off + mid))
 
 
 def make_tri(torch, nl, n, dtype, gen):

@@ -421,7 +421,7 @@ constexpr int IBS = 256;
 // issued while step v multiplies), the two pairs' chains interleaved.
 template <typename T, bool UP, bool TRB, int W>
 __device__ __forceinline__ void inv_apply_wave(const T *__restrict__ Inv, T *__restrict__ B, i64 ldb,
-                                               const T (*Xs)[BCW + 1], int c0, int cw) {
+                                               const T (*Xs)[BCW + 1], int c0, int cw, T (*Ys)[BCW + 1] = nullptr) {
     constexpr int NT = IBS / 16, PF = 3, NS = NT + 1;
     static_assert(NT == 16, "the pairing assumes 16 row tiles");
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
@@ -466,12 +466,96 @@ __device__ __forceinline__ void inv_apply_wave(const T *__restrict__ Inv, T *__r
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * it + acc_row<T>(lk, r);
+                if (Ys) Ys[row][lr] = acc[p][h][r];  // a copy of X for a following product
                 if (lr < cw) {
                     if (TRB) B[(c0 + lr) + (i64)row * ldb] = acc[p][h][r];
                     else B[row + (i64)(c0 + lr) * ldb] = acc[p][h][r];
                 }
             }
         }
+}
+
+// Xo -= A21 Xk on the wave's row tiles {W, W+4, W+8, W+12} (16 rows each) of an IBS x IBS block
+// A21 (column-major, lda) times the IBS x BCW panel Xk, both panels in LDS: 16 K-steps of 16
+// per tile, four tiles' chains interleaved, the A21 loads of step v + PF issued while step v
+// multiplies (the same operand roles as the inverse apply above).
+template <typename T, int W>
+__device__ __forceinline__ void sub_prod_wave(const T *__restrict__ A21, i64 lda, const T (*Xk)[BCW + 1],
+                                              T (*Xo)[BCW + 1]) {
+    constexpr int NKS = IBS / 16, PF = 3;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    typename Acc<T>::t acc[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) acc[p] = acc_zero<T>();
+    T a[PF][4][4];
+    auto load_step = [&](int ks, T (&dst)[4][4]) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const T *Ap = A21 + (16 * (W + 4 * p) + lr) + (i64)(16 * ks + lk) * lda;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) dst[p][u] = Ap[(i64)(4 * u) * lda];
+        }
+    };
+#pragma unroll
+    for (int v = 0; v < PF; ++v) load_step(v, a[v]);
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+        T cur[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cur[p][u] = a[ks % PF][p][u];
+        if (ks + PF < NKS) load_step(ks + PF, a[ks % PF]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int p = 0; p < 4; ++p) mfma16(acc[p], cur[p][u], Xk[16 * ks + lk + 4 * u][lr]);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xo[16 * (W + 4 * p) + acc_row<T>(lk, r)][lr] -= acc[p][r];
+}
+
+// A lower, left, untransposed 2*IBS-row block of the recursion in one launch, instead of base,
+// GEMM, base: X1 = Inv1 B1; B2 -= A21 X1; X2 = Inv2 B2 (Inv1, Inv2 the inverses of the two
+// diagonal blocks, A21 the block below the first).  A workgroup owns BCW right-hand sides
+// and keeps their 2*IBS rows (Xs) and X1 (Ys) in LDS throughout, so B is read and written
+// once and the three steps need no launches between them.  Dynamic LDS: 3 IBS (BCW+1) values.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_inv2_kernel(int m, const T *__restrict__ Inv1, const T *__restrict__ Inv2,
+                                                        const T *__restrict__ A21, i64 lda, T *__restrict__ B, i64 ldb) {
+    extern __shared__ __align__(16) unsigned char inv2_smem[];
+    T (*Xs)[BCW + 1] = reinterpret_cast<T (*)[BCW + 1]>(inv2_smem);  // 2 IBS rows
+    T (*Ys)[BCW + 1] = Xs + 2 * IBS;                                  // IBS rows
+    const int c0 = blockIdx.x * BCW, cw = min(BCW, m - c0);
+    const int t = threadIdx.x;
+    for (int idx = t; idx < 2 * IBS * BCW; idx += 256) {
+        const int r = idx % (2 * IBS), c = idx / (2 * IBS);
+        Xs[r][c] = c < cw ? B[r + (i64)(c0 + c) * ldb] : T(0);
+    }
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    switch (w) {
+    case 0: inv_apply_wave<T, false, false, 0>(Inv1, B, ldb, Xs, c0, cw, Ys); break;
+    case 1: inv_apply_wave<T, false, false, 1>(Inv1, B, ldb, Xs, c0, cw, Ys); break;
+    case 2: inv_apply_wave<T, false, false, 2>(Inv1, B, ldb, Xs, c0, cw, Ys); break;
+    default: inv_apply_wave<T, false, false, 3>(Inv1, B, ldb, Xs, c0, cw, Ys); break;
+    }
+    __syncthreads();
+    switch (w) {
+    case 0: sub_prod_wave<T, 0>(A21, lda, Ys, Xs + IBS); break;
+    case 1: sub_prod_wave<T, 1>(A21, lda, Ys, Xs + IBS); break;
+    case 2: sub_prod_wave<T, 2>(A21, lda, Ys, Xs + IBS); break;
+    default: sub_prod_wave<T, 3>(A21, lda, Ys, Xs + IBS); break;
+    }
+    __syncthreads();
+    switch (w) {
+    case 0: inv_apply_wave<T, false, false, 0>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw); break;
+    case 1: inv_apply_wave<T, false, false, 1>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw); break;
+    case 2: inv_apply_wave<T, false, false, 2>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw); break;
+    default: inv_apply_wave<T, false, false, 3>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw); break;
+    }
 }
 
 template <typename T, bool UP, bool TRB>
@@ -531,6 +615,22 @@ static hipError_t inv_apply_launch(bool up, bool trB, int m, const T *Inv, T *B,
     return hipGetLastError();
 }
 
+template <typename T>
+static hipError_t inv2_launch(int m, const T *Inv1, const T *A21, i64 lda, T *B, i64 ldb, hipStream_t s) {
+    ProfScope prof(PK_TRSM_BASE, s);
+    const size_t sh = sizeof(T) * 3 * IBS * (BCW + 1);
+    static bool attr[2] = {false, false};
+    if (sh > 64 * 1024 && !attr[sizeof(T) == 8]) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&trsm_inv2_kernel<T>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+        if (e != hipSuccess) return e;
+        attr[sizeof(T) == 8] = true;
+    }
+    const dim3 grid((unsigned)((m + BCW - 1) / BCW));
+    trsm_inv2_kernel<T><<<grid, 256, sh, s>>>(m, Inv1, Inv1 + (i64)IBS * IBS, A21, lda, B, ldb);
+    return hipGetLastError();
+}
+
 // Left-form flags of the base call for (side, uplo) of op(A), op(A)(i,j) = tr ? A(j,i) : A(i,j):
 // X op(A) = B  <=>  op(A)^T X^T = B^T.
 static inline void left_form(char side, char uplo, bool tr, bool *up, bool *trA, bool *trB) {
@@ -562,6 +662,10 @@ template <typename T>
 static hipError_t rec(char func, char side, char uplo, bool tr, i64 n, i64 m, const T *A, i64 lda, T *B, i64 ldb,
                       i64 threshold, const T *Dinv, const T *Inv, i64 off, hipStream_t s) {
     if (n <= threshold) return base_case<T>(func, side, uplo, tr, (int)n, (int)m, A, lda, B, ldb, Dinv, Inv, off, s);
+    // a lower, left, untransposed solve of two aligned base blocks: one fused launch
+    if (func == 'S' && side == 'L' && uplo == 'L' && !tr && Inv && n == 2 * IBS && threshold >= IBS &&
+        off % IBS == 0)
+        return inv2_launch<T>((int)m, Inv + (off / IBS) * (i64)IBS * IBS, A + IBS, lda, B, ldb, s);
     i64 mid;
     if ((n & (n - 1)) == 0) mid = n / 2;
     else { mid = 1; while (mid * 2 <= n) mid *= 2; }
