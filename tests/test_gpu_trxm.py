@@ -139,3 +139,21 @@ def test_rectrxm_interior_tiles_vs_oracle(side, uplo, trans, func):
     B = B0.copy(order="F")
     O.rectrxm(side, uplo, trans, 1.0, func, A, B, threshold=128)
     assert relerr(host(dB), B) < 1e-12
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("n,m", [(512, 64), (768, 37), (1280, 50)])
+def test_trsm_fused_two_block_base_vs_oracle(dt, n, m):
+    """Cut-off 256, left / lower / N: every aligned 512-row block of the recursion runs as one
+    launch (inverse-apply, the 256 x 256 update, inverse-apply).  n = 768 and 1280 put fused
+    blocks at offsets 0, 512 and beside 256-row base blocks; m not a multiple of 16 leaves a
+    partial workgroup of right-hand sides."""
+    rng = np.random.default_rng(n + m)
+    A = tri_matrix(rng, n, "L", dt)
+    B0 = F(rng.standard_normal((n, m)).astype(dt))
+    dB = nl.to_device(B0)
+    nl.unified_rectrxm_("L", "L", "N", 1.5, "S", nl.to_device(A), dB, cutoff=256)
+    B = B0.copy(order="F")
+    O.rectrxm("L", "L", "N", 1.5, "S", A, B, threshold=256)
+    tol = 1e-12 if dt == np.float64 else 1e-4
+    assert relerr(host(dB), B) < tol
