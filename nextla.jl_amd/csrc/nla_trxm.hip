@@ -419,24 +419,26 @@ constexpr int IBS = 256;
 // W is a template parameter, so every tile and step index is a constant and each tile has
 // its own accumulator: the steps are straight-line loads and MFMAs (the loads of step v + PF
 // issued while step v multiplies), the two pairs' chains interleaved.
-template <typename T, bool UP, bool TRB, int W>
+template <typename T, bool UP, bool TRB, int W, int NW = 4>
 __device__ __forceinline__ void inv_apply_wave(const T *__restrict__ Inv, T *__restrict__ B, i64 ldb,
                                                const T (*Xs)[BCW + 1], int c0, int cw, T (*Ys)[BCW + 1] = nullptr) {
     constexpr int NT = IBS / 16, PF = 3, NS = NT + 1;
     static_assert(NT == 16, "the pairing assumes 16 row tiles");
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-    constexpr int TA[2] = {W, 4 + W}, TB[2] = {NT - 1 - W, NT - 1 - (4 + W)};
+    static_assert(NW == 4 || NW == 8, "four or eight waves");
+    constexpr int NP = 8 / NW;  // pairs per wave: {W + NW q, 15 - (W + NW q)}
+    constexpr int TA[2] = {W, NP > 1 ? NW + W : W}, TB[2] = {NT - 1 - TA[0], NT - 1 - TA[1]};
     constexpr int FA[2] = {UP ? TA[0] : 0, UP ? TA[1] : 0}, FB[2] = {UP ? TB[0] : 0, UP ? TB[1] : 0};
     constexpr int CA[2] = {UP ? NT - TA[0] : TA[0] + 1, UP ? NT - TA[1] : TA[1] + 1};
     auto tile_of = [&](int p, int v) { return v < CA[p] ? TA[p] : TB[p]; };
     auto step_of = [&](int p, int v) { return v < CA[p] ? FA[p] + v : FB[p] + v - CA[p]; };
     typename Acc<T>::t acc[2][2];
 #pragma unroll
-    for (int p = 0; p < 2; ++p) acc[p][0] = acc[p][1] = acc_zero<T>();
+    for (int p = 0; p < NP; ++p) acc[p][0] = acc[p][1] = acc_zero<T>();
     T a[PF][2][4];
     auto load_step = [&](int v, T (&dst)[2][4]) {
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < NP; ++p) {
             const T *Ap = Inv + (16 * tile_of(p, v) + lr) + (i64)(16 * step_of(p, v) + lk) * IBS;
 #pragma unroll
             for (int u = 0; u < 4; ++u) dst[p][u] = Ap[(i64)(4 * u) * IBS];
@@ -448,18 +450,18 @@ __device__ __forceinline__ void inv_apply_wave(const T *__restrict__ Inv, T *__r
     for (int v = 0; v < NS; ++v) {
         T cur[2][4];
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < NP; ++p)
 #pragma unroll
             for (int u = 0; u < 4; ++u) cur[p][u] = a[v % PF][p][u];
         if (v + PF < NS) load_step(v + PF, a[v % PF]);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int p = 0; p < 2; ++p)
+            for (int p = 0; p < NP; ++p)
                 mfma16(acc[p][v < CA[p] ? 0 : 1], cur[p][u], Xs[16 * step_of(p, v) + lk + 4 * u][lr]);
     }
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int it = h == 0 ? TA[p] : TB[p];
@@ -479,19 +481,19 @@ __device__ __forceinline__ void inv_apply_wave(const T *__restrict__ Inv, T *__r
 // A21 (column-major, lda) times the IBS x BCW panel Xk, both panels in LDS: 16 K-steps of 16
 // per tile, four tiles' chains interleaved, the A21 loads of step v + PF issued while step v
 // multiplies (the same operand roles as the inverse apply above).
-template <typename T, int W>
+template <typename T, int W, int NW = 4>
 __device__ __forceinline__ void sub_prod_wave(const T *__restrict__ A21, i64 lda, const T (*Xk)[BCW + 1],
                                               T (*Xo)[BCW + 1]) {
-    constexpr int NKS = IBS / 16, PF = 3;
+    constexpr int NKS = IBS / 16, PF = 3, NTW = 16 / NW;  // row tiles W, W + NW, ...;
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-    typename Acc<T>::t acc[4];
+    typename Acc<T>::t acc[NTW];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) acc[p] = acc_zero<T>();
-    T a[PF][4][4];
-    auto load_step = [&](int ks, T (&dst)[4][4]) {
+    for (int p = 0; p < NTW; ++p) acc[p] = acc_zero<T>();
+    T a[PF][NTW][4];
+    auto load_step = [&](int ks, T (&dst)[NTW][4]) {
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const T *Ap = A21 + (16 * (W + 4 * p) + lr) + (i64)(16 * ks + lk) * lda;
+        for (int p = 0; p < NTW; ++p) {
+            const T *Ap = A21 + (16 * (W + NW * p) + lr) + (i64)(16 * ks + lk) * lda;
 #pragma unroll
             for (int u = 0; u < 4; ++u) dst[p][u] = Ap[(i64)(4 * u) * lda];
         }
@@ -500,21 +502,21 @@ __device__ __forceinline__ void sub_prod_wave(const T *__restrict__ A21, i64 lda
     for (int v = 0; v < PF; ++v) load_step(v, a[v]);
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
-        T cur[4][4];
+        T cur[NTW][4];
 #pragma unroll
-        for (int p = 0; p < 4; ++p)
+        for (int p = 0; p < NTW; ++p)
 #pragma unroll
             for (int u = 0; u < 4; ++u) cur[p][u] = a[ks % PF][p][u];
         if (ks + PF < NKS) load_step(ks + PF, a[ks % PF]);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int p = 0; p < 4; ++p) mfma16(acc[p], cur[p][u], Xk[16 * ks + lk + 4 * u][lr]);
+            for (int p = 0; p < NTW; ++p) mfma16(acc[p], cur[p][u], Xk[16 * ks + lk + 4 * u][lr]);
     }
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+    for (int p = 0; p < NTW; ++p)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Xo[16 * (W + 4 * p) + acc_row<T>(lk, r)][lr] -= acc[p][r];
+        for (int r = 0; r < 4; ++r) Xo[16 * (W + NW * p) + acc_row<T>(lk, r)][lr] -= acc[p][r];
 }
 
 // A lower, left, untransposed 2*IBS-row block of the recursion in one launch, instead of base,
@@ -523,39 +525,43 @@ __device__ __forceinline__ void sub_prod_wave(const T *__restrict__ A21, i64 lda
 // and keeps their 2*IBS rows (Xs) and X1 (Ys) in LDS throughout, so B is read and written
 // once and the three steps need no launches between them.  Dynamic LDS: 3 IBS (BCW+1) values.
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_inv2_kernel(int m, const T *__restrict__ Inv1, const T *__restrict__ Inv2,
+__global__ __launch_bounds__(512) void trsm_inv2_kernel(int m, const T *__restrict__ Inv1, const T *__restrict__ Inv2,
                                                         const T *__restrict__ A21, i64 lda, T *__restrict__ B, i64 ldb) {
     extern __shared__ __align__(16) unsigned char inv2_smem[];
     T (*Xs)[BCW + 1] = reinterpret_cast<T (*)[BCW + 1]>(inv2_smem);  // 2 IBS rows
     T (*Ys)[BCW + 1] = Xs + 2 * IBS;                                  // IBS rows
     const int c0 = blockIdx.x * BCW, cw = min(BCW, m - c0);
     const int t = threadIdx.x;
-    for (int idx = t; idx < 2 * IBS * BCW; idx += 256) {
+    for (int idx = t; idx < 2 * IBS * BCW; idx += 512) {
         const int r = idx % (2 * IBS), c = idx / (2 * IBS);
         Xs[r][c] = c < cw ? B[r + (i64)(c0 + c) * ldb] : T(0);
     }
     __syncthreads();
+    // eight waves, two per SIMD: each carries half the tiles of the four-wave kernel
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-    switch (w) {
-    case 0: inv_apply_wave<T, false, false, 0>(Inv1, B, ldb, Xs, c0, cw, Ys); break;
-    case 1: inv_apply_wave<T, false, false, 1>(Inv1, B, ldb, Xs, c0, cw, Ys); break;
-    case 2: inv_apply_wave<T, false, false, 2>(Inv1, B, ldb, Xs, c0, cw, Ys); break;
-    default: inv_apply_wave<T, false, false, 3>(Inv1, B, ldb, Xs, c0, cw, Ys); break;
+#define NLA_INV2_WAVES(STMT)                                                                                       \
+    switch (w) {                                                                                                   \
+    case 0: STMT(0); break;                                                                                        \
+    case 1: STMT(1); break;                                                                                        \
+    case 2: STMT(2); break;                                                                                        \
+    case 3: STMT(3); break;                                                                                        \
+    case 4: STMT(4); break;                                                                                        \
+    case 5: STMT(5); break;                                                                                        \
+    case 6: STMT(6); break;                                                                                        \
+    default: STMT(7); break;                                                                                       \
     }
+#define NLA_INV2_P1(W) inv_apply_wave<T, false, false, W, 8>(Inv1, B, ldb, Xs, c0, cw, Ys)
+#define NLA_INV2_P2(W) sub_prod_wave<T, W, 8>(A21, lda, Ys, Xs + IBS)
+#define NLA_INV2_P3(W) inv_apply_wave<T, false, false, W, 8>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw)
+    NLA_INV2_WAVES(NLA_INV2_P1)
     __syncthreads();
-    switch (w) {
-    case 0: sub_prod_wave<T, 0>(A21, lda, Ys, Xs + IBS); break;
-    case 1: sub_prod_wave<T, 1>(A21, lda, Ys, Xs + IBS); break;
-    case 2: sub_prod_wave<T, 2>(A21, lda, Ys, Xs + IBS); break;
-    default: sub_prod_wave<T, 3>(A21, lda, Ys, Xs + IBS); break;
-    }
+    NLA_INV2_WAVES(NLA_INV2_P2)
     __syncthreads();
-    switch (w) {
-    case 0: inv_apply_wave<T, false, false, 0>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw); break;
-    case 1: inv_apply_wave<T, false, false, 1>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw); break;
-    case 2: inv_apply_wave<T, false, false, 2>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw); break;
-    default: inv_apply_wave<T, false, false, 3>(Inv2, B + IBS, ldb, Xs + IBS, c0, cw); break;
-    }
+    NLA_INV2_WAVES(NLA_INV2_P3)
+#undef NLA_INV2_P1
+#undef NLA_INV2_P2
+#undef NLA_INV2_P3
+#undef NLA_INV2_WAVES
 }
 
 template <typename T, bool UP, bool TRB>
@@ -627,7 +633,7 @@ static hipError_t inv2_launch(int m, const T *Inv1, const T *A21, i64 lda, T *B,
         attr[sizeof(T) == 8] = true;
     }
     const dim3 grid((unsigned)((m + BCW - 1) / BCW));
-    trsm_inv2_kernel<T><<<grid, 256, sh, s>>>(m, Inv1, Inv1 + (i64)IBS * IBS, A21, lda, B, ldb);
+    trsm_inv2_kernel<T><<<grid, 512, sh, s>>>(m, Inv1, Inv1 + (i64)IBS * IBS, A21, lda, B, ldb);
     return hipGetLastError();
 }
 
