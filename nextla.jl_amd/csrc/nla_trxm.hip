@@ -410,15 +410,17 @@ constexpr int IBS = 256;
 
 // X = Inv B for one base block of IBS rows, in place: a workgroup owns BCW right-hand sides
 // and every row (it reads them all into LDS before writing any).  Inv is dense column-major
-// (ld IBS), zero outside its lower (UP false) or upper triangle.  Wave W computes the output
-// row tiles {W, 15-W} and {4+W, 11-W}: 34 K steps of 16 per wave, equal for every wave.
+// (ld IBS), zero outside its lower (UP false) or upper triangle.  With NW = 4 waves, wave W
+// computes the output row tiles {W, 15-W} and {4+W, 11-W} (34 K steps of 16 per wave); with
+// NW = 8 (two waves per SIMD, the launches below) the pair {W, 15-W} (17 steps): equal work
+// for every wave either way.
 //
 // A pair's two tiles need 17 K steps of 16 between them (tile x: first step f(x), count
 // c(x); lower f = 0, c = x + 1, upper f = x, c = 16 - x; c(x) + c(15 - x) = 17), so each pair
 // is one fixed 17-step sequence: the first c(a) steps accumulate tile a, the rest tile b.
 // W is a template parameter, so every tile and step index is a constant and each tile has
 // its own accumulator: the steps are straight-line loads and MFMAs (the loads of step v + PF
-// issued while step v multiplies), the two pairs' chains interleaved.
+// issued while step v multiplies), with four waves the two pairs' chains interleaved.
 template <typename T, bool UP, bool TRB, int W, int NW = 4>
 __device__ __forceinline__ void inv_apply_wave(const T *__restrict__ Inv, T *__restrict__ B, i64 ldb,
                                                const T (*Xs)[BCW + 1], int c0, int cw, T (*Ys)[BCW + 1] = nullptr) {
@@ -565,12 +567,12 @@ __global__ __launch_bounds__(512) void trsm_inv2_kernel(int m, const T *__restri
 }
 
 template <typename T, bool UP, bool TRB>
-__global__ __launch_bounds__(256) void trsm_inv_apply_kernel(int m, const T *__restrict__ Inv, T *__restrict__ B,
+__global__ __launch_bounds__(512) void trsm_inv_apply_kernel(int m, const T *__restrict__ Inv, T *__restrict__ B,
                                                              i64 ldb) {
     __shared__ T Xs[IBS][BCW + 1];
     const int c0 = blockIdx.x * BCW, cw = min(BCW, m - c0);
     const int t = threadIdx.x;
-    for (int idx = t; idx < IBS * BCW; idx += 256) {
+    for (int idx = t; idx < IBS * BCW; idx += 512) {
         int r, c;
         if (TRB) { c = idx % BCW; r = idx / BCW; } else { r = idx % IBS; c = idx / IBS; }
         T v = 0;
@@ -578,12 +580,17 @@ __global__ __launch_bounds__(256) void trsm_inv_apply_kernel(int m, const T *__r
         Xs[r][c] = v;
     }
     __syncthreads();
-    // the wave index is uniform across the wave: one scalar branch picks the wave's body
+    // eight waves, two per SIMD, one tile pair each; the wave index is uniform across the
+    // wave: one scalar branch picks the wave's body
     switch (__builtin_amdgcn_readfirstlane(t >> 6)) {
-    case 0: inv_apply_wave<T, UP, TRB, 0>(Inv, B, ldb, Xs, c0, cw); break;
-    case 1: inv_apply_wave<T, UP, TRB, 1>(Inv, B, ldb, Xs, c0, cw); break;
-    case 2: inv_apply_wave<T, UP, TRB, 2>(Inv, B, ldb, Xs, c0, cw); break;
-    default: inv_apply_wave<T, UP, TRB, 3>(Inv, B, ldb, Xs, c0, cw); break;
+    case 0: inv_apply_wave<T, UP, TRB, 0, 8>(Inv, B, ldb, Xs, c0, cw); break;
+    case 1: inv_apply_wave<T, UP, TRB, 1, 8>(Inv, B, ldb, Xs, c0, cw); break;
+    case 2: inv_apply_wave<T, UP, TRB, 2, 8>(Inv, B, ldb, Xs, c0, cw); break;
+    case 3: inv_apply_wave<T, UP, TRB, 3, 8>(Inv, B, ldb, Xs, c0, cw); break;
+    case 4: inv_apply_wave<T, UP, TRB, 4, 8>(Inv, B, ldb, Xs, c0, cw); break;
+    case 5: inv_apply_wave<T, UP, TRB, 5, 8>(Inv, B, ldb, Xs, c0, cw); break;
+    case 6: inv_apply_wave<T, UP, TRB, 6, 8>(Inv, B, ldb, Xs, c0, cw); break;
+    default: inv_apply_wave<T, UP, TRB, 7, 8>(Inv, B, ldb, Xs, c0, cw); break;
     }
 }
 
@@ -614,10 +621,10 @@ template <typename T>
 static hipError_t inv_apply_launch(bool up, bool trB, int m, const T *Inv, T *B, i64 ldb, hipStream_t s) {
     ProfScope prof(PK_TRSM_BASE, s);
     const dim3 grid((unsigned)((m + BCW - 1) / BCW));
-    if (!up && !trB) trsm_inv_apply_kernel<T, false, false><<<grid, 256, 0, s>>>(m, Inv, B, ldb);
-    else if (!up && trB) trsm_inv_apply_kernel<T, false, true><<<grid, 256, 0, s>>>(m, Inv, B, ldb);
-    else if (up && !trB) trsm_inv_apply_kernel<T, true, false><<<grid, 256, 0, s>>>(m, Inv, B, ldb);
-    else trsm_inv_apply_kernel<T, true, true><<<grid, 256, 0, s>>>(m, Inv, B, ldb);
+    if (!up && !trB) trsm_inv_apply_kernel<T, false, false><<<grid, 512, 0, s>>>(m, Inv, B, ldb);
+    else if (!up && trB) trsm_inv_apply_kernel<T, false, true><<<grid, 512, 0, s>>>(m, Inv, B, ldb);
+    else if (up && !trB) trsm_inv_apply_kernel<T, true, false><<<grid, 512, 0, s>>>(m, Inv, B, ldb);
+    else trsm_inv_apply_kernel<T, true, true><<<grid, 512, 0, s>>>(m, Inv, B, ldb);
     return hipGetLastError();
 }
 
