@@ -708,6 +708,9 @@ struct DagArgs {
 // not a DagArgs field: the Float64 kernel sits at its register limit and one more argument
 // makes it spill.  Set by the host before each launch (launches on one device are ordered).
 __device__ unsigned char *g_lpimg = nullptr;
+// 16-bit walkers: the trailing matrix in Float32 between steps (same positions and leading
+// dimension as the matrix; set by the host before each such launch).
+__device__ float *g_shadow = nullptr;
 
 // 16-bit walker support, defined with that walker below
 template <typename Tw, int NB> struct LdsLp;
@@ -1240,12 +1243,25 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
     const i64 ldc = g.ldc;
     T *Akj = Cm + (i64)k * NB + col * ldc;
     T *stg = stage_ptr_lp<T, Tw, NB>();
+    // Strips between steps live in the Float32 shadow: step 0 reads the matrix, later steps
+    // the shadow; a step writes its rows into the matrix (Float64) only for the strips of the
+    // next panel's tile column, which that panel reads, and the final R row (A1) always.  The
+    // Float32 view of the staging columns lies inside the Float64 one.
+    float *const sh32 = g_shadow;
+    const bool src32 = k > 0, dst64 = s / (NB / TW) == k + 1;
+    float *stgf = reinterpret_cast<float *>(stg - (threadIdx.x >> 6) * SW * Lds<T, NB>::LDS_S) +
+                  (threadIdx.x >> 6) * SW * Lds<float, NB>::LDS_S;
+    auto load_strip = [&](i64 row) {
+        if (src32) stage_load<float, NB>(stgf, sh32 + row + col * ldc, ldc);
+        else stage_load<T, NB>(stg, Cm + row + col * ldc, ldc);
+    };
     unsigned char *const lpimg = g_lpimg;
     auto img = [&](int i, int B) { return lpimg + (lp_tile_index(k, i, g.mt) * NBLK + B) * IMG; };
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     WStrip<float, NB> A1;
-    tiles_load_cvt<float, T, NB / 16>(A1.a, Akj, ldc, 0);
+    if (src32) tiles_load_cvt<float, float, NB / 16>(A1.a, sh32 + (i64)k * NB + col * ldc, ldc, 0);
+    else tiles_load_cvt<float, T, NB / 16>(A1.a, Akj, ldc, 0);
     // unmqr!('L','C') with geqrt(A_kk)'s blocks (their images carry the unit-lower V)
     lp_img_load<Tw, NB>(img(k, 0), 0);
     bool have0 = false;
@@ -1279,7 +1295,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            stage_load<T, NB>(stg, Aij, ldc);
+            load_strip((i64)i * NB);
         }
         bool next0 = false;
 #pragma unroll
@@ -1295,7 +1311,8 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-                stage_to_regs<T, float, NB>(stg, S);
+                if (src32) stage_to_regs<float, float, NB>(stgf, S);
+                else stage_to_regs<T, float, NB>(stg, S);
             }
             if (B == NBLK - 2 && threadIdx.x == 0) {
                 fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
@@ -1312,10 +1329,11 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
             apply_lp<Tw, NB, false>(S, &A1.a[2 * B], B & 1);
         }
         have0 = next0;
-        stage_store<T, float, NB>(stg, S, Aij, ldc);
+        if (dst64) stage_store<T, float, NB>(stg, S, Aij, ldc);
+        else stage_store<float, float, NB>(stgf, S, sh32 + (i64)i * NB + col * ldc, ldc);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
-        if (haveS) stage_load<T, NB>(stg, Aij + NB, ldc);
+        if (haveS) load_strip((i64)(i + 1) * NB);
     }
     if (pending) publish(mine, pending);
     tiles_store_cvt<float, T, NB / 16>(A1.a, Akj, ldc, 0);
@@ -1464,8 +1482,13 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
         if (!lpimg) return st;
     }
     DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr, (void *)C, ldc, mode == DAG_APPLY_Q ? 1 : 0};
-    if constexpr (sizeof(Tw) == 2)
+    if constexpr (sizeof(Tw) == 2) {
         NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpimg), &lpimg, sizeof(lpimg), 0, hipMemcpyHostToDevice, s));
+        // the trailing matrix's Float32 shadow (same positions and leading dimension as C)
+        float *shadow = (float *)scratch(SLOT_SHADOW, sizeof(float) * (size_t)ldc * (size_t)ncols, s, &st);
+        if (!shadow) return st;
+        NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_shadow), &shadow, sizeof(shadow), 0, hipMemcpyHostToDevice, s));
+    }
     unsigned long long *phase = nullptr;
     if (trace) {
         phase = (unsigned long long *)scratch(SLOT_W, 64 * sizeof(unsigned long long), s, &st);
