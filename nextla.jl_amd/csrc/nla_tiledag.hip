@@ -991,10 +991,21 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     const i64 ldc = g.ldc;
     T *Akj = Cm + (i64)k * NB + col * ldc;
     T *stg = stage_ptr<T, Tw, NB>();
+    // Strips between steps live in the Float32 shadow, as in the 16-bit walker: step 0 reads
+    // the matrix, later steps the shadow; rows go to the matrix only for the next panel's tile
+    // column and the final R row (A1).
+    float *const sh32 = g_shadow;
+    const bool src32 = k > 0, dst64 = s / (NB / TW) == k + 1;
+    float *stgf = stage_ptr<float, Tw, NB>();
+    auto load_strip = [&](i64 row) {
+        if (src32) stage_load<float, NB>(stgf, sh32 + row + col * ldc, ldc);
+        else stage_load<T, NB>(stg, Cm + row + col * ldc, ldc);
+    };
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     WStrip<Tw, NB> A1;
-    tiles_load_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
+    if (src32) tiles_load_cvt<Tw, float, NB / 16>(A1.a, sh32 + (i64)k * NB + col * ldc, ldc, 0);
+    else tiles_load_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
     VPre<T, NB> P;
     vload<T, NB>(P, Akk, lda, Tkk, ldt);
     vstore_cvt<T, Tw, NB, true>(P, 0, 0);
@@ -1036,7 +1047,7 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            stage_load<T, NB>(stg, Aij, ldc);
+            load_strip((i64)i * NB);
         }
         bool next0 = false;
 #pragma unroll
@@ -1048,7 +1059,8 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-                stage_to_regs<T, Tw, NB>(stg, S);
+                if (src32) stage_to_regs<float, Tw, NB>(stgf, S);
+                else stage_to_regs<T, Tw, NB>(stg, S);
             }
             if (B == NBLK - 2 && threadIdx.x == 0) {
                 fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
@@ -1071,10 +1083,11 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
             if (B + 1 < NBLK || next0) vstore_cvt<T, Tw, NB, false>(P, (B + 1) & 1, 0);
         }
         have0 = next0;
-        stage_store<T, Tw, NB>(stg, S, Aij, ldc);
+        if (dst64) stage_store<T, Tw, NB>(stg, S, Aij, ldc);
+        else stage_store<float, Tw, NB>(stgf, S, sh32 + (i64)i * NB + col * ldc, ldc);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
-        if (haveS) stage_load<T, NB>(stg, Aij + NB, ldc);
+        if (haveS) load_strip((i64)(i + 1) * NB);
     }
     if (pending) publish(mine, pending);
     tiles_store_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
@@ -1482,8 +1495,9 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
         if (!lpimg) return st;
     }
     DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr, (void *)C, ldc, mode == DAG_APPLY_Q ? 1 : 0};
-    if constexpr (sizeof(Tw) == 2) {
+    if constexpr (sizeof(Tw) == 2)
         NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpimg), &lpimg, sizeof(lpimg), 0, hipMemcpyHostToDevice, s));
+    if constexpr (sizeof(Tw) < sizeof(T)) {
         // the trailing matrix's Float32 shadow (same positions and leading dimension as C)
         float *shadow = (float *)scratch(SLOT_SHADOW, sizeof(float) * (size_t)ldc * (size_t)ncols, s, &st);
         if (!shadow) return st;
