@@ -90,6 +90,10 @@ __device__ void publish(int *p, int v) {
 // are staged once in LDS and shared by the four waves (four strips = TW columns).
 constexpr int WAVES = NTH / 64;
 constexpr int TW = SW * WAVES;  // matrix columns per workgroup task
+// Columns per walker task.  The 16-bit walkers (NB >= 128) give each wave two 16-column
+// strips: the block's V/T operands are read once for both, and the per-block image load and
+// barrier are paid once per 2 TW columns.
+template <typename Tw, int NB> constexpr int walk_width() { return sizeof(Tw) == 2 && NB >= 128 ? 2 * TW : TW; }
 
 template <typename T, int NB> struct Lds {
     static constexpr int LDV = NB + 2;                 // Vs[j * LDV + p] = V[p][j]
@@ -731,7 +735,7 @@ template <typename T> struct PanelTwT<T, __bf16> { using type = __bf16; };
 template <typename T, typename Tw> using PanelTw = typename PanelTwT<T, Tw>::type;
 template <typename T, int NB, typename Tw>
 __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
-    constexpr int WPT = NB / TW, NBLK = NB / IB;  // walker groups per tile column, stages
+    constexpr int WPT = NB / walk_width<Tw, NB>(), NBLK = NB / IB;  // walker groups per tile column, stages
     const int c0 = b * IB;
     int *mine = &g.panel_prog[k * NBLK + b];
     const int *prev = b > 0 ? &g.panel_prog[k * NBLK + b - 1] : nullptr;
@@ -1142,10 +1146,6 @@ static_assert(LdsLp<_Float16, 128>::IMG_BYTES == LdsLp<__bf16, 128>::IMG_BYTES &
 __host__ __device__ inline i64 lp_tile_index(int k, int i, int mt) {
     return (i64)k * mt - (i64)k * (k - 1) / 2 + (i - k);
 }
-template <typename T, typename Tw, int NB> __device__ __forceinline__ T *stage_ptr_lp() {
-    return reinterpret_cast<T *>(lds_base<unsigned char>() + LdsLp<Tw, NB>::SSTG_BYTES) +
-           (threadIdx.x >> 6) * SW * Lds<T, NB>::LDS_S;
-}
 // Float64 -> Tw through Float32: the hardware converts Float32 -> FP16 / BF16 in one
 // instruction, while a direct Float64 -> FP16 rounding is a long integer sequence (and the
 // compiler folds a plain two-step cast back into it; the empty asm keeps the Float32 value).
@@ -1199,45 +1199,116 @@ template <typename Tw, int NB> __device__ __forceinline__ void lp_img_load(const
         }
     }
 }
-// H^T (the reference's trans 'C') of the staged block applied to the wave's strip:
+// H^T (the reference's trans 'C') of the staged block applied to the wave's NS strips:
 //   W0 = V^T S (+ A1 rows),  W = T^T W0,  A1 rows -= W,  S -= V W.
-template <typename Tw, int NB, bool GE>
-__device__ __forceinline__ void apply_lp(WStrip<float, NB> &S, fx4 *a1, int buf) {
+// Every V / T operand read from LDS serves the NS strips' products.
+template <typename Tw, int NB, bool GE, int NS>
+__device__ __forceinline__ void apply_lp(WStrip<float, NB> *S, WStrip<float, NB> *A1, int B, int buf) {
     using L = LdsLp<Tw, NB>;
     using M = LpMfma<Tw>;
     typedef typename M::v4 v4;
     const Tw *sm = lds_base<Tw>() + buf * L::BUFS;
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-    fx4 w0[2] = {acc_zero<float>(), acc_zero<float>()};
+    fx4 w0[NS][2];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) w0[q][0] = w0[q][1] = acc_zero<float>();
 #pragma unroll
     for (int rt = 0; rt < NB / 16; ++rt) {
-        const v4 b = lp_cvt<Tw>(S.a[rt], 1.0f);
+        v4 a[2];
 #pragma unroll
-        for (int ot = 0; ot < 2; ++ot)
-            M::mma(w0[ot], *reinterpret_cast<const v4 *>(sm + L::VC + (16 * ot + lr) * L::LDC + 16 * rt + 4 * lk), b);
+        for (int ot = 0; ot < 2; ++ot) a[ot] = *reinterpret_cast<const v4 *>(sm + L::VC + (16 * ot + lr) * L::LDC + 16 * rt + 4 * lk);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const v4 b = lp_cvt<Tw>(S[q].a[rt], 1.0f);
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) M::mma(w0[q][ot], a[ot], b);
+        }
     }
     if (!GE) {
 #pragma unroll
-        for (int ot = 0; ot < 2; ++ot) w0[ot] += a1[ot];
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) w0[q][ot] += A1[q].a[2 * B + ot];
     }
-    fx4 wv[2] = {acc_zero<float>(), acc_zero<float>()};
+    fx4 wv[NS][2];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) wv[q][0] = wv[q][1] = acc_zero<float>();
 #pragma unroll
     for (int tq = 0; tq < 2; ++tq) {
-        const v4 b = lp_cvt<Tw>(w0[tq], 1.0f);
+        v4 a[2];
 #pragma unroll
-        for (int ot = 0; ot < 2; ++ot)
-            M::mma(wv[ot], *reinterpret_cast<const v4 *>(sm + L::TC + (16 * ot + lr) * L::LDR + 16 * tq + 4 * lk), b);
+        for (int ot = 0; ot < 2; ++ot) a[ot] = *reinterpret_cast<const v4 *>(sm + L::TC + (16 * ot + lr) * L::LDR + 16 * tq + 4 * lk);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const v4 b = lp_cvt<Tw>(w0[q][tq], 1.0f);
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) M::mma(wv[q][ot], a[ot], b);
+        }
     }
     if (!GE) {
 #pragma unroll
-        for (int ot = 0; ot < 2; ++ot) a1[ot] -= wv[ot];
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) A1[q].a[2 * B + ot] -= wv[q][ot];
     }
 #pragma unroll
     for (int tq = 0; tq < 2; ++tq) {
-        const v4 b = lp_cvt<Tw>(wv[tq], -1.0f);
+        v4 b[NS];
 #pragma unroll
-        for (int rt = 0; rt < NB / 16; ++rt)
-            M::mma(S.a[rt], *reinterpret_cast<const v4 *>(sm + L::VR + (16 * rt + lr) * L::LDR + 16 * tq + 4 * lk), b);
+        for (int q = 0; q < NS; ++q) b[q] = lp_cvt<Tw>(wv[q][tq], -1.0f);
+#pragma unroll
+        for (int rt = 0; rt < NB / 16; ++rt) {
+            const v4 a = *reinterpret_cast<const v4 *>(sm + L::VR + (16 * rt + lr) * L::LDR + 16 * tq + 4 * lk);
+#pragma unroll
+            for (int q = 0; q < NS; ++q) M::mma(S[q].a[rt], a, b[q]);
+        }
+    }
+}
+
+// Float32 accumulator strips <-> memory without staging.  In the 16x16x16 MFMA layout a lane
+// holds rows 4(l/16)..4(l/16)+3 of column l%16 of each 16-row tile: 16 contiguous bytes of a
+// column-major Float32 matrix (32 of a Float64 one), so a tile moves as one device-coherent
+// 16-byte buffer access per lane (two for Float64).  G: the strip's first element (the same
+// in every lane), ld its leading dimension; a 16-column strip of NB rows.
+template <typename Ts, int NB> __device__ __forceinline__ __amdgpu_buffer_rsrc_t strip_rsrc(const Ts *G, i64 ld) {
+    const unsigned long long gb = reinterpret_cast<unsigned long long>(G);
+    const unsigned glo = __builtin_amdgcn_readfirstlane((unsigned)gb), ghi = __builtin_amdgcn_readfirstlane((unsigned)(gb >> 32));
+    const unsigned bytes = (unsigned)((((i64)SW - 1) * ld + NB) * (i64)sizeof(Ts));
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(((unsigned long long)ghi << 32) | glo), 0,
+                                             __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+template <typename Ts, int NB> __device__ __forceinline__ void strip_load(fx4 *a, const Ts *G, i64 ld) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc<Ts, NB>(G, ld);
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NB / 16; ++rt) {
+        const int off = (int)(((i64)lr * ld + 16 * rt + 4 * lk) * (i64)sizeof(Ts));
+        if constexpr (sizeof(Ts) == 4) {
+            a[rt] = __builtin_bit_cast(fx4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, /*sc1*/ 16));
+        } else {
+            const f64x2 d0 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 16));
+            const f64x2 d1 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + 16, 0, 16));
+            a[rt] = fx4{(float)d0[0], (float)d0[1], (float)d1[0], (float)d1[1]};
+        }
+    }
+}
+template <typename Ts, int NB> __device__ __forceinline__ void strip_store(const fx4 *a, Ts *G, i64 ld) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc<Ts, NB>(G, ld);
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NB / 16; ++rt) {
+        const int off = (int)(((i64)lr * ld + 16 * rt + 4 * lk) * (i64)sizeof(Ts));
+        if constexpr (sizeof(Ts) == 4) {
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a[rt]), rsrc, off, 0, /*sc1*/ 16);
+        } else {
+            const f64x2 d0{(double)a[rt][0], (double)a[rt][1]}, d1{(double)a[rt][2], (double)a[rt][3]};
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d0), rsrc, off, 0, 16);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d1), rsrc, off + 16, 0, 16);
+        }
     }
 }
 
@@ -1247,34 +1318,42 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
     constexpr i64 IMG = LdsLp<Tw, NB>::IMG_BYTES;
     static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
     __shared__ int sh_ready;
+    constexpr int TWK = walk_width<Tw, NB>(), NS = TWK / TW;  // columns per task, strips per wave
     const int w = threadIdx.x >> 6;
-    const i64 col = (i64)s * TW + w * SW;
+    const i64 col = (i64)s * TWK + w * NS * SW;  // the wave's strip q starts at column col + 16 q
     const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
     int *mine = &g.strip_prog[k * g.nwalk + s];
     const int *pdone = &g.panel_prog[k * NBLK + NBLK - 1];
     T *Cm = reinterpret_cast<T *>(g.C);
     const i64 ldc = g.ldc;
-    T *Akj = Cm + (i64)k * NB + col * ldc;
-    T *stg = stage_ptr_lp<T, Tw, NB>();
     // Strips between steps live in the Float32 shadow: step 0 reads the matrix, later steps
     // the shadow; a step writes its rows into the matrix (Float64) only for the strips of the
-    // next panel's tile column, which that panel reads, and the final R row (A1) always.  The
-    // Float32 view of the staging columns lies inside the Float64 one.
+    // next panel's tile column, which that panel reads, and the final R row (A1) always.
+    // Strips move straight between memory and registers (strip_load / strip_store).
     float *const sh32 = g_shadow;
-    const bool src32 = k > 0, dst64 = s / (NB / TW) == k + 1;
-    float *stgf = reinterpret_cast<float *>(stg - (threadIdx.x >> 6) * SW * Lds<T, NB>::LDS_S) +
-                  (threadIdx.x >> 6) * SW * Lds<float, NB>::LDS_S;
-    auto load_strip = [&](i64 row) {
-        if (src32) stage_load<float, NB>(stgf, sh32 + row + col * ldc, ldc);
-        else stage_load<T, NB>(stg, Cm + row + col * ldc, ldc);
+    const bool src32 = k > 0, dst64 = s / (NB / TWK) == k + 1;
+    auto load_strips = [&](WStrip<float, NB> *X, int row_tile) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const i64 off = (i64)row_tile * NB + (col + q * SW) * ldc;
+            if (src32) strip_load<float, NB>(X[q].a, sh32 + off, ldc);
+            else strip_load<T, NB>(X[q].a, Cm + off, ldc);
+        }
+    };
+    auto store_strips = [&](const WStrip<float, NB> *X, int row_tile, bool f64) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const i64 off = (i64)row_tile * NB + (col + q * SW) * ldc;
+            if (f64) strip_store<T, NB>(X[q].a, Cm + off, ldc);
+            else strip_store<float, NB>(X[q].a, sh32 + off, ldc);
+        }
     };
     unsigned char *const lpimg = g_lpimg;
     auto img = [&](int i, int B) { return lpimg + (lp_tile_index(k, i, g.mt) * NBLK + B) * IMG; };
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
-    WStrip<float, NB> A1;
-    if (src32) tiles_load_cvt<float, float, NB / 16>(A1.a, sh32 + (i64)k * NB + col * ldc, ldc, 0);
-    else tiles_load_cvt<float, T, NB / 16>(A1.a, Akj, ldc, 0);
+    WStrip<float, NB> A1[NS];
+    load_strips(A1, k);
     // unmqr!('L','C') with geqrt(A_kk)'s blocks (their images carry the unit-lower V)
     lp_img_load<Tw, NB>(img(k, 0), 0);
     bool have0 = false;
@@ -1288,16 +1367,15 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
             have0 = ready_ge(pdone, k + 2, &sh_ready);
             if (have0) lp_img_load<Tw, NB>(img(k + 1, 0), 0);
         }
-        apply_lp<Tw, NB, true>(A1, nullptr, B & 1);
+        apply_lp<Tw, NB, true, NS>(A1, nullptr, B, B & 1);
     }
-    WStrip<float, NB> S;
-    bool haveS = false;
+    WStrip<float, NB> S[NS], N[NS];  // the strips of the current row, and the next row's in flight
+    bool haveS = false;  // N holds (or is loading) this row's strips
     int pending = 0;
     // the next-row prefetch flags, read a block early (as in the Float64 walker)
     __shared__ int sh_nx[2];
     int fl_p = 0, fl_s = 0;
     for (int i = k + 1; i < g.mt; ++i) {
-        T *Aij = Cm + (i64)i * NB + col * ldc;
         if (pending && (!have0 || !haveS)) {
             publish(mine, pending);
             pending = 0;
@@ -1308,7 +1386,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         }
         if (!haveS) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            load_strip((i64)i * NB);
+            load_strips(N, i);
         }
         bool next0 = false;
 #pragma unroll
@@ -1324,8 +1402,8 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-                if (src32) stage_to_regs<float, float, NB>(stgf, S);
-                else stage_to_regs<T, float, NB>(stg, S);
+#pragma unroll
+                for (int q = 0; q < NS; ++q) S[q] = N[q];
             }
             if (B == NBLK - 2 && threadIdx.x == 0) {
                 fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
@@ -1339,17 +1417,18 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
                 next0 = sh_nx[0] != 0;
                 if (next0) lp_img_load<Tw, NB>(img(i + 1, 0), 0);
             }
-            apply_lp<Tw, NB, false>(S, &A1.a[2 * B], B & 1);
+            if (B == NBLK - 1) {  // the next row's strip, when its previous step has released it
+                haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
+                if (haveS) load_strips(N, i + 1);
+            }
+            apply_lp<Tw, NB, false, NS>(S, A1, B, B & 1);
         }
         have0 = next0;
-        if (dst64) stage_store<T, float, NB>(stg, S, Aij, ldc);
-        else stage_store<float, float, NB>(stgf, S, sh32 + (i64)i * NB + col * ldc, ldc);
+        store_strips(S, i, dst64);
         pending = i + 1;
-        haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
-        if (haveS) load_strip((i64)(i + 1) * NB);
     }
     if (pending) publish(mine, pending);
-    tiles_store_cvt<float, T, NB / 16>(A1.a, Akj, ldc, 0);
+    store_strips(A1, k, true);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
@@ -1397,8 +1476,8 @@ __global__ void fill_int_kernel(int *p, int n, int v) {
 // cannot be claimed before they finish.  The weight 5 : 1 (a panel about 0.2 steps of column
 // distance ahead per step) is the best of a timed simulation of claim order and row
 // dependencies at 4096..16384 squared and 32768 x 8192; it is not critical between 0.1 and 0.3.
-static std::vector<int> build_tasks(int mt, int nt, int nb) {
-    const int kt = mt < nt ? mt : nt, spt = nb / TW, nblk = nb / IB;
+static std::vector<int> build_tasks(int mt, int nt, int nb, int tw) {
+    const int kt = mt < nt ? mt : nt, spt = nb / tw, nblk = nb / IB;
     struct Item {
         long long key;
         int k, j, sub, code;
@@ -1437,10 +1516,11 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
     using namespace dag;
     const int mt = (int)(m / NB), nt = (int)(n / NB), kt = mt < nt ? mt : nt;
     const int nblk = NB / IB;
-    const int nwalk = (int)((mode == DAG_FACTOR ? n : ncols) / TW);
+    constexpr int TWK = walk_width<Tw, NB>();
+    const int nwalk = (int)((mode == DAG_FACTOR ? n : ncols) / TWK);
     std::vector<int> tasks;
     if (mode == DAG_FACTOR) {
-        tasks = build_tasks(mt, nt, NB);
+        tasks = build_tasks(mt, nt, NB, TWK);
     } else if (mode == DAG_APPLY) {
         for (int k = 0; k < kt; ++k)
             for (int w = 0; w < nwalk; ++w) tasks.push_back(enc(TASK_WALK, k, w));
@@ -1599,14 +1679,15 @@ template int tile_apply_qt_dag<float>(i64, i64, i64, i64, const float *, i64, co
 
 // The factorization's task list (the claim order of the persistent launch), for inspection:
 // code = type << 30 | k << 18 | s with type 0 = PANEL(k, block stage s), 1 = WALK(k, strip
-// group s of TW = 64 columns).  Returns the number of tasks (codes written up to cap), or -1
+// group s of TW = 64 columns; the 16-bit-update factorization's list is the same with groups
+// of 128 columns when nb = 128).  Returns the number of tasks (codes written up to cap), or -1
 // for a shape the dataflow kernel does not take.
 NLA_API int64_t nla_tile_qr_schedule(int64_t m, int64_t n, int64_t nb, int32_t *codes, int64_t cap) {
     if ((nb != 64 && nb != 128) || m < nb || n < nb || m % nb || n % nb || m / nb >= 4096) {
         nla::set_error("tile QR schedule: nb must be 64 or 128 and m, n positive multiples of nb");
         return -1;
     }
-    const std::vector<int> v = nla::dag::build_tasks((int)(m / nb), (int)(n / nb), (int)nb);
+    const std::vector<int> v = nla::dag::build_tasks((int)(m / nb), (int)(n / nb), (int)nb, nla::dag::TW);
     for (size_t i = 0; i < v.size() && (int64_t)i < cap; ++i) codes[i] = v[i];
     return (int64_t)v.size();
 }
