@@ -1312,6 +1312,43 @@ template <typename Ts, int NB> __device__ __forceinline__ void strip_store(const
     }
 }
 
+// The next row's Float32 strips of the 16-bit walker arrive by LDS-DMA (no registers held
+// while the current row is applied).  A wave's NS * SW columns are staged in groups of
+// 256 / NB columns, one 1 KB DMA instruction each (64 lanes x 16 bytes), the groups 16 bytes
+// apart against bank conflicts; in the Float64 staging region, which this fits exactly at
+// NB = 128.
+template <typename Tw, int NB, int NS> struct LpStage {
+    static constexpr int CPD = 256 / NB, LPC = NB / 4, GRP = 1024 + 16, NG = NS * SW / CPD;
+    static_assert(NS * SW % CPD == 0, "whole groups per wave");
+    __device__ static unsigned char *base() {
+        return lds_base<unsigned char>() + LdsLp<Tw, NB>::SSTG_BYTES + (threadIdx.x >> 6) * NG * GRP;
+    }
+    // issue the loads of the wave's columns G, G + ld, ... (rows 0..NB-1)
+    __device__ static void load(const float *G, i64 ld) {
+        const int lane = threadIdx.x & 63;
+        unsigned char *b = base();
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // earlier reads of the staging are done
+#pragma unroll
+        for (int gi = 0; gi < NG; ++gi) {
+            const float *src = G + (i64)(gi * CPD + lane / LPC) * ld + (lane % LPC) * 4;
+            auto *d = (__attribute__((address_space(3))) void *)(b + gi * GRP);
+            __builtin_amdgcn_global_load_lds(src, d, 16, 0, /*sc1*/ 16);
+        }
+    }
+    // staged columns -> the wave's NS accumulator strips (after the loads' vmcnt wait)
+    __device__ static void read(WStrip<float, NB> *S) {
+        const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+        const unsigned char *b = base();
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const int c = q * SW + lr;
+            const unsigned char *cb = b + (c / CPD) * GRP + (c % CPD) * NB * 4 + 16 * lk;
+#pragma unroll
+            for (int rt = 0; rt < NB / 16; ++rt) S[q].a[rt] = *reinterpret_cast<const fx4 *>(cb + 64 * rt);
+        }
+    }
+};
+
 template <typename T, typename Tw, int NB>
 __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
     constexpr int NBLK = NB / IB;
@@ -1329,7 +1366,8 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
     // Strips between steps live in the Float32 shadow: step 0 reads the matrix, later steps
     // the shadow; a step writes its rows into the matrix (Float64) only for the strips of the
     // next panel's tile column, which that panel reads, and the final R row (A1) always.
-    // Strips move straight between memory and registers (strip_load / strip_store).
+    // Strips are stored straight from registers (strip_store); the next row's Float32 strips
+    // are prefetched into LDS (LpStage), step 0's Float64 ones loaded at the row start.
     float *const sh32 = g_shadow;
     const bool src32 = k > 0, dst64 = s / (NB / TWK) == k + 1;
     auto load_strips = [&](WStrip<float, NB> *X, int row_tile) {
@@ -1369,8 +1407,10 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         }
         apply_lp<Tw, NB, true, NS>(A1, nullptr, B, B & 1);
     }
-    WStrip<float, NB> S[NS], N[NS];  // the strips of the current row, and the next row's in flight
-    bool haveS = false;  // N holds (or is loading) this row's strips
+    using Stg = LpStage<Tw, NB, NS>;
+    static_assert(WAVES * Stg::NG * Stg::GRP <= TW * Lds<T, NB>::LDS_S * (int)sizeof(T), "staging fits the strip region");
+    WStrip<float, NB> S[NS];
+    bool haveS = false;  // this row's strips are staged (or being staged)
     int pending = 0;
     // the next-row prefetch flags, read a block early (as in the Float64 walker)
     __shared__ int sh_nx[2];
@@ -1384,9 +1424,9 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
             wait_ge(pdone, i + 1, g.err, sh_flag);
             lp_img_load<Tw, NB>(img(i, 0), 0);
         }
-        if (!haveS) {
+        if (!haveS && src32) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            load_strips(N, i);
+            Stg::load(sh32 + (i64)i * NB + col * ldc, ldc);
         }
         bool next0 = false;
 #pragma unroll
@@ -1402,8 +1442,8 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-#pragma unroll
-                for (int q = 0; q < NS; ++q) S[q] = N[q];
+                if (src32) Stg::read(S);
+                else load_strips(S, i);  // step 0 (no previous step to wait for)
             }
             if (B == NBLK - 2 && threadIdx.x == 0) {
                 fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
@@ -1419,7 +1459,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
             }
             if (B == NBLK - 1) {  // the next row's strip, when its previous step has released it
                 haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
-                if (haveS) load_strips(N, i + 1);
+                if (haveS && src32) Stg::load(sh32 + (i64)(i + 1) * NB + col * ldc, ldc);
             }
             apply_lp<Tw, NB, false, NS>(S, A1, B, B & 1);
         }
