@@ -979,6 +979,91 @@ __device__ __forceinline__ void vstore_cvt(const VPre<T, NB> &P, int buf, int j0
     }
 }
 
+// Float32 accumulator strips <-> memory without staging.  In the Float32-accumulator 16x16
+// MFMA layout (v_mfma_f32_16x16x4f32, _16x16x16_f16 / _bf16) a lane holds rows
+// 4(l/16)..4(l/16)+3 of column l%16 of each 16-row tile: 16 contiguous bytes of a
+// column-major Float32 matrix (32 of a Float64 one), so a tile moves as one device-coherent
+// 16-byte buffer access per lane (two for Float64).  G: the strip's first element (the same
+// in every lane), ld its leading dimension; a 16-column strip of NB rows.
+template <typename Ts, int NB> __device__ __forceinline__ __amdgpu_buffer_rsrc_t strip_rsrc(const Ts *G, i64 ld) {
+    const unsigned long long gb = reinterpret_cast<unsigned long long>(G);
+    const unsigned glo = __builtin_amdgcn_readfirstlane((unsigned)gb), ghi = __builtin_amdgcn_readfirstlane((unsigned)(gb >> 32));
+    const unsigned bytes = (unsigned)((((i64)SW - 1) * ld + NB) * (i64)sizeof(Ts));
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(((unsigned long long)ghi << 32) | glo), 0,
+                                             __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+template <typename Ts, int NB> __device__ __forceinline__ void strip_load(fx4 *a, const Ts *G, i64 ld) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc<Ts, NB>(G, ld);
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NB / 16; ++rt) {
+        const int off = (int)(((i64)lr * ld + 16 * rt + 4 * lk) * (i64)sizeof(Ts));
+        if constexpr (sizeof(Ts) == 4) {
+            a[rt] = __builtin_bit_cast(fx4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, /*sc1*/ 16));
+        } else {
+            const f64x2 d0 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 16));
+            const f64x2 d1 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + 16, 0, 16));
+            a[rt] = fx4{(float)d0[0], (float)d0[1], (float)d1[0], (float)d1[1]};
+        }
+    }
+}
+template <typename Ts, int NB> __device__ __forceinline__ void strip_store(const fx4 *a, Ts *G, i64 ld) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc<Ts, NB>(G, ld);
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rt = 0; rt < NB / 16; ++rt) {
+        const int off = (int)(((i64)lr * ld + 16 * rt + 4 * lk) * (i64)sizeof(Ts));
+        if constexpr (sizeof(Ts) == 4) {
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a[rt]), rsrc, off, 0, /*sc1*/ 16);
+        } else {
+            const f64x2 d0{(double)a[rt][0], (double)a[rt][1]}, d1{(double)a[rt][2], (double)a[rt][3]};
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d0), rsrc, off, 0, 16);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d1), rsrc, off + 16, 0, 16);
+        }
+    }
+}
+
+// The next row's Float32 strips of the reduced-precision walkers arrive by LDS-DMA (no
+// registers held while the current row is applied).  A wave's NS * SW columns are staged at
+// LDS byte offset BASE in groups of 256 / NB columns, one 1 KB DMA instruction each (64 lanes
+// x 16 bytes), the groups 16 bytes apart against bank conflicts; inside the Float64 staging
+// region of the kernel's layout.
+template <int BASE, int NB, int NS> struct StripStage {
+    static constexpr int CPD = 256 / NB, LPC = NB / 4, GRP = 1024 + 16, NG = NS * SW / CPD;
+    static_assert(NS * SW % CPD == 0, "whole groups per wave");
+    __device__ static unsigned char *base() {
+        return lds_base<unsigned char>() + BASE + (threadIdx.x >> 6) * NG * GRP;
+    }
+    // issue the loads of the wave's columns G, G + ld, ... (rows 0..NB-1)
+    __device__ static void load(const float *G, i64 ld) {
+        const int lane = threadIdx.x & 63;
+        unsigned char *b = base();
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // earlier reads of the staging are done
+#pragma unroll
+        for (int gi = 0; gi < NG; ++gi) {
+            const float *src = G + (i64)(gi * CPD + lane / LPC) * ld + (lane % LPC) * 4;
+            auto *d = (__attribute__((address_space(3))) void *)(b + gi * GRP);
+            __builtin_amdgcn_global_load_lds(src, d, 16, 0, /*sc1*/ 16);
+        }
+    }
+    // staged columns -> the wave's NS accumulator strips (after the loads' vmcnt wait)
+    __device__ static void read(WStrip<float, NB> *S) {
+        const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+        const unsigned char *b = base();
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const int c = q * SW + lr;
+            const unsigned char *cb = b + (c / CPD) * GRP + (c % CPD) * NB * 4 + 16 * lk;
+#pragma unroll
+            for (int rt = 0; rt < NB / 16; ++rt) S[q].a[rt] = *reinterpret_cast<const fx4 *>(cb + 64 * rt);
+        }
+    }
+};
+
 template <typename T, typename Tw, int NB>
 __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
     constexpr int NBLK = NB / IB;
@@ -996,20 +1081,23 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     T *Akj = Cm + (i64)k * NB + col * ldc;
     T *stg = stage_ptr<T, Tw, NB>();
     // Strips between steps live in the Float32 shadow, as in the 16-bit walker: step 0 reads
-    // the matrix, later steps the shadow; rows go to the matrix only for the next panel's tile
-    // column and the final R row (A1).
+    // the matrix (staged Float64), later steps the shadow (StripStage); rows go to the matrix
+    // only for the next panel's tile column and the final R row (A1), the others straight
+    // from the registers to the shadow.
+    static_assert(sizeof(Tw) == 4, "Float32 accumulator strips");
+    using Stg = StripStage<Lds<Tw, NB>::SSTG * (int)sizeof(Tw), NB, 1>;
+    static_assert(WAVES * Stg::NG * Stg::GRP <= TW * Lds<T, NB>::LDS_S * (int)sizeof(T), "staging fits the strip region");
     float *const sh32 = g_shadow;
     const bool src32 = k > 0, dst64 = s / (NB / TW) == k + 1;
-    float *stgf = stage_ptr<float, Tw, NB>();
     auto load_strip = [&](i64 row) {
-        if (src32) stage_load<float, NB>(stgf, sh32 + row + col * ldc, ldc);
+        if (src32) Stg::load(sh32 + row + col * ldc, ldc);
         else stage_load<T, NB>(stg, Cm + row + col * ldc, ldc);
     };
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     WStrip<Tw, NB> A1;
-    if (src32) tiles_load_cvt<Tw, float, NB / 16>(A1.a, sh32 + (i64)k * NB + col * ldc, ldc, 0);
-    else tiles_load_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
+    if (src32) strip_load<float, NB>(A1.a, sh32 + (i64)k * NB + col * ldc, ldc);
+    else strip_load<T, NB>(A1.a, Akj, ldc);
     VPre<T, NB> P;
     vload<T, NB>(P, Akk, lda, Tkk, ldt);
     vstore_cvt<T, Tw, NB, true>(P, 0, 0);
@@ -1063,7 +1151,7 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-                if (src32) stage_to_regs<float, Tw, NB>(stgf, S);
+                if (src32) Stg::read(&S);
                 else stage_to_regs<T, Tw, NB>(stg, S);
             }
             if (B == NBLK - 2 && threadIdx.x == 0) {
@@ -1088,13 +1176,13 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
         }
         have0 = next0;
         if (dst64) stage_store<T, Tw, NB>(stg, S, Aij, ldc);
-        else stage_store<float, Tw, NB>(stgf, S, sh32 + (i64)i * NB + col * ldc, ldc);
+        else strip_store<float, NB>(S.a, sh32 + (i64)i * NB + col * ldc, ldc);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
         if (haveS) load_strip((i64)(i + 1) * NB);
     }
     if (pending) publish(mine, pending);
-    tiles_store_cvt<Tw, T, NB / 16>(A1.a, Akj, ldc, 0);
+    strip_store<T, NB>(A1.a, Akj, ldc);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
@@ -1265,90 +1353,6 @@ __device__ __forceinline__ void apply_lp(WStrip<float, NB> *S, WStrip<float, NB>
     }
 }
 
-// Float32 accumulator strips <-> memory without staging.  In the 16x16x16 MFMA layout a lane
-// holds rows 4(l/16)..4(l/16)+3 of column l%16 of each 16-row tile: 16 contiguous bytes of a
-// column-major Float32 matrix (32 of a Float64 one), so a tile moves as one device-coherent
-// 16-byte buffer access per lane (two for Float64).  G: the strip's first element (the same
-// in every lane), ld its leading dimension; a 16-column strip of NB rows.
-template <typename Ts, int NB> __device__ __forceinline__ __amdgpu_buffer_rsrc_t strip_rsrc(const Ts *G, i64 ld) {
-    const unsigned long long gb = reinterpret_cast<unsigned long long>(G);
-    const unsigned glo = __builtin_amdgcn_readfirstlane((unsigned)gb), ghi = __builtin_amdgcn_readfirstlane((unsigned)(gb >> 32));
-    const unsigned bytes = (unsigned)((((i64)SW - 1) * ld + NB) * (i64)sizeof(Ts));
-    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(((unsigned long long)ghi << 32) | glo), 0,
-                                             __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
-}
-template <typename Ts, int NB> __device__ __forceinline__ void strip_load(fx4 *a, const Ts *G, i64 ld) {
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    typedef double f64x2 __attribute__((ext_vector_type(2)));
-    const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc<Ts, NB>(G, ld);
-    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int rt = 0; rt < NB / 16; ++rt) {
-        const int off = (int)(((i64)lr * ld + 16 * rt + 4 * lk) * (i64)sizeof(Ts));
-        if constexpr (sizeof(Ts) == 4) {
-            a[rt] = __builtin_bit_cast(fx4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, /*sc1*/ 16));
-        } else {
-            const f64x2 d0 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 16));
-            const f64x2 d1 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + 16, 0, 16));
-            a[rt] = fx4{(float)d0[0], (float)d0[1], (float)d1[0], (float)d1[1]};
-        }
-    }
-}
-template <typename Ts, int NB> __device__ __forceinline__ void strip_store(const fx4 *a, Ts *G, i64 ld) {
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-    typedef double f64x2 __attribute__((ext_vector_type(2)));
-    const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc<Ts, NB>(G, ld);
-    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int rt = 0; rt < NB / 16; ++rt) {
-        const int off = (int)(((i64)lr * ld + 16 * rt + 4 * lk) * (i64)sizeof(Ts));
-        if constexpr (sizeof(Ts) == 4) {
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a[rt]), rsrc, off, 0, /*sc1*/ 16);
-        } else {
-            const f64x2 d0{(double)a[rt][0], (double)a[rt][1]}, d1{(double)a[rt][2], (double)a[rt][3]};
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d0), rsrc, off, 0, 16);
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d1), rsrc, off + 16, 0, 16);
-        }
-    }
-}
-
-// The next row's Float32 strips of the 16-bit walker arrive by LDS-DMA (no registers held
-// while the current row is applied).  A wave's NS * SW columns are staged in groups of
-// 256 / NB columns, one 1 KB DMA instruction each (64 lanes x 16 bytes), the groups 16 bytes
-// apart against bank conflicts; in the Float64 staging region, which this fits exactly at
-// NB = 128.
-template <typename Tw, int NB, int NS> struct LpStage {
-    static constexpr int CPD = 256 / NB, LPC = NB / 4, GRP = 1024 + 16, NG = NS * SW / CPD;
-    static_assert(NS * SW % CPD == 0, "whole groups per wave");
-    __device__ static unsigned char *base() {
-        return lds_base<unsigned char>() + LdsLp<Tw, NB>::SSTG_BYTES + (threadIdx.x >> 6) * NG * GRP;
-    }
-    // issue the loads of the wave's columns G, G + ld, ... (rows 0..NB-1)
-    __device__ static void load(const float *G, i64 ld) {
-        const int lane = threadIdx.x & 63;
-        unsigned char *b = base();
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // earlier reads of the staging are done
-#pragma unroll
-        for (int gi = 0; gi < NG; ++gi) {
-            const float *src = G + (i64)(gi * CPD + lane / LPC) * ld + (lane % LPC) * 4;
-            auto *d = (__attribute__((address_space(3))) void *)(b + gi * GRP);
-            __builtin_amdgcn_global_load_lds(src, d, 16, 0, /*sc1*/ 16);
-        }
-    }
-    // staged columns -> the wave's NS accumulator strips (after the loads' vmcnt wait)
-    __device__ static void read(WStrip<float, NB> *S) {
-        const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-        const unsigned char *b = base();
-#pragma unroll
-        for (int q = 0; q < NS; ++q) {
-            const int c = q * SW + lr;
-            const unsigned char *cb = b + (c / CPD) * GRP + (c % CPD) * NB * 4 + 16 * lk;
-#pragma unroll
-            for (int rt = 0; rt < NB / 16; ++rt) S[q].a[rt] = *reinterpret_cast<const fx4 *>(cb + 64 * rt);
-        }
-    }
-};
-
 template <typename T, typename Tw, int NB>
 __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
     constexpr int NBLK = NB / IB;
@@ -1407,7 +1411,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         }
         apply_lp<Tw, NB, true, NS>(A1, nullptr, B, B & 1);
     }
-    using Stg = LpStage<Tw, NB, NS>;
+    using Stg = StripStage<LdsLp<Tw, NB>::SSTG_BYTES, NB, NS>;
     static_assert(WAVES * Stg::NG * Stg::GRP <= TW * Lds<T, NB>::LDS_S * (int)sizeof(T), "staging fits the strip region");
     WStrip<float, NB> S[NS];
     bool haveS = false;  // this row's strips are staged (or being staged)
