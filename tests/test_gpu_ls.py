@@ -85,6 +85,29 @@ def test_tile_geqrf_mixed_backward_error(m, n, nb, prec):
     assert lo < d < 10 * hi, d
 
 
+@pytest.mark.parametrize("prec", ["S", "H", "B"])
+def test_tile_geqrf_mixed_large_backward_error(prec):
+    """The benchmark size (16384^2, 128 tile rows and columns): every strip goes through the
+    Float32 shadow for up to 127 steps, with the launch's full concurrency.  A transport
+    defect of the shadow once showed only from 12288^2 up (NaN entries in R and T)."""
+    nl = _nl()
+    n = 16384
+    torch.manual_seed(5)
+    A0 = nl.zeros_cm(n, n)
+    A0.copy_(torch.randn(n, n, dtype=torch.float64, device="cuda").t())
+    dA = nl.zeros_cm(n, n)
+    dA.copy_(A0)
+    dT = nl.tile_geqrf_mixed(dA, 128, 32, prec)
+    QR = nl.zeros_cm(n, n)
+    QR.copy_(torch.triu(dA))
+    nl.tile_ormqr("N", dA, dT, QR, 128, 32)
+    torch.cuda.synchronize()
+    assert not torch.isnan(dA).any() and not torch.isnan(dT).any()
+    berr = (torch.linalg.norm(QR - A0) / torch.linalg.norm(A0)).item()
+    lo, hi = MIXED_BERR[prec]
+    assert lo < berr < hi, berr
+
+
 def test_tile_geqrf_mixed_precisions_order():
     """Float32 updates are more accurate than FP16, FP16 more than BF16 (unit roundoffs
     6e-8 < 4.9e-4 < 3.9e-3), on the same matrix."""
