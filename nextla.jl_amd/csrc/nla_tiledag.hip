@@ -1196,16 +1196,25 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
 // staged twice, column-major for V^T S and row-major for V W, so each lane's four A-operand
 // elements are one 8-byte LDS read in both products.
 template <typename Tw> struct LpMfma;
+// v_mfma_f32_16x16x32_{f16,bf16}: lane l holds A[l % 16][8 (l/16) + i] and B[8 (l/16) + i][l % 16],
+// i = 0..7; mma8 takes each as two halves of four (the products only need the same K order
+// in both operands).
 template <> struct LpMfma<_Float16> {
     typedef _Float16 v4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ void mma(fx4 &c, v4 a, v4 b) {
-        c = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ void mma8(fx4 &c, v4 alo, v4 ahi, v4 blo, v4 bhi) {
+        const v8 a = __builtin_shufflevector(alo, ahi, 0, 1, 2, 3, 4, 5, 6, 7);
+        const v8 b = __builtin_shufflevector(blo, bhi, 0, 1, 2, 3, 4, 5, 6, 7);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
 };
 template <> struct LpMfma<__bf16> {
     typedef __bf16 v4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ void mma(fx4 &c, v4 a, v4 b) {
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ void mma8(fx4 &c, v4 alo, v4 ahi, v4 blo, v4 bhi) {
+        const v8 a = __builtin_shufflevector(alo, ahi, 0, 1, 2, 3, 4, 5, 6, 7);
+        const v8 b = __builtin_shufflevector(blo, bhi, 0, 1, 2, 3, 4, 5, 6, 7);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 };
 template <typename Tw> __device__ __forceinline__ typename LpMfma<Tw>::v4 lp_cvt(const fx4 &x, float sign) {
@@ -1289,7 +1298,10 @@ template <typename Tw, int NB> __device__ __forceinline__ void lp_img_load(const
 }
 // H^T (the reference's trans 'C') of the staged block applied to the wave's NS strips:
 //   W0 = V^T S (+ A1 rows),  W = T^T W0,  A1 rows -= W,  S -= V W.
-// Every V / T operand read from LDS serves the NS strips' products.
+// Every V / T operand read from LDS serves the NS strips' products.  Each product has a
+// 32-deep K: a lane's eight K values are the four rows 4 (l/16)..+3 of two accumulator tiles
+// 16 rows apart (the Float32 strip / W0 / W registers as they are), and the A operands are
+// read at the same rows (two 8-byte LDS reads).
 template <typename Tw, int NB, bool GE, int NS>
 __device__ __forceinline__ void apply_lp(WStrip<float, NB> *S, WStrip<float, NB> *A1, int B, int buf) {
     using L = LdsLp<Tw, NB>;
@@ -1297,19 +1309,24 @@ __device__ __forceinline__ void apply_lp(WStrip<float, NB> *S, WStrip<float, NB>
     typedef typename M::v4 v4;
     const Tw *sm = lds_base<Tw>() + buf * L::BUFS;
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    auto rd = [&](int off) { return *reinterpret_cast<const v4 *>(sm + off); };
     fx4 w0[NS][2];
 #pragma unroll
     for (int q = 0; q < NS; ++q) w0[q][0] = w0[q][1] = acc_zero<float>();
 #pragma unroll
-    for (int rt = 0; rt < NB / 16; ++rt) {
-        v4 a[2];
+    for (int t = 0; t < NB / 32; ++t) {  // rows 32 t .. 32 t + 31 of the strip
+        v4 alo[2], ahi[2];
 #pragma unroll
-        for (int ot = 0; ot < 2; ++ot) a[ot] = *reinterpret_cast<const v4 *>(sm + L::VC + (16 * ot + lr) * L::LDC + 16 * rt + 4 * lk);
+        for (int ot = 0; ot < 2; ++ot) {
+            const int o = L::VC + (16 * ot + lr) * L::LDC + 32 * t + 4 * lk;
+            alo[ot] = rd(o);
+            ahi[ot] = rd(o + 16);
+        }
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
-            const v4 b = lp_cvt<Tw>(S[q].a[rt], 1.0f);
+            const v4 blo = lp_cvt<Tw>(S[q].a[2 * t], 1.0f), bhi = lp_cvt<Tw>(S[q].a[2 * t + 1], 1.0f);
 #pragma unroll
-            for (int ot = 0; ot < 2; ++ot) M::mma(w0[q][ot], a[ot], b);
+            for (int ot = 0; ot < 2; ++ot) M::mma8(w0[q][ot], alo[ot], ahi[ot], blo, bhi);
         }
     }
     if (!GE) {
@@ -1321,16 +1338,19 @@ __device__ __forceinline__ void apply_lp(WStrip<float, NB> *S, WStrip<float, NB>
     fx4 wv[NS][2];
 #pragma unroll
     for (int q = 0; q < NS; ++q) wv[q][0] = wv[q][1] = acc_zero<float>();
+    {
+        v4 alo[2], ahi[2];
 #pragma unroll
-    for (int tq = 0; tq < 2; ++tq) {
-        v4 a[2];
-#pragma unroll
-        for (int ot = 0; ot < 2; ++ot) a[ot] = *reinterpret_cast<const v4 *>(sm + L::TC + (16 * ot + lr) * L::LDR + 16 * tq + 4 * lk);
+        for (int ot = 0; ot < 2; ++ot) {
+            const int o = L::TC + (16 * ot + lr) * L::LDR + 4 * lk;
+            alo[ot] = rd(o);
+            ahi[ot] = rd(o + 16);
+        }
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
-            const v4 b = lp_cvt<Tw>(w0[q][tq], 1.0f);
+            const v4 blo = lp_cvt<Tw>(w0[q][0], 1.0f), bhi = lp_cvt<Tw>(w0[q][1], 1.0f);
 #pragma unroll
-            for (int ot = 0; ot < 2; ++ot) M::mma(wv[q][ot], a[ot], b);
+            for (int ot = 0; ot < 2; ++ot) M::mma8(wv[q][ot], alo[ot], ahi[ot], blo, bhi);
         }
     }
     if (!GE) {
@@ -1339,17 +1359,18 @@ __device__ __forceinline__ void apply_lp(WStrip<float, NB> *S, WStrip<float, NB>
 #pragma unroll
             for (int ot = 0; ot < 2; ++ot) A1[q].a[2 * B + ot] -= wv[q][ot];
     }
+    v4 blo[NS], bhi[NS];
 #pragma unroll
-    for (int tq = 0; tq < 2; ++tq) {
-        v4 b[NS];
+    for (int q = 0; q < NS; ++q) {
+        blo[q] = lp_cvt<Tw>(wv[q][0], -1.0f);
+        bhi[q] = lp_cvt<Tw>(wv[q][1], -1.0f);
+    }
 #pragma unroll
-        for (int q = 0; q < NS; ++q) b[q] = lp_cvt<Tw>(wv[q][tq], -1.0f);
+    for (int rt = 0; rt < NB / 16; ++rt) {
+        const int o = L::VR + (16 * rt + lr) * L::LDR + 4 * lk;
+        const v4 alo = rd(o), ahi = rd(o + 16);
 #pragma unroll
-        for (int rt = 0; rt < NB / 16; ++rt) {
-            const v4 a = *reinterpret_cast<const v4 *>(sm + L::VR + (16 * rt + lr) * L::LDR + 16 * tq + 4 * lk);
-#pragma unroll
-            for (int q = 0; q < NS; ++q) M::mma(S[q].a[rt], a, b[q]);
-        }
+        for (int q = 0; q < NS; ++q) M::mma8(S[q].a[rt], alo, ahi, blo[q], bhi[q]);
     }
 }
 
