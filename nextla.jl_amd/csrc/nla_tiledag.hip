@@ -1082,8 +1082,8 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     T *stg = stage_ptr<T, Tw, NB>();
     // Strips between steps live in the Float32 shadow, as in the 16-bit walker: step 0 reads
     // the matrix (staged Float64), later steps the shadow (StripStage); rows go to the matrix
-    // only for the next panel's tile column and the final R row (A1), the others straight
-    // from the registers to the shadow.
+    // only for the next panel's tile column and the final R row (A1).  Every row leaves
+    // straight from the registers.
     static_assert(sizeof(Tw) == 4, "Float32 accumulator strips");
     using Stg = StripStage<Lds<Tw, NB>::SSTG * (int)sizeof(Tw), NB, 1>;
     static_assert(WAVES * Stg::NG * Stg::GRP <= TW * Lds<T, NB>::LDS_S * (int)sizeof(T), "staging fits the strip region");
@@ -1175,7 +1175,10 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
             if (B + 1 < NBLK || next0) vstore_cvt<T, Tw, NB, false>(P, (B + 1) & 1, 0);
         }
         have0 = next0;
-        if (dst64) stage_store<T, Tw, NB>(stg, S, Aij, ldc);
+        // Both straight from the registers: the prefetch region of one wave lies inside another
+        // wave's Float64 staging columns, so a task that reads through StripStage must not
+        // stage Float64 (step 0, which stages its Float64 loads, never uses StripStage).
+        if (dst64) strip_store<T, NB>(S.a, Aij, ldc);
         else strip_store<float, NB>(S.a, sh32 + (i64)i * NB + col * ldc, ldc);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
