@@ -90,10 +90,12 @@ __device__ void publish(int *p, int v) {
 // are staged once in LDS and shared by the four waves (four strips = TW columns).
 constexpr int WAVES = NTH / 64;
 constexpr int TW = SW * WAVES;  // matrix columns per workgroup task
-// Columns per walker task.  The 16-bit walkers (NB >= 128) give each wave two 16-column
-// strips: the block's V/T operands are read once for both, and the per-block image load and
-// barrier are paid once per 2 TW columns.
-template <typename Tw, int NB> constexpr int walk_width() { return sizeof(Tw) == 2 && NB >= 128 ? 2 * TW : TW; }
+// Columns per walker task.  The reduced-precision walkers (Tw narrower than T, NB >= 128)
+// give each wave two 16-column strips: the block's V/T operands are read once for both, and
+// the per-block loads and barrier are paid once per 2 TW columns.
+template <typename T, typename Tw, int NB> constexpr int walk_width() {
+    return sizeof(Tw) < sizeof(T) && NB >= 128 ? 2 * TW : TW;
+}
 
 template <typename T, int NB> struct Lds {
     static constexpr int LDV = NB + 2;                 // Vs[j * LDV + p] = V[p][j]
@@ -429,6 +431,104 @@ __device__ __forceinline__ void apply_wave(WStrip<T, NB> &S, typename Acc<T>::t 
     }
 }
 
+// apply_wave (H^T only) for the wave's NS strips of the Float32-update walker: the same
+// products, every LDS operand read once for the NS strips; A1: the NS strips' A1 rows.
+template <typename T, int NB, bool GE, int NS, typename F = NoIssue>
+__device__ __forceinline__ void apply_wave_ns(WStrip<T, NB> *S, WStrip<T, NB> *A1, int B, int buf, F &&issue = F()) {
+    const T *sm = lds_base<T>() + buf * Lds<T, NB>::BUF;
+    constexpr int LDV = Lds<T, NB>::LDV;
+    constexpr int RT = NB / 16;
+    const T *Vs = sm + Lds<T, NB>::V;
+    const T *Ts = sm + Lds<T, NB>::TT;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    typename Acc<T>::t w0[NS][2];
+#pragma unroll
+    for (int st = 0; st < NS; ++st) w0[st][0] = w0[st][1] = acc_zero<T>();
+    {
+        auto vop = [&](int idx, int ot) {
+            const int rt = idx >> 2, r = idx & 3;
+            return Vs[(16 * ot + lr) * LDV + 16 * rt + acc_row<T>(lk, r)];
+        };
+        constexpr int N = RT * 4;
+        T va[3][2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) va[d][ot] = vop(d, ot);
+#pragma unroll
+        for (int idx = 0; idx < N; ++idx) {
+            if (idx + 2 < N) {
+#pragma unroll
+                for (int ot = 0; ot < 2; ++ot) va[(idx + 2) % 3][ot] = vop(idx + 2, ot);
+            }
+#pragma unroll
+            for (int st = 0; st < NS; ++st)
+#pragma unroll
+                for (int ot = 0; ot < 2; ++ot) mfma16(w0[st][ot], va[idx % 3][ot], S[st].a[idx >> 2][idx & 3]);
+            if (idx >= 2) issue(idx - 2);
+        }
+    }
+    if (!GE) {
+#pragma unroll
+        for (int st = 0; st < NS; ++st)
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) w0[st][ot][r] += A1[st].a[2 * B + ot][r];
+    }
+    typename Acc<T>::t wv[NS][2];
+#pragma unroll
+    for (int st = 0; st < NS; ++st) wv[st][0] = wv[st][1] = acc_zero<T>();
+    {
+        auto top = [&](int idx, int ot) {
+            const int q = 16 * (idx >> 2) + acc_row<T>(lk, idx & 3), j = 16 * ot + lr;
+            return Ts[q * (IB + 1) + j];
+        };
+        T ta[3][2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) ta[d][ot] = top(d, ot);
+#pragma unroll
+        for (int idx = 0; idx < 8; ++idx) {
+            if (idx + 2 < 8) {
+#pragma unroll
+                for (int ot = 0; ot < 2; ++ot) ta[(idx + 2) % 3][ot] = top(idx + 2, ot);
+            }
+#pragma unroll
+            for (int st = 0; st < NS; ++st)
+#pragma unroll
+                for (int ot = 0; ot < 2; ++ot) mfma16(wv[st][ot], ta[idx % 3][ot], w0[st][idx >> 2][idx & 3]);
+        }
+    }
+    T wn[NS][2][4];
+#pragma unroll
+    for (int st = 0; st < NS; ++st)
+#pragma unroll
+        for (int tq = 0; tq < 2; ++tq)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (!GE) A1[st].a[2 * B + tq][r] -= wv[st][tq][r];
+                wn[st][tq][r] = -wv[st][tq][r];
+            }
+    {
+        auto vop = [&](int idx) {
+            const int tq = idx / (4 * RT), r = (idx / RT) & 3, rt = idx % RT;
+            return Vs[(16 * tq + acc_row<T>(lk, r)) * LDV + 16 * rt + lr];
+        };
+        constexpr int N = 2 * 4 * RT;
+        T va[3];
+        va[0] = vop(0);
+        va[1] = vop(1);
+#pragma unroll
+        for (int idx = 0; idx < N; ++idx) {
+            if (idx + 2 < N) va[(idx + 2) % 3] = vop(idx + 2);
+#pragma unroll
+            for (int st = 0; st < NS; ++st) mfma16(S[st].a[idx % RT], va[idx % 3], wn[st][idx / (4 * RT)][(idx / RT) & 3]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Block factorization by the whole workgroup.  Wave w owns the eight columns j = 8w .. 8w+7
 // of the IB-column block: lane l holds column 8w + l / 8 on the RPT = NB/8 consecutive rows
@@ -735,7 +835,10 @@ template <typename T> struct PanelTwT<T, __bf16> { using type = __bf16; };
 template <typename T, typename Tw> using PanelTw = typename PanelTwT<T, Tw>::type;
 template <typename T, int NB, typename Tw>
 __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
-    constexpr int WPT = NB / walk_width<Tw, NB>(), NBLK = NB / IB;  // walker groups per tile column, stages
+    // walker groups per tile column (NB / the kernel's walk_width; from the launch, so that the
+    // Float64 and Float32-update kernels share this panel), stages
+    const int WPT = g.nwalk / g.nt;
+    constexpr int NBLK = NB / IB;
     const int c0 = b * IB;
     int *mine = &g.panel_prog[k * NBLK + b];
     const int *prev = b > 0 ? &g.panel_prog[k * NBLK + b - 1] : nullptr;
@@ -1069,8 +1172,9 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     constexpr int NBLK = NB / IB;
     static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
     __shared__ int sh_ready;
+    constexpr int TWK = walk_width<T, Tw, NB>(), NS = TWK / TW;  // columns per task, strips per wave
     const int w = threadIdx.x >> 6;
-    const i64 col = (i64)s * TW + w * SW;
+    const i64 col = (i64)s * TWK + w * NS * SW;  // the wave's strip q starts at column col + 16 q
     const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
     int *mine = &g.strip_prog[k * g.nwalk + s];
     const int *pdone = &g.panel_prog[k * NBLK + NBLK - 1];
@@ -1078,26 +1182,35 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
     T *Cm = reinterpret_cast<T *>(g.C);
     const i64 ldc = g.ldc;
-    T *Akj = Cm + (i64)k * NB + col * ldc;
-    T *stg = stage_ptr<T, Tw, NB>();
     // Strips between steps live in the Float32 shadow, as in the 16-bit walker: step 0 reads
-    // the matrix (staged Float64), later steps the shadow (StripStage); rows go to the matrix
-    // only for the next panel's tile column and the final R row (A1).  Every row leaves
-    // straight from the registers.
+    // the matrix (16-byte loads at the row start), later steps the shadow (prefetched through
+    // StripStage); rows go to the matrix only for the next panel's tile column and the final R
+    // row (A1).  Every row leaves straight from the registers.
     static_assert(sizeof(Tw) == 4, "Float32 accumulator strips");
-    using Stg = StripStage<Lds<Tw, NB>::SSTG * (int)sizeof(Tw), NB, 1>;
+    using Stg = StripStage<Lds<Tw, NB>::SSTG * (int)sizeof(Tw), NB, NS>;
     static_assert(WAVES * Stg::NG * Stg::GRP <= TW * Lds<T, NB>::LDS_S * (int)sizeof(T), "staging fits the strip region");
     float *const sh32 = g_shadow;
-    const bool src32 = k > 0, dst64 = s / (NB / TW) == k + 1;
-    auto load_strip = [&](i64 row) {
-        if (src32) Stg::load(sh32 + row + col * ldc, ldc);
-        else stage_load<T, NB>(stg, Cm + row + col * ldc, ldc);
+    const bool src32 = k > 0, dst64 = s / (NB / TWK) == k + 1;
+    auto load_strips = [&](WStrip<Tw, NB> *X, int row_tile) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const i64 off = (i64)row_tile * NB + (col + q * SW) * ldc;
+            if (src32) strip_load<float, NB>(X[q].a, sh32 + off, ldc);
+            else strip_load<T, NB>(X[q].a, Cm + off, ldc);
+        }
+    };
+    auto store_strips = [&](const WStrip<Tw, NB> *X, int row_tile, bool f64) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const i64 off = (i64)row_tile * NB + (col + q * SW) * ldc;
+            if (f64) strip_store<T, NB>(X[q].a, Cm + off, ldc);
+            else strip_store<float, NB>(X[q].a, sh32 + off, ldc);
+        }
     };
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
-    WStrip<Tw, NB> A1;
-    if (src32) strip_load<float, NB>(A1.a, sh32 + (i64)k * NB + col * ldc, ldc);
-    else strip_load<T, NB>(A1.a, Akj, ldc);
+    WStrip<Tw, NB> A1[NS];
+    load_strips(A1, k);
     VPre<T, NB> P;
     vload<T, NB>(P, Akk, lda, Tkk, ldt);
     vstore_cvt<T, Tw, NB, true>(P, 0, 0);
@@ -1111,15 +1224,15 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
             if (have0) vload<T, NB>(P, Akk + NB, lda, Tkk + IB, ldt);
         }
         __syncthreads();
-        apply_wave<Tw, NB, true>(A1, nullptr, false, B & 1);
+        apply_wave_ns<Tw, NB, true, NS>(A1, nullptr, B, B & 1);
         if (B + 1 < NBLK) vstore_cvt<T, Tw, NB, true>(P, (B + 1) & 1, (B + 1) * IB);
         else if (have0) vstore_cvt<T, Tw, NB, false>(P, 0, 0);
     }
     // Same row pipeline as the Float64 walker: strips staged (Float64) in LDS and converted
     // on the way to and from the registers, the next strip prefetched when released, a
     // finished row published at the next row's first block.
-    WStrip<Tw, NB> S;
-    bool haveS = false;
+    WStrip<Tw, NB> S[NS];
+    bool haveS = false;  // this row's strips are staged (or being staged)
     int pending = 0;
     // the next-row prefetch flags, read a block early (as in the Float64 walker)
     __shared__ int sh_nx[2];
@@ -1127,7 +1240,6 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     for (int i = k + 1; i < g.mt; ++i) {
         const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
         const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
-        T *Aij = Cm + (i64)i * NB + col * ldc;
         if (pending && (!have0 || !haveS)) {
             publish(mine, pending);
             pending = 0;
@@ -1137,9 +1249,9 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
             vload<T, NB>(P, Aik, lda, Tik, ldt);
             vstore_cvt<T, Tw, NB, false>(P, 0, 0);
         }
-        if (!haveS) {
+        if (!haveS && src32) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
-            load_strip((i64)i * NB);
+            Stg::load(sh32 + (i64)i * NB + col * ldc, ldc);
         }
         bool next0 = false;
 #pragma unroll
@@ -1151,8 +1263,8 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
                 }
-                if (src32) Stg::read(&S);
-                else stage_to_regs<T, Tw, NB>(stg, S);
+                if (src32) Stg::read(S);
+                else load_strips(S, i);  // step 0 (no previous step to wait for)
             }
             if (B == NBLK - 2 && threadIdx.x == 0) {
                 fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
@@ -1169,23 +1281,21 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
             if (!load_next && i + 1 < g.mt) load_next = next0 = sh_nx[0] != 0;
             // the next block's V and T, one element per product pair (into registers)
             static_assert(VPre<T, NB>::NV + VPre<T, NB>::NT <= NB / 16 * 4 - 2, "every load part gets a product pair");
-            apply_wave<Tw, NB, false>(S, &A1.a[2 * B], false, B & 1, [&](int part) {
+            apply_wave_ns<Tw, NB, false, NS>(S, A1, B, B & 1, [&](int part) {
                 if (load_next && part < VPre<T, NB>::NV + VPre<T, NB>::NT) vload_one<T, NB>(P, Vn, lda, Tn, ldt, part);
             });
             if (B + 1 < NBLK || next0) vstore_cvt<T, Tw, NB, false>(P, (B + 1) & 1, 0);
         }
         have0 = next0;
-        // Both straight from the registers: the prefetch region of one wave lies inside another
-        // wave's Float64 staging columns, so a task that reads through StripStage must not
-        // stage Float64 (step 0, which stages its Float64 loads, never uses StripStage).
-        if (dst64) strip_store<T, NB>(S.a, Aij, ldc);
-        else strip_store<float, NB>(S.a, sh32 + (i64)i * NB + col * ldc, ldc);
+        // Straight from the registers: the prefetch region of one wave lies inside another
+        // wave's Float64 staging columns, so this walker stages nothing in Float64.
+        store_strips(S, i, dst64);
         pending = i + 1;
         haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
-        if (haveS) load_strip((i64)(i + 1) * NB);
+        if (haveS && src32) Stg::load(sh32 + (i64)(i + 1) * NB + col * ldc, ldc);
     }
     if (pending) publish(mine, pending);
-    strip_store<T, NB>(A1.a, Akj, ldc);
+    store_strips(A1, k, true);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
@@ -1383,7 +1493,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
     constexpr i64 IMG = LdsLp<Tw, NB>::IMG_BYTES;
     static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
     __shared__ int sh_ready;
-    constexpr int TWK = walk_width<Tw, NB>(), NS = TWK / TW;  // columns per task, strips per wave
+    constexpr int TWK = walk_width<T, Tw, NB>(), NS = TWK / TW;  // columns per task, strips per wave
     const int w = threadIdx.x >> 6;
     const i64 col = (i64)s * TWK + w * NS * SW;  // the wave's strip q starts at column col + 16 q
     const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
@@ -1584,7 +1694,7 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
     using namespace dag;
     const int mt = (int)(m / NB), nt = (int)(n / NB), kt = mt < nt ? mt : nt;
     const int nblk = NB / IB;
-    constexpr int TWK = walk_width<Tw, NB>();
+    constexpr int TWK = walk_width<T, Tw, NB>();
     const int nwalk = (int)((mode == DAG_FACTOR ? n : ncols) / TWK);
     std::vector<int> tasks;
     if (mode == DAG_FACTOR) {
