@@ -117,6 +117,15 @@ template <typename T, int NB> struct Lds {
     static constexpr int TOTAL = PEND > WEND ? PEND : WEND;
 };
 
+// The Float32-update walker's V/T block image: one LDS buffer of the Float32 walker layout
+// (V column-major with LDV, then T with zeros below the diagonal), written once per block by the
+// panel; images are 1 KB-aligned in the scratch slot.
+template <int NB> struct F32Img {
+    static constexpr int BYTES = Lds<float, NB>::BUF * (int)sizeof(float);
+    static constexpr int STRIDE = (BYTES + 1023) / 1024 * 1024;
+    static_assert(BYTES % 16 == 0, "whole 16-byte pieces");
+};
+
 // LDS base with its address space visible at every use (a pointer passed through a call
 // would degrade to FLAT accesses).
 template <typename T> __device__ __forceinline__ T *lds_base() {
@@ -808,7 +817,7 @@ struct DagArgs {
     i64 ldc;
     int qmode;                  // walkers apply Q (reverse order) instead of Q^T
 };
-// 16-bit walkers: per (panel tile, block) the V/T image the panel writes.  A device global,
+// Reduced-precision walkers: per (panel tile, block) the V/T image the panel writes.  A device global,
 // not a DagArgs field: the Float64 kernel sits at its register limit and one more argument
 // makes it spill.  Set by the host before each launch (launches on one device are ordered).
 __device__ unsigned char *g_lpimg = nullptr;
@@ -826,13 +835,36 @@ __host__ __device__ inline i64 lp_tile_index(int k, int i, int mt);
 // this task's own previous step) and block b-1 of tile i (the trailing update of A_ik's
 // columns c0.. and the factorization of R_kk's rows above), so the stages of one panel
 // run as a pipeline over the tiles.  Stage 0 waits for the previous step's walkers instead.
-// Tw: the walkers' type; a 16-bit Tw makes the panel also write each block's image for them.
-// The kernels whose walkers read V and T directly share one instance (PanelTw maps their Tw
-// to T): that one out-of-line panel compiles as it did before the 16-bit variants existed.
+// Tw: the walkers' type; a Tw narrower than T makes the panel also write each block's image
+// for them (16-bit: write_lp_image; Float32 with a Float64 matrix: write_f32_image).  The
+// Float64 and Float32 factorizations share one instance (PanelTw maps their Tw to T): that
+// out-of-line panel compiles as it did before the image-reading variants existed.
 template <typename T, typename Tw> struct PanelTwT { using type = T; };
 template <typename T> struct PanelTwT<T, _Float16> { using type = _Float16; };
 template <typename T> struct PanelTwT<T, __bf16> { using type = __bf16; };
+template <> struct PanelTwT<double, float> { using type = float; };
 template <typename T, typename Tw> using PanelTw = typename PanelTwT<T, Tw>::type;
+// Panel side of F32Img: the block just factored (V, T of the panel's LDS, V unit lower for a
+// geqrt block) rounded to Float32 and written to dst with device-coherent stores.
+template <typename T, int NB> __device__ void write_f32_image(unsigned char *dst) {
+    using L = Lds<float, NB>;
+    constexpr int LDV = Lds<T, NB>::LDV;
+    static_assert(Lds<T, NB>::LDV == L::LDV, "one V column stride");
+    const T *Vs = lds_base<T>() + Lds<T, NB>::V;
+    const T *Ts = lds_base<T>() + Lds<T, NB>::TT;
+    float *d = reinterpret_cast<float *>(dst);
+    for (int e = threadIdx.x; e < L::BUF; e += NTH) {
+        float v = 0.0f;
+        if (e < L::TT) {
+            const int j = e / LDV, p = e % LDV;
+            if (p < NB) v = (float)Vs[j * LDV + p];
+        } else {
+            const int f = e - L::TT, q = f / (IB + 1), j = f % (IB + 1);
+            if (j < IB && q <= j) v = (float)Ts[q * (IB + 1) + j];
+        }
+        cst(&d[e], v);
+    }
+}
 template <typename T, int NB, typename Tw>
 __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
     // walker groups per tile column (NB / the kernel's walk_width; from the launch, so that the
@@ -853,9 +885,11 @@ __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 l
         T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
         if (i == k) geqrt_block<T, NB>(Akk, lda, Tik, ldt, c0);
         else tsqrt_block<T, NB>(Akk, lda, A + (i64)i * NB + (i64)k * NB * lda, lda, Tik, ldt, c0);
-        // the block's V and T for the 16-bit walkers
+        // the block's V and T for the reduced-precision walkers
         if constexpr (sizeof(Tw) == 2)
             write_lp_image<T, Tw, NB>(g_lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)LdsLp<Tw, NB>::IMG_BYTES);
+        else if constexpr (sizeof(Tw) < sizeof(T))
+            write_f32_image<T, NB>(g_lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)F32Img<NB>::STRIDE);
         publish(mine, i + 1);
         if (b == NBLK - 1 && g.ptrace && threadIdx.x == 0)
             g.ptrace[(size_t)k * g.mt + i] = __builtin_amdgcn_s_memrealtime();
@@ -1167,10 +1201,30 @@ template <int BASE, int NB, int NS> struct StripStage {
     }
 };
 
+// An image of BYTES (a multiple of 16) from global src into LDS dst with 1 KB LDS-DMA
+// instructions (waves take every fourth KB; the last one only its lanes inside BYTES);
+// completion is tracked by vmcnt like any vector load.
+template <int BYTES> __device__ __forceinline__ void img_dma(const unsigned char *src, unsigned char *dst) {
+    constexpr int NI = (BYTES + 1023) / 1024;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < NI; c += WAVES) {
+        const int ci = c + w;
+        if (ci < NI && ci * 1024 + lane * 16 < BYTES) {
+            auto *d = (__attribute__((address_space(3))) void *)(dst + ci * 1024);
+            __builtin_amdgcn_global_load_lds(src + ci * 1024 + lane * 16, d, 16, 0, /*sc1*/ 16);
+        }
+    }
+}
+
+// The Float32-update walker (Tw = Float32 over a Float64 matrix): the row pipeline of the
+// 16-bit walker with v_mfma_f32_16x16x4 products: each block's V/T image (F32Img, written by
+// the panel) copied into an LDS buffer by DMA one block ahead, two 16-column strips per wave.
 template <typename T, typename Tw, int NB>
 __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
     constexpr int NBLK = NB / IB;
     static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
+    static_assert(sizeof(Tw) == 4, "Float32 accumulator strips");
     __shared__ int sh_ready;
     constexpr int TWK = walk_width<T, Tw, NB>(), NS = TWK / TW;  // columns per task, strips per wave
     const int w = threadIdx.x >> 6;
@@ -1178,16 +1232,15 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
     int *mine = &g.strip_prog[k * g.nwalk + s];
     const int *pdone = &g.panel_prog[k * NBLK + NBLK - 1];
-    const T *Akk = A + (i64)k * NB + (i64)k * NB * lda;
-    const T *Tkk = Tm + (i64)k * IB + (i64)k * NB * ldt;
     T *Cm = reinterpret_cast<T *>(g.C);
     const i64 ldc = g.ldc;
     // Strips between steps live in the Float32 shadow, as in the 16-bit walker: step 0 reads
     // the matrix (16-byte loads at the row start), later steps the shadow (prefetched through
-    // StripStage); rows go to the matrix only for the next panel's tile column and the final R
-    // row (A1).  Every row leaves straight from the registers.
-    static_assert(sizeof(Tw) == 4, "Float32 accumulator strips");
+    // StripStage, behind the two image buffers); rows go to the matrix only for the next
+    // panel's tile column and the final R row (A1).  Every row leaves straight from the
+    // registers.
     using Stg = StripStage<Lds<Tw, NB>::SSTG * (int)sizeof(Tw), NB, NS>;
+    static_assert(2 * Lds<Tw, NB>::BUF <= Lds<Tw, NB>::SSTG, "the image buffers end before the staging");
     static_assert(WAVES * Stg::NG * Stg::GRP <= TW * Lds<T, NB>::LDS_S * (int)sizeof(T), "staging fits the strip region");
     float *const sh32 = g_shadow;
     const bool src32 = k > 0, dst64 = s / (NB / TWK) == k + 1;
@@ -1207,30 +1260,30 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
             else strip_store<float, NB>(X[q].a, sh32 + off, ldc);
         }
     };
+    unsigned char *const imgs = g_lpimg;
+    auto img_load = [&](int i, int B, int buf) {
+        img_dma<F32Img<NB>::BYTES>(imgs + (lp_tile_index(k, i, g.mt) * NBLK + B) * (i64)F32Img<NB>::STRIDE,
+                                   lds_base<unsigned char>() + buf * Lds<Tw, NB>::BUF * (int)sizeof(Tw));
+    };
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
     WStrip<Tw, NB> A1[NS];
     load_strips(A1, k);
-    VPre<T, NB> P;
-    vload<T, NB>(P, Akk, lda, Tkk, ldt);
-    vstore_cvt<T, Tw, NB, true>(P, 0, 0);
+    // unmqr!('L','C') with geqrt(A_kk)'s blocks (their images carry the unit-lower V)
+    img_load(k, 0, 0);
     bool have0 = false;
 #pragma unroll
     for (int B = 0; B < NBLK; ++B) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
         if (B + 1 < NBLK) {
-            vload<T, NB>(P, Akk + (i64)(B + 1) * IB * lda, lda, Tkk + (i64)(B + 1) * IB * ldt, ldt);
+            img_load(k, B + 1, (B + 1) & 1);
         } else if (k + 1 < g.mt) {
             have0 = ready_ge(pdone, k + 2, &sh_ready);
-            if (have0) vload<T, NB>(P, Akk + NB, lda, Tkk + IB, ldt);
+            if (have0) img_load(k + 1, 0, 0);
         }
-        __syncthreads();
         apply_wave_ns<Tw, NB, true, NS>(A1, nullptr, B, B & 1);
-        if (B + 1 < NBLK) vstore_cvt<T, Tw, NB, true>(P, (B + 1) & 1, (B + 1) * IB);
-        else if (have0) vstore_cvt<T, Tw, NB, false>(P, 0, 0);
     }
-    // Same row pipeline as the Float64 walker: strips staged (Float64) in LDS and converted
-    // on the way to and from the registers, the next strip prefetched when released, a
-    // finished row published at the next row's first block.
     WStrip<Tw, NB> S[NS];
     bool haveS = false;  // this row's strips are staged (or being staged)
     int pending = 0;
@@ -1238,16 +1291,13 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
     __shared__ int sh_nx[2];
     int fl_p = 0, fl_s = 0;
     for (int i = k + 1; i < g.mt; ++i) {
-        const T *Aik = A + (i64)i * NB + (i64)k * NB * lda;
-        const T *Tik = Tm + (i64)i * IB + (i64)k * NB * ldt;
         if (pending && (!have0 || !haveS)) {
             publish(mine, pending);
             pending = 0;
         }
         if (!have0) {
             wait_ge(pdone, i + 1, g.err, sh_flag);
-            vload<T, NB>(P, Aik, lda, Tik, ldt);
-            vstore_cvt<T, Tw, NB, false>(P, 0, 0);
+            img_load(i, 0, 0);
         }
         if (!haveS && src32) {
             if (prev) wait_ge(prev, i + 1, g.err, sh_flag);
@@ -1256,9 +1306,13 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
         bool next0 = false;
 #pragma unroll
         for (int B = 0; B < NBLK; ++B) {
+            if (B == NBLK - 1 && threadIdx.x == 0) {
+                sh_nx[0] = fl_p >= i + 2;
+                sh_nx[1] = fl_s >= i + 2;
+            }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this block's image, the strips, earlier stores
+            __syncthreads();
             if (B == 0) {
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the strip, the last row's stores
-                __syncthreads();
                 if (pending) {
                     if (threadIdx.x == 0) cst(mine, pending);
                     pending = 0;
@@ -1270,21 +1324,13 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *T
                 fl_p = i + 1 < g.mt ? ld_flag(pdone) : 0;
                 fl_s = i + 1 < g.mt && prev ? ld_flag(prev) : 0;
             }
-            if (B == NBLK - 1 && threadIdx.x == 0) {
-                sh_nx[0] = fl_p >= i + 2;
-                sh_nx[1] = fl_s >= i + 2;
+            if (B + 1 < NBLK) {
+                img_load(i, B + 1, (B + 1) & 1);
+            } else if (i + 1 < g.mt) {
+                next0 = sh_nx[0] != 0;
+                if (next0) img_load(i + 1, 0, 0);
             }
-            const T *Vn = B + 1 < NBLK ? Aik + (i64)(B + 1) * IB * lda : Aik + NB;
-            const T *Tn = B + 1 < NBLK ? Tik + (i64)(B + 1) * IB * ldt : Tik + IB;
-            __syncthreads();
-            bool load_next = B + 1 < NBLK;
-            if (!load_next && i + 1 < g.mt) load_next = next0 = sh_nx[0] != 0;
-            // the next block's V and T, one element per product pair (into registers)
-            static_assert(VPre<T, NB>::NV + VPre<T, NB>::NT <= NB / 16 * 4 - 2, "every load part gets a product pair");
-            apply_wave_ns<Tw, NB, false, NS>(S, A1, B, B & 1, [&](int part) {
-                if (load_next && part < VPre<T, NB>::NV + VPre<T, NB>::NT) vload_one<T, NB>(P, Vn, lda, Tn, ldt, part);
-            });
-            if (B + 1 < NBLK || next0) vstore_cvt<T, Tw, NB, false>(P, (B + 1) & 1, 0);
+            apply_wave_ns<Tw, NB, false, NS>(S, A1, B, B & 1);
         }
         have0 = next0;
         // Straight from the registers: the prefetch region of one wave lies inside another
@@ -1746,16 +1792,16 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
         NLA_HIP(hipMemsetAsync(tr, 0, sizeof(unsigned long long) * ((size_t)ntasks * 4 + (size_t)kt * mt), s));
     }
     unsigned char *lpimg = nullptr;
-    if constexpr (sizeof(Tw) == 2) {
+    if constexpr (sizeof(Tw) < sizeof(T)) {
         // every panel tile's NB/IB block images, written by the panels, read by the walkers
         const size_t ntiles = (size_t)kt * mt - (size_t)kt * (kt - 1) / 2;
-        lpimg = (unsigned char *)scratch(SLOT_LPIMG, ntiles * nblk * (size_t)LdsLp<Tw, NB>::IMG_BYTES, s, &st);
+        const size_t img_bytes = sizeof(Tw) == 2 ? (size_t)LdsLp<Tw, NB>::IMG_BYTES : (size_t)F32Img<NB>::STRIDE;
+        lpimg = (unsigned char *)scratch(SLOT_LPIMG, ntiles * nblk * img_bytes, s, &st);
         if (!lpimg) return st;
     }
     DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr, (void *)C, ldc, mode == DAG_APPLY_Q ? 1 : 0};
-    if constexpr (sizeof(Tw) == 2)
-        NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpimg), &lpimg, sizeof(lpimg), 0, hipMemcpyHostToDevice, s));
     if constexpr (sizeof(Tw) < sizeof(T)) {
+        NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpimg), &lpimg, sizeof(lpimg), 0, hipMemcpyHostToDevice, s));
         // the trailing matrix's Float32 shadow (same positions and leading dimension as C)
         float *shadow = (float *)scratch(SLOT_SHADOW, sizeof(float) * (size_t)ldc * (size_t)ncols, s, &st);
         if (!shadow) return st;
