@@ -155,19 +155,6 @@ __device__ __forceinline__ void tiles_store(const typename Acc<T>::t *a, T *G, i
         for (int r = 0; r < 4; ++r) cst(&G[(r0 + 16 * rt + acc_row<T>(lk, r)) + (i64)lr * ld], a[rt][r]);
 }
 
-// rows >= rmin of the same strip (rows above are left to whoever owns them)
-template <typename T, int NT>
-__device__ __forceinline__ void tiles_store_from(const typename Acc<T>::t *a, T *G, i64 ld, int rmin) {
-    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 16 * rt + acc_row<T>(lk, r);
-            if (row >= rmin) cst(&G[row + (i64)lr * ld], a[rt][r]);
-        }
-}
-
 // Strip staging.  A wave's 16-column strip moves between HBM and its MFMA registers through
 // its own SW columns of LDS (no workgroup synchronisation): loads go straight to LDS with
 // global_load_lds, one whole column (NB elements, 16 bytes per lane) per instruction, and
@@ -1073,48 +1060,9 @@ __device__ __forceinline__ void run_walk_q(int k, int s, T *A, i64 lda, T *Tm, i
 }
 
 // ---------------------------------------------------------------------------------
-// Mixed-precision walker: the panel (geqrt/tsqrt blocks and T) stays in T = Float64; the
-// trailing update of the strips (unmqr/tsmqr) runs in Tw = Float32 on v_mfma_f32_16x16x4:
-// strips are rounded to Tw on load and widened back on store, the reflector blocks and T
-// are rounded when staged.  Same pipeline as the Float64 walker, with V through registers.
-template <typename Tw, typename T, int NT>
-__device__ __forceinline__ void tiles_load_cvt(typename Acc<Tw>::t *a, const T *G, i64 ld, int r0) {
-    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) a[rt][r] = (Tw)cld(&G[(r0 + 16 * rt + acc_row<Tw>(lk, r)) + (i64)lr * ld]);
-}
-template <typename Tw, typename T, int NT>
-__device__ __forceinline__ void tiles_store_cvt(const typename Acc<Tw>::t *a, T *G, i64 ld, int r0) {
-    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int rt = 0; rt < NT; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) cst(&G[(r0 + 16 * rt + acc_row<Tw>(lk, r)) + (i64)lr * ld], (T)a[rt][r]);
-}
-template <typename T, typename Tw, int NB, bool GE>
-__device__ __forceinline__ void vstore_cvt(const VPre<T, NB> &P, int buf, int j0) {
-    Tw *sm = lds_base<Tw>() + buf * Lds<Tw, NB>::BUF;
-    constexpr int LDV = Lds<Tw, NB>::LDV;
-    Tw *Vs = sm + Lds<Tw, NB>::V;
-    Tw *Ts = sm + Lds<Tw, NB>::TT;
-#pragma unroll
-    for (int e = 0; e < VPre<T, NB>::NV; ++e) {
-        const int idx = threadIdx.x + e * NTH, p = idx % NB, j = idx / NB;
-        T v = P.v[e];
-        if (GE) {
-            const int ja = j0 + j;
-            v = p < ja ? T(0) : (p == ja ? T(1) : v);
-        }
-        Vs[j * LDV + p] = (Tw)v;
-    }
-#pragma unroll
-    for (int e = 0; e < VPre<T, NB>::NT; ++e) {
-        const int idx = threadIdx.x + e * NTH, q = idx % IB, j = idx / IB;
-        Ts[q * (IB + 1) + j] = q <= j ? (Tw)P.t[e] : Tw(0);
-    }
-}
+// Mixed-precision walkers: the panel (geqrt/tsqrt blocks and T) stays in T = Float64; the
+// trailing update of the strips (unmqr/tsmqr) runs with Float32 accumulators (Float32, FP16
+// or BF16 products), the strips kept in a Float32 shadow between steps.
 
 // Float32 accumulator strips <-> memory without staging.  In the Float32-accumulator 16x16
 // MFMA layout (v_mfma_f32_16x16x4f32, _16x16x16_f16 / _bf16) a lane holds rows
