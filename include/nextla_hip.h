@@ -178,7 +178,9 @@ int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, 
  * stay Float32 between products; V, T and each product's operands are rounded); the panels,
  * larfg norms/scaling and the T factors stay Float64.  prec 'D' is nla_dtile_geqrf.  'H'
  * assumes the matrix's entries are within FP16 range.  Needs nb in {64, 128}, ib = 32 and
- * 16-byte aligned columns. */
+ * 16-byte aligned columns.  Holds in the device scratch pool a Float32 copy of the trailing
+ * matrix (4 lda n bytes) and the panels' per-block V/T images (~21 KB per ib-block of each
+ * panel tile). */
 int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
                           int64_t ldt, char prec, void *stream);
 /* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
