@@ -161,6 +161,7 @@ int nla_stile_geqrf(int64_t m, int64_t n, int64_t nb, int64_t ib, float *A, int6
  * with nb in {64, 128}: code = type << 30 | k << 18 | s, type 0 = PANEL(step k, block stage s),
  * type 1 = WALK(step k, 64-column strip group s).  Writes min(count, cap) codes to the host
  * array `codes` and returns the count, or -1 for a shape the persistent launch does not take.
+ * (nla_dtile_geqrf_mixed with nb = 128 uses the same order over 128-column groups.)
  * Host-only; exposed so the order can be checked against the tasks' dependencies. */
 int64_t nla_tile_qr_schedule(int64_t m, int64_t n, int64_t nb, int32_t *codes, int64_t cap);
 /* Apply Q (trans 'N') or Q^T ('T'/'C') of nla_?tile_geqrf from the left to C (m x nrhs). */

@@ -1851,8 +1851,8 @@ template int tile_apply_qt_dag<float>(i64, i64, i64, i64, const float *, i64, co
 
 // The factorization's task list (the claim order of the persistent launch), for inspection:
 // code = type << 30 | k << 18 | s with type 0 = PANEL(k, block stage s), 1 = WALK(k, strip
-// group s of TW = 64 columns; the 16-bit-update factorization's list is the same with groups
-// of 128 columns when nb = 128).  Returns the number of tasks (codes written up to cap), or -1
+// group s of TW = 64 columns; the reduced-precision factorizations' (Float32 / FP16 / BF16
+// updates) list is the same with groups of 128 columns when nb = 128).  Returns the number of tasks (codes written up to cap), or -1
 // for a shape the dataflow kernel does not take.
 NLA_API int64_t nla_tile_qr_schedule(int64_t m, int64_t n, int64_t nb, int32_t *codes, int64_t cap) {
     if ((nb != 64 && nb != 128) || m < nb || n < nb || m % nb || n % nb || m / nb >= 4096) {
