@@ -58,7 +58,8 @@ MIXED_BERR = {"S": (1e-9, 2e-5), "H": (1e-6, 2e-2), "B": (1e-5, 1e-1)}
 
 
 @pytest.mark.parametrize("prec", ["S", "H", "B"])
-@pytest.mark.parametrize("m,n,nb", [(1024, 512, 128), (512, 512, 64), (2048, 256, 128)])
+@pytest.mark.parametrize("m,n,nb", [(1024, 512, 128), (512, 512, 64), (2048, 256, 128), (1152, 640, 128), (640, 1152, 128),
+                                    (576, 448, 64)])
 def test_tile_geqrf_mixed_backward_error(m, n, nb, prec):
     nl = _nl()
     rng = np.random.default_rng(9)
