@@ -1331,8 +1331,8 @@ template <typename Tw> __device__ __forceinline__ typename LpMfma<Tw>::v4 lp_cvt
     return r;
 }
 // LDS of the 16-bit walker, in Tw elements: per buffer V column-major (Vc[j * LDC + p]),
-// V row-major (Vr[p * LDR + j]) and T column-major (Tc[j * LDR + q] = T[q][j]); the Float64
-// strip staging columns follow the two buffers.
+// V row-major (Vr[p * LDR + j]) and T column-major (Tc[j * LDR + q] = T[q][j]); the next
+// row's strip prefetch (StripStage) follows the two buffers.
 // Each buffer is padded to whole KB: the panel writes exactly this image (IMG_BYTES) to HBM
 // once per (tile, block) and walkers copy it into a buffer with 1 KB LDS-DMA instructions.
 template <typename Tw, int NB> struct LdsLp {
