@@ -1094,7 +1094,11 @@ template <typename Ts, int NB> __device__ __forceinline__ void strip_load(fx4 *a
         }
     }
 }
-template <typename Ts, int NB> __device__ __forceinline__ void strip_store(const fx4 *a, Ts *G, i64 ld) {
+// AUX: the store's cache policy, device-coherent (sc1) with or without the non-temporal hint
+// (measured: it helps the 16-bit walkers' Float32 shadow rows, 1-2 %, and costs the Float32
+// walker's about as much).
+constexpr int ST_SC1 = 16, ST_SC1_NT = 18;
+template <typename Ts, int NB, int AUX = ST_SC1> __device__ __forceinline__ void strip_store(const fx4 *a, Ts *G, i64 ld) {
     typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
     typedef double f64x2 __attribute__((ext_vector_type(2)));
     const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc<Ts, NB>(G, ld);
@@ -1103,11 +1107,11 @@ template <typename Ts, int NB> __device__ __forceinline__ void strip_store(const
     for (int rt = 0; rt < NB / 16; ++rt) {
         const int off = (int)(((i64)lr * ld + 16 * rt + 4 * lk) * (i64)sizeof(Ts));
         if constexpr (sizeof(Ts) == 4) {
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a[rt]), rsrc, off, 0, /*sc1*/ 16);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a[rt]), rsrc, off, 0, AUX);
         } else {
             const f64x2 d0{(double)a[rt][0], (double)a[rt][1]}, d1{(double)a[rt][2], (double)a[rt][3]};
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d0), rsrc, off, 0, 16);
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d1), rsrc, off + 16, 0, 16);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d0), rsrc, off, 0, AUX);
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, d1), rsrc, off + 16, 0, AUX);
         }
     }
 }
@@ -1515,7 +1519,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, 
         for (int q = 0; q < NS; ++q) {
             const i64 off = (i64)row_tile * NB + (col + q * SW) * ldc;
             if (f64) strip_store<T, NB>(X[q].a, Cm + off, ldc);
-            else strip_store<float, NB>(X[q].a, sh32 + off, ldc);
+            else strip_store<float, NB, ST_SC1_NT>(X[q].a, sh32 + off, ldc);
         }
     };
     unsigned char *const lpimg = g_lpimg;
