@@ -900,7 +900,6 @@ template <typename T, int NB>
 __device__ __forceinline__ void run_walk(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
     constexpr int NBLK = NB / IB;
     static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
-    __shared__ int sh_ready;
     const int w = threadIdx.x >> 6;
     const i64 col = (i64)s * TW + w * SW;  // this wave's first matrix column
     const int *prev = k > 0 ? &g.strip_prog[(k - 1) * g.nwalk + s] : nullptr;
@@ -1078,7 +1077,6 @@ template <typename Ts, int NB> __device__ __forceinline__ __amdgpu_buffer_rsrc_t
                                              __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
 }
 template <typename Ts, int NB> __device__ __forceinline__ void strip_load(fx4 *a, const Ts *G, i64 ld) {
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
     typedef double f64x2 __attribute__((ext_vector_type(2)));
     const __amdgpu_buffer_rsrc_t rsrc = strip_rsrc<Ts, NB>(G, ld);
     const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
@@ -1173,7 +1171,8 @@ template <int BYTES> __device__ __forceinline__ void img_dma(const unsigned char
 // 16-bit walker with v_mfma_f32_16x16x4 products: each block's V/T image (F32Img, written by
 // the panel) copied into an LDS buffer by DMA one block ahead, two 16-column strips per wave.
 template <typename T, typename Tw, int NB>
-__device__ __forceinline__ void run_walk_mixed(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+__device__ __forceinline__ void run_walk_mixed(int k, int s, T * /*A*/, i64 /*lda*/, T * /*Tm*/, i64 /*ldt*/, DagArgs g,
+                                               int *sh_flag) {
     constexpr int NBLK = NB / IB;
     static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
     static_assert(sizeof(Tw) == 4, "Float32 accumulator strips");
@@ -1486,7 +1485,8 @@ __device__ __forceinline__ void apply_lp(WStrip<float, NB> *S, WStrip<float, NB>
 }
 
 template <typename T, typename Tw, int NB>
-__device__ __forceinline__ void run_walk_lp(int k, int s, T *A, i64 lda, T *Tm, i64 ldt, DagArgs g, int *sh_flag) {
+__device__ __forceinline__ void run_walk_lp(int k, int s, T * /*A*/, i64 /*lda*/, T * /*Tm*/, i64 /*ldt*/, DagArgs g,
+                                            int *sh_flag) {
     constexpr int NBLK = NB / IB;
     constexpr i64 IMG = LdsLp<Tw, NB>::IMG_BYTES;
     static_assert(NBLK % 2 == 0, "block count must be even for the buffer parity");
