@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
     __shared__ T red[4][BS + 1];
     __shared__ T ys[BS];
     __shared__ T tr[TRANS ? 4 * 16 * 64 : 1];  // TRANS: the lanes' partial sums, transposed
-    __shared__ int sh_b, sh_stop, sh_known[2];
+    __shared__ int sh_b, sh_known[2];
     const int nblk = n / BS, t = threadIdx.x, lane = t & 63, g = t >> 6;
     for (;;) {
         __syncthreads();
