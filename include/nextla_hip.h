@@ -8,14 +8,17 @@
  *  - Sizes are int64_t.  Char flags take the reference's values ('L'/'R', 'N'/'C'/'T', ...).
  *    For real types 'T' means the same as 'C' (src/larfb.jl:79-85, src/parfb.jl:96-100).
  *  - `stream` is a hipStream_t passed as void*; NULL means the default stream.  Every
- *    call is stream-ordered and returns without synchronising; a caller that needs the
- *    reference's synchronous semantics calls nla_synchronize(stream).
+ *    call is stream-ordered.  Calls return without synchronising unless their comment says
+ *    "Synchronises the stream"; a caller that needs the reference's synchronous semantics
+ *    calls nla_synchronize(stream).
  *  - Return status: 0 = success; -i = the i-th argument is illegal (the reference would
  *    throw ArgumentError; nla_last_error() returns the reference's message text);
- *    > 0 = a HIP runtime error code.
+ *    NLA_STATUS_WAIT_TIMEOUT = a device-side dependency wait of a persistent launch ran
+ *    out of time (results invalid); any other value > 0 = a HIP runtime error code.
  *  - The caller owns every buffer.  Workspace arguments accept the reference's sizes;
  *    the library never needs more than the reference asks for, and it keeps any extra
- *    device scratch in a per-device pool of its own.
+ *    device scratch in a pool of its own per (device, stream), so calls on different
+ *    streams may run concurrently.
  *  - Quick returns and k clamping are those of the reference routine cited on each line.
  * 'd' = Float64, 's' = Float32.
  */
@@ -27,6 +30,8 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+#define NLA_STATUS_WAIT_TIMEOUT 10001
 
 /* Message of the last non-zero status on this thread (the reference's ArgumentError text). */
 const char *nla_last_error(void);
@@ -46,7 +51,7 @@ void nla_prof_reset(void);
 int nla_prof_read(const char *kernel, double *total_ms, int64_t *launches);
 
 /* ---- Householder generation ------------------------------------------------------ */
-/* larfg!(n, alpha, x, incx, tau): src/larfg.jl:123-186.  alpha and tau are DEVICE scalars;
+/* larfg!(n, alpha, x, incx, tau): src/larfg.jl:43-106.  alpha and tau are DEVICE scalars;
  * on return *alpha = beta, x = v[2:n], *tau = tau.  (Julia shim returns (alpha, tau).) */
 int nla_dlarfg(int64_t n, double *alpha, double *x, int64_t incx, double *tau, void *stream);
 int nla_slarfg(int64_t n, float *alpha, float *x, int64_t incx, float *tau, void *stream);
@@ -152,7 +157,10 @@ int nla_sgemm_acc(int64_t N, int64_t R, int64_t M, float alpha, const float *X, 
 /* ---- Tile-QR driver (no reference function: the composition of geqrt!/unmqr!/tsqrt!/
  * tsmqr! that config "tile QR" describes, flat-tree PLASMA order) ------------------------- */
 /* Factor A (m x n, both multiples of nb) in nb x nb tiles with inner block ib.
- * T receives one ib x nb block per tile: tile (i,k) at T[i*ib, k*nb], ldt >= (m/nb)*ib. */
+ * T receives one ib x nb block per tile: tile (i,k) at T[i*ib, k*nb], ldt >= (m/nb)*ib.
+ * With ib = 32, nb in {64, 128} and 16-byte aligned columns the factorization is one
+ * persistent launch that synchronises the stream (it reads the launch's wait-timeout flag);
+ * other shapes are enqueued as per-step launches without synchronising. */
 int nla_dtile_geqrf(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
                     int64_t ldt, void *stream);
 int nla_stile_geqrf(int64_t m, int64_t n, int64_t nb, int64_t ib, float *A, int64_t lda, float *T,
@@ -164,7 +172,8 @@ int nla_stile_geqrf(int64_t m, int64_t n, int64_t nb, int64_t ib, float *A, int6
  * (nla_dtile_geqrf_mixed with nb = 128 uses the same order over 128-column groups.)
  * Host-only; exposed so the order can be checked against the tasks' dependencies. */
 int64_t nla_tile_qr_schedule(int64_t m, int64_t n, int64_t nb, int32_t *codes, int64_t cap);
-/* Apply Q (trans 'N') or Q^T ('T'/'C') of nla_?tile_geqrf from the left to C (m x nrhs). */
+/* Apply Q (trans 'N') or Q^T ('T'/'C') of nla_?tile_geqrf from the left to C (m x nrhs).
+ * Synchronises the stream when the factorization's shape takes the persistent launch. */
 int nla_dtile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, int64_t ib,
                     const double *A, int64_t lda, const double *T, int64_t ldt, double *C, int64_t ldc,
                     void *stream);
@@ -181,24 +190,28 @@ int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, 
  * assumes the matrix's entries are within FP16 range.  Needs nb in {64, 128}, ib = 32 and
  * 16-byte aligned columns.  Holds in the device scratch pool a Float32 copy of the trailing
  * matrix (4 lda n bytes) and the panels' per-block V/T images (~21 KB per ib-block of each
- * panel tile). */
+ * panel tile).  Synchronises the stream. */
 int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
                           int64_t ldt, char prec, void *stream);
 /* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
  * out.  Tile QR of a copy of A (prec 'D' plain; 'S' / 'H' / 'B' as nla_dtile_geqrf_mixed,
  * Float64 data only), the direct solution x0 = R^-1 (Q^T b)[0:n] (returned for maxit = 0; with
  * prec 'D' and maxit > 0, x0 = R^-1 R^-T (A^T b), the seminormal equations), then up to maxit
- * iterations of LSQR (Paige & Saunders) on the correction, right-preconditioned by R: min ||(A R^-1) y - (b - A x0)||,
- * x = x0 + R^-1 y, with every product by A in the working precision.  Stops when LSQR's
- * estimate of ||M^T r|| / (||M||_F ||r||) (M = A R^-1) reaches tol or the residual estimate
- * falls to tol ||b - A x0||.  iters / last_correction (may be NULL) receive the iterations
- * taken and the final estimate.  Synchronises the stream. */
+ * iterations of LSQR (Paige & Saunders) on the correction, right-preconditioned by R:
+ * min ||(A R^-1) y - (b - A x0)||, x = x0 + R^-1 y, with every product by A in the working
+ * precision.  Stops when LSQR's estimate of ||M^T r|| / (||M||_F ||r||) (M = A R^-1) reaches
+ * tol or the residual estimate falls to tol ||b - A x0||.  With a 16-bit factor ('H', 'B')
+ * the contraction of that estimate is watched from iteration 12 on: when the iterations
+ * still needed at the observed rate exceed what is left of maxit, A is factored again with
+ * Float32 updates and LSQR restarts from that factor's direct solution.  iters / last_correction / prec_used
+ * (may be NULL) receive the iterations taken over all restarts, the final estimate and the
+ * update precision of the factor that produced x.  Synchronises the stream. */
 int nla_dtile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *A, int64_t lda, const double *b,
                    double *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
-                   void *stream);
+                   char *prec_used, void *stream);
 int nla_stile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const float *A, int64_t lda, const float *b,
                    float *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
-                   void *stream);
+                   char *prec_used, void *stream);
 /* Triangular solve op(A) X = alpha B / X op(A) = alpha B (arguments as nla_drectrxm with
  * func 'S'; X overwrites B) in Float32 with Float64 iterative refinement: X0 from the
  * Float32 recursion (cutoff as there), then up to maxit steps X += op(A)^-1 (alpha B - op(A) X)
@@ -273,7 +286,8 @@ int nla_strmm_rrule(char side, char uplo, char trans, float alpha, int64_t n, in
                     int64_t lda, const float *B, int64_t ldb, float *Ybar, int64_t ldybar, float *Abar,
                     int64_t ldabar, int64_t cutoff, void *stream);
 /* Thin QR A = Q R (m x n, n <= m, both multiples of nb) from nla_?tile_geqrf's output (AF, T):
- * given Qbar (m x n, overwritten with Abar) and Rbar (n x n). */
+ * given Qbar (m x n, overwritten with Abar) and Rbar (n x n).  Synchronises the stream
+ * (its Q applications are persistent tile-QR launches). */
 int nla_dqr_rrule(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *AF, int64_t lda, const double *T,
                   int64_t ldt, double *Qbar, int64_t ldqbar, const double *Rbar, int64_t ldrbar, void *stream);
 int nla_sqr_rrule(int64_t m, int64_t n, int64_t nb, int64_t ib, const float *AF, int64_t lda, const float *T,

@@ -26,7 +26,7 @@ NLA_API int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int6
 namespace nla {
 
 template <typename T>
-hipError_t rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
+int rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
                          i64 ldb, i64 cutoff, hipStream_t s);
 
 // C (n x n) := tri(C) (keep the uplo triangle with its diagonal, zero the other one).
@@ -119,7 +119,7 @@ static int trsm_rrule_entry(char side, char uplo, char trans, T alpha, i64 n, i6
     hipStream_t s = as_stream(stream);
     const bool left = side == 'L', tr = is_trans(trans), upper = uplo == 'U';
     // Y = op(A)^-T Xbar (left) / Xbar op(A)^-T (right), in place of Xbar
-    NLA_HIP(rectrxm_async<T>(side, uplo, flip(trans), T(1), 'S', n, m, A, lda, Xbar, ldxb, cutoff, s));
+    NLA_ST(rectrxm_async<T>(side, uplo, flip(trans), T(1), 'S', n, m, A, lda, Xbar, ldxb, cutoff, s));
     // op(A)bar = -Y X^T (left) / -X^T Y (right); Abar is its (transposed for op = T) triangle
     if (left) {
         if (!tr) NLA_HIP(tri_product<T>(true, upper, n, m, T(-1), Xbar, ldxb, X, ldx, Abar, ldab, s));
@@ -155,7 +155,7 @@ static int trmm_rrule_entry(char side, char uplo, char trans, T alpha, i64 n, i6
         else NLA_HIP(tri_product<T>(false, upper, n, m, alpha, Ybar, ldyb, B, ldb, Abar, ldab, s));
     }
     // Bbar = alpha op(A)^T Ybar (left) / alpha Ybar op(A)^T (right), in place of Ybar
-    NLA_HIP(rectrxm_async<T>(side, uplo, flip(trans), alpha, 'M', n, m, A, lda, Ybar, ldyb, cutoff, s));
+    NLA_ST(rectrxm_async<T>(side, uplo, flip(trans), alpha, 'M', n, m, A, lda, Ybar, ldyb, cutoff, s));
     return 0;
 }
 
@@ -198,7 +198,7 @@ static int qr_rrule_entry(i64 m, i64 n, i64 nb, i64 ib, const T *AF, i64 lda, co
     add_kernel<T><<<grid2(m, n), 256, 0, s>>>((int)m, (int)n, W, m, Qbar, ldq);
     NLA_LAUNCH_CHECK();
     // Abar = (Qbar + Q copyltu(M)) R^-T, in place of Qbar
-    NLA_HIP(rectrxm_async<T>('R', 'U', 'T', T(1), 'S', n, m, AF, lda, Qbar, ldq, 0, s));
+    NLA_ST(rectrxm_async<T>('R', 'U', 'T', T(1), 'S', n, m, AF, lda, Qbar, ldq, 0, s));
     return 0;
 }
 

@@ -2,6 +2,8 @@
 // the small utility entry points of the C ABI.
 #include "nla_common.hpp"
 
+#include <map>
+#include <utility>
 #include <vector>
 
 namespace nla {
@@ -27,6 +29,11 @@ int arg_error(int pos, const char *fmt, ...) {
     return -pos;
 }
 
+int wait_timeout_error(const char *what) {
+    set_error("%s: a dependency wait exceeded its time limit (results are invalid)", what);
+    return STATUS_WAIT_TIMEOUT;
+}
+
 int hip_error(hipError_t e, const char *what) {
     set_error("HIP error %d (%s) in %s", int(e), hipGetErrorString(e), what);
     return int(e) > 0 ? int(e) : 1;
@@ -39,7 +46,9 @@ struct Pool {
     size_t size[NUM_SLOTS] = {};
 };
 std::mutex g_pool_mu;
-Pool g_pool[MAX_DEV];
+// One pool per (device, stream): calls enqueued on different streams never share scratch, so
+// they may run concurrently; calls on one stream are ordered and reuse the stream's buffers.
+std::map<std::pair<int, hipStream_t>, Pool> g_pool;
 }  // namespace
 
 void *scratch(ScratchSlot slot, size_t bytes, hipStream_t stream, int *status) {
@@ -51,13 +60,13 @@ void *scratch(ScratchSlot slot, size_t bytes, hipStream_t stream, int *status) {
         return nullptr;
     }
     std::lock_guard<std::mutex> lock(g_pool_mu);
-    Pool &p = g_pool[dev];
+    Pool &p = g_pool[std::make_pair(dev, stream)];
     if (bytes <= p.size[slot] && p.ptr[slot]) return p.ptr[slot];
     size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
     want = (want + 255) & ~size_t(255);
     if (p.ptr[slot]) {
+        // only this stream's work can use the old buffer: drain it, then replace the buffer
         e = hipStreamSynchronize(stream);
-        if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e != hipSuccess) { *status = hip_error(e, "scratch sync"); return nullptr; }
         (void)hipFree(p.ptr[slot]);
         p.ptr[slot] = nullptr;
@@ -150,6 +159,8 @@ NLA_API int64_t nla_scratch_bytes(void) {
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= nla::MAX_DEV) return 0;
     std::lock_guard<std::mutex> lock(nla::g_pool_mu);
     int64_t s = 0;
-    for (int i = 0; i < nla::NUM_SLOTS; ++i) s += int64_t(nla::g_pool[dev].size[i]);
+    for (auto &kv : nla::g_pool)
+        if (kv.first.first == dev)
+            for (int i = 0; i < nla::NUM_SLOTS; ++i) s += int64_t(kv.second.size[i]);
     return s;
 }

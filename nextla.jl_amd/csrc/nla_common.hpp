@@ -18,8 +18,11 @@ namespace nla {
 using i64 = int64_t;
 
 // ---------------------------------------------------------------------------------
-// Status handling: 0 ok, -i illegal argument i (message kept per thread), >0 HIP error.
+// Status handling: 0 ok, -i illegal argument i (message kept per thread), >0 HIP error,
+// STATUS_WAIT_TIMEOUT a device-side dependency wait of a persistent launch ran out of time.
+constexpr int STATUS_WAIT_TIMEOUT = 10001;
 void set_error(const char *fmt, ...);
+int wait_timeout_error(const char *what);
 int arg_error(int pos, const char *fmt, ...);
 int hip_error(hipError_t e, const char *what);
 
@@ -31,12 +34,19 @@ int hip_error(hipError_t e, const char *what);
 
 #define NLA_LAUNCH_CHECK() NLA_HIP(hipGetLastError())
 
+// Propagate a library status (0 ok, -i argument error, > 0 HIP or library error) unchanged.
+#define NLA_ST(expr)                                \
+    do {                                            \
+        int nla_st_ = (expr);                       \
+        if (nla_st_ != 0) return nla_st_;           \
+    } while (0)
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Per-device scratch pool.  Grows (never shrinks) under a mutex; growth synchronises the
-// stream first so in-flight kernels never see the old buffer freed.  Kernels of one call
-// share the slot they asked for; calls on different streams of one device must not overlap
-// on the same slot (slots are keyed by purpose).
+// Scratch pool per (device, stream), slots keyed by purpose.  Grows (never shrinks) under a
+// mutex; growth synchronises the stream first so its in-flight kernels never see the old
+// buffer freed.  Kernels of one call share the slot they asked for; calls on different
+// streams use different buffers and may overlap.
 enum ScratchSlot { SLOT_W = 0, SLOT_AUX = 1, SLOT_SCHED = 2, SLOT_MIXED = 3, SLOT_TRSM = 4, SLOT_RHS = 5, SLOT_LS = 6, SLOT_LPIMG = 7, SLOT_SHADOW = 8, NUM_SLOTS = 9 };
 void *scratch(ScratchSlot slot, size_t bytes, hipStream_t stream, int *status);
 

@@ -16,7 +16,7 @@
 namespace nla {
 
 template <typename T>
-hipError_t rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
+int rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
                          i64 ldb, i64 cutoff, hipStream_t s);
 
 // Dst (rows x cols, To) = alpha * Src (From); only the uplo triangle when tri != 0 ('L'/'U')
@@ -93,7 +93,7 @@ static int rectrsm_mixed_entry(char side, char uplo, char trans, double alpha, i
     NLA_LAUNCH_CHECK();
     convert_kernel<float, double><<<gb, 256, 0, s>>>((int)rows, (int)cols, 0, 1.0, B0, rows, R32, rows);
     NLA_LAUNCH_CHECK();
-    NLA_HIP(rectrxm_async<float>(side, uplo, trans, 1.0f, 'S', n, m, A32, n, R32, rows, cutoff, s));
+    NLA_ST(rectrxm_async<float>(side, uplo, trans, 1.0f, 'S', n, m, A32, n, R32, rows, cutoff, s));
     convert_kernel<double, float><<<gb, 256, 0, s>>>((int)rows, (int)cols, 0, 1.0f, R32, rows, B, ldb);
     NLA_LAUNCH_CHECK();
     double prev = 0;
@@ -101,11 +101,11 @@ static int rectrsm_mixed_entry(char side, char uplo, char trans, double alpha, i
         // R = B0 - op(A) X in Float64
         NLA_HIP(hipMemcpy2DAsync(Y, sizeof(double) * rows, B, sizeof(double) * ldb, sizeof(double) * rows, cols,
                                  hipMemcpyDeviceToDevice, s));
-        NLA_HIP(rectrxm_async<double>(side, uplo, trans, 1.0, 'M', n, m, A, lda, Y, rows, 256, s));
+        NLA_ST(rectrxm_async<double>(side, uplo, trans, 1.0, 'M', n, m, A, lda, Y, rows, 256, s));
         residual_kernel<<<gb, 256, 0, s>>>((int)rows, (int)cols, B0, rows, Y, rows, R32, rows);
         NLA_LAUNCH_CHECK();
         // D = op(A)^-1 R in Float32; X += D
-        NLA_HIP(rectrxm_async<float>(side, uplo, trans, 1.0f, 'S', n, m, A32, n, R32, rows, cutoff, s));
+        NLA_ST(rectrxm_async<float>(side, uplo, trans, 1.0f, 'S', n, m, A32, n, R32, rows, cutoff, s));
         NLA_HIP(hipMemsetAsync(sums, 0, 2 * sizeof(double), s));
         update_kernel<<<gb, 256, 0, s>>>((int)rows, (int)cols, R32, rows, B, ldb, sums);
         NLA_LAUNCH_CHECK();

@@ -23,7 +23,7 @@
 namespace nla {
 
 // ------------------------------------------------------------------------------------
-// larfg!: src/larfg.jl:123-186 (safe rescale loop kept; beta via hypot as LAPACK dlarfg).
+// larfg!: src/larfg.jl:43-106 (safe rescale loop kept; beta via hypot as LAPACK dlarfg).
 template <typename T> __device__ __forceinline__ T lamch_sfmin();
 template <> __device__ __forceinline__ double lamch_sfmin<double>() { return 2.2250738585072014e-308; }
 template <> __device__ __forceinline__ float lamch_sfmin<float>() { return 1.17549435e-38f; }
@@ -672,7 +672,7 @@ static bool valid_trans(char t) { return t == 'N' || t == 'C' || t == 'T'; }
 
 template <typename T> static int larfg_entry(i64 n, T *alpha, T *x, i64 incx, T *tau, void *stream) {
     hipStream_t s = as_stream(stream);
-    if (n <= 1) {  // src/larfg.jl:127-130
+    if (n <= 1) {  // src/larfg.jl:47-50
         NLA_HIP(hipMemsetAsync(tau, 0, sizeof(T), s));
         return 0;
     }
@@ -1024,7 +1024,17 @@ template <typename T> __global__ void fill_rand_kernel(T *p, i64 n, unsigned see
 template <typename T>
 static int batched_autotune_entry(i64 nclass, const i64 *m_values, const i64 *n_values, i64 samples,
                                   const i64 *cand, i64 ncand, i64 *ib_out, double *ms_out, void *stream) {
-    if (nclass < 0 || samples <= 0 || ncand <= 0) return arg_error(1, "need nclass >= 0, samples > 0 and candidates");
+    if (nclass < 0) return arg_error(1, "nclass must be non-negative, got %lld", (long long)nclass);
+    if (samples <= 0) return arg_error(4, "samples must be positive, got %lld", (long long)samples);
+    if (ncand <= 0 || !cand) return arg_error(6, "need at least one candidate ib");
+    // the batched kernel holds ib <= 64 (its LDS panel buffers and the 64-row T blocks)
+    for (i64 q = 0; q < ncand; ++q)
+        if (cand[q] < 1 || cand[q] > 64)
+            return arg_error(5, "candidate ib must be in 1..64, got %lld", (long long)cand[q]);
+    for (i64 c = 0; c < nclass; ++c)
+        if (m_values[c] <= 0 || n_values[c] <= 0 || n_values[c] > m_values[c])
+            return arg_error(2, "class %lld: need 0 < n <= m", (long long)c);
+    if (nclass == 0) return 0;
     hipStream_t s = as_stream(stream);
     int *counter = nullptr;
     BatchProblem<T> *dprob = nullptr;

@@ -720,14 +720,6 @@ template <typename T, int NB> __device__ __forceinline__ void geqrt_block(T *A, 
     __syncthreads();
 }
 
-__device__ unsigned long long *g_phase_trace = nullptr;  // debug: per-block phase stamps of one tile
-__device__ __forceinline__ void phase_stamp(int slot) {
-    if (g_phase_trace && threadIdx.x == 0) {
-        unsigned long long *p = g_phase_trace;
-        if (p[slot] == 0) p[slot] = __builtin_amdgcn_s_memrealtime();
-    }
-}
-
 // One ib-block of tsqrt!(NB, NB, IB, R, A2, T) (src/tsqrt.jl:146-155): the TS block QR of
 // columns c0..c0+IB-1 of [R; A2], then tsmqr!('L','C') of that block on the trailing columns
 // (A1 = rows c0..c0+IB-1 of R, A2).
@@ -749,9 +741,7 @@ __device__ __forceinline__ void tsqrt_block(T *R, i64 ldr, T *A2, i64 lda2, T *T
         Vs[j * LDV + p] = cld(&A2[p + (i64)(c0 + j) * lda2]);
     }
     __syncthreads();
-    phase_stamp(4 * (c0 / IB) + 0);
     block_factor<T, NB, false>(c0, nullptr, 0);
-    phase_stamp(4 * (c0 / IB) + 1);
     for (int idx = t; idx < NB * IB; idx += NTH) {
         const int p = idx % NB, j = idx / NB;
         cst(&A2[p + (i64)(c0 + j) * lda2], Vs[j * LDV + p]);
@@ -765,7 +755,6 @@ __device__ __forceinline__ void tsqrt_block(T *R, i64 ldr, T *A2, i64 lda2, T *T
         const int i = idx % IB, j = idx / IB;
         if (i <= j) cst(&Tm[i + (i64)(c0 + j) * ldt], Ts[i * (IB + 1) + j]);
     }
-    phase_stamp(4 * (c0 / IB) + 2);
     // The trailing columns of A2 move through the wave's strip staging columns (whole-column
     // LDS-DMA loads, column-contiguous stores), which overlay the R / scratch regions, free
     // once the block's R and T are stored.  No other block stage touches these columns of
@@ -786,7 +775,6 @@ __device__ __forceinline__ void tsqrt_block(T *R, i64 ldr, T *A2, i64 lda2, T *T
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    phase_stamp(4 * (c0 / IB) + 3);
 }
 
 // ---------------------------------------------------------------------------------
@@ -798,19 +786,15 @@ struct DagArgs {
     int *panel_prog;          // [kt][NB/IB]: tiles of column k done by block stage b
     int *strip_prog;          // [kt][nwalk]
     int *err;
-    unsigned long long *trace;  // optional timeline: [ntasks][4] = claim, -, end, workgroup
-    unsigned long long *ptrace; // optional: [kt][mt] panel tile completion times
     void *C;                    // matrix the walkers update (A itself when factoring)
     i64 ldc;
     int qmode;                  // walkers apply Q (reverse order) instead of Q^T
+    // Reduced-precision factorizations only (null otherwise): per (panel tile, block) the
+    // V/T image the panel writes and the walkers read, and for the 16-bit walkers the
+    // trailing matrix in Float32 between steps (same positions and leading dimension).
+    unsigned char *lpimg;
+    float *shadow;
 };
-// Reduced-precision walkers: per (panel tile, block) the V/T image the panel writes.  A device global,
-// not a DagArgs field: the Float64 kernel sits at its register limit and one more argument
-// makes it spill.  Set by the host before each launch (launches on one device are ordered).
-__device__ unsigned char *g_lpimg = nullptr;
-// 16-bit walkers: the trailing matrix in Float32 between steps (same positions and leading
-// dimension as the matrix; set by the host before each such launch).
-__device__ float *g_shadow = nullptr;
 
 // 16-bit walker support, defined with that walker below
 template <typename Tw, int NB> struct LdsLp;
@@ -874,12 +858,10 @@ __device__ __noinline__ void run_panel(int k, int b, T *A, i64 lda, T *Tm, i64 l
         else tsqrt_block<T, NB>(Akk, lda, A + (i64)i * NB + (i64)k * NB * lda, lda, Tik, ldt, c0);
         // the block's V and T for the reduced-precision walkers
         if constexpr (sizeof(Tw) == 2)
-            write_lp_image<T, Tw, NB>(g_lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)LdsLp<Tw, NB>::IMG_BYTES);
+            write_lp_image<T, Tw, NB>(g.lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)LdsLp<Tw, NB>::IMG_BYTES);
         else if constexpr (sizeof(Tw) < sizeof(T))
-            write_f32_image<T, NB>(g_lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)F32Img<NB>::STRIDE);
+            write_f32_image<T, NB>(g.lpimg + (lp_tile_index(k, i, g.mt) * NBLK + b) * (i64)F32Img<NB>::STRIDE);
         publish(mine, i + 1);
-        if (b == NBLK - 1 && g.ptrace && threadIdx.x == 0)
-            g.ptrace[(size_t)k * g.mt + i] = __builtin_amdgcn_s_memrealtime();
     }
 }
 
@@ -1193,7 +1175,7 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T * /*A*/, i64 /*ld
     using Stg = StripStage<Lds<Tw, NB>::SSTG * (int)sizeof(Tw), NB, NS>;
     static_assert(2 * Lds<Tw, NB>::BUF <= Lds<Tw, NB>::SSTG, "the image buffers end before the staging");
     static_assert(WAVES * Stg::NG * Stg::GRP <= TW * Lds<T, NB>::LDS_S * (int)sizeof(T), "staging fits the strip region");
-    float *const sh32 = g_shadow;
+    float *const sh32 = g.shadow;
     const bool src32 = k > 0, dst64 = s / (NB / TWK) == k + 1;
     auto load_strips = [&](WStrip<Tw, NB> *X, int row_tile) {
 #pragma unroll
@@ -1211,7 +1193,7 @@ __device__ __forceinline__ void run_walk_mixed(int k, int s, T * /*A*/, i64 /*ld
             else strip_store<float, NB>(X[q].a, sh32 + off, ldc);
         }
     };
-    unsigned char *const imgs = g_lpimg;
+    unsigned char *const imgs = g.lpimg;
     auto img_load = [&](int i, int B, int buf) {
         img_dma<F32Img<NB>::BYTES>(imgs + (lp_tile_index(k, i, g.mt) * NBLK + B) * (i64)F32Img<NB>::STRIDE,
                                    lds_base<unsigned char>() + buf * Lds<Tw, NB>::BUF * (int)sizeof(Tw));
@@ -1504,7 +1486,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T * /*A*/, i64 /*lda*/
     // next panel's tile column, which that panel reads, and the final R row (A1) always.
     // Strips are stored straight from registers (strip_store); the next row's Float32 strips
     // are prefetched into LDS (LpStage), step 0's Float64 ones loaded at the row start.
-    float *const sh32 = g_shadow;
+    float *const sh32 = g.shadow;
     const bool src32 = k > 0, dst64 = s / (NB / TWK) == k + 1;
     auto load_strips = [&](WStrip<float, NB> *X, int row_tile) {
 #pragma unroll
@@ -1522,7 +1504,7 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T * /*A*/, i64 /*lda*/
             else strip_store<float, NB, ST_SC1_NT>(X[q].a, sh32 + off, ldc);
         }
     };
-    unsigned char *const lpimg = g_lpimg;
+    unsigned char *const lpimg = g.lpimg;
     auto img = [&](int i, int B) { return lpimg + (lp_tile_index(k, i, g.mt) * NBLK + B) * IMG; };
     wait_ge(pdone, k + 1, g.err, sh_flag);
     if (prev) wait_ge(prev, k + 1, g.err, sh_flag);
@@ -1622,10 +1604,6 @@ __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i
         const int task = sh_task;
         if (task >= g.ntasks) break;
         const int code = g.tasks[task];
-        if (g.trace && threadIdx.x == 0) {
-            g.trace[(size_t)task * 4 + 0] = __builtin_amdgcn_s_memrealtime();
-            g.trace[(size_t)task * 4 + 3] = blockIdx.x;
-        }
         if (task_type(code) == TASK_PANEL) run_panel<T, NB, PanelTw<T, Tw>>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
         else if constexpr (sizeof(Tw) == sizeof(T)) {
             if (g.qmode) run_walk_q<T, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
@@ -1635,7 +1613,6 @@ __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i
         } else {
             run_walk_mixed<T, Tw, NB>(task_k(code), task_s(code), A, lda, Tm, ldt, g, &sh_flag);
         }
-        if (g.trace && threadIdx.x == 0) g.trace[(size_t)task * 4 + 2] = __builtin_amdgcn_s_memrealtime();
     }
 }
 
@@ -1733,16 +1710,6 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
         fill_int_kernel<<<(npp + 255) / 256, 256, 0, s>>>(pp, npp, mt);
         NLA_LAUNCH_CHECK();
     }
-    static const bool want_trace = getenv("NLA_TILEQR_TRACE") != nullptr;
-    const bool trace = want_trace && mode == DAG_FACTOR;
-    unsigned long long *tr = nullptr, *ptr = nullptr;
-    if (trace) {
-        tr = (unsigned long long *)scratch(SLOT_MIXED, sizeof(unsigned long long) * ((size_t)ntasks * 4 + (size_t)kt * mt),
-                                          s, &st);
-        if (!tr) return st;
-        ptr = tr + (size_t)ntasks * 4;
-        NLA_HIP(hipMemsetAsync(tr, 0, sizeof(unsigned long long) * ((size_t)ntasks * 4 + (size_t)kt * mt), s));
-    }
     unsigned char *lpimg = nullptr;
     if constexpr (sizeof(Tw) < sizeof(T)) {
         // every panel tile's NB/IB block images, written by the panels, read by the walkers
@@ -1751,21 +1718,14 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
         lpimg = (unsigned char *)scratch(SLOT_LPIMG, ntiles * nblk * img_bytes, s, &st);
         if (!lpimg) return st;
     }
-    DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, tr, ptr, (void *)C, ldc, mode == DAG_APPLY_Q ? 1 : 0};
+    float *shadow = nullptr;
     if constexpr (sizeof(Tw) < sizeof(T)) {
-        NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_lpimg), &lpimg, sizeof(lpimg), 0, hipMemcpyHostToDevice, s));
         // the trailing matrix's Float32 shadow (same positions and leading dimension as C)
-        float *shadow = (float *)scratch(SLOT_SHADOW, sizeof(float) * (size_t)ldc * (size_t)ncols, s, &st);
+        shadow = (float *)scratch(SLOT_SHADOW, sizeof(float) * (size_t)ldc * (size_t)ncols, s, &st);
         if (!shadow) return st;
-        NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_shadow), &shadow, sizeof(shadow), 0, hipMemcpyHostToDevice, s));
     }
-    unsigned long long *phase = nullptr;
-    if (trace) {
-        phase = (unsigned long long *)scratch(SLOT_W, 64 * sizeof(unsigned long long), s, &st);
-        if (!phase) return st;
-        NLA_HIP(hipMemsetAsync(phase, 0, 64 * sizeof(unsigned long long), s));
-    }
-    NLA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_phase_trace), &phase, sizeof(phase), 0, hipMemcpyHostToDevice, s));
+    DagArgs g{mt, nt, nwalk, ntasks, d_tasks, counter, pp, sp, err, (void *)C, ldc, mode == DAG_APPLY_Q ? 1 : 0,
+              lpimg, shadow};
     {
         ProfScope prof(PK_TILE_QR, s);
         tileqr_dag_kernel<T, NB, Tw><<<grid, NTH, sh, s>>>(A, lda, Tm, ldt, g);
@@ -1774,27 +1734,7 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
     int herr = 0;
     NLA_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
     NLA_HIP(hipStreamSynchronize(s));
-    if (trace) {
-        // "task type k s claim end wg" lines, then "panel k i time" lines
-        const char *path = getenv("NLA_TILEQR_TRACE");
-        std::vector<unsigned long long> h((size_t)ntasks * 4 + (size_t)kt * mt);
-        NLA_HIP(hipMemcpy(h.data(), tr, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        if (FILE *f = fopen(path, "w")) {
-            for (int i = 0; i < ntasks; ++i)
-                fprintf(f, "task %d %d %d %llu %llu %llu\n", tasks[i] >> 30, (tasks[i] >> 18) & 0xfff, tasks[i] & 0x3ffff,
-                        h[i * 4 + 0], h[i * 4 + 2], h[i * 4 + 3]);
-            for (int k = 0; k < kt; ++k)
-                for (int i = k; i < mt; ++i) fprintf(f, "panel %d %d %llu\n", k, i, h[(size_t)ntasks * 4 + (size_t)k * mt + i]);
-            unsigned long long ph[64];
-            if (hipMemcpy(ph, phase, sizeof ph, hipMemcpyDeviceToHost) == hipSuccess)
-                for (int i = 0; i < 16; ++i) fprintf(f, "phase %d %llu\n", i, ph[i]);
-            fclose(f);
-        }
-    }
-    if (herr) {
-        set_error("tile QR: a dependency wait exceeded its time limit (results are invalid)");
-        return -1;
-    }
+    if (herr) return wait_timeout_error("tile QR");
     return 0;
 }
 

@@ -21,7 +21,7 @@ template <typename T>
 int tile_apply_qt_dag(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, const T *Tm, i64 ldt, T *C, i64 ldc,
                       i64 ncols, hipStream_t s, bool *handled, bool qt);
 template <typename T>
-hipError_t rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
+int rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
                          i64 ldb, i64 cutoff, hipStream_t s);
 
 template <typename T>
@@ -40,7 +40,7 @@ static int tile_geqrf_entry(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, 
     {
         bool handled = false;
         const int r = tile_geqrf_dag<T>(m, n, nb, ib, A, lda, Tm, ldt, s, &handled);
-        if (handled) return r < 0 ? 1 : r;
+        if (handled) return r;
     }
     // other (nb, ib): host loop over per-step launches
     int st = 0;
@@ -88,7 +88,7 @@ static int tile_ormqr_entry(char trans, i64 m, i64 n, i64 nrhs, i64 nb, i64 ib, 
         bool handled = false;
         if (ncols == nrhs) {
             const int r = tile_apply_qt_dag<T>(m, n, nb, ib, A, lda, Tm, ldt, C, ldc, ncols, s, &handled, !notrans);
-            if (handled) return r < 0 ? 1 : r;
+            if (handled) return r;
         } else {
             int st = 0;
             T *P = (T *)scratch(SLOT_RHS, sizeof(T) * (size_t)m * ncols, s, &st);
@@ -97,7 +97,7 @@ static int tile_ormqr_entry(char trans, i64 m, i64 n, i64 nrhs, i64 nb, i64 ib, 
             NLA_HIP(hipMemcpy2DAsync(P, sizeof(T) * m, C, sizeof(T) * ldc, sizeof(T) * m, nrhs, hipMemcpyDeviceToDevice, s));
             const int r = tile_apply_qt_dag<T>(m, n, nb, ib, A, lda, Tm, ldt, P, m, ncols, s, &handled, !notrans);
             if (handled) {
-                if (r) return r < 0 ? 1 : r;
+                if (r) return r;
                 NLA_HIP(hipMemcpy2DAsync(C, sizeof(T) * ldc, P, sizeof(T) * m, sizeof(T) * m, nrhs, hipMemcpyDeviceToDevice, s));
                 return 0;
             }
@@ -136,7 +136,7 @@ static int tile_geqrf_mixed_entry(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 l
     bool handled = false;
     const int r = tile_geqrf_dag_mixed(m, n, nb, ib, A, lda, Tm, ldt, prec, as_stream(stream), &handled);
     if (!handled) return arg_error(3, "mixed precision needs nb in {64, 128}, ib = 32 and 16-byte aligned columns");
-    return r < 0 ? 1 : r;
+    return r;
 }
 
 // ---------------------------------------------------------------------------------
@@ -433,9 +433,18 @@ template <typename T> __global__ void axpy1_kernel(i64 n, const T *x, T *y) {
 // (x += R^-1 (Q^T r)[0:n]) diverges: kappa(A) times the factorization error near or above one.
 // Stops after maxit iterations or when LSQR's estimate of ||M^T r|| / (||M|| ||r||)
 // (M = A R^-1) falls to tol, or the residual estimate to tol ||b - A x0||.
+// A 16-bit factor ('H', 'B') is a usable preconditioner only while kappa(A) times its
+// backward error stays well below one.  While refining with one, the contraction of LSQR's
+// estimate is watched: from iteration STALL_START on, every STALL_SPAN iterations the
+// iterations still needed at the observed rate are projected, and when they exceed what is
+// left of maxit (or the estimate has stopped falling) A is factored again with Float32
+// updates and LSQR restarts from that factor's direct solution.  prec_used receives the
+// factor's precision of the final solve.
+constexpr int STALL_START = 12, STALL_SPAN = 6;
+
 template <typename T>
 static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, const T *b, T *x, char prec,
-                           i64 maxit, double tol, i64 *iters, double *last, void *stream) {
+                           i64 maxit, double tol, i64 *iters, double *last, char *prec_used, void *stream) {
     prec = up(prec);
     if (m < 0) return arg_error(1, "m must be non-negative");
     if (n < 0 || n > m) return arg_error(2, "n must be in 0..m");
@@ -448,6 +457,7 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     if (maxit < 0) return arg_error(10, "maxit must be non-negative");
     if (iters) *iters = 0;
     if (last) *last = 0;
+    if (prec_used) *prec_used = prec;
     if (m == 0 || n == 0) return 0;
     hipStream_t s = as_stream(stream);
     const i64 mt = m / nb, ldt = mt * ib;
@@ -469,27 +479,37 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     T *Dinv = (T *)(sync + nsync + (nsync & 1));  // 8-byte aligned
     NLA_HIP(hipMemsetAsync(sync, 0, sizeof(int) * nsync, s));
     // x := R^-1 x / R^-T x with R the upper triangle of the factor
-    auto rsolve = [&](bool trans, T *vec) -> hipError_t {
-        if (n % TRSV_BS == 0) return trsv_upper<T>(trans, (int)n, AF, m, Dinv, vec, sync, s);
+    auto rsolve = [&](bool trans, T *vec) -> int {
+        if (n % TRSV_BS == 0) {
+            const hipError_t e = trsv_upper<T>(trans, (int)n, AF, m, Dinv, vec, sync, s);
+            return e == hipSuccess ? 0 : hip_error(e, "trsv_upper");
+        }
         return rectrxm_async<T>('L', 'U', trans ? 'T' : 'N', T(1), 'S', n, 1, AF, m, vec, n, 0, s);
     };
-    NLA_HIP(hipMemcpy2DAsync(AF, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, s));
-    NLA_HIP(hipMemsetAsync(Tm, 0, sizeof(T) * (size_t)ldt * n, s));
-    int rc;
-    if (prec != 'D') rc = tile_geqrf_mixed_entry(m, n, nb, ib, (double *)AF, m, (double *)Tm, ldt, prec, s);
-    else rc = tile_geqrf_entry<T>(m, n, nb, ib, AF, m, Tm, ldt, s);
-    if (rc) return rc;
-    if (n % TRSV_BS == 0) {
-        diag_block_inv_kernel<T><<<(unsigned)(n / TRSV_BS), TRSV_BS, 0, s>>>(AF, m, Dinv);
-        NLA_LAUNCH_CHECK();
-    }
-    if (maxit == 0 || prec != 'D') {
-        // x0 = R^-1 (Q^T b)[0:n]
+    // the factor of a copy of A with trailing updates in precision p, and R's diagonal-block inverses
+    auto factor = [&](char p) -> int {
+        NLA_HIP(hipMemcpy2DAsync(AF, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, s));
+        NLA_HIP(hipMemsetAsync(Tm, 0, sizeof(T) * (size_t)ldt * n, s));
+        if (p != 'D') NLA_ST(tile_geqrf_mixed_entry(m, n, nb, ib, (double *)AF, m, (double *)Tm, ldt, p, s));
+        else NLA_ST(tile_geqrf_entry<T>(m, n, nb, ib, AF, m, Tm, ldt, s));
+        if (n % TRSV_BS == 0) {
+            diag_block_inv_kernel<T><<<(unsigned)(n / TRSV_BS), TRSV_BS, 0, s>>>(AF, m, Dinv);
+            NLA_LAUNCH_CHECK();
+        }
+        return 0;
+    };
+    // x0 = R^-1 (Q^T b)[0:n]
+    auto direct_start = [&]() -> int {
         NLA_HIP(hipMemsetAsync(Cb, 0, sizeof(T) * (size_t)m * W, s));
         NLA_HIP(hipMemcpyAsync(Cb, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
-        if ((rc = tile_ormqr_entry<T>('T', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s))) return rc;
-        NLA_HIP(rsolve(false, Cb));
+        NLA_ST(tile_ormqr_entry<T>('T', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s));
+        NLA_ST(rsolve(false, Cb));
         NLA_HIP(hipMemcpyAsync(x, Cb, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+        return 0;
+    };
+    NLA_ST(factor(prec));
+    if (maxit == 0 || prec != 'D') {
+        NLA_ST(direct_start());
         if (maxit == 0) {
             NLA_HIP(hipStreamSynchronize(s));
             return 0;
@@ -498,8 +518,8 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
         // x0 = R^-1 R^-T (A^T b)
         gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, b, x);
         NLA_LAUNCH_CHECK();
-        NLA_HIP(rsolve(true, x));
-        NLA_HIP(rsolve(false, x));
+        NLA_ST(rsolve(true, x));
+        NLA_ST(rsolve(false, x));
     }
     const dim3 gpart((unsigned)rb, (unsigned)nsplit);
     auto norm_read = [&](int nblocks, double *out) -> hipError_t {
@@ -509,89 +529,110 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
         if ((e = hipMemcpyAsync(out, dnorm, sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
         return hipStreamSynchronize(s);
     };
-    // M = A R^-1.  beta1 u1 = r0 = b - A x0;  alpha1 v1 = M^T u1 = R^-T (A^T u1)
-    NLA_HIP(hipMemcpyAsync(u, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
-    gemv_n_part_kernel<T><<<gpart, 256, 0, s>>>(m, n, A, lda, x, part, nsplit);
-    NLA_LAUNCH_CHECK();
-    gemv_n_finish_kernel<T><<<gm, 256, 0, s>>>(m, nsplit, part, T(1), u, bsum);  // u = A x0 - b
-    NLA_LAUNCH_CHECK();
-    double beta = 0, alpha = 0;
-    NLA_HIP(norm_read(gm, &beta));
-    const double r0norm = beta;
-    if (beta == 0) {
-        NLA_HIP(hipStreamSynchronize(s));
-        return 0;
-    }
-    scale_kernel<T><<<gm, 256, 0, s>>>(m, T(-1.0 / beta), u);  // u1 = (b - A x0) / beta1
-    NLA_LAUNCH_CHECK();
-    gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, u, q);
-    NLA_LAUNCH_CHECK();
-    NLA_HIP(rsolve(true, q));
-    NLA_HIP(hipMemsetAsync(v, 0, sizeof(T) * n, s));
-    lincomb_kernel<T><<<gn, 256, 0, s>>>(n, q, T(0), v, bsum);
-    NLA_LAUNCH_CHECK();
-    NLA_HIP(norm_read(gn, &alpha));
-    if (alpha == 0) {
-        NLA_HIP(hipStreamSynchronize(s));
-        return 0;
-    }
-    scale_kernel<T><<<gn, 256, 0, s>>>(n, T(1.0 / alpha), v);
-    NLA_LAUNCH_CHECK();
-    NLA_HIP(hipMemcpyAsync(w, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-    NLA_HIP(hipMemsetAsync(y, 0, sizeof(T) * n, s));
-    double phibar = beta, rhobar = alpha, anorm2 = 0, test2 = 1;
-    for (i64 it = 0; it < maxit; ++it) {
-        // beta u = M v - alpha u = A (R^-1 v) - alpha u
-        NLA_HIP(hipMemcpyAsync(t, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-        NLA_HIP(rsolve(false, t));
-        gemv_n_part_kernel<T><<<gpart, 256, 0, s>>>(m, n, A, lda, t, part, nsplit);
+    i64 done = 0;  // LSQR iterations taken over all restarts
+    // LSQR from the current x for at most `budget` iterations; x += R^-1 y at the end.
+    // watch: give up early (*stalled = true) when the projected iterations exceed the budget.
+    auto lsqr = [&](i64 budget, bool watch, bool *stalled) -> int {
+        *stalled = false;
+        // M = A R^-1.  beta1 u1 = r0 = b - A x;  alpha1 v1 = M^T u1 = R^-T (A^T u1)
+        NLA_HIP(hipMemcpyAsync(u, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
+        gemv_n_part_kernel<T><<<gpart, 256, 0, s>>>(m, n, A, lda, x, part, nsplit);
         NLA_LAUNCH_CHECK();
-        gemv_n_finish_kernel<T><<<gm, 256, 0, s>>>(m, nsplit, part, T(alpha), u, bsum);
+        gemv_n_finish_kernel<T><<<gm, 256, 0, s>>>(m, nsplit, part, T(1), u, bsum);  // u = A x - b
         NLA_LAUNCH_CHECK();
+        double beta = 0, alpha = 0;
         NLA_HIP(norm_read(gm, &beta));
-        anorm2 += alpha * alpha + beta * beta;
-        if (beta > 0) {
-            scale_kernel<T><<<gm, 256, 0, s>>>(m, T(1.0 / beta), u);
-            NLA_LAUNCH_CHECK();
-            // alpha v = M^T u - beta v = R^-T (A^T u) - beta v
-            gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, u, q);
-            NLA_LAUNCH_CHECK();
-            NLA_HIP(rsolve(true, q));
-            lincomb_kernel<T><<<gn, 256, 0, s>>>(n, q, T(beta), v, bsum);
-            NLA_LAUNCH_CHECK();
-            NLA_HIP(norm_read(gn, &alpha));
-            if (alpha > 0) {
-                scale_kernel<T><<<gn, 256, 0, s>>>(n, T(1.0 / alpha), v);
-                NLA_LAUNCH_CHECK();
-            }
-        } else {
-            alpha = 0;
-        }
-        // plane rotation eliminating beta from the lower bidiagonal
-        const double rho = std::hypot(rhobar, beta), c = rhobar / rho, sn = beta / rho;
-        const double theta = sn * alpha;
-        rhobar = -c * alpha;
-        const double phi = c * phibar;
-        phibar = sn * phibar;
-        lsqr_update_kernel<T><<<gn, 256, 0, s>>>(n, T(phi / rho), T(theta / rho), v, w, y);
+        const double r0norm = beta;
+        if (beta == 0) return 0;
+        scale_kernel<T><<<gm, 256, 0, s>>>(m, T(-1.0 / beta), u);  // u1 = (b - A x) / beta1
         NLA_LAUNCH_CHECK();
-        // ||M^T r|| / (||M||_F ||r||) = alpha |c| / ||B_k||_F;  ||r|| = phibar
-        test2 = anorm2 > 0 ? alpha * std::fabs(c) / std::sqrt(anorm2) : 0;
-        if (iters) *iters = it + 1;
-        if (last) *last = test2;
-        if (test2 <= tol || phibar <= tol * r0norm || alpha == 0 || beta == 0) break;
+        gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, u, q);
+        NLA_LAUNCH_CHECK();
+        NLA_ST(rsolve(true, q));
+        NLA_HIP(hipMemsetAsync(v, 0, sizeof(T) * n, s));
+        lincomb_kernel<T><<<gn, 256, 0, s>>>(n, q, T(0), v, bsum);
+        NLA_LAUNCH_CHECK();
+        NLA_HIP(norm_read(gn, &alpha));
+        if (alpha == 0) return 0;
+        scale_kernel<T><<<gn, 256, 0, s>>>(n, T(1.0 / alpha), v);
+        NLA_LAUNCH_CHECK();
+        NLA_HIP(hipMemcpyAsync(w, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+        NLA_HIP(hipMemsetAsync(y, 0, sizeof(T) * n, s));
+        double phibar = beta, rhobar = alpha, anorm2 = 0, test2 = 1;
+        std::vector<double> hist;
+        for (i64 it = 0; it < budget; ++it) {
+            // beta u = M v - alpha u = A (R^-1 v) - alpha u
+            NLA_HIP(hipMemcpyAsync(t, v, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+            NLA_ST(rsolve(false, t));
+            gemv_n_part_kernel<T><<<gpart, 256, 0, s>>>(m, n, A, lda, t, part, nsplit);
+            NLA_LAUNCH_CHECK();
+            gemv_n_finish_kernel<T><<<gm, 256, 0, s>>>(m, nsplit, part, T(alpha), u, bsum);
+            NLA_LAUNCH_CHECK();
+            NLA_HIP(norm_read(gm, &beta));
+            anorm2 += alpha * alpha + beta * beta;
+            if (beta > 0) {
+                scale_kernel<T><<<gm, 256, 0, s>>>(m, T(1.0 / beta), u);
+                NLA_LAUNCH_CHECK();
+                // alpha v = M^T u - beta v = R^-T (A^T u) - beta v
+                gemv_t_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(m, n, A, lda, u, q);
+                NLA_LAUNCH_CHECK();
+                NLA_ST(rsolve(true, q));
+                lincomb_kernel<T><<<gn, 256, 0, s>>>(n, q, T(beta), v, bsum);
+                NLA_LAUNCH_CHECK();
+                NLA_HIP(norm_read(gn, &alpha));
+                if (alpha > 0) {
+                    scale_kernel<T><<<gn, 256, 0, s>>>(n, T(1.0 / alpha), v);
+                    NLA_LAUNCH_CHECK();
+                }
+            } else {
+                alpha = 0;
+            }
+            // plane rotation eliminating beta from the lower bidiagonal
+            const double rho = std::hypot(rhobar, beta), c = rhobar / rho, sn = beta / rho;
+            const double theta = sn * alpha;
+            rhobar = -c * alpha;
+            const double phi = c * phibar;
+            phibar = sn * phibar;
+            lsqr_update_kernel<T><<<gn, 256, 0, s>>>(n, T(phi / rho), T(theta / rho), v, w, y);
+            NLA_LAUNCH_CHECK();
+            // ||M^T r|| / (||M||_F ||r||) = alpha |c| / ||B_k||_F;  ||r|| = phibar
+            test2 = anorm2 > 0 ? alpha * std::fabs(c) / std::sqrt(anorm2) : 0;
+            ++done;
+            if (iters) *iters = done;
+            if (last) *last = test2;
+            if (test2 <= tol || phibar <= tol * r0norm || alpha == 0 || beta == 0) break;
+            hist.push_back(test2);
+            const i64 k = (i64)hist.size();
+            if (watch && k >= STALL_START && (k - STALL_START) % STALL_SPAN == 0) {
+                const double rate = std::pow(test2 / hist[k - 1 - STALL_SPAN], 1.0 / STALL_SPAN);
+                const double need = rate < 1 ? std::log(tol / test2) / std::log(rate) : 1e30;
+                if (need > double(budget - it - 1)) {
+                    *stalled = true;
+                    break;
+                }
+            }
+        }
+        // x = x + R^-1 y
+        NLA_ST(rsolve(false, y));
+        axpy1_kernel<T><<<gn, 256, 0, s>>>(n, y, x);
+        NLA_LAUNCH_CHECK();
+        return 0;
+    };
+    bool stalled = false;
+    NLA_ST(lsqr(maxit, prec == 'H' || prec == 'B', &stalled));
+    if (stalled && done < maxit) {
+        // the 16-bit factor does not precondition A well enough: Float32 updates instead,
+        // from that factor's own direct solution (a restart from the stalled iterate leaves
+        // LSQR's attainable accuracy at the size of the correction it still has to make)
+        NLA_ST(factor('S'));
+        if (prec_used) *prec_used = 'S';
+        NLA_ST(direct_start());
+        NLA_ST(lsqr(maxit - done, false, &stalled));
     }
-    // x = x0 + R^-1 y
-    NLA_HIP(rsolve(false, y));
-    axpy1_kernel<T><<<gn, 256, 0, s>>>(n, y, x);
-    NLA_LAUNCH_CHECK();
     int herr = 0;
     NLA_HIP(hipMemcpyAsync(&herr, sync + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     NLA_HIP(hipStreamSynchronize(s));
-    if (herr) {
-        set_error("tile_gels: a triangular-solve dependency wait exceeded its time limit (results are invalid)");
-        return 1;
-    }
+    if (herr) return wait_timeout_error("tile_gels triangular solve");
     return 0;
 }
 
@@ -603,13 +644,15 @@ NLA_API int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, 
 }
 NLA_API int nla_dtile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const double *A, int64_t lda, const double *b,
                            double *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
-                           void *stream) {
-    return nla::tile_gels_entry<double>(m, n, nb, ib, A, lda, b, x, prec, maxit, tol, iters, last_correction, stream);
+                           char *prec_used, void *stream) {
+    return nla::tile_gels_entry<double>(m, n, nb, ib, A, lda, b, x, prec, maxit, tol, iters, last_correction, prec_used,
+                                        stream);
 }
 NLA_API int nla_stile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const float *A, int64_t lda, const float *b,
                            float *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
-                           void *stream) {
-    return nla::tile_gels_entry<float>(m, n, nb, ib, A, lda, b, x, prec, maxit, tol, iters, last_correction, stream);
+                           char *prec_used, void *stream) {
+    return nla::tile_gels_entry<float>(m, n, nb, ib, A, lda, b, x, prec, maxit, tol, iters, last_correction, prec_used,
+                                       stream);
 }
 
 NLA_API int nla_dtile_geqrf(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,

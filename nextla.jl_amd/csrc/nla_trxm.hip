@@ -803,15 +803,14 @@ static int base_entry(char func, char side, char uplo, i64 n, i64 m, const T *A,
 
 // Exposed for the mixed-precision and differentiation code.
 template <typename T>
-hipError_t rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
+int rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
                          i64 ldb, i64 cutoff, hipStream_t s) {
-    int st = rectrxm_entry<T>(side, uplo, trans, alpha, func, n, m, A, lda, B, ldb, cutoff, (void *)s);
-    return st == 0 ? hipSuccess : hipErrorInvalidValue;
+    return rectrxm_entry<T>(side, uplo, trans, alpha, func, n, m, A, lda, B, ldb, cutoff, (void *)s);
 }
-template hipError_t rectrxm_async<double>(char, char, char, double, char, i64, i64, const double *, i64, double *, i64,
-                                          i64, hipStream_t);
-template hipError_t rectrxm_async<float>(char, char, char, float, char, i64, i64, const float *, i64, float *, i64, i64,
-                                         hipStream_t);
+template int rectrxm_async<double>(char, char, char, double, char, i64, i64, const double *, i64, double *, i64, i64,
+                                   hipStream_t);
+template int rectrxm_async<float>(char, char, char, float, char, i64, i64, const float *, i64, float *, i64, i64,
+                                  hipStream_t);
 
 }  // namespace nla
 

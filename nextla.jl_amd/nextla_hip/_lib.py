@@ -37,7 +37,7 @@ _SIGS = {
     "trsm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
     "trmm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
     "qr_rrule": [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp],
-    "tile_gels": [i64, i64, i64, i64, vp, i64, vp, vp, ch, i64, dbl, vp, vp, vp],
+    "tile_gels": [i64, i64, i64, i64, vp, i64, vp, vp, ch, i64, dbl, vp, vp, vp, vp],
     "batched_qr_trsm": [i64, vp, vp, ctypes.c_int, vp],
     "batched_autotune": [i64, vp, vp, i64, vp, i64, vp, vp, vp],
 }

@@ -96,14 +96,14 @@ def _ptr(t):
 
 # ------------------------------------------------------------------------ Householder / panel
 def larfg_(n, alpha, x, incx, tau):
-    """larfg!(n, alpha, x, incx, tau) (src/larfg.jl:123-186); alpha and tau are 1-element
+    """larfg!(n, alpha, x, incx, tau) (src/larfg.jl:43-106); alpha and tau are 1-element
     device tensors, overwritten with beta and tau."""
     p = _p(x, alpha, tau)
     call(p, "larfg", n, alpha.data_ptr(), x.data_ptr(), incx, tau.data_ptr(), _stream())
 
 
 def larfg(alpha, x, incx=1):
-    """larfg!(alpha, x, incx, tau) convenience form (src/larfg.jl:140-146): returns (beta, tau)."""
+    """larfg!(alpha, x, incx, tau) convenience form (src/larfg.jl:140-145): returns (beta, tau)."""
     a = torch.tensor([alpha], dtype=x.dtype, device=x.device)
     t = torch.zeros(1, dtype=x.dtype, device=x.device)
     larfg_(x.numel() // max(incx, 1) + 1 if incx > 0 else 1, a, x, incx, t)
@@ -351,8 +351,19 @@ def trmm(side, uplo, transa, diag, A, B, alpha=1.0):
     return B
 
 
+def _gemm_shapes(X, Y, C, name):
+    """C (N x M) += X (N x R) * Y (R x M): the operands' inner and outer sizes must agree."""
+    if X.dim() != 2 or Y.dim() != 2 or C.dim() != 2:
+        raise ArgumentError(f"{name}: matrices must be 2-dimensional")
+    if X.shape[1] != Y.shape[0]:
+        raise ArgumentError(f"{name}: inner dimensions differ ({X.shape[1]} and {Y.shape[0]})")
+    if C.shape[0] != X.shape[0] or C.shape[1] != Y.shape[1]:
+        raise ArgumentError(f"{name}: result is {tuple(C.shape)}, product is ({X.shape[0]}, {Y.shape[1]})")
+
+
 def GEMM_ADD_(A, B, C):
     """GEMM_ADD!(A, B, C): C += A*B (src/matmul.jl:69-74)."""
+    _gemm_shapes(A, B, C, "GEMM_ADD!")
     p = _p(A, B, C)
     pa, lda = _m(A)
     pb, ldb = _m(B)
@@ -363,6 +374,7 @@ def GEMM_ADD_(A, B, C):
 
 def GEMM_SUB_(A, B, C):
     """GEMM_SUB!(A, B, C): A -= B*C (src/matmul.jl:76-81)."""
+    _gemm_shapes(B, C, A, "GEMM_SUB!")
     p = _p(A, B, C)
     pa, lda = _m(A)
     pb, ldb = _m(B)
@@ -428,23 +440,33 @@ def tile_geqrf_mixed(A, nb, ib, prec="S"):
     return T
 
 
-def tile_gels(A, b, nb=128, ib=32, prec="D", maxit=100, tol=1e-15):
+def tile_gels(A, b, nb=128, ib=32, prec="D", maxit=100, tol=1e-15, info=None):
     """Least-squares solution of min ||A x - b||: tile QR (prec 'S' / 'H' / 'B', Float64
     only: Float32 / FP16 / BF16 trailing updates); the direct solution x0 = R^-1 (Q^T b)[0:n]
     (returned for maxit = 0; prec 'D' with maxit > 0 starts from R^-1 R^-T A^T b instead),
-    corrected by R-preconditioned LSQR with products by A in the working precision.  Returns (x, iterations, final estimate of ||M^T r|| / (||M|| ||r||)
-    for M = A R^-1)."""
+    corrected by R-preconditioned LSQR with products by A in the working precision.  A 16-bit
+    factor whose refinement stalls is replaced by a Float32-update factor (the library does
+    this by itself).  Returns (x, iterations, final estimate of ||M^T r|| / (||M|| ||r||) for
+    M = A R^-1); when `info` is a dict it receives "prec_used", the update precision of the
+    factor that produced x."""
     import ctypes
 
-    p = _p(A, b)
     m, n = A.shape
+    if b.numel() < m:
+        raise ArgumentError(f"b must have at least m = {m} entries, got {b.numel()}")
+    if m < n:
+        raise ArgumentError(f"least squares needs m >= n, got m = {m}, n = {n}")
+    p = _p(A, b)
     x = torch.zeros(n, dtype=A.dtype, device=A.device)
     pa, lda = _m(A)
     it = ctypes.c_int64(0)
     last = ctypes.c_double(0)
+    used = ctypes.c_char(b" ")
     call(p, "tile_gels", m, n, nb, ib, pa, lda, _ptr(b.contiguous()), _ptr(x), _c(prec), int(maxit), float(tol),
          ctypes.cast(ctypes.pointer(it), ctypes.c_void_p), ctypes.cast(ctypes.pointer(last), ctypes.c_void_p),
-         _stream())
+         ctypes.cast(ctypes.pointer(used), ctypes.c_void_p), _stream())
+    if info is not None:
+        info["prec_used"] = used.value.decode()
     return x, it.value, last.value
 
 

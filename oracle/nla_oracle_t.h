@@ -39,12 +39,12 @@ static R P(nrm2)(int64_t n, const R *x, int64_t incx) {
     return scale * (R)sqrt((double)ssq);
 }
 
-/* larfg!: src/larfg.jl:123-186, including the safmin rescale loop.
+/* larfg!: src/larfg.jl:43-106, including the safmin rescale loop.
  * On return *alpha holds beta and x holds v[2:n].
  * Deviation (LAPACK dlarfg semantics): beta uses hypot instead of
  * sqrt(alpha^2 + xnorm^2), and the rescale loop stops after 20 passes.  With the
  * reference's form, |alpha|, ||x|| ~ 1e-300 underflow beta to 0 and the loop at
- * src/larfg.jl:150 never ends. */
+ * src/larfg.jl:70-77 never ends. */
 void P(larfg)(int64_t n, R *alpha, R *x, int64_t incx, R *tau) {
     if (n <= 1) { *tau = 0; return; }
     R xnorm = P(nrm2)(n - 1, x, incx);

@@ -84,7 +84,7 @@ def lamch(dtype, cmach):
 
 
 def larfg(alpha, x):
-    """larfg! (src/larfg.jl:123-186): returns (beta, tau); x is overwritten with v[2:n]."""
+    """larfg! (src/larfg.jl:43-106): returns (beta, tau); x is overwritten with v[2:n]."""
     p = _p(x)
     a = np.array([alpha], dtype=x.dtype)
     t = np.zeros(1, dtype=x.dtype)

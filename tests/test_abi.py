@@ -87,3 +87,33 @@ def test_python_mirror_raises_argument_error():
     A = torch.zeros(4, 4, dtype=torch.float64)  # host tensor: rejected before any call
     with pytest.raises(nl.ArgumentError):
         nl.tsmqr("L", "N", A, A, A, A)
+
+
+def test_batched_autotune_rejects_bad_candidates():
+    # a candidate outside 1..64 would never advance the block loop (ib <= 0) or overrun the
+    # 64-row T blocks (ib > 64): rejected on the host before any allocation or launch
+    I64 = ctypes.c_int64
+    ms, ns = (I64 * 1)(512), (I64 * 1)(64)
+    out = (I64 * 1)()
+    for bad in (0, -4, 65):
+        cand = (I64 * 2)(16, bad)
+        for f in ("nla_dbatched_autotune", "nla_sbatched_autotune"):
+            r = st(f, 1, ms, ns, 4, cand, 2, out, NULL, NULL)
+            assert r == (-5, f"candidate ib must be in 1..64, got {bad}"), r
+    cand = (I64 * 1)(16)
+    r = st("nla_sbatched_autotune", 1, ms, (I64 * 1)(600), 4, cand, 1, out, NULL, NULL)
+    assert r == (-2, "class 0: need 0 < n <= m")
+    assert st("nla_sbatched_autotune", 1, ms, ns, 0, cand, 1, out, NULL, NULL)[0] == -4
+
+
+def test_python_mirror_shape_checks():
+    torch = pytest.importorskip("torch")
+    z = lambda m, n: torch.zeros(m, n, dtype=torch.float64)
+    with pytest.raises(nl.ArgumentError, match="inner dimensions"):
+        nl.GEMM_ADD_(z(4, 3), z(2, 5), z(4, 5))
+    with pytest.raises(nl.ArgumentError, match="result is"):
+        nl.GEMM_ADD_(z(4, 3), z(3, 5), z(4, 6))
+    with pytest.raises(nl.ArgumentError, match="inner dimensions"):
+        nl.GEMM_SUB_(z(4, 5), z(4, 3), z(2, 5))
+    with pytest.raises(nl.ArgumentError, match="at least m"):
+        nl.tile_gels(z(256, 128), torch.zeros(100, dtype=torch.float64))

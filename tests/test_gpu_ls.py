@@ -127,7 +127,8 @@ def test_tile_geqrf_mixed_precisions_order():
     assert errs["S"] < errs["H"] / 100 < errs["B"] / 100, errs
 
 
-@pytest.mark.parametrize("prec,kappa", [("D", 1e2), ("D", 1e6), ("S", 1e2), ("S", 1e6), ("H", 1e2), ("B", 1e2)])
+@pytest.mark.parametrize("prec,kappa", [("D", 1e2), ("D", 1e6), ("S", 1e2), ("S", 1e6), ("H", 1e2), ("B", 1e2),
+                                        ("H", 1e6), ("B", 1e6)])
 def test_gels_refinement_vs_lapack(prec, kappa):
     nl = _nl()
     rng = np.random.default_rng(int(kappa) % 1000 + "DSHB".index(prec))
