@@ -7,17 +7,19 @@ One step = restore B from its pristine copy (device to device) + the solve; inpu
 resident in HBM before timing starts.  TFLOP/s use the LAPACK count n^2 m.
 
 Extra fields: Float32 TRSM and Float64 TRMM of the same shape, the solve's backward error,
-the mixed-precision solve (Float32 + Float64 refinement), Float64 and Float32-update tile QR
-(flat-tree geqrt/unmqr/tsqrt/tsmqr) at 16384^2 with backward errors, the batched uneven
-workload (4096 problems, one graph-replayed launch), adjoint timings, optionally (--ls) the
-32768 x 8192 least-squares workload; the roofline of the dominant kernel (the MFMA GEMM of
-the off-diagonal updates, timed live with hipEvents on its launch stream in a second pass of
-the same steps, so that the event packets do not slow the pass that gives `value`; HBM
-traffic from the committed counter profile) and the CPU oracle timed on a bounded sample of
-the solve.
+the mixed-precision solve (Float32 + Float64 refinement), Float64 and reduced-precision-update
+tile QR (flat-tree geqrt/unmqr/tsqrt/tsmqr) at 16384^2 with backward errors, the same
+factorizations compared at equal backward error as refined solves of a 16384^2 system, the
+batched uneven workload (4096 problems, one graph-replayed launch), adjoint timings, the
+32768 x 8192 least-squares workload; the roofline of the dominant kernel (the 128 x 128-tile
+MFMA GEMM of the level-1/2 off-diagonal updates, timed live with hipEvents on its launch stream
+in a second pass of the same steps, so that the event packets do not slow the pass that gives
+`value`; HBM traffic from the committed counter profile) and CPU baselines on the host cores
+(the oracle's TRSM on a bounded sample, vendor LAPACK legs, the 1024^2 tile QR), each the
+median of 5 runs after a warm-up.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--qr-n N] [--ls]
-                    [--extras trsm32,trmm,trsm_mixed,qr,batched,adjoints,ls] [--no-extras]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--qr-n N]
+                    [--extras trsm32,trmm,trsm_mixed,qr,qr_solve,batched,adjoints,ls] [--no-extras]
 """
 import argparse
 import json
@@ -51,6 +53,18 @@ def gemm_flops_of_recursion(n, m, cut, off=0):
             + gemm_flops_of_recursion(rem, m, cut, off + mid))
 
 
+def gemm128_flops_of_recursion(n, m, cut, off=0):
+    """Flops of those GEMM launches of the same recursion that run on 128 x 128 tiles (the
+    library picks them when the update's C has >= 512 such tiles: C is rem x m here)."""
+    if n <= cut or (256 <= cut < 512 and n == 512 and off % 256 == 0):
+        return 0
+    mid = n // 2 if (n & (n - 1)) == 0 else 1 << (n.bit_length() - 1)
+    rem = n - mid
+    big = ((rem + 127) // 128) * ((m + 127) // 128) >= 512
+    return ((2 * mid * rem * m if big else 0) + gemm128_flops_of_recursion(mid, m, cut, off)
+            + gemm128_flops_of_recursion(rem, m, cut, off + mid))
+
+
 def make_tri(torch, nl, n, dtype, gen):
     X = torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64) / (n ** 0.5)
     L = torch.tril(X, -1)
@@ -69,31 +83,92 @@ def time_steps(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(n, cut, target_s=12.0):
-    """The oracle's recursive TRSM (a C/OpenMP restatement of unified_rectrxm!) on the host
-    cores, on n x n with a reduced number of right-hand sides chosen for ~target_s seconds."""
+def host_info():
+    """CPU model, logical CPUs available to this process, physical cores (lscpu)."""
+    import subprocess
+
+    model, phys = None, None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {l.split(":", 1)[0].strip(): l.split(":", 1)[1].strip() for l in out.splitlines() if ":" in l}
+        model = kv.get("Model name")
+        phys = int(kv.get("Core(s) per socket", "0")) * int(kv.get("Socket(s)", "0")) or None
+    except Exception:
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except Exception:
+            model = "unknown"
+    return {"model": model, "logical_cpus_available": len(os.sched_getaffinity(0)), "physical_cores": phys}
+
+
+def median_of(fn, runs=5):
+    """1 warm-up, then the median wall time of `runs` runs (the CPU-baseline protocol)."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_baseline(n, cut, target_s=2.5):
+    """CPU baselines on the host cores, each 1 warm-up + median of 5 runs:
+    (value) the oracle's recursive TRSM (a C/OpenMP restatement of unified_rectrxm!) on n x n
+    with a bounded number of the 4096 right-hand sides, chosen for ~target_s per run;
+    (legs) vendor LAPACK/BLAS through scipy (OpenBLAS) dtrsm on the same sample, the tile QR
+    workload 1024^2 (nb 128, ib 32) through the oracle, and LAPACK dgeqrf of the same matrix."""
     import numpy as np
+    from scipy.linalg import blas, lapack
 
     import oracle as O
 
+    try:
+        from threadpoolctl import threadpool_info
+        blas_threads = max((d.get("num_threads", 0) for d in threadpool_info() if d.get("user_api") == "blas"),
+                           default=None)
+    except Exception:
+        blas_threads = None
     rng = np.random.default_rng(7)
     A = np.tril(rng.standard_normal((n, n)) / np.sqrt(n), -1)
     A[np.diag_indices(n)] = rng.uniform(1, 2, n)
     A = np.asfortranarray(A)
-    m = 32
-    B = np.asfortranarray(rng.standard_normal((n, m)))
+    B = np.asfortranarray(rng.standard_normal((n, 32)))
     t0 = time.perf_counter()
-    O.rectrxm("L", "L", "N", 1.0, "S", A, B, threshold=cut)
-    dt = time.perf_counter() - t0
-    rate = n * n * m / dt
+    O.rectrxm("L", "L", "N", 1.0, "S", A, B.copy(order="F"), threshold=cut)
+    rate = n * n * 32 / (time.perf_counter() - t0)
     m2 = int(max(32, min(4096, rate * target_s / (n * n))))
     B = np.asfortranarray(rng.standard_normal((n, m2)))
-    t0 = time.perf_counter()
-    O.rectrxm("L", "L", "N", 1.0, "S", A, B, threshold=cut)
-    dt = time.perf_counter() - t0
-    return {"value": round(n * n * m2 / dt / 1e12, 6), "unit": "TFLOP/s", "cores": O.num_threads(), "kind": "port",
+    dt = median_of(lambda: O.rectrxm("L", "L", "N", 1.0, "S", A, B.copy(order="F"), threshold=cut))
+    dt_l = median_of(lambda: blas.dtrsm(1.0, A, B, side=0, lower=1, trans_a=0, diag=0))
+    # tile QR 1024^2, nb 128, ib 32 (oracle) and LAPACK dgeqrf
+    Q = np.asfortranarray(rng.standard_normal((1024, 1024)))
+    qflops = 2.0 * 1024 ** 3 - 2.0 / 3.0 * 1024 ** 3
+
+    def tqr():
+        Qw = Q.copy(order="F")
+        O.tile_geqrf(Qw, 128, 32)
+    dt_q = median_of(tqr)
+    dt_qf = median_of(lambda: lapack.dgeqrf(Q.copy(order="F")))
+    info = host_info()
+    threads = O.num_threads()
+    return {"value": round(n * n * m2 / dt / 1e12, 6), "unit": "TFLOP/s", "cores": threads, "kind": "port",
             "sample": f"oracle unified_rectrxm TRSM L/L/N Float64 n={n} with {m2} of the 4096 right-hand sides, "
-                      f"cut-off {cut}, {dt:.1f} s"}
+                      f"cut-off {cut}; median of 5 runs after 1 warm-up, {dt:.2f} s each; {threads} OpenMP threads",
+            "host": info,
+            "legs": [
+                {"what": f"LAPACK/BLAS dtrsm (scipy, OpenBLAS) on the same {n} x {m2} sample", "kind": "vendor",
+                 "value": round(n * n * m2 / dt_l / 1e12, 6), "unit": "TFLOP/s", "s": round(dt_l, 3),
+                 "threads": blas_threads},
+                {"what": "tile QR 1024^2, nb 128, ib 32 (geqrt/unmqr/tsqrt/tsmqr), oracle", "kind": "port",
+                 "value": round(qflops / dt_q / 1e12, 6), "unit": "TFLOP/s", "s": round(dt_q, 4), "threads": threads},
+                {"what": "LAPACK dgeqrf 1024^2 (scipy, OpenBLAS)", "kind": "vendor",
+                 "value": round(qflops / dt_qf / 1e12, 6), "unit": "TFLOP/s", "s": round(dt_qf, 4),
+                 "threads": blas_threads},
+            ]}
 
 
 def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
@@ -114,14 +189,16 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
         for prec in ("D", "S", "H", "B"):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            x, it, last = nl.tile_gels(A, b, nb, ib, prec, maxit=200, tol=1e-15)
+            info = {}
+            x, it, last = nl.tile_gels(A, b, nb, ib, prec, maxit=200, tol=1e-15, info=info)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             r = b - A @ x
             # ||A^T r|| / (||A||_2 ||r||) with ||A||_2 = s_1 = 1 by construction
             eta = (torch.linalg.norm(A.T @ r) / torch.linalg.norm(r)).item()
-            row[prec] = {"s": round(dt, 3), "iterations": it, "eta": eta, "x": x}
-            log(f"[bench] LS kappa={kappa:g} prec {prec}: {dt:.3f} s, {it} steps, eta {eta}")
+            row[prec] = {"s": round(dt, 3), "iterations": it, "eta": eta, "factor_used": info["prec_used"], "x": x}
+            log(f"[bench] LS kappa={kappa:g} prec {prec}: {dt:.3f} s, {it} steps, eta {eta}, "
+                f"factor {info['prec_used']}")
         xd = row["D"]["x"]
         for p in ("S", "H", "B"):
             row[p]["rel_diff_vs_f64_solution"] = (torch.linalg.norm(row[p]["x"] - xd) / torch.linalg.norm(xd)).item()
@@ -130,6 +207,43 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
             v.pop("x")
         out[f"kappa_{kappa:g}"] = {"f64": row["D"], "mixed_f32": row["S"], "mixed_f16": row["H"], "mixed_bf16": row["B"]}
         del A, b
+    return out
+
+
+def qr_solve_equal_error(torch, nl, gen, n=16384, kappa=1e2, nb=128, ib=32):
+    """Tile QR at 16384^2 compared at equal backward error: solve A x = b (A = U diag(s) V^T, s
+    geometric from 1 to 1/kappa, b = A x0) with the Float64 factorization and with the Float32 /
+    FP16 / BF16-update factorizations, each refined by R-preconditioned LSQR (products by A in
+    Float64) until LSQR's estimate reaches 1e-15; the time covers factorization + refinement
+    and every row states the normwise backward error ||b - A x|| / (||A||_2 ||x|| + ||b||)."""
+    U = torch.linalg.qr(torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64))[0]
+    V = torch.linalg.qr(torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64))[0]
+    sv = kappa ** (-torch.arange(n, device="cuda", dtype=torch.float64) / (n - 1))
+    A = nl.colmajor((U * sv) @ V.T)
+    del U, V
+    x0 = torch.randn(n, generator=gen, device="cuda", dtype=torch.float64)
+    b = A @ x0
+    out = {"n": n, "kappa": kappa, "nb": nb, "ib": ib,
+           "time": "factorization + direct start + LSQR refinement, one call, wall clock"}
+    for prec, tag in (("D", "f64"), ("S", "mixed_f32"), ("H", "mixed_f16"), ("B", "mixed_bf16")):
+        info = {}
+        nl.tile_gels(A, b, nb, ib, prec, maxit=100, tol=1e-15, info=info)   # warm-up
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, it, last = nl.tile_gels(A, b, nb, ib, prec, maxit=100, tol=1e-15, info=info)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[1]
+        be = (torch.linalg.norm(b - A @ x) / (torch.linalg.norm(x) + torch.linalg.norm(b))).item()  # ||A||_2 = 1
+        out[tag] = {"ms": round(t * 1e3, 2), "iterations": it, "factor_used": info["prec_used"],
+                    "backward_error": be, "forward_error": (torch.linalg.norm(x - x0) / torch.linalg.norm(x0)).item()}
+        log(f"[bench] QR solve {n}^2 kappa {kappa:g} prec {prec}: {t*1e3:.1f} ms, {it} its, berr {be:.2e}, "
+            f"factor {info['prec_used']}")
+    for tag in ("mixed_f32", "mixed_f16", "mixed_bf16"):
+        out[tag]["vs_f64"] = round(out["f64"]["ms"] / out[tag]["ms"], 3)
+    del A, b
     return out
 
 
@@ -242,8 +356,9 @@ def main():
     ap.add_argument("--cutoff", type=int, default=256)
     ap.add_argument("--qr-n", type=int, default=16384)
     ap.add_argument("--ls", action="store_true", help="also run the 32768 x 8192 least-squares workload")
-    ap.add_argument("--extras", default="trsm32,trmm,trsm_mixed,qr,batched,adjoints",
-                    help="comma-separated extra workloads: trsm32, trmm, trsm_mixed, qr, batched, adjoints, ls")
+    ap.add_argument("--extras", default="trsm32,trmm,trsm_mixed,qr,qr_solve,batched,adjoints,ls",
+                    help="comma-separated extra workloads: trsm32, trmm, trsm_mixed, qr, qr_solve, batched, "
+                         "adjoints, ls")
     ap.add_argument("--qr-no-check", action="store_true", help="skip the tile-QR backward-error check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -293,14 +408,19 @@ def main():
     sec_prof = (time.perf_counter() - t0) / args.steps
     nl.prof_enable(False)
     gemm_ms, gemm_n = nl.prof_read("gemm")
+    g128_ms, g128_n = nl.prof_read("gemm128")
     base_ms, base_n = nl.prof_read("trsm_base")
     nl.prof_reset()
     value = flops / sec / 1e12
     gemm_flops = gemm_flops_of_recursion(n, m, cut)
-    achieved = gemm_flops * args.steps / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    g128_flops = gemm128_flops_of_recursion(n, m, cut)
+    all_gemm_tf = gemm_flops * args.steps / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    # the roofline's kernel: gemm_kernel<double,128,128,...>, the dominant launches of the solve
+    achieved = g128_flops * args.steps / (g128_ms * 1e-3) / 1e12 if g128_ms > 0 else None
     log(f"[bench] TRSM f64 n={n} m={m} cut={cut}: {sec*1e3:.3f} ms/step, {value:.2f} TFLOP/s "
         f"({sec_prof*1e3:.3f} ms/step with the launch profiler); "
-        f"gemm {gemm_ms/args.steps:.3f} ms/step over {gemm_n//args.steps} launches ({achieved} TFLOP/s); "
+        f"gemm {gemm_ms/args.steps:.3f} ms/step over {gemm_n//args.steps} launches ({all_gemm_tf} TFLOP/s); "
+        f"128x128 gemm {g128_ms/args.steps:.3f} ms/step over {g128_n//args.steps} launches ({achieved} TFLOP/s); "
         f"base {base_ms/args.steps:.3f} ms/step")
 
     # backward error of the timed solve: ||A X - B|| / (||A|| ||X||)
@@ -394,6 +514,8 @@ def main():
         extras["tile_qr"] = {"n": qn, "nb": nb, "ib": ib, "flops": "2n^3 - 2n^3/3 (LAPACK count)", **res}
         del Q0, Qw
 
+    if "qr_solve" in chosen:
+        extras["tile_qr_solve_equal_error"] = qr_solve_equal_error(torch, nl, gen, n=args.qr_n)
     if "batched" in chosen:
         extras["batched"] = batched(torch, nl, gen)
     if "adjoints" in chosen:
@@ -405,7 +527,7 @@ def main():
     tpath = os.path.join(REPO, "profiles", "trsm_f64_gemm_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f).get("hbm_bytes_per_launch_128")
 
     out = {
         "metric": "tile-QR & TRSM TFLOP/s (% of dtype peak), mixed vs FP64 at equal backward err",
@@ -426,10 +548,16 @@ def main():
         "roofline": {"bound": "mfma", "achieved": None if achieved is None else round(achieved, 3),
                      "peak": PEAK["f64"], "unit": "TFLOP/s",
                      "frac": None if achieved is None else round(achieved / PEAK["f64"], 4),
-                     "traffic": traffic, "kernel": "gemm_kernel<double> (off-diagonal GEMM_SUB! updates)",
-                     "launches_per_step": gemm_n // max(1, args.steps),
-                     "avg_launch_ms": round(gemm_ms / max(1, gemm_n), 4),
-                     "measured": "HIP events on the launch stream around every GEMM launch, over a "
+                     "traffic": traffic,
+                     "kernel": "nla::gemm_kernel<double, 128, 128, 16, false, false, false, 1> (the level-1/2 "
+                               "off-diagonal GEMM_SUB! updates of the recursion)",
+                     "launches_per_step": g128_n // max(1, args.steps),
+                     "flop_per_step": g128_flops,
+                     "avg_launch_ms": round(g128_ms / max(1, g128_n), 4),
+                     "all_gemm_launches": {"launches_per_step": gemm_n // max(1, args.steps),
+                                           "tflops": None if all_gemm_tf is None else round(all_gemm_tf, 3),
+                                           "avg_launch_ms": round(gemm_ms / max(1, gemm_n), 4)},
+                     "measured": "HIP events on the launch stream around every 128x128-tile GEMM launch, over a "
                                  "second pass of the same steps (ms_per_step there: "
                                  f"{round(sec_prof * 1e3, 4)})"},
         "extras": extras,

@@ -44,7 +44,8 @@ int64_t nla_scratch_bytes(void);
 
 /* Launch profiler (no reference counterpart; measurement hook for bench.py): while enabled,
  * each kernel family brackets its launches with hipEvents on the launch stream.
- * nla_prof_read sums elapsed time and launch count for one family: "gemm", "trsm_base",
+ * nla_prof_read sums elapsed time and launch count for one family: "gemm" (every GEMM
+ * launch), "gemm128" (the 128 x 128-tile GEMM launches alone), "trsm_base",
  * "trmm_base", "ge_apply", "ts_apply", "panel_qr", "ts_panel", "tile_qr". */
 int nla_prof_enable(int on);
 void nla_prof_reset(void);

@@ -89,7 +89,7 @@ bool g_prof_enabled = false;
 struct ProfRec { int kind; hipEvent_t a, b; };
 std::vector<ProfRec> g_prof;
 std::vector<hipEvent_t> g_prof_free;
-const char *g_prof_names[PK_COUNT] = {"gemm", "trsm_base", "trmm_base", "ge_apply", "ts_apply",
+const char *g_prof_names[PK_COUNT] = {"gemm", "gemm128", "trsm_base", "trmm_base", "ge_apply", "ts_apply",
                                       "panel_qr", "ts_panel", "tile_qr", "other"};
 }  // namespace
 

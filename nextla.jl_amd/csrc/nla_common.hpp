@@ -55,7 +55,7 @@ void *scratch(ScratchSlot slot, size_t bytes, hipStream_t stream, int *status);
 bool prof_on();
 void prof_push(int kind, hipEvent_t a, hipEvent_t b);
 hipEvent_t prof_event();
-enum ProfKind { PK_GEMM = 0, PK_TRSM_BASE, PK_TRMM_BASE, PK_GE_APPLY, PK_TS_APPLY, PK_PANEL, PK_TS_PANEL,
+enum ProfKind { PK_GEMM = 0, PK_GEMM128, PK_TRSM_BASE, PK_TRMM_BASE, PK_GE_APPLY, PK_TS_APPLY, PK_PANEL, PK_TS_PANEL,
                 PK_TILE_QR, PK_OTHER, PK_COUNT };
 struct ProfScope {
     int kind;

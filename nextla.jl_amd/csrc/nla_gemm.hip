@@ -198,10 +198,13 @@ hipError_t gemm(char ta, char tb, i64 M, i64 N, i64 K, T alpha, const T *A, i64 
         if (beta == T(1)) return hipSuccess;
         // C = beta * C through a K=0 product: the kernel reads nothing from A/B.
     }
-    ProfScope prof(PK_GEMM, s);
     const i64 big_tiles = ((M + 127) / 128) * ((N + 127) / 128);
-    if (big_tiles >= 512)
+    // "gemm" counts every launch; "gemm128" the 128 x 128-tile launches alone (one kernel)
+    ProfScope prof(PK_GEMM, s);
+    if (big_tiles >= 512) {
+        ProfScope prof128(PK_GEMM128, s);
         return launch_sized<T, 128, 128, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    }
     return launch_sized<T, 64, 64, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
