@@ -134,6 +134,16 @@ int nla_drectrxm(char side, char uplo, char trans, double alpha, char func, int6
 int nla_srectrxm(char side, char uplo, char trans, float alpha, char func, int64_t n, int64_t m,
                  const float *A, int64_t lda, float *B, int64_t ldb, int64_t cutoff, void *stream);
 
+/* No reference counterpart: nla_drectrxm for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as nla_drectrxm; it calls no hipMalloc/hipFree and allocates nothing:
+ * its workspace is a library-owned 18 MiB device array per GPU (held until the process ends),
+ * enough for n <= 8192 with the default cut-off 256 (n <= 73728 with other cut-offs).  A call
+ * that would need more returns -6 with a message naming the sizes.  All calls on one GPU share
+ * that array: calls that overlap on one GPU (e.g. on two streams) race. */
+int nla_drectrxm_noalloc(char side, char uplo, char trans, double alpha, char func, int64_t n, int64_t m,
+                         const double *A, int64_t lda, double *B, int64_t ldb, int64_t cutoff, void *stream);
+
 /* Base-case kernels LeftLowerTRSM!/LeftUpperTRSM!/RightLowerTRSM!/RightUpperTRSM!
  * (src/trsm.jl:128-150) and the TRMM counterparts (src/trmm.jl:332-389): one diagonal
  * block, B n x m (side 'L') or m x n (side 'R'); n <= 1024 as in src/trsm.jl:10-12. */

@@ -93,7 +93,11 @@ const char *g_prof_names[PK_COUNT] = {"gemm", "gemm128", "trsm_base", "trmm_base
                                       "panel_qr", "ts_panel", "tile_qr", "other"};
 }  // namespace
 
-bool prof_on() { return g_prof_enabled; }
+// Calls that must not allocate (the no-allocation entry point) switch the profiler off on
+// their thread: creating its events could allocate.
+static thread_local int g_prof_suppress = 0;
+void prof_suppress(bool on) { g_prof_suppress += on ? 1 : -1; }
+bool prof_on() { return g_prof_enabled && g_prof_suppress == 0; }
 
 hipEvent_t prof_event() {
     std::lock_guard<std::mutex> lock(g_prof_mu);

@@ -53,6 +53,7 @@ void *scratch(ScratchSlot slot, size_t bytes, hipStream_t stream, int *status);
 // Launch-time profiler: when enabled, every kernel family brackets its launches with a
 // pair of hipEvents on the launch stream; nla_prof_read() sums the elapsed times.
 bool prof_on();
+void prof_suppress(bool on);
 void prof_push(int kind, hipEvent_t a, hipEvent_t b);
 hipEvent_t prof_event();
 enum ProfKind { PK_GEMM = 0, PK_GEMM128, PK_TRSM_BASE, PK_TRMM_BASE, PK_GE_APPLY, PK_TS_APPLY, PK_PANEL, PK_TS_PANEL,

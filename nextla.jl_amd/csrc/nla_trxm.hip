@@ -728,9 +728,20 @@ static hipError_t scale(i64 rows, i64 cols, T alpha, T *B, i64 ldb, hipStream_t 
     return hipGetLastError();
 }
 
+// Device workspace the solve needs beside A and B: the 32 x 32 inverses of every diagonal
+// block of op(A) and, with a cut-off in [IBS, 2 IBS), the IBS x IBS inverses as well.
+template <typename T> static size_t rectrxm_ws_bytes(char func, i64 n, i64 thr) {
+    if (func != 'S' || thr < DB) return 0;
+    const i64 nd = (n + DB - 1) / DB * DB * DB;
+    const i64 ninv = (thr >= IBS && thr < 2 * IBS) ? n / IBS : 0;
+    return sizeof(T) * (size_t)(nd + ninv * IBS * IBS);
+}
+
+// ws = nullptr: the workspace comes from the per-(device, stream) scratch pool; otherwise
+// the caller's buffer of ws_cap bytes is used and a solve that needs more is refused.
 template <typename T>
 static int rectrxm_entry(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
-                         i64 ldb, i64 cutoff, void *stream) {
+                         i64 ldb, i64 cutoff, void *stream, void *ws = nullptr, size_t ws_cap = 0) {
     side = up(side); uplo = up(uplo); trans = up(trans); func = up(func);
     if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);
     if (uplo != 'L' && uplo != 'U') return arg_error(2, "uplo must be 'L' or 'U', got '%c'", uplo);
@@ -746,6 +757,10 @@ static int rectrxm_entry(char side, char uplo, char trans, T alpha, char func, i
     if (n > INT32_MAX || m > INT32_MAX) return arg_error(6, "dimension exceeds 2^31-1");
     i64 thr = cutoff == 0 ? (func == 'S' ? 256 : 16) : cutoff;
     if (thr > 1024) return arg_error(12, "cutoff must be <= 1024 (base kernels hold n <= 1024 rows, src/trsm.jl:10-12)");
+    const size_t need = rectrxm_ws_bytes<T>(func, n, thr);
+    if (ws && need > ws_cap)
+        return arg_error(6, "n = %lld with cut-off %lld needs %zu bytes of device workspace; this entry point holds "
+                            "%zu", (long long)n, (long long)thr, need, ws_cap);
     bool tr = false;
     if (trans == 'T' || trans == 'C') { tr = true; uplo = (uplo == 'L') ? 'U' : 'L'; }  // src/rectrxm.jl:56-59
     hipStream_t s = as_stream(stream);
@@ -758,7 +773,7 @@ static int rectrxm_entry(char side, char uplo, char trans, T alpha, char func, i
         const i64 nd = (n + DB - 1) / DB * DB * DB;
         const i64 ninv = (thr >= IBS && thr < 2 * IBS) ? n / IBS : 0;
         int st = 0;
-        T *D = (T *)scratch(SLOT_TRSM, sizeof(T) * (size_t)(nd + ninv * IBS * IBS), s, &st);
+        T *D = ws ? (T *)ws : (T *)scratch(SLOT_TRSM, need, s, &st);
         if (!D) return st;
         bool upf, trA, trB;
         left_form(side, uplo, tr, &upf, &trA, &trB);
@@ -837,4 +852,58 @@ NLA_API int nla_dtrmm_base(char side, char uplo, int64_t n, int64_t m, const dou
 NLA_API int nla_strmm_base(char side, char uplo, int64_t n, int64_t m, const float *A, int64_t lda, float *B,
                            int64_t ldb, void *stream) {
     return nla::base_entry<float>('M', side, uplo, n, m, A, lda, B, ldb, stream);
+}
+
+// ------------------------------------------------------------------------------------
+// No-allocation variant of nla_drectrxm for callers that cannot allocate device memory
+// (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Its
+// workspace is this file-scope device array, one per GPU, held until the process ends;
+// every call on a GPU shares that GPU's copy, so calls that overlap on one GPU race.
+// Nothing in it needs clearing: each solve writes every workspace entry it reads.
+constexpr size_t NOALLOC_BYTES = size_t(18) << 20;   // n <= 8192 with the 256-row bases
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEV = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[NOALLOC_MAX_DEV] = {};
+}  // namespace
+
+NLA_API int nla_drectrxm_noalloc(char side, char uplo, char trans, double alpha, char func, int64_t n, int64_t m,
+                                 const double *A, int64_t lda, double *B, int64_t ldb, int64_t cutoff,
+                                 void *stream) {
+    // sizes first, on the host: a solve that needs more than the array holds is refused
+    {
+        const char f = nla::up(func);
+        const int64_t thr = cutoff == 0 ? (f == 'S' ? 256 : 16) : cutoff;
+        if (cutoff >= 0 && thr <= 1024 && n > 0 && m > 0) {
+            const size_t need = nla::rectrxm_ws_bytes<double>(f, n, thr);
+            if (need > NOALLOC_BYTES)
+                return nla::arg_error(6, "n = %lld with cut-off %lld needs %zu bytes of device workspace; the "
+                                         "no-allocation entry point holds %zu", (long long)n, (long long)thr, need,
+                                      NOALLOC_BYTES);
+        }
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return nla::hip_error(e, "hipGetDevice");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEV) return nla::hip_error(hipErrorInvalidDevice, "device index");
+    void *ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!g_noalloc_ptr[dev]) {
+            // this device's copy of the array (the name itself is a host-side stand-in)
+            e = hipGetSymbolAddress(&g_noalloc_ptr[dev], HIP_SYMBOL(scratch));
+            if (e != hipSuccess) {
+                g_noalloc_ptr[dev] = nullptr;
+                return nla::hip_error(e, "hipGetSymbolAddress(scratch)");
+            }
+        }
+        ptr = g_noalloc_ptr[dev];
+    }
+    nla::prof_suppress(true);
+    const int st = nla::rectrxm_entry<double>(side, uplo, trans, alpha, func, n, m, A, lda, B, ldb, cutoff, stream,
+                                              ptr, NOALLOC_BYTES);
+    nla::prof_suppress(false);
+    return st;
 }

@@ -117,3 +117,10 @@ def test_python_mirror_shape_checks():
         nl.GEMM_SUB_(z(4, 5), z(4, 3), z(2, 5))
     with pytest.raises(nl.ArgumentError, match="at least m"):
         nl.tile_gels(z(256, 128), torch.zeros(100, dtype=torch.float64))
+
+
+def test_noalloc_entry_rejects_oversize_on_host():
+    # the size check runs before any HIP call: n = 16384 with the 256-row bases needs 36 MiB
+    r = st("nla_drectrxm_noalloc", b"L", b"L", b"N", ctypes.c_double(1.0), b"S", 16384, 4, NULL, 16384, NULL,
+           16384, 0, NULL)
+    assert r[0] == -6 and "37748736 bytes" in r[1] and "holds 18874368" in r[1], r
