@@ -51,39 +51,66 @@ int nla_prof_enable(int on);
 void nla_prof_reset(void);
 int nla_prof_read(const char *kernel, double *total_ms, int64_t *launches);
 
+/* ---- Element types of the QR-family routines ------------------------------------------
+ * Each routine below has four entry points: 'd' Float64, 's' Float32, 'z' ComplexF64,
+ * 'c' ComplexF32.  Complex elements are two reals, real part first (Julia's Complex{T},
+ * numpy's complex128/complex64).  For complex types 'C' means the conjugate transpose and
+ * 'T' is rejected; for real types 'T' means the same as 'C'. */
+typedef struct nla_zcomplex { double re, im; } nla_zcomplex;
+typedef struct nla_ccomplex { float re, im; } nla_ccomplex;
+
 /* ---- Householder generation ------------------------------------------------------ */
 /* larfg!(n, alpha, x, incx, tau): src/larfg.jl:43-106.  alpha and tau are DEVICE scalars;
  * on return *alpha = beta, x = v[2:n], *tau = tau.  (Julia shim returns (alpha, tau).) */
 int nla_dlarfg(int64_t n, double *alpha, double *x, int64_t incx, double *tau, void *stream);
 int nla_slarfg(int64_t n, float *alpha, float *x, int64_t incx, float *tau, void *stream);
+int nla_zlarfg(int64_t n, nla_zcomplex *alpha, nla_zcomplex *x, int64_t incx, nla_zcomplex *tau, void *stream);
+int nla_clarfg(int64_t n, nla_ccomplex *alpha, nla_ccomplex *x, int64_t incx, nla_ccomplex *tau, void *stream);
 
 /* ---- Panel QR and T factor --------------------------------------------------------- */
 /* geqr2!(m, n, A, tau, work): src/geqr2.jl:107-121.  work (length n) is accepted, unused. */
 int nla_dgeqr2(int64_t m, int64_t n, double *A, int64_t lda, double *tau, double *work, void *stream);
 int nla_sgeqr2(int64_t m, int64_t n, float *A, int64_t lda, float *tau, float *work, void *stream);
+int nla_zgeqr2(int64_t m, int64_t n, nla_zcomplex *A, int64_t lda, nla_zcomplex *tau, nla_zcomplex *work, void *stream);
+int nla_cgeqr2(int64_t m, int64_t n, nla_ccomplex *A, int64_t lda, nla_ccomplex *tau, nla_ccomplex *work, void *stream);
 
-/* larft!(direct, storev, n, k, V, tau, T): src/larft.jl:205-213.  direct='F', storev='C'
- * are implemented (the forms the QR path uses); the others return -1 / -2. */
+/* larft!(direct, storev, n, k, V, tau, T): src/larft.jl:205-213.  direct='F' with storev 'C'
+ * (V n x k) or 'R' (V k x n); direct='B' returns -1. */
 int nla_dlarft(char direct, char storev, int64_t n, int64_t k, const double *V, int64_t ldv,
                const double *tau, double *T, int64_t ldt, void *stream);
 int nla_slarft(char direct, char storev, int64_t n, int64_t k, const float *V, int64_t ldv,
                const float *tau, float *T, int64_t ldt, void *stream);
+int nla_zlarft(char direct, char storev, int64_t n, int64_t k, const nla_zcomplex *V, int64_t ldv,
+               const nla_zcomplex *tau, nla_zcomplex *T, int64_t ldt, void *stream);
+int nla_clarft(char direct, char storev, int64_t n, int64_t k, const nla_ccomplex *V, int64_t ldv,
+               const nla_ccomplex *tau, nla_ccomplex *T, int64_t ldt, void *stream);
 
 /* geqrt!(m, n, ib, A, T, tau, work): src/geqrt.jl:22-68.  T is ib x min(m,n) (ldt >= ib). */
 int nla_dgeqrt(int64_t m, int64_t n, int64_t ib, double *A, int64_t lda, double *T, int64_t ldt,
                double *tau, double *work, void *stream);
 int nla_sgeqrt(int64_t m, int64_t n, int64_t ib, float *A, int64_t lda, float *T, int64_t ldt,
                float *tau, float *work, void *stream);
+int nla_zgeqrt(int64_t m, int64_t n, int64_t ib, nla_zcomplex *A, int64_t lda, nla_zcomplex *T, int64_t ldt,
+               nla_zcomplex *tau, nla_zcomplex *work, void *stream);
+int nla_cgeqrt(int64_t m, int64_t n, int64_t ib, nla_ccomplex *A, int64_t lda, nla_ccomplex *T, int64_t ldt,
+               nla_ccomplex *tau, nla_ccomplex *work, void *stream);
 
 /* ---- Block-reflector application ----------------------------------------------------- */
 /* larfb!(side, trans, direct, storev, m, n, k, V, ldv, T, C, work): src/larfb.jl:37-142.
- * direct='F', storev='C' only, as in the reference (src/larfb.jl:42-44). */
+ * direct='F', storev='C' only, as in the reference (src/larfb.jl:42-44).  Blocks wider than
+ * 64 reflectors are applied as 64-wide sub-blocks, which assumes T is larft!'s factor of V. */
 int nla_dlarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
                const double *V, int64_t ldv, const double *T, int64_t ldt, double *C, int64_t ldc,
                double *work, int64_t ldwork, void *stream);
 int nla_slarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
                const float *V, int64_t ldv, const float *T, int64_t ldt, float *C, int64_t ldc,
                float *work, int64_t ldwork, void *stream);
+int nla_zlarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
+               const nla_zcomplex *V, int64_t ldv, const nla_zcomplex *T, int64_t ldt, nla_zcomplex *C, int64_t ldc,
+               nla_zcomplex *work, int64_t ldwork, void *stream);
+int nla_clarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
+               const nla_ccomplex *V, int64_t ldv, const nla_ccomplex *T, int64_t ldt, nla_ccomplex *C, int64_t ldc,
+               nla_ccomplex *work, int64_t ldwork, void *stream);
 
 /* unmqr!(side, trans, m, n, k, ib, A, lda, T, C, work): src/unmqr.jl:52-149.
  * a_cols = size(A,2), used by the clamp k = min(k, nq, size(A,2)) (src/unmqr.jl:83). */
@@ -93,27 +120,70 @@ int nla_dunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t i
 int nla_sunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const float *A,
                int64_t lda, int64_t a_cols, const float *T, int64_t ldt, float *C, int64_t ldc,
                float *work, void *stream);
+int nla_zunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const nla_zcomplex *A,
+               int64_t lda, int64_t a_cols, const nla_zcomplex *T, int64_t ldt, nla_zcomplex *C, int64_t ldc,
+               nla_zcomplex *work, void *stream);
+int nla_cunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const nla_ccomplex *A,
+               int64_t lda, int64_t a_cols, const nla_ccomplex *T, int64_t ldt, nla_ccomplex *C, int64_t ldc,
+               nla_ccomplex *work, void *stream);
 
-/* ---- Triangle-on-square kernels --------------------------------------------------- */
+/* ---- Triangle-on-square and triangle-on-triangle kernels ---------------------------- */
 /* tsqrt!(m, n, ib, A1, A2, T, tau, work): src/tsqrt.jl:128-156.  m1 = size(A1,1);
  * k = min(m1, n) reflectors (src/tsqrt.jl:139-140). */
 int nla_dtsqrt(int64_t m, int64_t n, int64_t ib, double *A1, int64_t lda1, int64_t m1, double *A2,
                int64_t lda2, double *T, int64_t ldt, double *tau, double *work, void *stream);
 int nla_stsqrt(int64_t m, int64_t n, int64_t ib, float *A1, int64_t lda1, int64_t m1, float *A2,
                int64_t lda2, float *T, int64_t ldt, float *tau, float *work, void *stream);
+int nla_ztsqrt(int64_t m, int64_t n, int64_t ib, nla_zcomplex *A1, int64_t lda1, int64_t m1, nla_zcomplex *A2,
+               int64_t lda2, nla_zcomplex *T, int64_t ldt, nla_zcomplex *tau, nla_zcomplex *work, void *stream);
+int nla_ctsqrt(int64_t m, int64_t n, int64_t ib, nla_ccomplex *A1, int64_t lda1, int64_t m1, nla_ccomplex *A2,
+               int64_t lda2, nla_ccomplex *T, int64_t ldt, nla_ccomplex *tau, nla_ccomplex *work, void *stream);
 
 /* tsmqr!(side, trans, m1, n1, m2, n2, k, ib, A1, A2, V, T, work): src/tsmqr.jl:47-146.
- * v_cols = size(V,2), t_cols = size(T,2) for the clamp at src/tsmqr.jl:94-96. */
-int nla_dtsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k, int64_t ib,
-               double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V, int64_t ldv,
+ * v_cols = size(V,2), t_cols = size(T,2) for the clamp at src/tsmqr.jl:94-96.  ib <= 256. */
+int nla_dtsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,
+               int64_t ib, double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V, int64_t ldv,
                int64_t v_cols, const double *T, int64_t ldt, int64_t t_cols, double *work, void *stream);
-int nla_stsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k, int64_t ib,
-               float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V, int64_t ldv,
+int nla_stsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,
+               int64_t ib, float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V, int64_t ldv,
                int64_t v_cols, const float *T, int64_t ldt, int64_t t_cols, float *work, void *stream);
+int nla_ztsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,
+               int64_t ib, nla_zcomplex *A1, int64_t lda1, nla_zcomplex *A2, int64_t lda2, const nla_zcomplex *V, int64_t ldv,
+               int64_t v_cols, const nla_zcomplex *T, int64_t ldt, int64_t t_cols, nla_zcomplex *work, void *stream);
+int nla_ctsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,
+               int64_t ib, nla_ccomplex *A1, int64_t lda1, nla_ccomplex *A2, int64_t lda2, const nla_ccomplex *V, int64_t ldv,
+               int64_t v_cols, const nla_ccomplex *T, int64_t ldt, int64_t t_cols, nla_ccomplex *work, void *stream);
+
+/* ttqrt!(m, n, ib, A1, A2, T, tau, work): src/ttqrt.jl:81-118.  QR of [A1; A2] with both
+ * m x n upper triangular (trapezoidal); column r's reflector has rows 0..min(r+1, m)-1 of A2. */
+int nla_dttqrt(int64_t m, int64_t n, int64_t ib, double *A1, int64_t lda1, double *A2, int64_t lda2,
+               double *T, int64_t ldt, double *tau, double *work, void *stream);
+int nla_sttqrt(int64_t m, int64_t n, int64_t ib, float *A1, int64_t lda1, float *A2, int64_t lda2,
+               float *T, int64_t ldt, float *tau, float *work, void *stream);
+int nla_zttqrt(int64_t m, int64_t n, int64_t ib, nla_zcomplex *A1, int64_t lda1, nla_zcomplex *A2, int64_t lda2,
+               nla_zcomplex *T, int64_t ldt, nla_zcomplex *tau, nla_zcomplex *work, void *stream);
+int nla_cttqrt(int64_t m, int64_t n, int64_t ib, nla_ccomplex *A1, int64_t lda1, nla_ccomplex *A2, int64_t lda2,
+               nla_ccomplex *T, int64_t ldt, nla_ccomplex *tau, nla_ccomplex *work, void *stream);
+
+/* ttmqr!(side, trans, m1, n1, m2, n2, k, ib, A1, A2, V, T, work): src/ttmqr.jl:1-95 (block i
+ * uses the first min(i+kb, m2) rows -- columns for side 'R' -- of A2 and V, pentagon order
+ * min(kb, m2 - i - 1) as src/ttmqr.jl:70-74, 83-87).  ib <= 256. */
+int nla_dttmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,
+               int64_t ib, double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V, int64_t ldv,
+               const double *T, int64_t ldt, double *work, void *stream);
+int nla_sttmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,
+               int64_t ib, float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V, int64_t ldv,
+               const float *T, int64_t ldt, float *work, void *stream);
+int nla_zttmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,
+               int64_t ib, nla_zcomplex *A1, int64_t lda1, nla_zcomplex *A2, int64_t lda2, const nla_zcomplex *V, int64_t ldv,
+               const nla_zcomplex *T, int64_t ldt, nla_zcomplex *work, void *stream);
+int nla_cttmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,
+               int64_t ib, nla_ccomplex *A1, int64_t lda1, nla_ccomplex *A2, int64_t lda2, const nla_ccomplex *V, int64_t ldv,
+               const nla_ccomplex *T, int64_t ldt, nla_ccomplex *work, void *stream);
 
 /* parfb!(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, A2, V, T, work):
- * src/parfb.jl:44-191.  direct='F', storev='C', l=0 are implemented (the tsmqr! path);
- * other forms return -3 / -4 / -10.  work is k x n2 (side L) or m2 x k (side R). */
+ * src/parfb.jl:44-191.  Every side / trans / direct / storev with pentagon order l; applies
+ * I - V T V^H exactly for any T of the stored triangle (upper 'F', lower 'B').  k <= 256. */
 int nla_dparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2,
                int64_t n2, int64_t k, int64_t l, double *A1, int64_t lda1, double *A2, int64_t lda2,
                const double *V, int64_t ldv, const double *T, int64_t ldt, double *work, int64_t ldwork,
@@ -122,6 +192,44 @@ int nla_sparfb(char side, char trans, char direct, char storev, int64_t m1, int6
                int64_t n2, int64_t k, int64_t l, float *A1, int64_t lda1, float *A2, int64_t lda2,
                const float *V, int64_t ldv, const float *T, int64_t ldt, float *work, int64_t ldwork,
                void *stream);
+int nla_zparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2,
+               int64_t n2, int64_t k, int64_t l, nla_zcomplex *A1, int64_t lda1, nla_zcomplex *A2, int64_t lda2,
+               const nla_zcomplex *V, int64_t ldv, const nla_zcomplex *T, int64_t ldt, nla_zcomplex *work, int64_t ldwork,
+               void *stream);
+int nla_cparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2,
+               int64_t n2, int64_t k, int64_t l, nla_ccomplex *A1, int64_t lda1, nla_ccomplex *A2, int64_t lda2,
+               const nla_ccomplex *V, int64_t ldv, const nla_ccomplex *T, int64_t ldt, nla_ccomplex *work, int64_t ldwork,
+               void *stream);
+
+/* pamm!(op, side, storev, direct, m, n, k, l, A1, A2, V, W): src/pamm.jl:49-100.  op 'W':
+ * W = A1 + op(V) A2 (side L, W k x n) / A1 + A2 op(V) (side R, W m x k); op 'A':
+ * A2 -= op(V) W / W op(V); m x n = size(A2); V pentagonal of order l.  k <= 256. */
+int nla_dpamm(char op, char side, char storev, char direct, int64_t m, int64_t n, int64_t k,
+              int64_t l, const double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V, int64_t ldv,
+              double *W, int64_t ldw, void *stream);
+int nla_spamm(char op, char side, char storev, char direct, int64_t m, int64_t n, int64_t k,
+              int64_t l, const float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V, int64_t ldv,
+              float *W, int64_t ldw, void *stream);
+int nla_zpamm(char op, char side, char storev, char direct, int64_t m, int64_t n, int64_t k,
+              int64_t l, const nla_zcomplex *A1, int64_t lda1, nla_zcomplex *A2, int64_t lda2, const nla_zcomplex *V, int64_t ldv,
+              nla_zcomplex *W, int64_t ldw, void *stream);
+int nla_cpamm(char op, char side, char storev, char direct, int64_t m, int64_t n, int64_t k,
+              int64_t l, const nla_ccomplex *A1, int64_t lda1, nla_ccomplex *A2, int64_t lda2, const nla_ccomplex *V, int64_t ldv,
+              nla_ccomplex *W, int64_t ldw, void *stream);
+
+/* pemv!(trans, storev, m, n, l, alpha, A, X, beta, Y, work): src/pemv.jl:39-181.
+ * storev 'C' with trans 'T'/'C': Y (n) = alpha op(A) X + beta Y; storev 'R' with trans 'N':
+ * Y (m) = alpha A X + beta Y; A m x n pentagonal of order l (l = 1 counts as 0).  alpha and
+ * beta are passed by value; beta = 0 does not read Y; alpha = beta = 0 leaves Y unchanged.
+ * work is accepted, unused. */
+int nla_dpemv(char trans, char storev, int64_t m, int64_t n, int64_t l, double alpha, const double *A,
+              int64_t lda, const double *x, double beta, double *y, double *work, void *stream);
+int nla_spemv(char trans, char storev, int64_t m, int64_t n, int64_t l, float alpha, const float *A,
+              int64_t lda, const float *x, float beta, float *y, float *work, void *stream);
+int nla_zpemv(char trans, char storev, int64_t m, int64_t n, int64_t l, nla_zcomplex alpha, const nla_zcomplex *A,
+              int64_t lda, const nla_zcomplex *x, nla_zcomplex beta, nla_zcomplex *y, nla_zcomplex *work, void *stream);
+int nla_cpemv(char trans, char storev, int64_t m, int64_t n, int64_t l, nla_ccomplex alpha, const nla_ccomplex *A,
+              int64_t lda, const nla_ccomplex *x, nla_ccomplex beta, nla_ccomplex *y, nla_ccomplex *work, void *stream);
 
 /* ---- Recursive TRSM / TRMM ------------------------------------------------------ */
 /* unified_rectrxm!(side, uplo, trans, alpha, func, A, B): src/rectrxm.jl:43-76.

@@ -76,9 +76,78 @@ struct ProfScope {
 inline bool is_trans(char t) { return t == 'T' || t == 'C' || t == 't' || t == 'c'; }
 inline char up(char c) { return (c >= 'a' && c <= 'z') ? char(c - 32) : c; }
 
+// ---------------------------------------------------------------------------------
+// Complex element type of the QR-family routines (ComplexF64 / ComplexF32: two reals, real
+// part first, the layout Julia, numpy and torch use).  The real types use the same helper
+// names, so one kernel body serves both: conj_ is the identity on reals.
+template <typename R> struct alignas(2 * sizeof(R)) cplx {
+    R re, im;
+    cplx() = default;
+    __host__ __device__ constexpr cplx(R r) : re(r), im(0) {}
+    __host__ __device__ constexpr cplx(R r, R i) : re(r), im(i) {}
+    __host__ __device__ cplx &operator+=(cplx b) { re += b.re; im += b.im; return *this; }
+    __host__ __device__ cplx &operator-=(cplx b) { re -= b.re; im -= b.im; return *this; }
+    __host__ __device__ cplx &operator*=(cplx b) {
+        const R r = re * b.re - im * b.im;
+        im = re * b.im + im * b.re;
+        re = r;
+        return *this;
+    }
+    __host__ __device__ cplx &operator/=(cplx b);
+};
+template <typename R> __host__ __device__ inline cplx<R> operator+(cplx<R> a, cplx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R> __host__ __device__ inline cplx<R> operator-(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R> __host__ __device__ inline cplx<R> operator-(cplx<R> a) { return {-a.re, -a.im}; }
+template <typename R> __host__ __device__ inline cplx<R> operator*(cplx<R> a, cplx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R> __host__ __device__ inline cplx<R> operator*(R a, cplx<R> b) { return {a * b.re, a * b.im}; }
+template <typename R> __host__ __device__ inline cplx<R> operator*(cplx<R> a, R b) { return {a.re * b, a.im * b}; }
+template <typename R> __host__ __device__ inline cplx<R> operator/(cplx<R> a, cplx<R> b) {
+    const R d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+template <typename R> __host__ __device__ inline cplx<R> operator/(cplx<R> a, R b) { return {a.re / b, a.im / b}; }
+template <typename R> __host__ __device__ inline cplx<R> &cplx<R>::operator/=(cplx<R> b) { return *this = *this / b; }
+template <typename R> __host__ __device__ inline bool operator==(cplx<R> a, cplx<R> b) { return a.re == b.re && a.im == b.im; }
+template <typename R> __host__ __device__ inline bool operator!=(cplx<R> a, cplx<R> b) { return !(a == b); }
+
+template <typename T> struct RealOf { typedef T t; };
+template <typename R> struct RealOf<cplx<R>> { typedef R t; };
+template <typename T> struct IsCplx { static constexpr bool v = false; };
+template <typename R> struct IsCplx<cplx<R>> { static constexpr bool v = true; };
+
+__host__ __device__ inline double conj_(double x) { return x; }
+__host__ __device__ inline float conj_(float x) { return x; }
+template <typename R> __host__ __device__ inline cplx<R> conj_(cplx<R> x) { return {x.re, -x.im}; }
+__host__ __device__ inline double re_(double x) { return x; }
+__host__ __device__ inline float re_(float x) { return x; }
+template <typename R> __host__ __device__ inline R re_(cplx<R> x) { return x.re; }
+__host__ __device__ inline double im_(double) { return 0; }
+__host__ __device__ inline float im_(float) { return 0; }
+template <typename R> __host__ __device__ inline R im_(cplx<R> x) { return x.im; }
+__host__ __device__ inline double abs2_(double x) { return x * x; }
+__host__ __device__ inline float abs2_(float x) { return x * x; }
+template <typename R> __host__ __device__ inline R abs2_(cplx<R> x) { return x.re * x.re + x.im * x.im; }
+// largest |component|: a scale factor for overflow-safe norms
+__host__ __device__ inline double absmax_(double x) { return x < 0 ? -x : x; }
+__host__ __device__ inline float absmax_(float x) { return x < 0 ? -x : x; }
+template <typename R> __host__ __device__ inline R absmax_(cplx<R> x) {
+    const R a = x.re < 0 ? -x.re : x.re, b = x.im < 0 ? -x.im : x.im;
+    return a > b ? a : b;
+}
+// element built from real and imaginary parts (the imaginary part is dropped for reals)
+template <typename T> __host__ __device__ inline T mkc_(typename RealOf<T>::t re, typename RealOf<T>::t im);
+template <> __host__ __device__ inline double mkc_<double>(double re, double) { return re; }
+template <> __host__ __device__ inline float mkc_<float>(float re, float) { return re; }
+template <> __host__ __device__ inline cplx<double> mkc_<cplx<double>>(double re, double im) { return {re, im}; }
+template <> __host__ __device__ inline cplx<float> mkc_<cplx<float>>(float re, float im) { return {re, im}; }
+
 template <typename T> struct Tname;
 template <> struct Tname<double> { static constexpr const char *s = "Float64"; };
 template <> struct Tname<float> { static constexpr const char *s = "Float32"; };
+template <> struct Tname<cplx<double>> { static constexpr const char *s = "ComplexF64"; };
+template <> struct Tname<cplx<float>> { static constexpr const char *s = "ComplexF32"; };
 
 // ---------------------------------------------------------------------------------
 // Device helpers.
@@ -87,6 +156,14 @@ constexpr int WAVE = 64;
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+template <typename R> __device__ __forceinline__ cplx<R> wave_sum(cplx<R> v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v.re += __shfl_xor(v.re, off, WAVE);
+        v.im += __shfl_xor(v.im, off, WAVE);
+    }
     return v;
 }
 
@@ -106,7 +183,7 @@ template <typename T> __device__ __forceinline__ T block_sum(T v, T *red) {
     __syncthreads();
     if (lane == 0) red[w] = v;
     __syncthreads();
-    T s = 0;
+    T s = T(0);
     for (int i = 0; i < nw; ++i) s += red[i];
     return s;
 }

@@ -23,105 +23,133 @@
 namespace nla {
 
 // ------------------------------------------------------------------------------------
-// larfg!: src/larfg.jl:43-106 (safe rescale loop kept; beta via hypot as LAPACK dlarfg).
-template <typename T> __device__ __forceinline__ T lamch_sfmin();
+// larfg!: src/larfg.jl:43-106 with its complex branch (safe rescale loop kept; beta via
+// hypot for reals as LAPACK dlarfg, via a scaled 3-term norm for complex elements).
+template <typename R> __device__ __forceinline__ R lamch_sfmin();
 template <> __device__ __forceinline__ double lamch_sfmin<double>() { return 2.2250738585072014e-308; }
 template <> __device__ __forceinline__ float lamch_sfmin<float>() { return 1.17549435e-38f; }
-template <typename T> __device__ __forceinline__ T lamch_eps();
+template <typename R> __device__ __forceinline__ R lamch_eps();
 template <> __device__ __forceinline__ double lamch_eps<double>() { return 1.1102230246251565e-16; }
 template <> __device__ __forceinline__ float lamch_eps<float>() { return 5.96046448e-08f; }
 
+// sqrt(a^2 + b^2 + c^2) without overflow or underflow of the squares
+template <typename R> __device__ __forceinline__ R hyp3(R a, R b, R c) {
+    a = fabs(a); b = fabs(b); c = fabs(c);
+    const R mx = fmax(a, fmax(b, c));
+    if (mx == R(0)) return R(0);
+    a /= mx; b /= mx; c /= mx;
+    return mx * sqrt(a * a + b * b + c * c);
+}
+template <typename T> __device__ __forceinline__ typename RealOf<T>::t beta_of(typename RealOf<T>::t alphr,
+                                                                              typename RealOf<T>::t alphi,
+                                                                              typename RealOf<T>::t xnorm) {
+    if constexpr (IsCplx<T>::v) return -copysign(hyp3(alphr, alphi, xnorm), alphr);
+    else return -copysign(hypot(alphr, xnorm), alphr);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void larfg_kernel(int n, T *alpha, T *x, i64 incx, T *tau) {
-    __shared__ T red[16];
+    typedef typename RealOf<T>::t R;
+    __shared__ R red[16];
     __shared__ T sh_scale;
     const int t = threadIdx.x;
     const int nx = n - 1;
-    // two-pass overflow-safe norm: max |x|, then sum (x/max)^2
-    T mx = 0;
-    for (int i = t; i < nx; i += 256) mx = fmax(mx, fabs(x[i * incx]));
+    // two-pass overflow-safe norm: max |component|, then sum |x/max|^2
+    R mx = 0;
+    for (int i = t; i < nx; i += 256) mx = fmax(mx, absmax_(x[i * incx]));
     mx = wave_max(mx);
     __syncthreads();
     if ((t & 63) == 0) red[t >> 6] = mx;
     __syncthreads();
     mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    T ss = 0;
+    R ss = 0;
     if (mx > 0)
-        for (int i = t; i < nx; i += 256) { T v = x[i * incx] / mx; ss += v * v; }
+        for (int i = t; i < nx; i += 256) ss += abs2_(x[i * incx] / mx);
     ss = block_sum(ss, red);
-    T xnorm = mx * sqrt(ss);
+    R xnorm = mx * sqrt(ss);
     const T a0 = *alpha;
-    if (xnorm == T(0)) {
-        if (t == 0) *tau = 0;
+    R alphr = re_(a0), alphi = im_(a0);
+    if (xnorm == R(0) && alphi == R(0)) {
+        if (t == 0) *tau = T(0);
         return;
     }
-    T alphr = a0;
-    T beta = -copysign(hypot(alphr, xnorm), alphr);
-    const T safmin = lamch_sfmin<T>() / lamch_eps<T>();
-    const T rsafmn = T(1) / safmin;
+    R beta = beta_of<T>(alphr, alphi, xnorm);
+    const R safmin = lamch_sfmin<R>() / lamch_eps<R>();
+    const R rsafmn = R(1) / safmin;
     int knt = 0;
-    T a = a0;
     if (fabs(beta) < safmin) {
         while (fabs(beta) < safmin && knt < 20) {
             ++knt;
-            beta *= rsafmn; alphr *= rsafmn; a *= rsafmn;
+            beta *= rsafmn; alphr *= rsafmn; alphi *= rsafmn;
         }
-        T f = 1;
+        R f = 1;
         for (int j = 0; j < knt; ++j) f *= rsafmn;
-        for (int i = t; i < nx; i += 256) x[i * incx] *= f;
+        for (int i = t; i < nx; i += 256) x[i * incx] = x[i * incx] * f;
         __syncthreads();
         mx = 0;
-        for (int i = t; i < nx; i += 256) mx = fmax(mx, fabs(x[i * incx]));
+        for (int i = t; i < nx; i += 256) mx = fmax(mx, absmax_(x[i * incx]));
         mx = wave_max(mx);
         __syncthreads();
         if ((t & 63) == 0) red[t >> 6] = mx;
         __syncthreads();
         mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
         ss = 0;
-        for (int i = t; i < nx; i += 256) { T v = x[i * incx] / mx; ss += v * v; }
+        for (int i = t; i < nx; i += 256) ss += abs2_(x[i * incx] / mx);
         ss = block_sum(ss, red);
         xnorm = mx * sqrt(ss);
-        beta = -copysign(hypot(alphr, xnorm), alphr);
+        beta = beta_of<T>(alphr, alphi, xnorm);
     }
-    if (t == 0) sh_scale = T(1) / (a - beta);
+    if (t == 0) sh_scale = T(1) / (mkc_<T>(alphr, alphi) - T(beta));
     __syncthreads();
     const T s = sh_scale;
     for (int i = t; i < nx; i += 256) x[i * incx] *= s;
     if (t == 0) {
-        *tau = (beta - alphr) / beta;
+        *tau = mkc_<T>((beta - alphr) / beta, -alphi / beta);
         for (int j = 0; j < knt; ++j) beta *= safmin;
-        *alpha = beta;
+        *alpha = T(beta);
     }
+}
+
+// Householder vector of one column in the panel kernels: alpha = the diagonal entry, xs =
+// sum |x|^2 of the rest.  The reference's inline larfg (src/geqr2.jl:38-73,
+// src/tsqrt.jl:53-75): no reflector (tau = 0, beta = alpha) when xs = 0 and alpha is real.
+template <typename T>
+__device__ __forceinline__ void reflector(T alpha, typename RealOf<T>::t xs, bool make, T *sh) {
+    typedef typename RealOf<T>::t R;
+    const R alphr = re_(alpha), alphi = im_(alpha);
+    T tv = T(0), beta = alpha, scale = T(0);
+    if (make && !(xs == R(0) && alphi == R(0))) {
+        const R b = -copysign(sqrt(alphr * alphr + alphi * alphi + xs), alphr);
+        tv = mkc_<T>((b - alphr) / b, -alphi / b);
+        scale = T(1) / (alpha - T(b));
+        beta = T(b);
+    }
+    sh[0] = tv; sh[1] = beta; sh[2] = scale;
 }
 
 // ------------------------------------------------------------------------------------
 // Panel QR (geqr2! + larft!('F','C') fused).  geqr2_kernel! semantics (src/geqr2.jl:23-105):
-// plain sum-of-squares norm, beta = -sign(alpha) sqrt(alpha^2 + |x|^2), tau = (beta-alpha)/beta,
-// x /= (alpha - beta); the in-panel update uses w = A^T v with v(1) = 1.
+// plain sum-of-squares norm, beta = -sign(re alpha) sqrt(|alpha|^2 + |x|^2), tau = (beta -
+// alpha)/beta (conjugated imaginary part), x /= (alpha - beta); a column of length 1 gets
+// tau = 0; the in-panel update applies H^H: A -= conj(tau) v (v^H A), v(1) = 1.
 // If Tm != nullptr also writes T (larft_kernel! F/C, src/larft.jl:55-126): for column i,
-// T[0:i, i] = -tau_i * T[0:i,0:i] * (V^T v_i), T[i,i] = tau_i.  LAPACK's lastv trimming
+// T[0:i, i] = -tau_i * T[0:i,0:i] * (V^H v_i), T[i,i] = tau_i.  LAPACK's lastv trimming
 // only skips exact-zero products, so the dense sums here give the same T.
+// red: 16 entries of T (used as real scratch).
 template <typename T>
 __device__ void panel_qr_dev(int m, int n, T *__restrict__ A, i64 lda, T *__restrict__ tau, T *__restrict__ Tm,
                              i64 ldt, T *z, T *tcol, T *red, T *sh) {
-    // z[n]: v^T A[:, j] for every panel column j; tcol[n] (only with Tm)
+    typedef typename RealOf<T>::t R;
+    // z[n]: v^H A[:, j] for every panel column j; tcol[n] (only with Tm)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int k = min(m, n);
     for (int i = 0; i < k; ++i) {
         const int len = m - i;
         T *col = A + i + (i64)i * lda;  // column i from row i
-        T xs = 0;
-        for (int p = 1 + t; p < len; p += 256) { T v = col[p]; xs += v * v; }
-        xs = block_sum(xs, red);
-        if (t == 0) {
-            T alpha = col[0], tv = 0, beta = alpha, scale = 0;
-            if (len > 1 && xs != T(0)) {
-                beta = -copysign(sqrt(alpha * alpha + xs), alpha);
-                tv = (beta - alpha) / beta;
-                scale = T(1) / (alpha - beta);
-            }
-            sh[0] = tv; sh[1] = beta; sh[2] = scale;
-        }
+        R xs = 0;
+        for (int p = 1 + t; p < len; p += 256) xs += abs2_(col[p]);
+        xs = block_sum(xs, reinterpret_cast<R *>(red));
+        if (t == 0) reflector<T>(col[0], xs, len > 1, sh);
         __syncthreads();
         const T tv = sh[0], beta = sh[1], scale = sh[2];
         if (tv != T(0)) {
@@ -129,34 +157,36 @@ __device__ void panel_qr_dev(int m, int n, T *__restrict__ A, i64 lda, T *__rest
         }
         __syncthreads();
         if (t == 0) { col[0] = beta; tau[i] = tv; }
-        // z_j = A[i, j] + sum_{p>=1} A[i+p, j] v_p for all j != i (j < i feeds T, j > i the update)
+        // z_j = A[i, j] + sum_{p>=1} conj(v_p) A[i+p, j] = v^H a_j for all j != i (j > i: the
+        // update; j < i: conj(z_j) = v_j^H v_i feeds T)
         const int jlo = Tm ? 0 : i + 1;
         if (tv != T(0)) {
             for (int j = jlo + w; j < n; j += 4) {
                 if (j == i) continue;
                 const T *cj = A + i + (i64)j * lda;
-                T s = 0;
-                for (int p = 1 + lane; p < len; p += 64) s += cj[p] * col[p];
+                T s = T(0);
+                for (int p = 1 + lane; p < len; p += 64) s += conj_(col[p]) * cj[p];
                 s = wave_sum(s);
                 if (lane == 0) z[j] = s + cj[0];
             }
         }
         __syncthreads();
         if (tv != T(0)) {
-            // trailing in-panel update A[i+p, j] -= tau v_p z_j
+            // trailing in-panel update A[i+p, j] -= conj(tau) v_p z_j
+            const T ct = conj_(tv);
             const int nc = n - i - 1;
             for (int idx = t; idx < nc * len; idx += 256) {
                 const int p = idx % len, j = i + 1 + idx / len;
                 const T vp = p == 0 ? T(1) : col[p];
-                A[i + p + (i64)j * lda] -= tv * vp * z[j];
+                A[i + p + (i64)j * lda] -= ct * vp * z[j];
             }
         }
         if (Tm) {
-            // T[0:i, i] = -tau * T[0:i, 0:i] * z[0:i]
-            if (t < i) tcol[t] = tv != T(0) ? -tv * z[t] : T(0);
+            // T[0:i, i] = -tau * T[0:i, 0:i] * conj(z[0:i])
+            if (t < i) tcol[t] = tv != T(0) ? -tv * conj_(z[t]) : T(0);
             __syncthreads();
             if (t < i) {
-                T s = 0;
+                T s = T(0);
                 if (tv != T(0))
                     for (int q = t; q < i; ++q) s += Tm[t + (i64)q * ldt] * tcol[q];
                 Tm[t + (i64)i * ldt] = s;
@@ -179,69 +209,71 @@ __global__ __launch_bounds__(256) void panel_qr_kernel(int m, int n, T *__restri
 }
 
 // ------------------------------------------------------------------------------------
-// TS panel: one ib-block of tsqrt_block_kernel! (src/tsqrt.jl:46-126).
-// Column r = ii + i of [A1 (upper); A2 (m x n)]: alpha = A1[r,r], x = A2[:, r].
-template <typename T>
+// TS / TT panel: one ib-block of tsqrt_block_kernel! (src/tsqrt.jl:46-126) or
+// ttqrt_block_kernel! (src/ttqrt.jl:2-79).  Column r = ii + i of [A1 (upper); A2]:
+// alpha = A1[r,r], x = A2[0:mi, r] with mi = m (TS) or min(r + 1, m) (TT, A2 upper
+// triangular); the block's later columns get H^H; the T column uses the first mj = m (TS)
+// or min(r, m) (TT) rows of A2: T[p, r] = -tau sum_q conj(A2[q, ii+p]) A2[q, r].
+template <typename T, bool TT>
 __global__ __launch_bounds__(256) void ts_panel_kernel(int m, int ii, int sb, T *__restrict__ A1, i64 lda1,
                                                        T *__restrict__ A2, i64 lda2, T *__restrict__ Tm, i64 ldt,
                                                        T *__restrict__ tau) {
-    __shared__ T red[16];
+    typedef typename RealOf<T>::t R;
+    __shared__ R red[16];
     __shared__ T z[256];     // block width sb <= 256
     __shared__ T tcol[256];
     __shared__ T sh[4];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int i = 0; i < sb; ++i) {
         const int r = ii + i;
+        const int mi = TT ? min(r + 1, m) : m, mj = TT ? min(r, m) : m;
         T *x = A2 + (i64)r * lda2;
-        T xs = 0;
-        for (int p = t; p < m; p += 256) { T v = x[p]; xs += v * v; }
+        R xs = 0;
+        for (int p = t; p < mi; p += 256) xs += abs2_(x[p]);
         xs = block_sum(xs, red);
-        if (t == 0) {
-            T alpha = A1[r + (i64)r * lda1], tv = 0, beta = alpha, scale = 0;
-            if (xs != T(0)) {
-                beta = -copysign(sqrt(alpha * alpha + xs), alpha);
-                tv = (beta - alpha) / beta;
-                scale = T(1) / (alpha - beta);
-            }
-            sh[0] = tv; sh[1] = beta; sh[2] = scale;
-        }
+        if (t == 0) reflector<T>(A1[r + (i64)r * lda1], xs, true, sh);
         __syncthreads();
         const T tv = sh[0], beta = sh[1], scale = sh[2];
         if (tv != T(0))
-            for (int p = t; p < m; p += 256) x[p] *= scale;
+            for (int p = t; p < mi; p += 256) x[p] *= scale;
         __syncthreads();
         if (t == 0) { A1[r + (i64)r * lda1] = beta; tau[r] = tv; }
-        // z_j = sum_q A2[q, ii+j] x[q] for block columns j != i; A1 row term only for j > i
+        // j > i: z_j = sum_{q < mi} conj(x_q) A2[q, ii+j] (the update's v^H a_j without A1's row);
+        // j < i: z_j = sum_{q < mj} conj(A2[q, ii+j]) x_q (v_j^H v_r for the T column)
         if (tv != T(0)) {
             for (int j = w; j < sb; j += 4) {
                 if (j == i) continue;
                 const T *cj = A2 + (i64)(ii + j) * lda2;
-                T s = 0;
-                for (int p = lane; p < m; p += 64) s += cj[p] * x[p];
+                T s = T(0);
+                if (j > i)
+                    for (int p = lane; p < mi; p += 64) s += conj_(x[p]) * cj[p];
+                else
+                    for (int p = lane; p < mj; p += 64) s += conj_(cj[p]) * x[p];
                 s = wave_sum(s);
                 if (lane == 0) z[j] = s;
             }
         }
         __syncthreads();
         if (tv != T(0)) {
+            const T ct = conj_(tv);
             const int nc = sb - i - 1;
-            // A1[r, r+1+j] -= tau * (A1[r, r+1+j] + z) ; A2[:, r+1+j] -= tau x (A1 + z)
-            for (int idx = t; idx < nc * m; idx += 256) {
-                const int p = idx % m, j = i + 1 + idx / m;
+            // w = A1[r, r+1+j] + z ;  A2[0:mi, r+1+j] -= conj(tau) x w ;  A1[r, r+1+j] -= conj(tau) w
+            for (int idx = t; idx < nc * mi; idx += 256) {
+                const int p = idx % mi, j = i + 1 + idx / mi;
                 const T wj = A1[r + (i64)(ii + j) * lda1] + z[j];
-                A2[p + (i64)(ii + j) * lda2] -= tv * x[p] * wj;
+                A2[p + (i64)(ii + j) * lda2] -= ct * x[p] * wj;
             }
             __syncthreads();
             for (int j = i + 1 + t; j < sb; j += 256) {
                 T *a1 = &A1[r + (i64)(ii + j) * lda1];
-                *a1 -= tv * (*a1 + z[j]);
+                *a1 -= ct * (*a1 + z[j]);
             }
         }
         // T column: T[p, r] = -tau * sum_q T[p, ii+q] z_q  (p < i), T[i, r] = tau
         if (t < i) tcol[t] = tv != T(0) ? -tv * z[t] : T(0);
         __syncthreads();
         if (t < i) {
-            T s = 0;
+            T s = T(0);
             if (tv != T(0))
                 for (int q = t; q < i; ++q) s += Tm[t + (i64)(ii + q) * ldt] * tcol[q];
             Tm[t + (i64)r * ldt] = s;
@@ -252,8 +284,10 @@ __global__ __launch_bounds__(256) void ts_panel_kernel(int m, int ii, int sb, T 
 }
 
 // ------------------------------------------------------------------------------------
-// larft!('F','C') standalone: T = upper k x k with T[0:i,i] = -tau_i T[0:i,0:i] V^T v_i.
-template <typename T>
+// larft!('F', storev) standalone (src/larft.jl:55-126): T upper k x k with
+// T[0:i,i] = -tau_i T[0:i,0:i] (V^H v_i) (storev 'C', V n x k) or -tau_i T[0:i,0:i] (V v_i^H)
+// over the rows of V (storev 'R', V k x n).
+template <typename T, bool ROW>
 __global__ __launch_bounds__(256) void larft_kernel(int n, int k, const T *__restrict__ V, i64 ldv,
                                                     const T *__restrict__ tau, T *__restrict__ Tm, i64 ldt) {
     extern __shared__ __align__(16) unsigned char dyn_raw[];
@@ -261,16 +295,20 @@ __global__ __launch_bounds__(256) void larft_kernel(int n, int k, const T *__res
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     for (int i = 0; i < k; ++i) {
         const T ti = tau[i];
-        // z_p = V[i, p] + sum_{r > i} V[r, p] V[r, i] for p < i
+        // C: z_p = conj(V[i, p]) + sum_{r > i} conj(V[r, p]) V[r, i];  R: z_p = V[p, i] + sum_{r > i} V[p, r] conj(V[i, r])
         for (int p = w; p < i; p += 4) {
-            T s = 0;
-            for (int r = i + 1 + lane; r < n; r += 64) s += V[r + (i64)p * ldv] * V[r + (i64)i * ldv];
+            T s = T(0);
+            for (int r = i + 1 + lane; r < n; r += 64) {
+                if (ROW) s += V[p + (i64)r * ldv] * conj_(V[i + (i64)r * ldv]);
+                else s += conj_(V[r + (i64)p * ldv]) * V[r + (i64)i * ldv];
+            }
             s = wave_sum(s);
-            if (lane == 0) z[p] = ti == T(0) ? T(0) : -ti * (V[i + (i64)p * ldv] + s);
+            const T head = ROW ? V[p + (i64)i * ldv] : conj_(V[i + (i64)p * ldv]);
+            if (lane == 0) z[p] = ti == T(0) ? T(0) : -ti * (head + s);
         }
         __syncthreads();
         for (int p = t; p < i; p += 256) {
-            T s = 0;
+            T s = T(0);
             if (ti != T(0))
                 for (int q = p; q < i; ++q) s += Tm[p + (i64)q * ldt] * z[q];
             Tm[p + (i64)i * ldt] = s;
@@ -283,38 +321,44 @@ __global__ __launch_bounds__(256) void larft_kernel(int n, int k, const T *__res
 
 // ------------------------------------------------------------------------------------
 // Strip kernels.  View element (r, c) of a matrix stored at p with leading dimension ld:
-// TR ? p[c + r*ld] : p[r + c*ld].
-template <typename T, bool TR> __device__ __forceinline__ T &vat(T *p, i64 ld, int r, int c) {
-    return TR ? p[c + (i64)r * ld] : p[r + (i64)c * ld];
-}
+// TR ? conj(p[c + r*ld]) : p[r + c*ld]  (the conjugate-transposed view for right-side calls).
 template <typename T, bool TR> __device__ __forceinline__ T vget(const T *p, i64 ld, int r, int c) {
-    return TR ? p[c + (i64)r * ld] : p[r + (i64)c * ld];
+    return TR ? conj_(p[c + (i64)r * ld]) : p[r + (i64)c * ld];
+}
+template <typename T, bool TR> __device__ __forceinline__ void vsub(T *p, i64 ld, int r, int c, T s) {
+    if (TR) p[c + (i64)r * ld] -= conj_(s);
+    else p[r + (i64)c * ld] -= s;
 }
 
 constexpr int CW = 16;    // strip width (columns of the left-form view)
 constexpr int KBM = 64;   // reflectors per applied sub-block
 constexpr int RC = 64;    // rows per staged chunk
 
-// Apply op(H) = I - Vh op(T) Vh^T of kb reflectors from the left to rows [Top (kb); Bot (mb)]
-// of the strip c0..c0+cw-1 (left-form views, TRC = stored transposed).
+// Apply op(H) = I - Vh op(T) Vh^H of kb reflectors from the left to rows [Top (kb); Bot (mb)]
+// of the strip c0..c0+cw-1 (left-form views, TRC = conjugate-transposed storage).
 //   GE: Vh = [V1 unit lower (kb x kb) ; V2 (mb x kb)]   (larfb!, src/larfb.jl:57-98)
 //   TS: Vh = [I ; V2 (mb x kb)]                         (parfb!, src/parfb.jl:121-129,189)
-// W0 = Vh^T C, W = op(T) W0 (op(T) = T for notrans, T^T otherwise), C -= Vh W.
+// W0 = Vh^H C, W = op(T) W0 (op(T) = T for notrans, T^H otherwise), C -= Vh W.
+// TS with l > 0 (ttmqr!, parfb! with a triangular V): V2 is pentagonal, its last l rows the
+// first l rows of a upper triangle; column j here is column jofs + j of that block, and
+// V2[r, j] = 0 for r >= mb - l and r - (mb - l) > jofs + j (src/pamm.jl:104-127,306-325).
 template <typename T, bool GE, bool TRC>
 __device__ void apply_strip(int kb, int mb, const T *__restrict__ V1, const T *__restrict__ V2, i64 ldv,
                             const T *__restrict__ Tb, i64 ldt, bool notrans, T *Top, i64 ldtop, T *Bot, i64 ldbot,
-                            int c0, int cw, T *sV, T *sC, T *sW0, T *sW) {
+                            int c0, int cw, T *sV, T *sC, T *sW0, T *sW, int l = 0, int jofs = 0) {
     const int t = threadIdx.x;
     const int oc = t % CW, oj = t / CW;  // thread -> W0 entries (oj + 16 q, oc)
+    const int rz = mb - l;               // first row of the triangular part (l > 0)
     T acc[KBM / 16];
 #pragma unroll
-    for (int q = 0; q < KBM / 16; ++q) acc[q] = 0;
+    for (int q = 0; q < KBM / 16; ++q) acc[q] = T(0);
     for (int r0 = 0; r0 < mb; r0 += RC) {
         const int rc = min(RC, mb - r0);
         __syncthreads();
         for (int idx = t; idx < rc * kb; idx += 256) {
             const int r = idx % rc, j = idx / rc;
-            sV[j * RC + r] = V2[(r0 + r) + (i64)j * ldv];
+            const bool zero = !GE && l > 0 && r0 + r >= rz && r0 + r - rz > jofs + j;
+            sV[j * RC + r] = zero ? T(0) : conj_(V2[(r0 + r) + (i64)j * ldv]);
         }
         for (int idx = t; idx < rc * CW; idx += 256) {
             int r, c;
@@ -326,7 +370,7 @@ __device__ void apply_strip(int kb, int mb, const T *__restrict__ V1, const T *_
         for (int q = 0; q < KBM / 16; ++q) {
             const int j = oj + 16 * q;
             if (j < kb) {
-                T s = 0;
+                T s = T(0);
                 for (int r = 0; r < rc; ++r) s += sV[j * RC + r] * sC[oc * RC + r];
                 acc[q] += s;
             }
@@ -341,7 +385,8 @@ __device__ void apply_strip(int kb, int mb, const T *__restrict__ V1, const T *_
             if (oc < cw) {
                 s += vget<T, TRC>(Top, ldtop, j, c0 + oc);
                 if (GE)
-                    for (int r = j + 1; r < kb; ++r) s += V1[r + (i64)j * ldv] * vget<T, TRC>(Top, ldtop, r, c0 + oc);
+                    for (int r = j + 1; r < kb; ++r)
+                        s += conj_(V1[r + (i64)j * ldv]) * vget<T, TRC>(Top, ldtop, r, c0 + oc);
             }
             sW0[j * CW + oc] = s;
         }
@@ -351,9 +396,9 @@ __device__ void apply_strip(int kb, int mb, const T *__restrict__ V1, const T *_
     for (int q = 0; q < KBM / 16; ++q) {
         const int j = oj + 16 * q;
         if (j < kb) {
-            T s = 0;
+            T s = T(0);
             if (notrans) for (int p = j; p < kb; ++p) s += Tb[j + (i64)p * ldt] * sW0[p * CW + oc];
-            else         for (int p = 0; p <= j; ++p) s += Tb[p + (i64)j * ldt] * sW0[p * CW + oc];
+            else         for (int p = 0; p <= j; ++p) s += conj_(Tb[p + (i64)j * ldt]) * sW0[p * CW + oc];
             sW[j * CW + oc] = s;
         }
     }
@@ -364,14 +409,16 @@ __device__ void apply_strip(int kb, int mb, const T *__restrict__ V1, const T *_
         T s = sW[r * CW + c];
         if (GE)
             for (int j = 0; j < r; ++j) s += V1[r + (i64)j * ldv] * sW[j * CW + c];
-        vat<T, TRC>(Top, ldtop, r, c0 + c) -= s;
+        vsub<T, TRC>(Top, ldtop, r, c0 + c, s);
     }
     for (int idx = t; idx < mb * cw; idx += 256) {
         int r, c;
         if (TRC) { c = idx % cw; r = idx / cw; } else { r = idx % mb; c = idx / mb; }
-        T s = 0;
-        for (int j = 0; j < kb; ++j) s += V2[r + (i64)j * ldv] * sW[j * CW + c];
-        vat<T, TRC>(Bot, ldbot, r, c0 + c) -= s;
+        T s = T(0);
+        // pentagonal V2: row r >= rz has entries only in columns jofs + j >= r - rz
+        const int jlo = (!GE && l > 0 && r >= rz) ? max(0, r - rz - jofs) : 0;
+        for (int j = jlo; j < kb; ++j) s += V2[r + (i64)j * ldv] * sW[j * CW + c];
+        vsub<T, TRC>(Bot, ldbot, r, c0 + c, s);
     }
     __syncthreads();
 }
@@ -408,12 +455,15 @@ __global__ __launch_bounds__(256) void ge_apply_kernel(int m, int ncols, int k, 
     }
 }
 
-// tsmqr! (left form): k reflectors of V (m2 x k) in blocks of ib; block i updates rows
-// i..i+kb-1 of A1 and all m2 rows of A2 (parfb!, l = 0).
+// tsmqr! / ttmqr! (left form): k reflectors of V (m2 x k) in blocks of ib; block i updates
+// rows i..i+kb-1 of A1 and the first mb rows of A2 (parfb!, src/tsmqr.jl:123-145,
+// src/ttmqr.jl:64-92).  mode 0: mb = m2 and pentagon order l = lfix (tsmqr!: 0; one
+// parfb! block of order lfix); mode 1 (ttmqr!): mb = min(i + kb, m2), l = min(kb, m2 - i - 1).
 template <typename T, bool TRC>
 __global__ __launch_bounds__(256) void ts_apply_kernel(int m2, int ncols, int k, int ib, const T *__restrict__ V,
                                                        i64 ldv, const T *__restrict__ Tm, i64 ldt, bool notrans,
-                                                       bool forward, T *A1, i64 lda1, T *A2, i64 lda2) {
+                                                       bool forward, T *A1, i64 lda1, T *A2, i64 lda2, int mode,
+                                                       int lfix) {
     __shared__ T sV[KBM * RC];
     __shared__ T sC[CW * RC];
     __shared__ T sW0[KBM * CW];
@@ -425,6 +475,8 @@ __global__ __launch_bounds__(256) void ts_apply_kernel(int m2, int ncols, int k,
         const int b = forward ? bb : nblk - 1 - bb;
         const int i = b * ib;
         const int kb = min(ib, k - i);
+        const int mb = mode == 1 ? min(i + kb, m2) : m2;
+        const int l = mode == 1 ? min(kb, max(0, m2 - i - 1)) : lfix;
         const int nsub = (kb + KBM - 1) / KBM;
         for (int ss = 0; ss < nsub; ++ss) {
             const int s = forward ? ss : nsub - 1 - ss;
@@ -433,9 +485,132 @@ __global__ __launch_bounds__(256) void ts_apply_kernel(int m2, int ncols, int k,
             const T *Vb = V + (i64)row * ldv;
             const T *Tb = Tm + j0 + (i64)(i + j0) * ldt;
             T *Top = TRC ? A1 + (i64)row * lda1 : A1 + row;
-            apply_strip<T, false, TRC>(kk, m2, nullptr, Vb, ldv, Tb, ldt, notrans, Top, lda1, A2, lda2, c0, cw, sV, sC,
-                                       sW0, sW);
+            apply_strip<T, false, TRC>(kk, mb, nullptr, Vb, ldv, Tb, ldt, notrans, Top, lda1, A2, lda2, c0, cw, sV, sC,
+                                       sW0, sW, l, j0);
         }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// General parfb! / pamm! (src/parfb.jl:44-191, src/pamm.jl:49-473): one block of k
+// reflectors in any direct / storev with pentagon order l and an arbitrary T (parfb! applies
+// I - Vc T Vc^H exactly; no sub-block splitting, so T need not come from larft!).  Left
+// form: A1 (k x ncols), A2 (rows x ncols); right-side calls use conjugate-transposed views.
+// Column form of V: Vc(r, j) = V(r, j) (storev 'C') or conj(V(j, r)) (storev 'R'), with the
+// structural zeros of direct 'F' (last l rows: first l rows of an upper triangle) or 'B'
+// (first l rows: last l rows of a lower triangle).  T upper ('F') or lower ('B').
+// Phases (flags): PW: W = A1 + Vc^H A2;  PT: W = op(T) W, A1 -= W;  PA: A2 -= Vc W.  W lives in
+// LDS, or in the caller's W (pamm!'s 'W' output / 'A' input; Wg != nullptr).
+constexpr int PCW = 8;       // strip width of the general kernel
+constexpr int PKMAX = 256;   // largest k of one general block
+
+template <typename T> __device__ __forceinline__ T pent_vc(const T *V, i64 ldv, bool row, int r, int j) {
+    return row ? conj_(V[j + (i64)r * ldv]) : V[r + (i64)j * ldv];
+}
+__device__ __forceinline__ bool pent_nz(bool fwd, int rows, int k, int l, int r, int j) {
+    return fwd ? !(r >= rows - l && r - (rows - l) > j) : !(r < l && j > k - l + r);
+}
+
+enum { PENT_W = 1, PENT_T = 2, PENT_A = 4 };
+
+template <typename T, bool TRC>
+__global__ __launch_bounds__(256) void pent_apply_kernel(int rows, int ncols, int k, int l, bool fwd, bool row,
+                                                         bool notrans, int phases, const T *__restrict__ V, i64 ldv,
+                                                         const T *__restrict__ Tm, i64 ldt, T *A1, i64 lda1, T *A2,
+                                                         i64 lda2, T *Wg, i64 ldw) {
+    __shared__ T sW0[PKMAX * PCW];
+    __shared__ T sW[PKMAX * PCW];
+    const int t = threadIdx.x;
+    const int c0 = blockIdx.x * PCW;
+    const int cw = min(PCW, ncols - c0);
+    // W0 = A1 + Vc^H A2, or the caller's W
+    for (int idx = t; idx < k * PCW; idx += 256) {
+        const int j = idx / PCW, c = idx % PCW;
+        T s = T(0);
+        if (c < cw) {
+            if (phases & PENT_W) {
+                s = vget<T, TRC>(A1, lda1, j, c0 + c);
+                for (int r = 0; r < rows; ++r)
+                    if (pent_nz(fwd, rows, k, l, r, j)) s += conj_(pent_vc(V, ldv, row, r, j)) * vget<T, TRC>(A2, lda2, r, c0 + c);
+            } else {
+                s = vget<T, TRC>(Wg, ldw, j, c0 + c);
+            }
+        }
+        sW0[j * PCW + c] = s;
+    }
+    __syncthreads();
+    if (phases & PENT_T) {
+        for (int idx = t; idx < k * PCW; idx += 256) {
+            const int i = idx / PCW, c = idx % PCW;
+            T s = T(0);
+            for (int j = 0; j < k; ++j) {
+                // element (i, j) of op(T) within T's triangle
+                const int a = notrans ? i : j, b = notrans ? j : i;
+                if (fwd ? a > b : a < b) continue;
+                const T tv = Tm[a + (i64)b * ldt];
+                s += (notrans ? tv : conj_(tv)) * sW0[j * PCW + c];
+            }
+            sW[i * PCW + c] = s;
+        }
+        __syncthreads();
+        for (int idx = t; idx < k * cw; idx += 256) {
+            const int i = idx / cw, c = idx % cw;
+            vsub<T, TRC>(A1, lda1, i, c0 + c, sW[i * PCW + c]);
+        }
+    } else {
+        for (int idx = t; idx < k * PCW; idx += 256) sW[idx] = sW0[idx];
+        __syncthreads();
+    }
+    if ((phases & PENT_W) && !(phases & PENT_T) && Wg) {
+        // pamm! 'W': the product is the output
+        for (int idx = t; idx < k * cw; idx += 256) {
+            const int j = idx / cw, c = idx % cw;
+            const T v = sW[j * PCW + c];
+            if (TRC) Wg[c0 + c + (i64)j * ldw] = conj_(v);
+            else Wg[j + (i64)(c0 + c) * ldw] = v;
+        }
+    }
+    if (phases & PENT_A) {
+        for (int idx = t; idx < rows * cw; idx += 256) {
+            int r, c;
+            if (TRC) { c = idx % cw; r = idx / cw; } else { r = idx % rows; c = idx / rows; }
+            T s = T(0);
+            for (int j = 0; j < k; ++j)
+                if (pent_nz(fwd, rows, k, l, r, j)) s += pent_vc(V, ldv, row, r, j) * sW[j * PCW + c];
+            vsub<T, TRC>(A2, lda2, r, c0 + c, s);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// pemv! (src/pemv.jl:39-181): y = alpha op(A) x + beta y over a pentagonal A (m x n).
+// COLS (storev 'C', trans 'T'/'C'): y has n entries, column j's rows r >= m - l with
+// r - (m - l) > j are zero (j < l); !COLS (storev 'R', trans 'N'): y has m entries, row r's
+// columns c >= n - l with c - (n - l) > r are zero (r < l).  beta = 0 does not read y.
+template <typename T, bool COLS>
+__global__ __launch_bounds__(256) void pemv_kernel(int m, int n, int l, bool conjA, T alpha, const T *__restrict__ A,
+                                                   i64 lda, const T *__restrict__ x, T beta, T *__restrict__ y) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (COLS) {
+        // one wave per output entry: a column of A against x (coalesced down the column)
+        const int j = blockIdx.x * 4 + w;
+        if (j >= n) return;
+        const int rlim = (j < l) ? m - l + j + 1 : m;
+        T s = T(0);
+        for (int r = lane; r < rlim; r += 64) {
+            const T a = A[r + (i64)j * lda];
+            s += (conjA ? conj_(a) : a) * x[r];
+        }
+        s = wave_sum(s);
+        if (lane == 0) y[j] = alpha * s + (beta == T(0) ? T(0) : beta * y[j]);
+    } else {
+        // one thread per output entry: threads of a wave read consecutive rows of a column
+        const int r = blockIdx.x * 256 + t;
+        if (r >= m) return;
+        const int clim = (r < l) ? n - l + r + 1 : n;
+        T s = T(0);
+        for (int c = 0; c < clim; ++c) s += A[r + (i64)c * lda] * x[c];
+        y[r] = alpha * s + (beta == T(0) ? T(0) : beta * y[r]);
     }
 }
 
@@ -451,14 +626,15 @@ hipError_t launch_panel_qr(i64 m, i64 n, T *A, i64 lda, T *tau, T *Tm, i64 ldt, 
 }
 
 template <typename T>
-hipError_t launch_ts_panel(i64 m, i64 ii, i64 sb, T *A1, i64 lda1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau,
+hipError_t launch_ts_panel(bool tt, i64 m, i64 ii, i64 sb, T *A1, i64 lda1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau,
                            hipStream_t s) {
     ProfScope prof(PK_TS_PANEL, s);
-    ts_panel_kernel<T><<<1, 256, 0, s>>>((int)m, (int)ii, (int)sb, A1, lda1, A2, lda2, Tm, ldt, tau);
+    if (tt) ts_panel_kernel<T, true><<<1, 256, 0, s>>>((int)m, (int)ii, (int)sb, A1, lda1, A2, lda2, Tm, ldt, tau);
+    else ts_panel_kernel<T, false><<<1, 256, 0, s>>>((int)m, (int)ii, (int)sb, A1, lda1, A2, lda2, Tm, ldt, tau);
     return hipGetLastError();
 }
 
-// Left-form GE apply on an m x ncols view (trc: C stored transposed).
+// Left-form GE apply on an m x ncols view (trc: C stored conjugate-transposed).
 template <typename T>
 hipError_t launch_ge_apply(bool trc, i64 m, i64 ncols, i64 k, i64 ib, const T *V, i64 ldv, const T *Tm, i64 ldt,
                            bool notrans, bool forward, T *C, i64 ldc, hipStream_t s) {
@@ -471,13 +647,32 @@ hipError_t launch_ge_apply(bool trc, i64 m, i64 ncols, i64 k, i64 ib, const T *V
 }
 
 template <typename T>
-hipError_t launch_ts_apply(bool trc, i64 m2, i64 ncols, i64 k, i64 ib, const T *V, i64 ldv, const T *Tm, i64 ldt,
-                           bool notrans, bool forward, T *A1, i64 lda1, T *A2, i64 lda2, hipStream_t s) {
+hipError_t launch_ts_apply_mode(bool trc, i64 m2, i64 ncols, i64 k, i64 ib, const T *V, i64 ldv, const T *Tm, i64 ldt,
+                                bool notrans, bool forward, T *A1, i64 lda1, T *A2, i64 lda2, int mode, int lfix,
+                                hipStream_t s) {
     if (m2 <= 0 || ncols <= 0 || k <= 0) return hipSuccess;
     dim3 grid((unsigned)((ncols + CW - 1) / CW)), block(256);
     ProfScope prof(PK_TS_APPLY, s);
-    if (trc) ts_apply_kernel<T, true><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2);
-    else ts_apply_kernel<T, false><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2);
+    if (trc) ts_apply_kernel<T, true><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2, mode, lfix);
+    else ts_apply_kernel<T, false><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2, mode, lfix);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ts_apply(bool trc, i64 m2, i64 ncols, i64 k, i64 ib, const T *V, i64 ldv, const T *Tm, i64 ldt,
+                           bool notrans, bool forward, T *A1, i64 lda1, T *A2, i64 lda2, hipStream_t s) {
+    return launch_ts_apply_mode<T>(trc, m2, ncols, k, ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2, 0, 0, s);
+}
+
+template <typename T>
+hipError_t launch_pent(bool trc, int rows, int ncols, int k, int l, bool fwd, bool row, bool notrans, int phases,
+                       const T *V, i64 ldv, const T *Tm, i64 ldt, T *A1, i64 lda1, T *A2, i64 lda2, T *W, i64 ldw,
+                       hipStream_t s) {
+    if (rows < 0 || ncols <= 0 || k <= 0) return hipSuccess;
+    dim3 grid((unsigned)((ncols + PCW - 1) / PCW)), block(256);
+    ProfScope prof(PK_TS_APPLY, s);
+    if (trc) pent_apply_kernel<T, true><<<grid, block, 0, s>>>(rows, ncols, k, l, fwd, row, notrans, phases, V, ldv, Tm, ldt, A1, lda1, A2, lda2, W, ldw);
+    else pent_apply_kernel<T, false><<<grid, block, 0, s>>>(rows, ncols, k, l, fwd, row, notrans, phases, V, ldv, Tm, ldt, A1, lda1, A2, lda2, W, ldw);
     return hipGetLastError();
 }
 
@@ -506,7 +701,7 @@ hipError_t tsqrt_async(i64 m, i64 n, i64 ib, i64 k, T *A1, i64 lda1, T *A2, i64 
                        hipStream_t s) {
     for (i64 ii = 0; ii < k; ii += ib) {
         const i64 sb = ib < k - ii ? ib : k - ii;
-        hipError_t e = launch_ts_panel<T>(m, ii, sb, A1, lda1, A2, lda2, Tm, ldt, tau, s);
+        hipError_t e = launch_ts_panel<T>(false, m, ii, sb, A1, lda1, A2, lda2, Tm, ldt, tau, s);
         if (e != hipSuccess) return e;
         if (n > ii + sb) {
             // tsmqr!('L','C', sb, n-ii-sb, m, n-ii-sb, sb, ib, A1[ii:, ii+sb:], A2[:, ii+sb:], A2[:, ii:], T[:, ii:])
@@ -518,20 +713,39 @@ hipError_t tsqrt_async(i64 m, i64 n, i64 ib, i64 k, T *A1, i64 lda1, T *A2, i64 
     return hipSuccess;
 }
 
-template hipError_t launch_ge_apply<double>(bool, i64, i64, i64, i64, const double *, i64, const double *, i64, bool,
-                                            bool, double *, i64, hipStream_t);
-template hipError_t launch_ge_apply<float>(bool, i64, i64, i64, i64, const float *, i64, const float *, i64, bool, bool,
-                                           float *, i64, hipStream_t);
-template hipError_t launch_ts_apply<double>(bool, i64, i64, i64, i64, const double *, i64, const double *, i64, bool,
-                                            bool, double *, i64, double *, i64, hipStream_t);
-template hipError_t launch_ts_apply<float>(bool, i64, i64, i64, i64, const float *, i64, const float *, i64, bool, bool,
-                                           float *, i64, float *, i64, hipStream_t);
-template hipError_t geqrt_async<double>(i64, i64, i64, double *, i64, double *, i64, double *, hipStream_t);
-template hipError_t geqrt_async<float>(i64, i64, i64, float *, i64, float *, i64, float *, hipStream_t);
-template hipError_t tsqrt_async<double>(i64, i64, i64, i64, double *, i64, double *, i64, double *, i64, double *,
-                                        hipStream_t);
-template hipError_t tsqrt_async<float>(i64, i64, i64, i64, float *, i64, float *, i64, float *, i64, float *,
-                                       hipStream_t);
+// ttqrt! body: src/ttqrt.jl:101-117 -- per ib-block the TT panel, then parfb!('L','C','F','C')
+// of the block on the trailing columns with the first mi = min(ii + sb, m) rows of A2 and
+// pentagon order l = min(sb, mi - ii).
+template <typename T>
+hipError_t ttqrt_async(i64 m, i64 n, i64 ib, T *A1, i64 lda1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau, hipStream_t s) {
+    const i64 k = m < n ? m : n;
+    for (i64 ii = 0; ii < k; ii += ib) {
+        const i64 sb = ib < k - ii ? ib : k - ii;
+        hipError_t e = launch_ts_panel<T>(true, m, ii, sb, A1, lda1, A2, lda2, Tm, ldt, tau, s);
+        if (e != hipSuccess) return e;
+        if (n > ii + sb) {
+            const i64 mi = ii + sb < m ? ii + sb : m;
+            const i64 l = mi - ii > 0 ? (sb < mi - ii ? sb : mi - ii) : 0;
+            e = launch_ts_apply_mode<T>(false, mi, n - ii - sb, sb, sb, A2 + ii * lda2, lda2, Tm + ii * ldt, ldt, false,
+                                        true, A1 + ii + (ii + sb) * lda1, lda1, A2 + (ii + sb) * lda2, lda2, 0, (int)l, s);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+#define NLA_QR_INST(T)                                                                                                 \
+    template hipError_t launch_ge_apply<T>(bool, i64, i64, i64, i64, const T *, i64, const T *, i64, bool, bool, T *, \
+                                           i64, hipStream_t);                                                          \
+    template hipError_t launch_ts_apply<T>(bool, i64, i64, i64, i64, const T *, i64, const T *, i64, bool, bool, T *, \
+                                           i64, T *, i64, hipStream_t);                                                \
+    template hipError_t geqrt_async<T>(i64, i64, i64, T *, i64, T *, i64, T *, hipStream_t);                          \
+    template hipError_t tsqrt_async<T>(i64, i64, i64, i64, T *, i64, T *, i64, T *, i64, T *, hipStream_t);
+NLA_QR_INST(double)
+NLA_QR_INST(float)
+NLA_QR_INST(cplx<double>)
+NLA_QR_INST(cplx<float>)
+#undef NLA_QR_INST
 
 // ------------------------------------------------------------------------------------
 // Batched uneven problems: one persistent launch over a device work list.  Task t < count is
@@ -688,7 +902,7 @@ template <typename T> static int geqr2_entry(i64 m, i64 n, T *A, i64 lda, T *tau
     if (n < 0) return arg_error(2, "illegal value of n: %lld", (long long)n);   // src/geqr2.jl:111-113
     if (m == 0 || n == 0) return 0;
     if (lda < m) return arg_error(4, "lda must be >= max(1,m)");
-    if (n > 8192) return arg_error(2, "geqr2 panel holds at most 8192 columns, got %lld", (long long)n);
+    if (n > 4096) return arg_error(2, "geqr2 panel holds at most 4096 columns, got %lld", (long long)n);
     if (m > INT32_MAX) return arg_error(1, "m exceeds 2^31-1");
     NLA_HIP(launch_panel_qr<T>(m, n, A, lda, tau, (T *)nullptr, 0, as_stream(stream)));
     return 0;
@@ -701,15 +915,17 @@ static int larft_entry(char direct, char storev, i64 n, i64 k, const T *V, i64 l
     if (n == 0) return 0;  // src/larft.jl:206-208
     if (direct != 'F' && direct != 'B') return arg_error(1, "direct must be 'F' or 'B', got '%c'", direct);
     if (storev != 'C' && storev != 'R') return arg_error(2, "storev must be 'C' or 'R', got '%c'", storev);
-    if (direct != 'F') return arg_error(1, "direct='B' is not implemented (the QR path uses 'F')");
-    if (storev != 'C') return arg_error(2, "storev='R' is not implemented (the QR path uses 'C')");
+    if (direct != 'F') return arg_error(1, "direct='B' is not implemented (the QR routines use 'F')");
     if (n < 0) return arg_error(3, "n must be non-negative");
     if (k < 0) return arg_error(4, "k must be non-negative");
     if (k == 0) return 0;
-    if (ldv < n) return arg_error(6, "ldv must be >= n");
+    if (ldv < (storev == 'C' ? n : k)) return arg_error(6, "ldv must be >= %lld", (long long)(storev == 'C' ? n : k));
     if (ldt < k) return arg_error(9, "ldt must be >= k");
     if (k > 4096) return arg_error(4, "k must be <= 4096");
-    larft_kernel<T><<<1, 256, (size_t)k * sizeof(T), as_stream(stream)>>>((int)n, (int)k, V, ldv, tau, Tm, ldt);
+    hipStream_t s = as_stream(stream);
+    const size_t sh = (size_t)k * sizeof(T);
+    if (storev == 'C') larft_kernel<T, false><<<1, 256, sh, s>>>((int)n, (int)k, V, ldv, tau, Tm, ldt);
+    else larft_kernel<T, true><<<1, 256, sh, s>>>((int)n, (int)k, V, ldv, tau, Tm, ldt);
     NLA_LAUNCH_CHECK();
     return 0;
 }
@@ -723,6 +939,7 @@ static int larfb_entry(char side, char trans, char direct, char storev, i64 m, i
         return arg_error(storev != 'C' ? 4 : 3, "larfb! currently supports only forward, columnwise reflectors");
     if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);
     if (!valid_trans(trans)) return arg_error(2, "trans must be 'N', 'C', or 'T', got '%c'", trans);
+    if (IsCplx<T>::v && trans == 'T') return arg_error(2, "trans='T' is not valid for complex element types");
     const i64 q = side == 'L' ? m : n;
     if (k > q) return arg_error(7, "k must be <= %lld", (long long)q);
     if (ldv < q) return arg_error(9, "ldv must be >= %lld", (long long)q);
@@ -758,6 +975,7 @@ static int unmqr_entry(char side, char trans, i64 m, i64 n, i64 k, i64 ib, const
     if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);  // src/unmqr.jl:54-56
     const i64 nq = side == 'L' ? m : n;
     if (!valid_trans(trans)) return arg_error(2, "trans must be 'N', 'C', or 'T', got '%c'", trans);  // :66-68
+    if (IsCplx<T>::v && trans == 'T') return arg_error(2, "trans='T' is not valid for complex element types");
     if (m < 0) return arg_error(3, "m must be non-negative, got %lld", (long long)m);                 // :70-72
     if (n < 0) return arg_error(4, "n must be non-negative, got %lld", (long long)n);                 // :74-76
     if (k < 0) return arg_error(5, "k must be non-negative, got %lld", (long long)k);                 // :78-80
@@ -795,12 +1013,59 @@ static int tsqrt_entry(i64 m, i64 n, i64 ib, T *A1, i64 lda1, i64 m1, T *A2, i64
     return 0;
 }
 
+// ttqrt!(m, n, ib, A1, A2, T, tau, work): src/ttqrt.jl:81-118.
+template <typename T>
+static int ttqrt_entry(i64 m, i64 n, i64 ib, T *A1, i64 lda1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau, void *stream) {
+    if (m < 0) return arg_error(1, "illegal value of m");    // src/ttqrt.jl:82-84
+    if (n < 0) return arg_error(2, "illegal value of n");    // :86-88
+    if (ib < 0) return arg_error(3, "illegal value of ib");  // :90-92
+    if (m == 0 || n == 0 || ib == 0) return 0;               // :95-97
+    const i64 k = m < n ? m : n;
+    if (lda1 < m) return arg_error(5, "lda1 must be >= m");
+    if (lda2 < m) return arg_error(7, "lda2 must be >= m");
+    if (ldt < (ib < k ? ib : k)) return arg_error(9, "ldt must be >= min(ib, k)");
+    if (ib > 256) return arg_error(3, "ib must be <= 256 (one panel workgroup), got %lld", (long long)ib);
+    if (m > INT32_MAX || n > INT32_MAX) return arg_error(1, "dimension exceeds 2^31-1");
+    NLA_HIP(ttqrt_async<T>(m, n, ib, A1, lda1, A2, lda2, Tm, ldt, tau, as_stream(stream)));
+    return 0;
+}
+
+// The ib-block loop of tsmqr!/ttmqr! on any T: blocks of up to 64 reflectors run in the strip
+// kernel (one launch for every block); wider blocks are applied one block per launch by the
+// general kernel, which applies I - V T V^H exactly without splitting the block.
+template <typename T>
+static int ts_blocks(bool tt, char side, char trans, i64 m1, i64 n1, i64 m2, i64 n2, i64 k, i64 ib, T *A1, i64 lda1,
+                     T *A2, i64 lda2, const T *V, i64 ldv, const T *Tm, i64 ldt, hipStream_t s) {
+    const bool notrans = trans == 'N';
+    const bool fwd = (side == 'L' && !notrans) || (side == 'R' && notrans);  // src/tsmqr.jl:104-114
+    const bool L = side == 'L';
+    const i64 rows = L ? m2 : n2, ncols = L ? n1 : m1;   // left form
+    if (ib <= KBM) {
+        NLA_HIP(launch_ts_apply_mode<T>(!L, rows, ncols, k, ib, V, ldv, Tm, ldt, L ? notrans : !notrans, fwd, A1, lda1,
+                                        A2, lda2, tt ? 1 : 0, 0, s));
+        return 0;
+    }
+    const i64 nblk = (k + ib - 1) / ib;
+    for (i64 bb = 0; bb < nblk; ++bb) {
+        const i64 i = (fwd ? bb : nblk - 1 - bb) * ib;
+        const i64 kb = ib < k - i ? ib : k - i;
+        const i64 mb = tt ? (i + kb < rows ? i + kb : rows) : rows;
+        const i64 l = tt ? (rows - i - 1 > 0 ? (kb < rows - i - 1 ? kb : rows - i - 1) : 0) : 0;
+        T *a1 = L ? A1 + i : A1 + i * lda1;
+        NLA_HIP(launch_pent<T>(!L, (int)mb, (int)ncols, (int)kb, (int)l, true, false, L ? notrans : !notrans,
+                               PENT_W | PENT_T | PENT_A, V + i * ldv, ldv, Tm + i * ldt, ldt, a1, lda1, A2, lda2,
+                               (T *)nullptr, 0, s));
+    }
+    return 0;
+}
+
 template <typename T>
 static int tsmqr_entry(char side, char trans, i64 m1, i64 n1, i64 m2, i64 n2, i64 k, i64 ib, T *A1, i64 lda1, T *A2,
                        i64 lda2, const T *V, i64 ldv, i64 v_cols, const T *Tm, i64 ldt, i64 t_cols, void *stream) {
     side = up(side); trans = up(trans);
     if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);  // src/tsmqr.jl:51-53
     if (!valid_trans(trans)) return arg_error(2, "trans must be 'N', 'C', or 'T', got '%c'", trans);  // :55-57
+    if (IsCplx<T>::v && trans == 'T') return arg_error(2, "trans='T' is not valid for complex element types");
     if (m1 < 0) return arg_error(3, "m1 must be non-negative, got %lld", (long long)m1);              // :59-61
     if (n1 < 0) return arg_error(4, "n1 must be non-negative, got %lld", (long long)n1);              // :63-65
     if (m2 < 0 || (m2 != m1 && side == 'R')) {                                                        // :67-73
@@ -825,16 +1090,39 @@ static int tsmqr_entry(char side, char trans, i64 m1, i64 n1, i64 m2, i64 n2, i6
     const i64 vr = side == 'L' ? m2 : n2;
     if (ldv < vr) return arg_error(14, "ldv must be >= %lld", (long long)vr);
     if (ldt < (ib < k ? ib : k)) return arg_error(17, "ldt must be >= min(ib, k)");
-    const bool notrans = trans == 'N';
-    const bool fwd = (side == 'L' && !notrans) || (side == 'R' && notrans);  // src/tsmqr.jl:104-114
-    hipStream_t s = as_stream(stream);
-    if (side == 'L')
-        NLA_HIP(launch_ts_apply<T>(false, m2, n1, k, ib, V, ldv, Tm, ldt, notrans, fwd, A1, lda1, A2, lda2, s));
-    else
-        NLA_HIP(launch_ts_apply<T>(true, n2, m1, k, ib, V, ldv, Tm, ldt, !notrans, fwd, A1, lda1, A2, lda2, s));
-    return 0;
+    if (ib > PKMAX) return arg_error(8, "ib must be <= %d", PKMAX);
+    return ts_blocks<T>(false, side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, Tm, ldt,
+                        as_stream(stream));
 }
 
+// ttmqr!(side, trans, m1, n1, m2, n2, k, ib, A1, A2, V, T, work): src/ttmqr.jl:1-95.
+template <typename T>
+static int ttmqr_entry(char side, char trans, i64 m1, i64 n1, i64 m2, i64 n2, i64 k, i64 ib, T *A1, i64 lda1, T *A2,
+                       i64 lda2, const T *V, i64 ldv, const T *Tm, i64 ldt, void *stream) {
+    side = up(side); trans = up(trans);
+    if (side != 'L' && side != 'R') return arg_error(1, "illegal value of side");                  // src/ttmqr.jl:3-5
+    if (!valid_trans(trans)) return arg_error(2, "illegal value of trans");                        // :7-9
+    if (IsCplx<T>::v && trans == 'T') return arg_error(2, "trans='T' is not valid for complex element types");
+    if (m1 < 0) return arg_error(3, "illegal value of m1");                                         // :11-13
+    if (n1 < 0) return arg_error(4, "illegal value of n1");                                         // :15-17
+    if (m2 < 0 || (m2 != m1 && side == 'R')) return arg_error(5, "illegal value of m2");            // :19-21
+    if (n2 < 0 || (n2 != n1 && side == 'L')) return arg_error(6, "illegal value of n2");            // :23-25
+    if (k < 0) return arg_error(7, "illegal value of k");                                           // :27-29
+    const i64 kmax = side == 'L' ? m1 : n1;                                                         // :31-32
+    if (k > kmax) k = kmax;
+    if (ib < 0) return arg_error(8, "illegal value of ib");                                         // :34-36
+    if (m1 == 0 || n1 == 0 || m2 == 0 || n2 == 0 || k == 0 || ib == 0) return 0;                    // :39-41
+    if (lda1 < m1) return arg_error(10, "lda1 must be >= m1");
+    if (lda2 < m2) return arg_error(12, "lda2 must be >= m2");
+    const i64 vr = side == 'L' ? m2 : n2;
+    if (ldv < vr) return arg_error(14, "ldv must be >= %lld", (long long)vr);
+    if (ldt < (ib < k ? ib : k)) return arg_error(16, "ldt must be >= min(ib, k)");
+    if (ib > PKMAX) return arg_error(8, "ib must be <= %d", PKMAX);
+    return ts_blocks<T>(true, side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, Tm, ldt,
+                        as_stream(stream));
+}
+
+// parfb!(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, A2, V, T, work): src/parfb.jl:44-191.
 template <typename T>
 static int parfb_entry(char side, char trans, char direct, char storev, i64 m1, i64 n1, i64 m2, i64 n2, i64 k, i64 l,
                        T *A1, i64 lda1, T *A2, i64 lda2, const T *V, i64 ldv, const T *Tm, i64 ldt, void *stream) {
@@ -850,17 +1138,81 @@ static int parfb_entry(char side, char trans, char direct, char storev, i64 m1, 
     if (k < 0) return arg_error(9, "illegal value of k");                                    // :79-81
     if (l < 0 || l > k) return arg_error(10, "illegal value of l");                          // :83-85
     if (m1 == 0 || n1 == 0 || n2 == 0 || k == 0) return 0;                                   // :87-89
-    if (direct != 'F') return arg_error(3, "direct='B' is not implemented (tsmqr! uses 'F')");
-    if (storev != 'C') return arg_error(4, "storev='R' is not implemented (tsmqr! uses 'C')");
-    if (l != 0) return arg_error(10, "l > 0 (triangular V, ttmqr!) is not implemented");
-    if (m2 == 0) return 0;
-    if (lda1 < m1) return arg_error(12, "lda1 must be >= m1");
-    if (lda2 < m2) return arg_error(14, "lda2 must be >= m2");
-    const bool notrans = trans == 'N';
+    if (IsCplx<T>::v && trans == 'T') return arg_error(2, "trans='T' is not valid for complex element types");
+    if (k > PKMAX) return arg_error(9, "k must be <= %d", PKMAX);
+    const bool L = side == 'L';
+    const i64 rows = L ? m2 : n2, ncols = L ? n2 : m2;   // left form
+    if (rows == 0 || ncols == 0) return 0;   // no rows of A2 to combine with
+    if (lda1 < (L ? k : m2)) return arg_error(12, "lda1 too small");
+    if (rows > 0 && lda2 < m2) return arg_error(14, "lda2 must be >= m2");
+    if (rows > 0 && ldv < (storev == 'C' ? rows : k)) return arg_error(16, "ldv too small");
+    if (ldt < k) return arg_error(18, "ldt must be >= k");
+    const bool notrans = (trans == 'N') == L;   // right-side calls flip op(T) in the left form
     hipStream_t s = as_stream(stream);
-    // one block of k reflectors: side L uses rows 0..k-1 of A1, side R columns 0..k-1
-    if (side == 'L') NLA_HIP(launch_ts_apply<T>(false, m2, n2, k, k, V, ldv, Tm, ldt, notrans, true, A1, lda1, A2, lda2, s));
-    else NLA_HIP(launch_ts_apply<T>(true, n2, m2, k, k, V, ldv, Tm, ldt, !notrans, true, A1, lda1, A2, lda2, s));
+    if (direct == 'F' && storev == 'C' && k <= KBM) {
+        NLA_HIP(launch_ts_apply_mode<T>(!L, rows, ncols, k, k, V, ldv, Tm, ldt, notrans, true, A1, lda1, A2, lda2, 0,
+                                        (int)l, s));
+        return 0;
+    }
+    NLA_HIP(launch_pent<T>(!L, (int)rows, (int)ncols, (int)k, (int)l, direct == 'F', storev == 'R', notrans,
+                           PENT_W | PENT_T | PENT_A, V, ldv, Tm, ldt, A1, lda1, A2, lda2, (T *)nullptr, 0, s));
+    return 0;
+}
+
+// pamm!(op, side, storev, direct, m, n, k, l, A1, A2, V, W): src/pamm.jl:49-100.
+template <typename T>
+static int pamm_entry(char op, char side, char storev, char direct, i64 m, i64 n, i64 k, i64 l, const T *A1, i64 lda1,
+                      T *A2, i64 lda2, const T *V, i64 ldv, T *W, i64 ldw, void *stream) {
+    op = up(op); side = up(side); storev = up(storev); direct = up(direct);
+    if (op != 'W' && op != 'A') return arg_error(1, "op must be 'W' or 'A', got '%c'", op);                // src/pamm.jl:51-53
+    if (side != 'L' && side != 'R') return arg_error(2, "side must be 'L' or 'R', got '%c'", side);        // :54-56
+    if (storev != 'C' && storev != 'R') return arg_error(3, "storev must be 'C' or 'R', got '%c'", storev); // :57-59
+    if (direct != 'F' && direct != 'B') return arg_error(4, "direct must be 'F' or 'B', got '%c'", direct); // :60-62
+    if (m < 0) return arg_error(5, "m must be non-negative, got %lld", (long long)m);                      // :65-67
+    if (n < 0) return arg_error(6, "n must be non-negative, got %lld", (long long)n);                      // :68-70
+    if (k < 0) return arg_error(7, "k must be non-negative, got %lld", (long long)k);                      // :71-73
+    if (l < 0) return arg_error(8, "l must be non-negative, got %lld", (long long)l);                      // :74-76
+    if (m == 0 || n == 0 || k == 0) return 0;                                                               // :80-82
+    if (l > k) return arg_error(8, "l must be <= k");
+    if (k > PKMAX) return arg_error(7, "k must be <= %d", PKMAX);
+    const bool L = side == 'L';
+    const i64 rows = L ? m : n, ncols = L ? n : m;
+    if (lda2 < m) return arg_error(12, "lda2 must be >= m");
+    if (ldv < (storev == 'C' ? rows : k)) return arg_error(14, "ldv too small");
+    if (ldw < (L ? k : m)) return arg_error(16, "ldw too small");
+    if (op == 'W' && lda1 < (L ? k : m)) return arg_error(10, "lda1 too small");
+    NLA_HIP(launch_pent<T>(!L, (int)rows, (int)ncols, (int)k, (int)l, direct == 'F', storev == 'R', true,
+                           op == 'W' ? PENT_W : PENT_A, V, ldv, (const T *)nullptr, 0, const_cast<T *>(A1), lda1, A2,
+                           lda2, W, ldw, as_stream(stream)));
+    return 0;
+}
+
+// pemv!(trans, storev, m, n, l, alpha, A, X, beta, Y, work): src/pemv.jl:39-181.
+template <typename T>
+static int pemv_entry(char trans, char storev, i64 m, i64 n, i64 l, T alpha, const T *A, i64 lda, const T *x, T beta,
+                      T *y, void *stream) {
+    trans = up(trans); storev = up(storev);
+    if (trans != 'N' && trans != 'T' && trans != 'C') return arg_error(1, "illegal value of trans");   // src/pemv.jl:40-42
+    if (storev != 'C' && storev != 'R') return arg_error(2, "illegal value of storev");               // :44-46
+    if (!((storev == 'C' && trans != 'N') || (storev == 'R' && trans == 'N')))                         // :48-50
+        return arg_error(1, "illegal values of trans/storev");
+    if (m < 0) return arg_error(3, "illegal value of m");                                               // :52-54
+    if (n < 0) return arg_error(4, "illegal value of n");                                               // :56-58
+    if (l > (m < n ? m : n) || l < 0) return arg_error(5, "illegal value of l");                       // :60-62
+    if (m == 0 || n == 0) return 0;                                                                      // :64-66
+    if (alpha == T(0) && beta == T(0)) return 0;                                                         // :68-70
+    if (l == 1) l = 0;                                                                                   // :72-74
+    if (lda < m) return arg_error(8, "lda must be >= m");
+    if (m > INT32_MAX || n > INT32_MAX) return arg_error(3, "dimension exceeds 2^31-1");
+    hipStream_t s = as_stream(stream);
+    if (storev == 'C') {
+        pemv_kernel<T, true><<<(unsigned)((n + 3) / 4), 256, 0, s>>>((int)m, (int)n, (int)l, trans == 'C', alpha, A, lda,
+                                                                       x, beta, y);
+    } else {
+        pemv_kernel<T, false><<<(unsigned)((m + 255) / 256), 256, 0, s>>>((int)m, (int)n, (int)l, false, alpha, A, lda,
+                                                                           x, beta, y);
+    }
+    NLA_LAUNCH_CHECK();
     return 0;
 }
 
@@ -868,99 +1220,81 @@ static int parfb_entry(char side, char trans, char direct, char storev, i64 m1, 
 
 using namespace nla;
 
-NLA_API int nla_dlarfg(int64_t n, double *alpha, double *x, int64_t incx, double *tau, void *stream) {
-    return larfg_entry<double>(n, alpha, x, incx, tau, stream);
-}
-NLA_API int nla_slarfg(int64_t n, float *alpha, float *x, int64_t incx, float *tau, void *stream) {
-    return larfg_entry<float>(n, alpha, x, incx, tau, stream);
-}
-NLA_API int nla_dgeqr2(int64_t m, int64_t n, double *A, int64_t lda, double *tau, double *work, void *stream) {
-    (void)work;
-    return geqr2_entry<double>(m, n, A, lda, tau, stream);
-}
-NLA_API int nla_sgeqr2(int64_t m, int64_t n, float *A, int64_t lda, float *tau, float *work, void *stream) {
-    (void)work;
-    return geqr2_entry<float>(m, n, A, lda, tau, stream);
-}
-NLA_API int nla_dlarft(char direct, char storev, int64_t n, int64_t k, const double *V, int64_t ldv, const double *tau,
-                       double *T, int64_t ldt, void *stream) {
-    return larft_entry<double>(direct, storev, n, k, V, ldv, tau, T, ldt, stream);
-}
-NLA_API int nla_slarft(char direct, char storev, int64_t n, int64_t k, const float *V, int64_t ldv, const float *tau,
-                       float *T, int64_t ldt, void *stream) {
-    return larft_entry<float>(direct, storev, n, k, V, ldv, tau, T, ldt, stream);
-}
-NLA_API int nla_dgeqrt(int64_t m, int64_t n, int64_t ib, double *A, int64_t lda, double *T, int64_t ldt, double *tau,
-                       double *work, void *stream) {
-    (void)work;
-    return geqrt_entry<double>(m, n, ib, A, lda, T, ldt, tau, stream);
-}
-NLA_API int nla_sgeqrt(int64_t m, int64_t n, int64_t ib, float *A, int64_t lda, float *T, int64_t ldt, float *tau,
-                       float *work, void *stream) {
-    (void)work;
-    return geqrt_entry<float>(m, n, ib, A, lda, T, ldt, tau, stream);
-}
-NLA_API int nla_dlarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
-                       const double *V, int64_t ldv, const double *T, int64_t ldt, double *C, int64_t ldc, double *work,
-                       int64_t ldwork, void *stream) {
-    (void)work; (void)ldwork;
-    return larfb_entry<double>(side, trans, direct, storev, m, n, k, V, ldv, T, ldt, C, ldc, stream);
-}
-NLA_API int nla_slarfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,
-                       const float *V, int64_t ldv, const float *T, int64_t ldt, float *C, int64_t ldc, float *work,
-                       int64_t ldwork, void *stream) {
-    (void)work; (void)ldwork;
-    return larfb_entry<float>(side, trans, direct, storev, m, n, k, V, ldv, T, ldt, C, ldc, stream);
-}
-NLA_API int nla_dunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const double *A, int64_t lda,
-                       int64_t a_cols, const double *T, int64_t ldt, double *C, int64_t ldc, double *work,
-                       void *stream) {
-    (void)work;
-    return unmqr_entry<double>(side, trans, m, n, k, ib, A, lda, a_cols, T, ldt, C, ldc, stream);
-}
-NLA_API int nla_sunmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const float *A, int64_t lda,
-                       int64_t a_cols, const float *T, int64_t ldt, float *C, int64_t ldc, float *work, void *stream) {
-    (void)work;
-    return unmqr_entry<float>(side, trans, m, n, k, ib, A, lda, a_cols, T, ldt, C, ldc, stream);
-}
-NLA_API int nla_dtsqrt(int64_t m, int64_t n, int64_t ib, double *A1, int64_t lda1, int64_t m1, double *A2,
-                       int64_t lda2, double *T, int64_t ldt, double *tau, double *work, void *stream) {
-    (void)work;
-    return tsqrt_entry<double>(m, n, ib, A1, lda1, m1, A2, lda2, T, ldt, tau, stream);
-}
-NLA_API int nla_stsqrt(int64_t m, int64_t n, int64_t ib, float *A1, int64_t lda1, int64_t m1, float *A2, int64_t lda2,
-                       float *T, int64_t ldt, float *tau, float *work, void *stream) {
-    (void)work;
-    return tsqrt_entry<float>(m, n, ib, A1, lda1, m1, A2, lda2, T, ldt, tau, stream);
-}
-NLA_API int nla_dtsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k, int64_t ib,
-                       double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V, int64_t ldv,
-                       int64_t v_cols, const double *T, int64_t ldt, int64_t t_cols, double *work, void *stream) {
-    (void)work;
-    return tsmqr_entry<double>(side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, v_cols, T, ldt, t_cols,
-                               stream);
-}
-NLA_API int nla_stsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k, int64_t ib,
-                       float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V, int64_t ldv, int64_t v_cols,
-                       const float *T, int64_t ldt, int64_t t_cols, float *work, void *stream) {
-    (void)work;
-    return tsmqr_entry<float>(side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, v_cols, T, ldt, t_cols,
-                              stream);
-}
-NLA_API int nla_dparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2, int64_t n2,
-                       int64_t k, int64_t l, double *A1, int64_t lda1, double *A2, int64_t lda2, const double *V,
-                       int64_t ldv, const double *T, int64_t ldt, double *work, int64_t ldwork, void *stream) {
-    (void)work; (void)ldwork;
-    return parfb_entry<double>(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, lda1, A2, lda2, V, ldv, T, ldt,
-                               stream);
-}
-NLA_API int nla_sparfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2, int64_t n2,
-                       int64_t k, int64_t l, float *A1, int64_t lda1, float *A2, int64_t lda2, const float *V,
-                       int64_t ldv, const float *T, int64_t ldt, float *work, int64_t ldwork, void *stream) {
-    (void)work; (void)ldwork;
-    return parfb_entry<float>(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, lda1, A2, lda2, V, ldv, T, ldt,
-                              stream);
-}
+// Exported entry points for one element type (prefix p, element type T).
+#define NLA_QR_EXPORTS(p, T)                                                                                           \
+    NLA_API int nla_##p##larfg(int64_t n, T *alpha, T *x, int64_t incx, T *tau, void *stream) {                       \
+        return larfg_entry<T>(n, alpha, x, incx, tau, stream);                                                         \
+    }                                                                                                                  \
+    NLA_API int nla_##p##geqr2(int64_t m, int64_t n, T *A, int64_t lda, T *tau, T *work, void *stream) {              \
+        (void)work;                                                                                                    \
+        return geqr2_entry<T>(m, n, A, lda, tau, stream);                                                              \
+    }                                                                                                                  \
+    NLA_API int nla_##p##larft(char direct, char storev, int64_t n, int64_t k, const T *V, int64_t ldv, const T *tau, \
+                               T *Tm, int64_t ldt, void *stream) {                                                     \
+        return larft_entry<T>(direct, storev, n, k, V, ldv, tau, Tm, ldt, stream);                                     \
+    }                                                                                                                  \
+    NLA_API int nla_##p##geqrt(int64_t m, int64_t n, int64_t ib, T *A, int64_t lda, T *Tm, int64_t ldt, T *tau,       \
+                               T *work, void *stream) {                                                                \
+        (void)work;                                                                                                    \
+        return geqrt_entry<T>(m, n, ib, A, lda, Tm, ldt, tau, stream);                                                 \
+    }                                                                                                                  \
+    NLA_API int nla_##p##larfb(char side, char trans, char direct, char storev, int64_t m, int64_t n, int64_t k,      \
+                               const T *V, int64_t ldv, const T *Tm, int64_t ldt, T *C, int64_t ldc, T *work,          \
+                               int64_t ldwork, void *stream) {                                                         \
+        (void)work; (void)ldwork;                                                                                      \
+        return larfb_entry<T>(side, trans, direct, storev, m, n, k, V, ldv, Tm, ldt, C, ldc, stream);                  \
+    }                                                                                                                  \
+    NLA_API int nla_##p##unmqr(char side, char trans, int64_t m, int64_t n, int64_t k, int64_t ib, const T *A,        \
+                               int64_t lda, int64_t a_cols, const T *Tm, int64_t ldt, T *C, int64_t ldc, T *work,      \
+                               void *stream) {                                                                         \
+        (void)work;                                                                                                    \
+        return unmqr_entry<T>(side, trans, m, n, k, ib, A, lda, a_cols, Tm, ldt, C, ldc, stream);                      \
+    }                                                                                                                  \
+    NLA_API int nla_##p##tsqrt(int64_t m, int64_t n, int64_t ib, T *A1, int64_t lda1, int64_t m1, T *A2,              \
+                               int64_t lda2, T *Tm, int64_t ldt, T *tau, T *work, void *stream) {                      \
+        (void)work;                                                                                                    \
+        return tsqrt_entry<T>(m, n, ib, A1, lda1, m1, A2, lda2, Tm, ldt, tau, stream);                                 \
+    }                                                                                                                  \
+    NLA_API int nla_##p##tsmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,      \
+                               int64_t ib, T *A1, int64_t lda1, T *A2, int64_t lda2, const T *V, int64_t ldv,          \
+                               int64_t v_cols, const T *Tm, int64_t ldt, int64_t t_cols, T *work, void *stream) {      \
+        (void)work;                                                                                                    \
+        return tsmqr_entry<T>(side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, v_cols, Tm, ldt, t_cols, \
+                              stream);                                                                                 \
+    }                                                                                                                  \
+    NLA_API int nla_##p##ttqrt(int64_t m, int64_t n, int64_t ib, T *A1, int64_t lda1, T *A2, int64_t lda2, T *Tm,     \
+                               int64_t ldt, T *tau, T *work, void *stream) {                                           \
+        (void)work;                                                                                                    \
+        return ttqrt_entry<T>(m, n, ib, A1, lda1, A2, lda2, Tm, ldt, tau, stream);                                     \
+    }                                                                                                                  \
+    NLA_API int nla_##p##ttmqr(char side, char trans, int64_t m1, int64_t n1, int64_t m2, int64_t n2, int64_t k,      \
+                               int64_t ib, T *A1, int64_t lda1, T *A2, int64_t lda2, const T *V, int64_t ldv,          \
+                               const T *Tm, int64_t ldt, T *work, void *stream) {                                      \
+        (void)work;                                                                                                    \
+        return ttmqr_entry<T>(side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, Tm, ldt, stream);        \
+    }                                                                                                                  \
+    NLA_API int nla_##p##parfb(char side, char trans, char direct, char storev, int64_t m1, int64_t n1, int64_t m2,   \
+                               int64_t n2, int64_t k, int64_t l, T *A1, int64_t lda1, T *A2, int64_t lda2, const T *V, \
+                               int64_t ldv, const T *Tm, int64_t ldt, T *work, int64_t ldwork, void *stream) {         \
+        (void)work; (void)ldwork;                                                                                      \
+        return parfb_entry<T>(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, lda1, A2, lda2, V, ldv, Tm, ldt,  \
+                              stream);                                                                                 \
+    }                                                                                                                  \
+    NLA_API int nla_##p##pamm(char op, char side, char storev, char direct, int64_t m, int64_t n, int64_t k,          \
+                              int64_t l, const T *A1, int64_t lda1, T *A2, int64_t lda2, const T *V, int64_t ldv,      \
+                              T *W, int64_t ldw, void *stream) {                                                       \
+        return pamm_entry<T>(op, side, storev, direct, m, n, k, l, A1, lda1, A2, lda2, V, ldv, W, ldw, stream);        \
+    }                                                                                                                  \
+    NLA_API int nla_##p##pemv(char trans, char storev, int64_t m, int64_t n, int64_t l, T alpha, const T *A,          \
+                              int64_t lda, const T *x, T beta, T *y, T *work, void *stream) {                          \
+        (void)work;                                                                                                    \
+        return pemv_entry<T>(trans, storev, m, n, l, alpha, A, lda, x, beta, y, stream);                               \
+    }
+NLA_QR_EXPORTS(d, double)
+NLA_QR_EXPORTS(s, float)
+NLA_QR_EXPORTS(z, cplx<double>)
+NLA_QR_EXPORTS(c, cplx<float>)
+#undef NLA_QR_EXPORTS
 
 namespace nla {
 // Host-side validation of a work list (copied to the host once; a device list is the

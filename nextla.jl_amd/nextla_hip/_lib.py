@@ -28,6 +28,10 @@ _SIGS = {
     "tsqrt": [i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, vp, vp, vp],
     "tsmqr": [ch, ch, i64, i64, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, i64, vp, i64, i64, vp, vp],
     "parfb": [ch, ch, ch, ch, i64, i64, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp],
+    "ttqrt": [i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, vp, vp],
+    "ttmqr": [ch, ch, i64, i64, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, vp],
+    "pamm": [ch, ch, ch, ch, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp],
+    "pemv": [ch, ch, i64, i64, i64, "R", vp, i64, vp, "R", vp, vp, vp],
     "rectrxm": [ch, ch, ch, "R", ch, i64, i64, vp, i64, vp, i64, i64, vp],
     "trsm_base": [ch, ch, i64, i64, vp, i64, vp, i64, vp],
     "trmm_base": [ch, ch, i64, i64, vp, i64, vp, i64, vp],
@@ -47,6 +51,21 @@ _SIGS_D = {
     "rectrsm_mixed": [ch, ch, ch, dbl, i64, i64, vp, i64, vp, i64, i64, i64, dbl, vp, vp, vp],
     "rectrxm_noalloc": [ch, ch, ch, dbl, ch, i64, i64, vp, i64, vp, i64, i64, vp],
 }
+
+# the QR-family routines also exist for complex element types ('z' ComplexF64, 'c' ComplexF32)
+QR_FAMILY = ("larfg", "geqr2", "larft", "geqrt", "larfb", "unmqr", "tsqrt", "tsmqr", "parfb", "ttqrt", "ttmqr",
+             "pamm", "pemv")
+
+
+class ZComplex(ctypes.Structure):
+    """nla_zcomplex: a ComplexF64 passed by value."""
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class CComplex(ctypes.Structure):
+    """nla_ccomplex: a ComplexF32 passed by value."""
+    _fields_ = [("re", ctypes.c_float), ("im", ctypes.c_float)]
+
 
 _lib = None
 
@@ -77,7 +96,8 @@ def load():
     lib.nla_tile_qr_schedule.argtypes = [i64, i64, i64, ctypes.POINTER(ctypes.c_int32), i64]
     lib.nla_tile_qr_schedule.restype = i64
     for name, sig in _SIGS.items():
-        for p, real in (("d", dbl), ("s", flt)):
+        kinds = (("d", dbl), ("s", flt)) + ((("z", ZComplex), ("c", CComplex)) if name in QR_FAMILY else ())
+        for p, real in kinds:
             f = getattr(lib, f"nla_{p}{name}")
             f.argtypes = [real if a == "R" else a for a in sig]
             f.restype = ctypes.c_int

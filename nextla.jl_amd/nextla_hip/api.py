@@ -15,7 +15,8 @@ from ._lib import ArgumentError, call, load
 __all__ = [
     "ArgumentError", "colmajor", "empty_cm", "zeros_cm", "to_device", "to_host",
     "larfg_", "larfg", "geqr2_", "geqr2", "larft_", "larft", "larfb_", "larfb", "geqrt_", "geqrt",
-    "unmqr_", "unmqr", "tsqrt_", "tsqrt", "tsmqr_", "tsmqr", "parfb_",
+    "unmqr_", "unmqr", "tsqrt_", "tsqrt", "tsmqr_", "tsmqr", "parfb_", "parfb", "ttqrt_", "ttqrt", "ttmqr_",
+    "ttmqr", "pamm_", "pemv_", "pemv",
     "unified_rectrxm_", "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_", "trsm", "trmm",
     "GEMM_ADD_", "GEMM_SUB_", "tile_geqrf", "tile_ormqr", "tile_geqrf_mixed", "tile_gels", "rectrsm_mixed_",
@@ -74,7 +75,30 @@ def _p(*ts):
         return "d"
     if dt == torch.float32:
         return "s"
-    raise ArgumentError(f"unsupported element type {dt} (Float64 and Float32 are implemented)")
+    if dt == torch.complex128:
+        return "z"
+    if dt == torch.complex64:
+        return "c"
+    raise ArgumentError(f"unsupported element type {dt} (Float64, Float32, ComplexF64, ComplexF32)")
+
+
+def _real_only(p, what):
+    if p in "zc":
+        raise ArgumentError(f"{what} is implemented for Float64 and Float32 (the reference's unified_rec "
+                            f"requires T <: AbstractFloat, src/rectrxm.jl:101)")
+
+
+def _scalar(p, v):
+    """A scalar argument passed by value for element type prefix p."""
+    from ._lib import CComplex, ZComplex
+
+    if p == "z":
+        v = complex(v)
+        return ZComplex(v.real, v.imag)
+    if p == "c":
+        v = complex(v)
+        return CComplex(v.real, v.imag)
+    return float(v)
 
 
 def _m(t):
@@ -108,6 +132,73 @@ def larfg(alpha, x, incx=1):
     t = torch.zeros(1, dtype=x.dtype, device=x.device)
     larfg_(x.numel() // max(incx, 1) + 1 if incx > 0 else 1, a, x, incx, t)
     return a.item(), t.item()
+
+
+def ttqrt_(m, n, ib, A1, A2, T, tau, work=None):
+    """ttqrt!(m, n, ib, A1, A2, T, tau, work) (src/ttqrt.jl:81-118)."""
+    p = _p(A1, A2, T, tau)
+    p1, l1 = _m(A1)
+    p2, l2 = _m(A2)
+    pt, ldt = _m(T)
+    call(p, "ttqrt", m, n, ib, p1, l1, p2, l2, pt, ldt, tau.data_ptr(), _ptr(work), _stream())
+
+
+def ttqrt(A1, A2, T):
+    """ttqrt!(A, B, T) convenience form (src/ttqrt.jl:133-144): returns (A1, A2, T, tau)."""
+    m, n = A1.shape
+    if A2.shape[1] != n:
+        raise ArgumentError(f"A and B must have the same number of columns: got {n} and {A2.shape[1]}")
+    tau = torch.zeros(min(m, n), dtype=A1.dtype, device=A1.device)
+    ttqrt_(m, n, T.shape[0], A1, A2, T, tau)
+    return A1, A2, T, tau
+
+
+def ttmqr_(side, trans, m1, n1, m2, n2, k, ib, A1, A2, V, T, work=None):
+    """ttmqr!(side, trans, m1, n1, m2, n2, k, ib, A1, A2, V, T, work) (src/ttmqr.jl:1-95)."""
+    p = _p(A1, A2, V, T)
+    p1, l1 = _m(A1)
+    p2, l2 = _m(A2)
+    pv, ldv = _m(V)
+    pt, ldt = _m(T)
+    call(p, "ttmqr", _c(side), _c(trans), m1, n1, m2, n2, k, ib, p1, l1, p2, l2, pv, ldv, pt, ldt, _ptr(work),
+         _stream())
+
+
+def ttmqr(side, trans, A1, A2, V, T):
+    """ttmqr!(side, trans, A1, A2, V, T) convenience form (src/ttmqr.jl:114-134): ib = size(T,1),
+    k = min(size(T,2), side == 'L' ? m1 : n1, size(V,2))."""
+    m1, n1 = A1.shape
+    m2, n2 = A2.shape
+    ib = T.shape[0]
+    k = min(T.shape[1], m1 if side == "L" else n1, V.shape[1])
+    ttmqr_(side, trans, m1, n1, m2, n2, k, ib, A1, A2, V, T)
+    return A1, A2
+
+
+def pamm_(op, side, storev, direct, m, n, k, l, A1, A2, V, W):
+    """pamm!(op, side, storev, direct, m, n, k, l, A1, A2, V, W) (src/pamm.jl:49-100)."""
+    p = _p(A1, A2, V, W)
+    p1, l1 = _m(A1)
+    p2, l2 = _m(A2)
+    pv, ldv = _m(V)
+    pw, ldw = _m(W)
+    call(p, "pamm", _c(op), _c(side), _c(storev), _c(direct), m, n, k, l, p1, l1, p2, l2, pv, ldv, pw, ldw,
+         _stream())
+
+
+def pemv_(trans, storev, m, n, l, alpha, A, x, beta, y, work=None):
+    """pemv!(trans, storev, m, n, l, alpha, A, X, beta, Y, work) (src/pemv.jl:39-181)."""
+    p = _p(A, x, y)
+    pa, lda = _m(A)
+    call(p, "pemv", _c(trans), _c(storev), m, n, l, _scalar(p, alpha), pa, lda, x.data_ptr(), _scalar(p, beta),
+         y.data_ptr(), _ptr(work), _stream())
+
+
+def pemv(trans, storev, alpha, A, x, beta, y):
+    """pemv!(trans, storev, alpha, A, X, beta, Y) convenience form (src/pemv.jl:211-224): l = min(m, n)."""
+    m, n = A.shape
+    pemv_(trans, storev, m, n, min(m, n), alpha, A, x, beta, y)
+    return y
 
 
 def geqr2_(m, n, A, tau, work=None):
@@ -268,11 +359,21 @@ def parfb_(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, A2, V, T, work
          ldt, _ptr(work), 0, _stream())
 
 
+def parfb(side, trans, direct, storev, A1, A2, V, T):
+    """parfb!(side, trans, direct, storev, A1, A2, V, T) convenience form (src/parfb.jl:222-238):
+    k = size(T,1), l = size(V,2)."""
+    m1, n1 = A1.shape
+    m2, n2 = A2.shape
+    parfb_(side, trans, direct, storev, m1, n1, m2, n2, T.shape[0], V.shape[1], A1, A2, V, T)
+    return A1, A2
+
+
 # --------------------------------------------------------------------- TRSM / TRMM / GEMM
 def unified_rectrxm_(side, uplo, transpose, alpha, func, A, B, cutoff=0):
     """unified_rectrxm!(side, uplo, transpose, alpha, func, A, B) (src/rectrxm.jl:43-76).
     cutoff = recursion threshold (0: the reference's 256 for 'S', 16 for 'M')."""
     p = _p(A, B)
+    _real_only(p, "unified_rectrxm!")
     pa, lda = _m(A)
     pb, ldb = _m(B)
     n = A.shape[0]
