@@ -247,6 +247,37 @@ def qr_solve_equal_error(torch, nl, gen, n=16384, kappa=1e2, nb=128, ib=32):
     return out
 
 
+def lu_factor(torch, nl, gen, n=8192):
+    """Recursive LU with partial pivoting (getrf2!, src/lu.jl:185-300) of an n x n N(0,1)
+    matrix, Float64: TFLOP/s with the LAPACK count 2n^3/3 and the residual ||P A - L U|| / ||A||
+    estimated on 4 random vectors."""
+    A0 = nl.colmajor(torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64))
+    A = nl.colmajor(torch.empty_like(A0))
+    ipiv = torch.zeros(n, dtype=torch.int64, device="cuda")
+    holder = {}
+
+    def run():
+        A.copy_(A0)
+        holder["r"] = nl.getrf2_(A, ipiv)
+    t = time_steps(torch, run, 3, 1)
+    # residual on random vectors: ||P A x - L (U x)|| / (||A|| ||x||)
+    X = torch.randn((n, 4), generator=gen, device="cuda", dtype=torch.float64)
+    U = torch.triu(A)
+    L = torch.tril(A, -1) + torch.eye(n, device="cuda", dtype=torch.float64)
+    piv = ipiv.cpu().numpy() - 1
+    import numpy as np
+    p = np.arange(n)
+    for i in range(n):
+        p[[i, piv[i]]] = p[[piv[i], i]]
+    PAx = A0[torch.from_numpy(p).cuda()] @ X
+    res = (torch.linalg.norm(PAx - L @ (U @ X)) / (torch.linalg.norm(A0) * torch.linalg.norm(X))).item()
+    flops = 2.0 / 3.0 * n ** 3
+    log(f"[bench] LU getrf2 {n}^2 f64: {t*1e3:.2f} ms, {flops / t / 1e12:.2f} TFLOP/s, residual {res:.2e}")
+    return {"n": n, "ms": round(t * 1e3, 3), "tflops": round(flops / t / 1e12, 3),
+            "frac_of_f64_peak": round(flops / t / 1e12 / PEAK["f64"], 4), "residual": res,
+            "flops": "2n^3/3 (LAPACK count)", "info": holder["r"][2]}
+
+
 def batched(torch, nl, gen, count=4096, nrhs=32):
     """Batched uneven workload: `count` Float32 problems, m uniform over multiples of 32 in
     [64, 1024], n uniform over multiples of 16 in [16, min(128, m)]; geqrt! of each A and a
@@ -356,8 +387,8 @@ def main():
     ap.add_argument("--cutoff", type=int, default=256)
     ap.add_argument("--qr-n", type=int, default=16384)
     ap.add_argument("--ls", action="store_true", help="also run the 32768 x 8192 least-squares workload")
-    ap.add_argument("--extras", default="trsm32,trmm,trsm_mixed,qr,qr_solve,batched,adjoints,ls",
-                    help="comma-separated extra workloads: trsm32, trmm, trsm_mixed, qr, qr_solve, batched, "
+    ap.add_argument("--extras", default="trsm32,trmm,trsm_mixed,qr,qr_solve,lu,batched,adjoints,ls",
+                    help="comma-separated extra workloads: trsm32, trmm, trsm_mixed, qr, qr_solve, lu, batched, "
                          "adjoints, ls")
     ap.add_argument("--qr-no-check", action="store_true", help="skip the tile-QR backward-error check")
     ap.add_argument("--no-cpu", action="store_true")
@@ -516,6 +547,8 @@ def main():
 
     if "qr_solve" in chosen:
         extras["tile_qr_solve_equal_error"] = qr_solve_equal_error(torch, nl, gen, n=args.qr_n)
+    if "lu" in chosen:
+        extras["lu"] = lu_factor(torch, nl, gen)
     if "batched" in chosen:
         extras["batched"] = batched(torch, nl, gen)
     if "adjoints" in chosen:

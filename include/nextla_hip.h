@@ -273,6 +273,22 @@ int nla_dgemm_acc(int64_t N, int64_t R, int64_t M, double alpha, const double *X
 int nla_sgemm_acc(int64_t N, int64_t R, int64_t M, float alpha, const float *X, int64_t ldx,
                   const float *Y, int64_t ldy, float *C, int64_t ldc, void *stream);
 
+/* ---- Recursive LU (src/lu.jl) --------------------------------------------------------- */
+/* getrf2!(A, ipiv, info): src/lu.jl:185-300 (LAPACK xgetrf2's recursion, partial pivoting).
+ * A (m x n) is overwritten with L (unit lower, below the diagonal) and U; ipiv (DEVICE,
+ * min(m,n) entries) receives the 1-based pivot rows; info (DEVICE, one entry) receives the
+ * first i with U(i,i) exactly zero (1-based), 0 if none.  The off-diagonal updates run on
+ * nla_?rectrxm's TRSM and the MFMA GEMM. */
+int nla_dgetrf2(int64_t m, int64_t n, double *A, int64_t lda, int64_t *ipiv, int64_t *info, void *stream);
+int nla_sgetrf2(int64_t m, int64_t n, float *A, int64_t lda, int64_t *ipiv, int64_t *info, void *stream);
+/* laswp(A, first, last, ipiv, incx): src/lu.jl:470-522.  Rows first..last (1-based) of the n
+ * columns of A interchanged with rows ipiv(k) (DEVICE, 1-based), in order for incx > 0 and in
+ * reverse for incx < 0; incx = 0 does nothing. */
+int nla_dlaswp(int64_t n, double *A, int64_t lda, int64_t k1, int64_t k2, const int64_t *ipiv, int64_t incx,
+               void *stream);
+int nla_slaswp(int64_t n, float *A, int64_t lda, int64_t k1, int64_t k2, const int64_t *ipiv, int64_t incx,
+               void *stream);
+
 /* ---- Tile-QR driver (no reference function: the composition of geqrt!/unmqr!/tsqrt!/
  * tsmqr! that config "tile QR" describes, flat-tree PLASMA order) ------------------------- */
 /* Factor A (m x n, both multiples of nb) in nb x nb tiles with inner block ib.

@@ -44,6 +44,8 @@ _SIGS = {
     "tile_gels": [i64, i64, i64, i64, vp, i64, vp, vp, ch, i64, dbl, vp, vp, vp, vp],
     "batched_qr_trsm": [i64, vp, vp, ctypes.c_int, vp],
     "batched_autotune": [i64, vp, vp, i64, vp, i64, vp, vp, vp],
+    "getrf2": [i64, i64, vp, i64, vp, vp, vp],
+    "laswp": [i64, vp, i64, i64, i64, vp, i64, vp],
 }
 # Float64-only entry points
 _SIGS_D = {

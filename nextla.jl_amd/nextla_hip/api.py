@@ -20,6 +20,7 @@ __all__ = [
     "unified_rectrxm_", "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_", "trsm", "trmm",
     "GEMM_ADD_", "GEMM_SUB_", "tile_geqrf", "tile_ormqr", "tile_geqrf_mixed", "tile_gels", "rectrsm_mixed_",
+    "getrf2_", "lu_", "laswp_",
     "NextLAMatrix",
     "synchronize",
 ]
@@ -481,6 +482,36 @@ def GEMM_SUB_(A, B, C):
     pb, ldb = _m(B)
     pc, ldc = _m(C)
     call(p, "gemm_acc", A.shape[0], C.shape[0], A.shape[1], -1.0, pb, ldb, pc, ldc, pa, lda, _stream())
+    return A
+
+
+# --------------------------------------------------------------------------------- LU
+def getrf2_(A, ipiv):
+    """getrf2!(A, ipiv, info) (src/lu.jl:185-300): A overwritten with L and U, ipiv (int64
+    device vector of min(m,n)) with the 1-based pivot rows.  Returns (A, ipiv, info)."""
+    p = _p(A)
+    _real_only(p, "getrf2!")
+    m, n = A.shape
+    if ipiv.dtype != torch.int64 or not ipiv.is_cuda or ipiv.numel() < min(m, n):
+        raise ArgumentError("ipiv must be an int64 device vector of length min(m, n)")
+    pa, lda = _m(A)
+    info = torch.zeros(1, dtype=torch.int64, device=A.device)
+    call(p, "getrf2", m, n, pa, lda, ipiv.data_ptr(), info.data_ptr(), _stream())
+    return A, ipiv, int(info.item())
+
+
+def lu_(A):
+    """lu!(A, RowMaximum()) (src/lu.jl:48-58): getrf2! with a fresh ipiv; returns (A, ipiv, info)."""
+    ipiv = torch.zeros(max(1, min(A.shape)), dtype=torch.int64, device=A.device)
+    return getrf2_(A, ipiv)
+
+
+def laswp_(A, first, last, ipiv, incx=1):
+    """laswp(A, first, last, ipiv, incx) (src/lu.jl:470-522); ipiv 1-based int64 on the device."""
+    p = _p(A)
+    _real_only(p, "laswp")
+    pa, lda = _m(A)
+    call(p, "laswp", A.shape[1], pa, lda, int(first), int(last), ipiv.data_ptr(), int(incx), _stream())
     return A
 
 

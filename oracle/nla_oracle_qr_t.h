@@ -531,5 +531,82 @@ void P(pemv)(char trans, char storev, int64_t m, int64_t n, int64_t l, const R *
     }
 }
 
+/* ------------------------------------------------------------------------------------------
+ * Recursive LU with partial pivoting (src/lu.jl:185-300, LAPACK xgetrf2) and the row
+ * interchanges it applies (laswp, src/lu.jl:470-522, LAPACK xlaswp).  ipiv is 1-based as in
+ * the reference; info = the first i with U(i,i) exactly zero (1-based), 0 otherwise. */
+
+/* laswp(A, k1, k2, ipiv, incx): rows k1..k2 (1-based) interchanged with ipiv(k) in order
+ * (incx > 0) or in reverse (incx < 0); incx = 0 does nothing. */
+void P(laswp)(int64_t n, R *A, int64_t lda, int64_t k1, int64_t k2, const int64_t *ipiv, int64_t incx) {
+    if (incx == 0) return;
+    for (int64_t c = 0; c < n; ++c) {
+        if (incx > 0) {
+            for (int64_t i = k1; i <= k2; ++i) {
+                int64_t ip = ipiv[i - 1];
+                if (ip != i) { R t = QAT(A, lda, i - 1, c); QAT(A, lda, i - 1, c) = QAT(A, lda, ip - 1, c); QAT(A, lda, ip - 1, c) = t; }
+            }
+        } else {
+            for (int64_t i = k2; i >= k1; --i) {
+                int64_t ip = ipiv[i - 1];
+                if (ip != i) { R t = QAT(A, lda, i - 1, c); QAT(A, lda, i - 1, c) = QAT(A, lda, ip - 1, c); QAT(A, lda, ip - 1, c) = t; }
+            }
+        }
+    }
+}
+
+static RR P(q_cabs1)(R x) { RR a = RE(x), b = IM(x); return (a < 0 ? -a : a) + (b < 0 ? -b : b); }
+
+/* getrf2!: src/lu.jl:185-300.  Returns info. */
+int64_t P(getrf2)(int64_t m, int64_t n, R *A, int64_t lda, int64_t *ipiv) {
+    if (m <= 0 || n <= 0) return 0;
+    if (m == 1) {
+        ipiv[0] = 1;
+        return (RE(QAT(A, lda, 0, 0)) == 0 && IM(QAT(A, lda, 0, 0)) == 0) ? 1 : 0;
+    }
+    if (n == 1) {
+        RR sfmin = (sizeof(RR) == 8) ? (RR)DBL_MIN : (RR)FLT_MIN;
+        int64_t p = 0;
+        RR dmax = P(q_cabs1)(QAT(A, lda, 0, 0));
+        for (int64_t i = 1; i < m; ++i)
+            if (P(q_cabs1)(QAT(A, lda, i, 0)) > dmax) { p = i; dmax = P(q_cabs1)(QAT(A, lda, i, 0)); }
+        ipiv[0] = p + 1;
+        if (RE(QAT(A, lda, p, 0)) == 0 && IM(QAT(A, lda, p, 0)) == 0) return 1;
+        if (p != 0) { R t = QAT(A, lda, 0, 0); QAT(A, lda, 0, 0) = QAT(A, lda, p, 0); QAT(A, lda, p, 0) = t; }
+        R d = QAT(A, lda, 0, 0);
+        if (P(q_abs)(d) >= sfmin) {
+            R r = (R)1 / d;
+            for (int64_t i = 1; i < m; ++i) QAT(A, lda, i, 0) *= r;
+        } else {
+            for (int64_t i = 1; i < m; ++i) QAT(A, lda, i, 0) /= d;
+        }
+        return 0;
+    }
+    int64_t mn = m < n ? m : n;
+    int64_t n1 = mn / 2, n2 = n - n1;
+    int64_t info = P(getrf2)(m, n1, A, lda, ipiv);
+    /* interchanges on [A12; A22] */
+    P(laswp)(n2, &QAT(A, lda, 0, n1), lda, 1, n1, ipiv, 1);
+    /* A12 = L11^-1 A12 (L11 unit lower) */
+    for (int64_t c = 0; c < n2; ++c)
+        for (int64_t i = 0; i < n1; ++i) {
+            R acc = QAT(A, lda, i, n1 + c);
+            for (int64_t q = 0; q < i; ++q) acc -= QAT(A, lda, i, q) * QAT(A, lda, q, n1 + c);
+            QAT(A, lda, i, n1 + c) = acc;
+        }
+    /* A22 -= A21 A12 */
+    for (int64_t c = 0; c < n2; ++c)
+        for (int64_t i = n1; i < m; ++i) {
+            R acc = 0;
+            for (int64_t q = 0; q < n1; ++q) acc += QAT(A, lda, i, q) * QAT(A, lda, q, n1 + c);
+            QAT(A, lda, i, n1 + c) -= acc;
+        }
+    int64_t iinfo = P(getrf2)(m - n1, n2, &QAT(A, lda, n1, n1), lda, ipiv + n1);
+    if (info == 0 && iinfo > 0) info = iinfo + n1;
+    for (int64_t i = n1; i < mn; ++i) ipiv[i] += n1;
+    P(laswp)(n1, A, lda, n1 + 1, mn, ipiv, 1);
+    return info;
+}
+
 #undef QAT
 #undef ABS2

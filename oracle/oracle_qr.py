@@ -30,6 +30,8 @@ _SIGS = {
     "tsqrt": [i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, vp, vp],
     "ttqrt": [i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, vp],
     "pemv": [ch, ch, i64, i64, i64, vp, vp, i64, vp, vp, vp],
+    "laswp": [i64, vp, i64, i64, i64, vp, i64],
+    "getrf2": [i64, i64, vp, i64, vp],
 }
 _done = False
 
@@ -42,7 +44,7 @@ def lib():
             for name, sig in _SIGS.items():
                 f = getattr(L, f"nlaq_{p}{name}")
                 f.argtypes = sig
-                f.restype = None
+                f.restype = i64 if name == "getrf2" else None
         _done = True
     return L
 
@@ -154,3 +156,17 @@ def pemv(trans, storev, l, alpha, A, x, beta, y):
     a = np.array([alpha], dtype=A.dtype)
     b = np.array([beta], dtype=A.dtype)
     _f(A, "pemv")(_c(trans), _c(storev), m, n, l, _ptr(a), _ptr(A), _ld(A), _ptr(x), _ptr(b), _ptr(y))
+
+
+def laswp(A, k1, k2, ipiv, incx=1):
+    """laswp(A, first, last, ipiv, incx) (src/lu.jl:470-522); ipiv 1-based int64."""
+    ip = np.ascontiguousarray(ipiv, dtype=np.int64)
+    _f(A, "laswp")(A.shape[1], _ptr(A), _ld(A), k1, k2, ip.ctypes.data_as(vp), incx)
+
+
+def getrf2(A):
+    """getrf2! (src/lu.jl:185-300): A overwritten with L and U; returns (ipiv 1-based, info)."""
+    m, n = A.shape
+    ipiv = np.zeros(max(min(m, n), 1), dtype=np.int64)
+    info = _f(A, "getrf2")(m, n, _ptr(A), _ld(A), ipiv.ctypes.data_as(vp))
+    return ipiv[: min(m, n)], int(info)

@@ -326,3 +326,36 @@ def test_real_instances_match_real_oracle(dt):
     Q.tsqrt(b1a, c1a, Ta)
     O.tsqrt(b1b, c1b, Tb)
     assert rel(b1a, b1b) < TOL[dt] and rel(c1a, c1b) < TOL[dt] and rel(Ta, Tb) < TOL[dt]
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 5), (7, 1), (10, 10), (64, 40), (40, 64), (129, 129)])
+def test_getrf2_vs_lapack(dt, m, n):
+    """getrf2! (src/lu.jl:185-300) against LAPACK xgetrf: same pivots, same factors."""
+    rng = np.random.default_rng(m * 100 + n)
+    A = rnd(rng, (m, n), dt)
+    lu, piv, info = getattr(lapack, PRE[dt] + "getrf")(A.copy())
+    Ao = A.copy(order="F")
+    ipiv, info2 = Q.getrf2(Ao)
+    assert np.array_equal(ipiv, piv + 1) and info2 == info == 0
+    assert rel(Ao, lu) < 100 * TOL[dt]
+
+
+def test_getrf2_singular_info_and_laswp():
+    A = np.asfortranarray(np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [1.0, 1.0, 1.0]]))
+    lu, piv, info = lapack.dgetrf(A.copy())
+    Ao = A.copy(order="F")
+    ipiv, info2 = Q.getrf2(Ao)
+    assert info == info2 == 3 and np.array_equal(ipiv, piv + 1)
+    # laswp forward then backward is the identity
+    rng = np.random.default_rng(2)
+    B = F(rng.standard_normal((6, 4)))
+    ip = np.array([3, 5, 3, 6, 5, 6], dtype=np.int64)
+    Bo = B.copy(order="F")
+    Q.laswp(Bo, 1, 6, ip, 1)
+    want = B.copy()
+    for i in range(6):
+        want[[i, ip[i] - 1]] = want[[ip[i] - 1, i]]
+    assert np.array_equal(Bo, want)
+    Q.laswp(Bo, 1, 6, ip, -1)
+    assert np.array_equal(Bo, B)
