@@ -252,6 +252,15 @@ int nla_srectrxm(char side, char uplo, char trans, float alpha, char func, int64
 int nla_drectrxm_noalloc(char side, char uplo, char trans, double alpha, char func, int64_t n, int64_t m,
                          const double *A, int64_t lda, double *B, int64_t ldb, int64_t cutoff, void *stream);
 
+/* Host-pointer forms (NextLAMatrix wraps a host Matrix, src/NextLAMatrix.jl:4-32; the corrected
+ * 4-Char BLAS.trsm!/trmm! forwards of src/NextLAMatrix.jl:27-28 call these): A and B are HOST
+ * column-major arrays; they are copied to the device, solved / multiplied as nla_?rectrxm and
+ * B is copied back.  Synchronises the stream. */
+int nla_hdrectrxm(char side, char uplo, char trans, double alpha, char func, int64_t n, int64_t m,
+                  const double *A, int64_t lda, double *B, int64_t ldb, int64_t cutoff, void *stream);
+int nla_hsrectrxm(char side, char uplo, char trans, float alpha, char func, int64_t n, int64_t m,
+                  const float *A, int64_t lda, float *B, int64_t ldb, int64_t cutoff, void *stream);
+
 /* Base-case kernels LeftLowerTRSM!/LeftUpperTRSM!/RightLowerTRSM!/RightUpperTRSM!
  * (src/trsm.jl:128-150) and the TRMM counterparts (src/trmm.jl:332-389): one diagonal
  * block, B n x m (side 'L') or m x n (side 'R'); n <= 1024 as in src/trsm.jl:10-12. */

@@ -47,7 +47,7 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // mutex; growth synchronises the stream first so its in-flight kernels never see the old
 // buffer freed.  Kernels of one call share the slot they asked for; calls on different
 // streams use different buffers and may overlap.
-enum ScratchSlot { SLOT_W = 0, SLOT_AUX = 1, SLOT_SCHED = 2, SLOT_MIXED = 3, SLOT_TRSM = 4, SLOT_RHS = 5, SLOT_LS = 6, SLOT_LPIMG = 7, SLOT_SHADOW = 8, SLOT_LU = 9, NUM_SLOTS = 10 };
+enum ScratchSlot { SLOT_W = 0, SLOT_AUX = 1, SLOT_SCHED = 2, SLOT_MIXED = 3, SLOT_TRSM = 4, SLOT_RHS = 5, SLOT_LS = 6, SLOT_LPIMG = 7, SLOT_SHADOW = 8, SLOT_LU = 9, SLOT_HOST = 10, NUM_SLOTS = 11 };
 void *scratch(ScratchSlot slot, size_t bytes, hipStream_t stream, int *status);
 
 // Launch-time profiler: when enabled, every kernel family brackets its launches with a

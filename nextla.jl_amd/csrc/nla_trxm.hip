@@ -854,6 +854,46 @@ NLA_API int nla_strmm_base(char side, char uplo, int64_t n, int64_t m, const flo
     return nla::base_entry<float>('M', side, uplo, n, m, A, lda, B, ldb, stream);
 }
 
+namespace nla {
+// ------------------------------------------------------------------------------------
+// Host-pointer variant (NextLAMatrix wraps a host Matrix, src/NextLAMatrix.jl:4-32): A and B
+// are copied into the stream's scratch, solved / multiplied there, and B is copied back.
+template <typename T>
+static int h_rectrxm_entry(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
+                           i64 ldb, i64 cutoff, void *stream) {
+    side = up(side);
+    if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);
+    if (n < 0) return arg_error(6, "n must be non-negative, got %lld", (long long)n);
+    if (m < 0) return arg_error(7, "m must be non-negative, got %lld", (long long)m);
+    if (n == 0 || m == 0) return 0;
+    if (lda < n) return arg_error(9, "lda must be >= n");
+    const i64 rows = side == 'L' ? n : m, cols = side == 'L' ? m : n;
+    if (ldb < rows) return arg_error(11, "ldb must be >= %lld", (long long)rows);
+    if (!A || !B) return arg_error(8, "A and B must be host pointers");
+    hipStream_t s = as_stream(stream);
+    int st = 0;
+    T *dA = (T *)scratch(SLOT_HOST, sizeof(T) * (size_t)(n * n + rows * cols), s, &st);
+    if (!dA) return st;
+    T *dB = dA + n * n;
+    NLA_HIP(hipMemcpy2DAsync(dA, sizeof(T) * n, A, sizeof(T) * lda, sizeof(T) * n, n, hipMemcpyHostToDevice, s));
+    NLA_HIP(hipMemcpy2DAsync(dB, sizeof(T) * rows, B, sizeof(T) * ldb, sizeof(T) * rows, cols, hipMemcpyHostToDevice, s));
+    NLA_ST(rectrxm_entry<T>(side, uplo, trans, alpha, func, n, m, dA, n, dB, rows, cutoff, stream));
+    NLA_HIP(hipMemcpy2DAsync(B, sizeof(T) * ldb, dB, sizeof(T) * rows, sizeof(T) * rows, cols, hipMemcpyDeviceToHost, s));
+    NLA_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // namespace nla
+
+NLA_API int nla_hdrectrxm(char side, char uplo, char trans, double alpha, char func, int64_t n, int64_t m,
+                          const double *A, int64_t lda, double *B, int64_t ldb, int64_t cutoff, void *stream) {
+    return nla::h_rectrxm_entry<double>(side, uplo, trans, alpha, func, n, m, A, lda, B, ldb, cutoff, stream);
+}
+NLA_API int nla_hsrectrxm(char side, char uplo, char trans, float alpha, char func, int64_t n, int64_t m,
+                          const float *A, int64_t lda, float *B, int64_t ldb, int64_t cutoff, void *stream) {
+    return nla::h_rectrxm_entry<float>(side, uplo, trans, alpha, func, n, m, A, lda, B, ldb, cutoff, stream);
+}
+
 // ------------------------------------------------------------------------------------
 // No-allocation variant of nla_drectrxm for callers that cannot allocate device memory
 // (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Its

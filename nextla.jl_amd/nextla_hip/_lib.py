@@ -103,6 +103,10 @@ def load():
             f = getattr(lib, f"nla_{p}{name}")
             f.argtypes = [real if a == "R" else a for a in sig]
             f.restype = ctypes.c_int
+    for p, real in (("d", dbl), ("s", flt)):
+        f = getattr(lib, f"nla_h{p}rectrxm")
+        f.argtypes = [ch, ch, ch, real, ch, i64, i64, vp, i64, vp, i64, i64, vp]
+        f.restype = ctypes.c_int
     for name, sig in _SIGS_D.items():
         f = getattr(lib, f"nla_d{name}")
         f.argtypes = sig

@@ -631,11 +631,21 @@ class NextLAMatrix:
         if diag == "U":
             a = a.copy(order="F")
             np.fill_diagonal(a, 1.0)
-        bt = B.data if isinstance(B, NextLAMatrix) else B
-        dA = to_device(a)
-        dB = to_device(bt)
-        unified_rectrxm_(side, uplo, transa, alpha, func, dA, dB)
-        out = to_host(dB)
+        b = B.data if isinstance(B, NextLAMatrix) else np.asarray(B)
+        dt = np.result_type(a.dtype, b.dtype)
+        if dt not in (np.float64, np.float32):
+            raise ArgumentError(f"unsupported element type {dt} (Float64 and Float32 are implemented)")
+        a = np.asfortranarray(a, dtype=dt)
+        out = np.asfortranarray(b, dtype=dt).copy(order="F")
+        n = a.shape[0]
+        m = out.shape[1] if side == "L" else out.shape[0]
+        p = "d" if dt == np.float64 else "s"
+        # host-pointer entry point: copies in, computes on the GPU, copies out, synchronises
+        st = getattr(load(), f"nla_h{p}rectrxm")(_c(side), _c(uplo), _c(transa), float(alpha), _c(func), n, m,
+                                                 a.ctypes.data, max(1, a.shape[0]), out.ctypes.data,
+                                                 max(1, out.shape[0]), 0, _stream())
+        from ._lib import check
+        check(st, f"nla_h{p}rectrxm")
         if isinstance(B, NextLAMatrix):
             B.data[...] = out
             return B

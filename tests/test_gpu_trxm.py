@@ -157,3 +157,37 @@ def test_trsm_fused_two_block_base_vs_oracle(dt, n, m):
     O.rectrxm("L", "L", "N", 1.5, "S", A, B, threshold=256)
     tol = 1e-12 if dt == np.float64 else 1e-4
     assert relerr(host(dB), B) < tol
+
+
+@pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("side", "LR")
+@pytest.mark.parametrize("uplo", "LU")
+@pytest.mark.parametrize("func", "SM")
+def test_host_pointer_rectrxm(dt, side, uplo, func):
+    """nla_h?rectrxm (NextLAMatrix's host-matrix path): host A and B with leading dimensions
+    larger than their rows, against the oracle."""
+    import ctypes
+
+    rng = np.random.default_rng(hash((side, uplo, func)) % 991)
+    n, m = 300, 37
+    Abig = F(np.zeros((n + 5, n), dtype=dt))
+    Abig[:n, :] = tri_matrix(rng, n, uplo, dt)
+    rows, cols = (n, m) if side == "L" else (m, n)
+    Bbig = F(rng.standard_normal((rows + 3, cols)).astype(dt))
+    B0 = Bbig[:rows, :].copy(order="F")
+    lib = nl.load()
+    st = getattr(lib, f"nla_h{P[dt]}rectrxm")(side.encode(), uplo.encode(), b"T", 0.5, func.encode(), n, m,
+                                             Abig.ctypes.data, n + 5, Bbig.ctypes.data, rows + 3, 64,
+                                             ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert st == 0, lib.nla_last_error()
+    want = B0.copy(order="F")
+    O.rectrxm(side, uplo, "T", 0.5, func, F(Abig[:n, :]), want, threshold=64)
+    assert relerr(Bbig[:rows, :], want) < (1e-12 if dt == np.float64 else 1e-4)
+    # NextLAMatrix's 4-Char forward with a unit diagonal
+    Bm = nl.NextLAMatrix(B0.copy())
+    Au = F(Abig[:n, :].copy())
+    getattr(nl.NextLAMatrix, "trsm" if func == "S" else "trmm")(side, uplo, "N", "U", 1.0, nl.NextLAMatrix(Au), Bm)
+    np.fill_diagonal(Au, 1.0)
+    want2 = B0.copy(order="F")
+    O.rectrxm(side, uplo, "N", 1.0, func, Au, want2)
+    assert relerr(Bm.data, want2) < (1e-12 if dt == np.float64 else 1e-4)
