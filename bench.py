@@ -171,6 +171,32 @@ def cpu_baseline(n, cut, target_s=2.5):
             ]}
 
 
+def cutoff_sweep(torch, nl, n, m, funcs="SM", dtypes=("f64", "f32"), cuts=(16, 32, 64, 128, 256, 512)):
+    """TFLOP/s (n^2 m) of unified_rectrxm! L/L/N, TRSM and TRMM, Float64 and Float32, over the
+    recursion cut-off (src/rectrxm.jl:52,63), one JSON line."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(8192)
+    A64 = make_tri(torch, nl, n, torch.float64, gen)
+    B64 = nl.colmajor(torch.randn((m, n), generator=gen, device="cuda", dtype=torch.float64).t())
+    out = {"n": n, "m": m, "tflops": {}}
+    for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+        if tag not in dtypes:
+            continue
+        A = nl.colmajor(A64.to(dt))
+        B0 = nl.colmajor(B64.to(dt))
+        B = nl.colmajor(torch.empty_like(B0))
+        for func in funcs:
+            for cut in cuts:
+                def step():
+                    B.copy_(B0)
+                    nl.unified_rectrxm_("L", "L", "N", 1.0, func, A, B, cutoff=cut)
+                t = time_steps(torch, step, 5, 2)
+                key = f"{'trsm' if func == 'S' else 'trmm'}_{tag}_cut{cut}"
+                out["tflops"][key] = round(float(n) * n * m / t / 1e12, 3)
+                log(f"[sweep] {key}: {out['tflops'][key]} TFLOP/s ({t*1e3:.3f} ms incl. the copy of B)")
+    print(json.dumps(out), flush=True)
+
+
 def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
     """Mixed-precision tile QR + least squares: A = U diag(s) V^T (U, V from QR of Gaussian,
     s geometric from 1 to 1/kappa), b = A x0 + 1e-8 N(0,1); tile QR with Float64 / Float32 /
@@ -393,6 +419,11 @@ def main():
     ap.add_argument("--qr-no-check", action="store_true", help="skip the tile-QR backward-error check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--cutoff-sweep", action="store_true",
+                    help="only time TRSM and TRMM (Float64, Float32) at n x m over cut-offs 16..512 and print JSON")
+    ap.add_argument("--sweep-funcs", default="SM", help="functions of the cut-off sweep: S and/or M")
+    ap.add_argument("--sweep-dtypes", default="f64,f32", help="element types of the cut-off sweep")
+    ap.add_argument("--sweep-cuts", default="16,32,64,128,256,512", help="cut-offs of the sweep")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -405,6 +436,10 @@ def main():
 
     nl.load()
     n, m, cut = args.n, args.m, args.cutoff
+    if args.cutoff_sweep:
+        cutoff_sweep(torch, nl, n, m, args.sweep_funcs, args.sweep_dtypes.split(","),
+                     [int(c) for c in args.sweep_cuts.split(",")])
+        return
     gen = torch.Generator(device="cuda")
     gen.manual_seed(8192)
     A = make_tri(torch, nl, n, torch.float64, gen)
@@ -479,11 +514,14 @@ def main():
         del A32, B032, B32
 
     if "trmm" in chosen:
-        def trmm():
-            B.copy_(B0)
-            nl.unified_rectrxm_("L", "L", "N", 1.0, "M", A, B, cutoff=0)
-        tm = time_steps(torch, trmm, max(3, args.steps // 2), 1)
-        extras["trmm_f64_tflops"] = round(flops / tm / 1e12, 3)
+        for cut_m, key in ((0, "trmm_f64_tflops"), (16, "trmm_f64_tflops_cutoff16")):
+            def trmm():
+                B.copy_(B0)
+                nl.unified_rectrxm_("L", "L", "N", 1.0, "M", A, B, cutoff=cut_m)
+            tm = time_steps(torch, trmm, max(3, args.steps // 2), 1)
+            extras[key] = round(flops / tm / 1e12, 3)
+        log(f"[bench] TRMM f64: {extras['trmm_f64_tflops']} TFLOP/s (cut-off 256), "
+            f"{extras['trmm_f64_tflops_cutoff16']} (the reference's 16)")
 
     if "trsm_mixed" in chosen:
         # mixed-precision TRSM (Float32 solve + Float64 refinement), same problem

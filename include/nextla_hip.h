@@ -235,8 +235,10 @@ int nla_cpemv(char trans, char storev, int64_t m, int64_t n, int64_t l, nla_ccom
 /* unified_rectrxm!(side, uplo, trans, alpha, func, A, B): src/rectrxm.jl:43-76.
  * func 'S' = solve op(A) X = alpha B / X op(A) = alpha B, 'M' = B := alpha op(A) B / alpha B op(A).
  * n = size(A,1); m = the other dimension of B (B is n x m for side 'L', m x n for 'R').
- * cutoff = recursion threshold (src/rectrxm.jl:52,63); 0 selects the reference's
- * 256 ('S') / 16 ('M').  Only the uplo triangle of A is read. */
+ * cutoff = recursion threshold (src/rectrxm.jl:52,63); 0 selects this library's default 256
+ * for both functions (the reference's thresholds are 256 for 'S' and 16 for 'M'; 16 is still
+ * accepted).  With a cut-off in [256, 512) the base blocks of 256 rows run as MFMA triangular
+ * products.  Only the uplo triangle of A is read. */
 int nla_drectrxm(char side, char uplo, char trans, double alpha, char func, int64_t n, int64_t m,
                  const double *A, int64_t lda, double *B, int64_t ldb, int64_t cutoff, void *stream);
 int nla_srectrxm(char side, char uplo, char trans, float alpha, char func, int64_t n, int64_t m,

@@ -594,6 +594,29 @@ __global__ __launch_bounds__(512) void trsm_inv_apply_kernel(int m, const T *__r
     }
 }
 
+// The aligned IBS x IBS diagonal blocks of the left-form matrix op'(A) themselves (zero outside
+// the triangle), dense column-major: the TRMM base B_b := op'(A)_bb B_b is then the same
+// triangular MFMA product as the solve's X = Inv B (trsm_inv_apply_kernel).
+template <typename T, bool UP, bool TRA>
+__global__ void tri_blocks_kernel(int nblocks, const T *__restrict__ A, i64 lda, T *__restrict__ Out) {
+    const i64 total = (i64)nblocks * IBS * IBS;
+    for (i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x; idx < total; idx += (i64)gridDim.x * blockDim.x) {
+        const i64 b = idx / ((i64)IBS * IBS), e = idx % ((i64)IBS * IBS);
+        const int r = (int)(e % IBS), c = (int)(e / IBS);
+        const i64 o = b * IBS;
+        Out[idx] = (UP ? r <= c : r >= c) ? opa<T, TRA>(A, lda, (int)(o + r), (int)(o + c)) : T(0);
+    }
+}
+
+template <typename T>
+static hipError_t tri_blocks(bool up, bool trA, int nblocks, const T *A, i64 lda, T *Out, hipStream_t s) {
+    if (!up && !trA) tri_blocks_kernel<T, false, false><<<2048, 256, 0, s>>>(nblocks, A, lda, Out);
+    else if (!up && trA) tri_blocks_kernel<T, false, true><<<2048, 256, 0, s>>>(nblocks, A, lda, Out);
+    else if (up && !trA) tri_blocks_kernel<T, true, false><<<2048, 256, 0, s>>>(nblocks, A, lda, Out);
+    else tri_blocks_kernel<T, true, true><<<2048, 256, 0, s>>>(nblocks, A, lda, Out);
+    return hipGetLastError();
+}
+
 template <typename T> __global__ void identity_blocks_kernel(int nblocks, T *Inv) {
     const i64 total = (i64)nblocks * IBS * IBS;
     for (i64 idx = blockIdx.x * (i64)blockDim.x + threadIdx.x; idx < total; idx += (i64)gridDim.x * blockDim.x) {
@@ -658,7 +681,8 @@ static hipError_t base_case(char func, char side, char uplo, bool tr, int n, int
                             const T *Dinv, const T *Inv, i64 diag_off, hipStream_t s) {
     bool up, trA, trB;
     left_form(side, uplo, tr, &up, &trA, &trB);
-    if (func == 'S' && Inv && n == IBS && (diag_off % IBS) == 0)
+    // 'S': Inv holds the inverted IBS-blocks; 'M': the blocks themselves (same product)
+    if (Inv && n == IBS && (diag_off % IBS) == 0)
         return inv_apply_launch<T>(up, trB, m, Inv + (diag_off / IBS) * (i64)IBS * IBS, B, ldb, s);
     if (func == 'S' && Dinv && (diag_off % DB) == 0 && blk_ok<T>(n))
         return blk_launch<T>(up, trA, trB, n, m, A, lda, B, ldb, Dinv + (diag_off / DB) * DB * DB, s);
@@ -731,7 +755,8 @@ static hipError_t scale(i64 rows, i64 cols, T alpha, T *B, i64 ldb, hipStream_t 
 // Device workspace the solve needs beside A and B: the 32 x 32 inverses of every diagonal
 // block of op(A) and, with a cut-off in [IBS, 2 IBS), the IBS x IBS inverses as well.
 template <typename T> static size_t rectrxm_ws_bytes(char func, i64 n, i64 thr) {
-    if (func != 'S' || thr < DB) return 0;
+    if (func == 'M') return (thr >= IBS && thr < 2 * IBS && n >= IBS) ? sizeof(T) * (size_t)(n / IBS) * IBS * IBS : 0;
+    if (thr < DB) return 0;
     const i64 nd = (n + DB - 1) / DB * DB * DB;
     const i64 ninv = (thr >= IBS && thr < 2 * IBS) ? n / IBS : 0;
     return sizeof(T) * (size_t)(nd + ninv * IBS * IBS);
@@ -755,7 +780,9 @@ static int rectrxm_entry(char side, char uplo, char trans, T alpha, char func, i
     if (ldb < brows) return arg_error(11, "ldb must be >= %lld", (long long)brows);
     if (cutoff < 0) return arg_error(12, "cutoff must be non-negative");
     if (n > INT32_MAX || m > INT32_MAX) return arg_error(6, "dimension exceeds 2^31-1");
-    i64 thr = cutoff == 0 ? (func == 'S' ? 256 : 16) : cutoff;
+    // 0: this library's default, 256 for both (measured best at n = 8192; the reference's
+    // thresholds are 256 for 'S' and 16 for 'M', src/rectrxm.jl:52,63)
+    i64 thr = cutoff == 0 ? 256 : cutoff;
     if (thr > 1024) return arg_error(12, "cutoff must be <= 1024 (base kernels hold n <= 1024 rows, src/trsm.jl:10-12)");
     const size_t need = rectrxm_ws_bytes<T>(func, n, thr);
     if (ws && need > ws_cap)
@@ -784,6 +811,18 @@ static int rectrxm_entry(char side, char uplo, char trans, T alpha, char func, i
             NLA_HIP(inv_blocks<T>(upf, trA, (int)ninv, A, lda, D, D + nd, s));
             Inv = D + nd;
         }
+    }
+    if (func == 'M' && thr >= IBS && thr < 2 * IBS && n >= IBS) {
+        // TRMM with 256-row bases: every aligned diagonal block once, for the MFMA base product
+        const i64 nblk = n / IBS;
+        int st = 0;
+        T *D = ws ? (T *)ws : (T *)scratch(SLOT_TRSM, need, s, &st);
+        if (!D) return st;
+        bool upf, trA, trB;
+        left_form(side, uplo, tr, &upf, &trA, &trB);
+        ProfScope prof(PK_TRMM_BASE, s);
+        NLA_HIP(tri_blocks<T>(upf, trA, (int)nblk, A, lda, D, s));
+        Inv = D;
     }
     NLA_HIP(rec<T>(func, side, uplo, tr, n, m, A, lda, B, ldb, thr, Dinv, Inv, 0, s));
     if (func == 'M') NLA_HIP(scale<T>(rows, cols, alpha, B, ldb, s));
@@ -915,7 +954,7 @@ NLA_API int nla_drectrxm_noalloc(char side, char uplo, char trans, double alpha,
     // sizes first, on the host: a solve that needs more than the array holds is refused
     {
         const char f = nla::up(func);
-        const int64_t thr = cutoff == 0 ? (f == 'S' ? 256 : 16) : cutoff;
+        const int64_t thr = cutoff == 0 ? 256 : cutoff;
         if (cutoff >= 0 && thr <= 1024 && n > 0 && m > 0) {
             const size_t need = nla::rectrxm_ws_bytes<double>(f, n, thr);
             if (need > NOALLOC_BYTES)

@@ -372,7 +372,8 @@ def parfb(side, trans, direct, storev, A1, A2, V, T):
 # --------------------------------------------------------------------- TRSM / TRMM / GEMM
 def unified_rectrxm_(side, uplo, transpose, alpha, func, A, B, cutoff=0):
     """unified_rectrxm!(side, uplo, transpose, alpha, func, A, B) (src/rectrxm.jl:43-76).
-    cutoff = recursion threshold (0: the reference's 256 for 'S', 16 for 'M')."""
+    cutoff = recursion threshold (0: the library default 256 for both functions; the reference's
+    are 256 for 'S' and 16 for 'M')."""
     p = _p(A, B)
     _real_only(p, "unified_rectrxm!")
     pa, lda = _m(A)
