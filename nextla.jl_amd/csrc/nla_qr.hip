@@ -772,6 +772,137 @@ template <typename T> struct BatchProblem {
 constexpr int BATCH_RHS = 32;    // right-hand sides staged per LDS pass
 constexpr int BATCH_MAXN = 512;  // largest triangular order / panel width handled per problem
 
+// LDS-resident block of the batched geqrt!: the ib-block's panel (mr x sb) is factored in LDS
+// and the trailing update W = V^T C, W = T^T W, C -= V W (larfb!('L','C'), src/larfb.jl:57-98)
+// runs on MFMA 16x16x4 tiles with V and W in LDS.  PCAP = panel elements held (128 KiB),
+// WCOLS = trailing columns per W chunk.
+template <typename T> struct BqCfg;
+template <> struct BqCfg<float> { static constexpr int PCAP = 32768, WCOLS = 128; };
+template <> struct BqCfg<double> { static constexpr int PCAP = 16384, WCOLS = 64; };
+constexpr int BQ_SBMAX = 32;   // widest block of the LDS path
+
+template <typename T> __device__ __forceinline__ bool bq_fits(int mr, int sb) {
+    return sb <= BQ_SBMAX && (mr + 1) * sb <= BqCfg<T>::PCAP;
+}
+template <typename T> static constexpr size_t bq_lds_elems() {
+    return (size_t)BqCfg<T>::PCAP + 64 + (size_t)BQ_SBMAX * BqCfg<T>::WCOLS + BQ_SBMAX * BQ_SBMAX + 64 + 64 + 16 + 4 + 64;
+}
+
+// C (mr x nc, global) := C - V (T^T (V^T C)); V unit lower (mr x sb) in LDS (ld LDV), T (sb x sb,
+// ld BQ_SBMAX) and W (sb x nc, ld BQ_SBMAX) in LDS.  Four waves share the tiles.
+template <typename T>
+__device__ void bq_apply(int mr, int sb, int nc, const T *Vs, int LDV, const T *Ts, T *C, i64 ldc, T *Ws) {
+    typedef typename Acc<T>::t AT;
+    constexpr int REG = BQ_SBMAX * BqCfg<T>::WCOLS / 256;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, lr = lane & 15, lk = lane >> 4;
+    const int ng = (sb + 15) / 16, nct = (nc + 15) / 16, nrt = (mr + 15) / 16;
+    // W = V^T C: one 16 x 16 tile per (reflector group, column tile)
+    for (int tile = w; tile < ng * nct; tile += 4) {
+        const int g = tile % ng, ct = tile / ng;
+        const int j = 16 * g + lr, c = 16 * ct + lr;
+        const bool jv = j < sb, cv = c < nc;
+        const T *Vcol = Vs + j * LDV;
+        const T *Ccol = C + (i64)(cv ? c : 0) * ldc;
+        AT acc = acc_zero<T>();
+        int k0 = 0;
+        for (; k0 + 16 <= mr; k0 += 16) {
+            T a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = k0 + 4 * q + lk;
+                a[q] = jv ? Vcol[r] : T(0);
+                b[q] = cv ? Ccol[r] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) mfma16(acc, a[q], b[q]);
+        }
+        for (; k0 < mr; k0 += 4) {
+            const int r = k0 + lk;
+            const T a = (jv && r < mr) ? Vcol[r] : T(0);
+            const T b = (cv && r < mr) ? Ccol[r] : T(0);
+            mfma16(acc, a, b);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int jj = 16 * g + acc_row<T>(lk, e);
+            if (jj < sb && cv) Ws[jj + c * BQ_SBMAX] = acc[e];
+        }
+    }
+    __syncthreads();
+    // W = T^T W (lower triangular T^T)
+    {
+        T reg[REG];
+        int q = 0;
+        for (int idx = t; idx < sb * nc; idx += 256, ++q) {
+            const int j = idx % sb, c = idx / sb;
+            T s = T(0);
+            for (int p = 0; p <= j; ++p) s += Ts[p + j * BQ_SBMAX] * Ws[p + c * BQ_SBMAX];
+            reg[q] = s;
+        }
+        __syncthreads();
+        q = 0;
+        for (int idx = t; idx < sb * nc; idx += 256, ++q) Ws[(idx % sb) + (idx / sb) * BQ_SBMAX] = reg[q];
+    }
+    __syncthreads();
+    // C -= V W: one 16 x 16 tile per (row tile, column tile)
+    for (int tile = w; tile < nrt * nct; tile += 4) {
+        const int rt = tile % nrt, ct = tile / nrt;
+        const int r = 16 * rt + lr, c = 16 * ct + lr;
+        const bool rv = r < mr, cv = c < nc;
+        AT acc = acc_zero<T>();
+        for (int k0 = 0; k0 < sb; k0 += 4) {
+            const int kk = k0 + lk;
+            const T a = (rv && kk < sb) ? Vs[r + kk * LDV] : T(0);
+            const T b = (cv && kk < sb) ? Ws[kk + c * BQ_SBMAX] : T(0);
+            mfma16(acc, a, b);
+        }
+        if (cv) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int rr = 16 * rt + acc_row<T>(lk, e);
+                if (rr < mr) C[rr + (i64)c * ldc] -= acc[e];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// One ib-block on the LDS path: panel (mr x sb at `panel`) -> LDS, geqr2 + larft there, back to
+// memory with its T, then the trailing columns (ncols of them, right of the panel).
+template <typename T>
+__device__ void bq_block(int mr, int sb, int ncols, T *panel, i64 lda, T *Tg, i64 ldt, T *smem) {
+    const int LDV = mr | 1;  // odd: the column-strided LDS reads of the MFMA operands spread over banks
+    T *Vs = smem, *Ws = Vs + BqCfg<T>::PCAP + 64, *Ts = Ws + BQ_SBMAX * BqCfg<T>::WCOLS;
+    T *z = Ts + BQ_SBMAX * BQ_SBMAX, *tcol = z + 64, *red = tcol + 64, *sh = red + 16, *tau = sh + 4;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < mr * sb; idx += 256) {
+        const int r = idx % mr, c = idx / mr;
+        Vs[r + c * LDV] = panel[r + (i64)c * lda];
+    }
+    __syncthreads();
+    panel_qr_dev<T>(mr, sb, Vs, LDV, tau, Ts, BQ_SBMAX, z, tcol, red, sh);
+    for (int idx = t; idx < mr * sb; idx += 256) {
+        const int r = idx % mr, c = idx / mr;
+        panel[r + (i64)c * lda] = Vs[r + c * LDV];
+    }
+    for (int idx = t; idx < sb * sb; idx += 256) {
+        const int r = idx % sb, c = idx / sb;
+        Tg[r + (i64)c * ldt] = r <= c ? Ts[r + c * BQ_SBMAX] : T(0);
+    }
+    __syncthreads();
+    // V as a unit lower trapezoid for the update (R above the diagonal is already stored)
+    for (int idx = t; idx < sb * sb; idx += 256) {
+        const int r = idx % sb, c = idx / sb;
+        if (r <= c) Vs[r + c * LDV] = r == c ? T(1) : T(0);
+    }
+    __syncthreads();
+    T *C = panel + (i64)sb * lda;
+    for (int cc = 0; cc < ncols; cc += BqCfg<T>::WCOLS) {
+        const int nc = ncols - cc < BqCfg<T>::WCOLS ? ncols - cc : BqCfg<T>::WCOLS;
+        bq_apply<T>(mr, sb, nc, Vs, LDV, Ts, C + (i64)cc * lda, lda, Ws);
+    }
+}
+
 template <typename T> __device__ void batched_geqrt(const BatchProblem<T> &P, T *smem) {
     T *sV = smem, *sC = sV + KBM * RC, *sW0 = sC + CW * RC, *sW = sW0 + KBM * CW;
     T *z = sW + KBM * CW, *tcol = z + 64, *red = tcol + 64, *sh = red + 16, *tau = sh + 4;
@@ -780,6 +911,10 @@ template <typename T> __device__ void batched_geqrt(const BatchProblem<T> &P, T 
     for (int i = 0; i < k; i += ib) {
         const int sb = ib < k - i ? ib : k - i;
         T *panel = P.A + i + (i64)i * P.lda;
+        if (bq_fits<T>(m - i, sb)) {
+            bq_block<T>(m - i, sb, n - i - sb, panel, P.lda, P.Tm + (i64)i * P.ldt, P.ldt, smem);
+            continue;
+        }
         panel_qr_dev<T>(m - i, sb, panel, P.lda, tau, P.Tm + (i64)i * P.ldt, P.ldt, z, tcol, red, sh);
         __syncthreads();
         const int ncols = n - i - sb;
@@ -821,7 +956,9 @@ template <typename T> __device__ void batched_trsm(const BatchProblem<T> &P, T *
 template <typename T> static size_t batched_lds_bytes() {
     const size_t geq = (size_t)KBM * RC + (size_t)CW * RC + 2 * (size_t)KBM * CW + 64 + 64 + 16 + 4 + 64;
     const size_t trs = (size_t)BATCH_MAXN * BATCH_RHS;
-    return sizeof(T) * (geq > trs ? geq : trs);
+    size_t e = geq > trs ? geq : trs;
+    if (bq_lds_elems<T>() > e) e = bq_lds_elems<T>();
+    return sizeof(T) * e;
 }
 
 template <typename T>
