@@ -18,6 +18,7 @@
 #include "nla_common.hpp"
 #include "nla_qr.hpp"
 
+#include <utility>
 #include <vector>
 
 namespace nla {
@@ -772,134 +773,286 @@ template <typename T> struct BatchProblem {
 constexpr int BATCH_RHS = 32;    // right-hand sides staged per LDS pass
 constexpr int BATCH_MAXN = 512;  // largest triangular order / panel width handled per problem
 
-// LDS-resident block of the batched geqrt!: the ib-block's panel (mr x sb) is factored in LDS
-// and the trailing update W = V^T C, W = T^T W, C -= V W (larfb!('L','C'), src/larfb.jl:57-98)
-// runs on MFMA 16x16x4 tiles with V and W in LDS.  PCAP = panel elements held (128 KiB),
-// WCOLS = trailing columns per W chunk.
-template <typename T> struct BqCfg;
-template <> struct BqCfg<float> { static constexpr int PCAP = 32768, WCOLS = 128; };
-template <> struct BqCfg<double> { static constexpr int PCAP = 16384, WCOLS = 64; };
-constexpr int BQ_SBMAX = 32;   // widest block of the LDS path
+// Register-resident blocks of the batched geqrt!.  An ib-block of width sb <= 16 over
+// mr <= 1024 rows is factored with its columns in registers: thread t holds rows t, t + 256,
+// t + 512 and t + 768 of the block, so a column step costs two workgroup reductions (the
+// norm below the diagonal; the products v^T a_j of the new reflector with every other
+// column, geqr2!, src/geqr2.jl:23-105) and no LDS traffic for the panel itself.  T
+// (larft!('F','C'), src/larft.jl:55-126) is built in the same pass: thread j < 16 keeps row j
+// of T in registers.  The trailing columns then take W = V^T C, W = T^T W, C -= V W
+// (larfb!('L','T'), src/larfb.jl:57-98) on MFMA 16x16x4 tiles, V staged in LDS, C read and
+// written in place, W in LDS.  Real element types only (the batched entry points are s / d).
+constexpr int RP_SB = 16;            // widest block of the register path
+constexpr int RP_NQ = 4;             // rows per thread: problems of up to 1024 rows
+constexpr int RP_VLD = RP_SB + 1;    // odd row stride of V in LDS (spreads the MFMA reads over banks)
+constexpr int RP_WC = 32;            // trailing columns per W chunk (two column tiles)
 
-template <typename T> __device__ __forceinline__ bool bq_fits(int mr, int sb) {
-    return sb <= BQ_SBMAX && (mr + 1) * sb <= BqCfg<T>::PCAP;
-}
-template <typename T> static constexpr size_t bq_lds_elems() {
-    return (size_t)BqCfg<T>::PCAP + 64 + (size_t)BQ_SBMAX * BqCfg<T>::WCOLS + BQ_SBMAX * BQ_SBMAX + 64 + 64 + 16 + 4 + 64;
+__device__ __forceinline__ bool rp_fits(int mr, int sb) { return sb <= RP_SB && mr <= 256 * RP_NQ; }
+template <typename T> static constexpr size_t rp_lds_elems() {
+    // zred, two norm slots, T, W partials (4 waves x RP_WC / 16 tiles), W, V
+    return 64 + 16 + 256 + 4 * 16 * RP_WC + 16 * RP_WC + (size_t)256 * RP_NQ * RP_VLD;
 }
 
-// C (mr x nc, global) := C - V (T^T (V^T C)); V unit lower (mr x sb) in LDS (ld LDV), T (sb x sb,
-// ld BQ_SBMAX) and W (sb x nc, ld BQ_SBMAX) in LDS.  Four waves share the tiles.
+// C (mr x nc, nc <= RP_WC) -= V (T^T (V^T C)); Vs: V (rows padded with zeros to a multiple of
+// 256, columns sb..15 zero) row-major with stride RP_VLD; Tl: T row-major 16 x 16.
+// Both products stream C from memory; each wave keeps two 16-row steps of every column tile
+// in flight, so a wave has up to 32 independent loads outstanding.
 template <typename T>
-__device__ void bq_apply(int mr, int sb, int nc, const T *Vs, int LDV, const T *Ts, T *C, i64 ldc, T *Ws) {
+__device__ void rp_apply(int mr, int sb, int nc, const T *Vs, const T *Tl, T *Wp, T *Ws, T *C, i64 ldc) {
     typedef typename Acc<T>::t AT;
-    constexpr int REG = BQ_SBMAX * BqCfg<T>::WCOLS / 256;
+    constexpr int NCT = RP_WC / 16;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, lr = lane & 15, lk = lane >> 4;
-    const int ng = (sb + 15) / 16, nct = (nc + 15) / 16, nrt = (mr + 15) / 16;
-    // W = V^T C: one 16 x 16 tile per (reflector group, column tile)
-    for (int tile = w; tile < ng * nct; tile += 4) {
-        const int g = tile % ng, ct = tile / ng;
-        const int j = 16 * g + lr, c = 16 * ct + lr;
-        const bool jv = j < sb, cv = c < nc;
-        const T *Vcol = Vs + j * LDV;
-        const T *Ccol = C + (i64)(cv ? c : 0) * ldc;
-        AT acc = acc_zero<T>();
-        int k0 = 0;
-        for (; k0 + 16 <= mr; k0 += 16) {
-            T a[4], b[4];
+    const int nct = (nc + 15) >> 4;           // column tiles, 1..NCT
+    const int nkt = (mr + 15) >> 4;           // 16-row steps
+    // W partials: wave w sums row steps w, w + 4, ... of V^T C for every column tile.
+    // Lane (lr, lk) takes rows 16 s + 4 lk + q of step s in the q-th MFMA (4 consecutive rows).
+    {
+        AT acc[NCT];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = k0 + 4 * q + lk;
-                a[q] = jv ? Vcol[r] : T(0);
-                b[q] = cv ? Ccol[r] : T(0);
+        for (int ct = 0; ct < NCT; ++ct) acc[ct] = acc_zero<T>();
+        for (int s = w; s < nkt; s += 8) {
+            T a[8], b[NCT][8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int st = s + 4 * (q >> 2), r = 16 * st + 4 * lk + (q & 3);
+                const bool ok = st < nkt;
+                a[q] = ok ? Vs[r * RP_VLD + lr] : T(0);
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) {
+                    const int c = 16 * ct + lr;
+                    b[ct][q] = (ok && r < mr && c < nc) ? C[r + (i64)c * ldc] : T(0);
+                }
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) mfma16(acc, a[q], b[q]);
-        }
-        for (; k0 < mr; k0 += 4) {
-            const int r = k0 + lk;
-            const T a = (jv && r < mr) ? Vcol[r] : T(0);
-            const T b = (cv && r < mr) ? Ccol[r] : T(0);
-            mfma16(acc, a, b);
+            for (int ct = 0; ct < NCT; ++ct)
+                if (ct < nct) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) mfma16(acc[ct], a[q], b[ct][q]);
+                }
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int jj = 16 * g + acc_row<T>(lk, e);
-            if (jj < sb && cv) Ws[jj + c * BQ_SBMAX] = acc[e];
-        }
-    }
-    __syncthreads();
-    // W = T^T W (lower triangular T^T)
-    {
-        T reg[REG];
-        int q = 0;
-        for (int idx = t; idx < sb * nc; idx += 256, ++q) {
-            const int j = idx % sb, c = idx / sb;
-            T s = T(0);
-            for (int p = 0; p <= j; ++p) s += Ts[p + j * BQ_SBMAX] * Ws[p + c * BQ_SBMAX];
-            reg[q] = s;
-        }
-        __syncthreads();
-        q = 0;
-        for (int idx = t; idx < sb * nc; idx += 256, ++q) Ws[(idx % sb) + (idx / sb) * BQ_SBMAX] = reg[q];
-    }
-    __syncthreads();
-    // C -= V W: one 16 x 16 tile per (row tile, column tile)
-    for (int tile = w; tile < nrt * nct; tile += 4) {
-        const int rt = tile % nrt, ct = tile / nrt;
-        const int r = 16 * rt + lr, c = 16 * ct + lr;
-        const bool rv = r < mr, cv = c < nc;
-        AT acc = acc_zero<T>();
-        for (int k0 = 0; k0 < sb; k0 += 4) {
-            const int kk = k0 + lk;
-            const T a = (rv && kk < sb) ? Vs[r + kk * LDV] : T(0);
-            const T b = (cv && kk < sb) ? Ws[kk + c * BQ_SBMAX] : T(0);
-            mfma16(acc, a, b);
-        }
-        if (cv) {
+        for (int ct = 0; ct < NCT; ++ct)
+            if (ct < nct) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int rr = 16 * rt + acc_row<T>(lk, e);
-                if (rr < mr) C[rr + (i64)c * ldc] -= acc[e];
+                for (int e = 0; e < 4; ++e) Wp[(w * NCT + ct) * 256 + acc_row<T>(lk, e) * 16 + lr] = acc[ct][e];
+            }
+    }
+    __syncthreads();
+    // W = the four waves' partials summed in a fixed order, then W := T^T W into the partials' space
+    for (int idx = t; idx < 16 * nc; idx += 256) {
+        const int p = idx & 15, c = idx >> 4;
+        const int o = (c >> 4) * 256 + p * 16 + (c & 15);
+        Ws[p * RP_WC + c] = (Wp[o] + Wp[NCT * 256 + o]) + (Wp[2 * NCT * 256 + o] + Wp[3 * NCT * 256 + o]);
+    }
+    __syncthreads();
+    for (int idx = t; idx < 16 * nc; idx += 256) {
+        const int j = idx & 15, c = idx >> 4;
+        T s = T(0);
+        if (j < sb)
+            for (int p = 0; p <= j; ++p) s += Tl[p * 16 + j] * Ws[p * RP_WC + c];
+        Wp[j * RP_WC + c] = s;
+    }
+    __syncthreads();
+    // C -= V W as (C^T -= W^T V^T) tiles: lane (lr, lk) updates row 16 rt + lr of columns
+    // 16 ct + acc_row(lk, e), so each column's 16 rows are contiguous.  Two row tiles per pass,
+    // their loads issued before the products.
+    for (int rt0 = w; rt0 < nkt; rt0 += 8) {
+        T cv[2][NCT][4];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            const int r = 16 * (rt0 + 4 * rb) + lr;
+            const bool rv = rt0 + 4 * rb < nkt && r < mr;
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int c = 16 * ct + acc_row<T>(lk, e);
+                    cv[rb][ct][e] = (rv && c < nc) ? C[r + (i64)c * ldc] : T(0);
+                }
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            const int rt = rt0 + 4 * rb;
+            if (rt >= nkt) continue;
+            const int r = 16 * rt + lr;
+            const bool rv = r < mr;
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+                if (ct >= nct) continue;
+                AT acc = acc_zero<T>();
+#pragma unroll
+                for (int k0 = 0; k0 < 16; k0 += 4)
+                    mfma16(acc, Wp[(k0 + lk) * RP_WC + 16 * ct + lr], Vs[r * RP_VLD + k0 + lk]);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int c = 16 * ct + acc_row<T>(lk, e);
+                    if (rv && c < nc) C[r + (i64)c * ldc] = cv[rb][ct][e] - acc[e];
+                }
             }
         }
     }
     __syncthreads();
 }
 
-// One ib-block on the LDS path: panel (mr x sb at `panel`) -> LDS, geqr2 + larft there, back to
-// memory with its T, then the trailing columns (ncols of them, right of the panel).
+// Column I of a register-resident block (see rp_block); a template so that every register
+// index is a constant.
+template <typename T, int I>
+__device__ __forceinline__ void rp_column(T (&a)[RP_NQ][RP_SB], T (&Trow)[RP_SB], int mr, int sb, int nq, T *nred,
+                                          T *zred) {
+    constexpr int i = I;
+    if (i >= sb) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // sum of squares below the diagonal (rows >= mr hold zeros) and alpha = A[i, i]
+    T xs = T(0);
+#pragma unroll
+    for (int q = 0; q < RP_NQ; ++q)
+        if (q > 0 || t > i) xs += a[q][i] * a[q][i];
+    xs = wave_sum(xs);
+    T *nr = nred + 8 * (i & 1);   // alternating slots: a step without a reflector has one barrier
+    if (lane == 0) nr[w] = xs;
+    if (t == i) nr[4] = a[0][i];
+    __syncthreads();
+    const T xsum = (nr[0] + nr[1]) + (nr[2] + nr[3]);
+    const T alpha = nr[4];
+    // the reflector, identically in every thread (src/geqr2.jl:38-73)
+    T tau = T(0), beta = alpha, scale = T(0);
+    if (mr - i > 1 && xsum != T(0)) {
+        beta = -copysign(sqrt(alpha * alpha + xsum), alpha);
+        tau = (beta - alpha) / beta;
+        scale = T(1) / (alpha - beta);
+    }
+    if (tau != T(0)) {
+        T v[RP_NQ];
+#pragma unroll
+        for (int q = 0; q < RP_NQ; ++q) {
+            if (q == 0) {
+                v[0] = t > i ? a[0][i] * scale : (t == i ? T(1) : T(0));
+                if (t > i) a[0][i] = v[0];
+            } else {
+                v[q] = a[q][i] * scale;
+                a[q][i] = v[q];
+            }
+        }
+        // p_j = v^T a_j for every j != i: j > i feeds the update, j < i the T column
+        T p[RP_SB];
+#pragma unroll
+        for (int j = 0; j < RP_SB; ++j) p[j] = T(0);
+#pragma unroll
+        for (int q = 0; q < RP_NQ; ++q)
+            if (q < nq) {
+#pragma unroll
+                for (int j = 0; j < RP_SB; ++j)
+                    if (j != i) p[j] += v[q] * a[q][j];
+            }
+        // transposing butterfly: after it lane l holds the wave's sum of p_{(l >> 2) & 15}
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool hi = lane & 32;
+            const T send = hi ? p[k] : p[k + 8];
+            p[k] = (hi ? p[k + 8] : p[k]) + __shfl_xor(send, 32, WAVE);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool hi = lane & 16;
+            const T send = hi ? p[k] : p[k + 4];
+            p[k] = (hi ? p[k + 4] : p[k]) + __shfl_xor(send, 16, WAVE);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const bool hi = lane & 8;
+            const T send = hi ? p[k] : p[k + 2];
+            p[k] = (hi ? p[k + 2] : p[k]) + __shfl_xor(send, 8, WAVE);
+        }
+        {
+            const bool hi = lane & 4;
+            const T send = hi ? p[0] : p[1];
+            p[0] = (hi ? p[1] : p[0]) + __shfl_xor(send, 4, WAVE);
+        }
+        p[0] += __shfl_xor(p[0], 2, WAVE);
+        p[0] += __shfl_xor(p[0], 1, WAVE);
+        if ((lane & 3) == 0) zred[((lane >> 2) & 15) * 4 + w] = p[0];
+        __syncthreads();
+        // A[:, j] -= tau v z_j for j > i (H^T applied to the block's later columns)
+#pragma unroll
+        for (int j = i + 1; j < RP_SB; ++j) {
+            const T u = tau * ((zred[4 * j] + zred[4 * j + 1]) + (zred[4 * j + 2] + zred[4 * j + 3]));
+#pragma unroll
+            for (int q = 0; q < RP_NQ; ++q)
+                if (q < nq) a[q][j] -= v[q] * u;
+        }
+        // T[0:i, i] = T[0:i, 0:i] (-tau z[0:i]), T[i, i] = tau
+        T s = T(0);
+#pragma unroll
+        for (int q = 0; q < i; ++q)
+            if (q >= t) s += Trow[q] * (-tau * ((zred[4 * q] + zred[4 * q + 1]) + (zred[4 * q + 2] + zred[4 * q + 3])));
+        if (t < i) Trow[i] = s;
+        if (t == i) Trow[i] = tau;
+    }
+    if (t == i) a[0][i] = beta;
+    }
+
+template <typename T, int... I>
+__device__ __forceinline__ void rp_columns(T (&a)[RP_NQ][RP_SB], T (&Trow)[RP_SB], int mr, int sb, int nq, T *nred,
+                                           T *zred, std::integer_sequence<int, I...>) {
+    (rp_column<T, I>(a, Trow, mr, sb, nq, nred, zred), ...);
+}
+
+// One ib-block (mr x sb at `panel`, ncols trailing columns to its right) on the register path.
 template <typename T>
-__device__ void bq_block(int mr, int sb, int ncols, T *panel, i64 lda, T *Tg, i64 ldt, T *smem) {
-    const int LDV = mr | 1;  // odd: the column-strided LDS reads of the MFMA operands spread over banks
-    T *Vs = smem, *Ws = Vs + BqCfg<T>::PCAP + 64, *Ts = Ws + BQ_SBMAX * BqCfg<T>::WCOLS;
-    T *z = Ts + BQ_SBMAX * BQ_SBMAX, *tcol = z + 64, *red = tcol + 64, *sh = red + 16, *tau = sh + 4;
-    const int t = threadIdx.x;
-    for (int idx = t; idx < mr * sb; idx += 256) {
-        const int r = idx % mr, c = idx / mr;
-        Vs[r + c * LDV] = panel[r + (i64)c * lda];
+__device__ void rp_block(int mr, int sb, int ncols, T *panel, i64 lda, T *Tg, i64 ldt, T *smem) {
+    static_assert(!IsCplx<T>::v, "real element types only");
+    T *zred = smem, *nred = zred + 64, *Tl = nred + 16, *Wp = Tl + 256, *Ws = Wp + 4 * 16 * RP_WC, *Vs = Ws + 16 * RP_WC;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int nq = (mr + 255) >> 8;
+    T a[RP_NQ][RP_SB];
+#pragma unroll
+    for (int q = 0; q < RP_NQ; ++q) {
+        const int r = t + 256 * q;
+#pragma unroll
+        for (int j = 0; j < RP_SB; ++j) a[q][j] = (r < mr && j < sb) ? panel[r + (i64)j * lda] : T(0);
     }
-    __syncthreads();
-    panel_qr_dev<T>(mr, sb, Vs, LDV, tau, Ts, BQ_SBMAX, z, tcol, red, sh);
-    for (int idx = t; idx < mr * sb; idx += 256) {
-        const int r = idx % mr, c = idx / mr;
-        panel[r + (i64)c * lda] = Vs[r + c * LDV];
+    T Trow[RP_SB];
+#pragma unroll
+    for (int j = 0; j < RP_SB; ++j) Trow[j] = T(0);
+    rp_columns<T>(a, Trow, mr, sb, nq, nred, zred, std::make_integer_sequence<int, RP_SB>{});
+
+    // R and V back to memory, T to memory (upper triangle; zeros below) and to LDS
+#pragma unroll
+    for (int q = 0; q < RP_NQ; ++q) {
+        const int r = t + 256 * q;
+        if (r < mr) {
+#pragma unroll
+            for (int j = 0; j < RP_SB; ++j)
+                if (j < sb) panel[r + (i64)j * lda] = a[q][j];
+        }
     }
-    for (int idx = t; idx < sb * sb; idx += 256) {
-        const int r = idx % sb, c = idx / sb;
-        Tg[r + (i64)c * ldt] = r <= c ? Ts[r + c * BQ_SBMAX] : T(0);
+    if (t < sb) {
+#pragma unroll
+        for (int j = 0; j < RP_SB; ++j) {
+            const T v = j >= t ? Trow[j] : T(0);
+            if (j < sb) Tg[t + (i64)j * ldt] = v;
+            Tl[t * 16 + j] = v;
+        }
+    } else if (t < 16) {
+#pragma unroll
+        for (int j = 0; j < RP_SB; ++j) Tl[t * 16 + j] = T(0);
     }
-    __syncthreads();
-    // V as a unit lower trapezoid for the update (R above the diagonal is already stored)
-    for (int idx = t; idx < sb * sb; idx += 256) {
-        const int r = idx % sb, c = idx / sb;
-        if (r <= c) Vs[r + c * LDV] = r == c ? T(1) : T(0);
+    if (ncols <= 0) return;
+    // V (unit lower trapezoid, zero columns past sb) to LDS for the update
+#pragma unroll
+    for (int q = 0; q < RP_NQ; ++q) {
+        if (q < nq) {
+            const int r = t + 256 * q;
+#pragma unroll
+            for (int j = 0; j < RP_SB; ++j)
+                Vs[r * RP_VLD + j] = j < sb ? (r > j ? a[q][j] : (r == j ? T(1) : T(0))) : T(0);
+        }
     }
     __syncthreads();
     T *C = panel + (i64)sb * lda;
-    for (int cc = 0; cc < ncols; cc += BqCfg<T>::WCOLS) {
-        const int nc = ncols - cc < BqCfg<T>::WCOLS ? ncols - cc : BqCfg<T>::WCOLS;
-        bq_apply<T>(mr, sb, nc, Vs, LDV, Ts, C + (i64)cc * lda, lda, Ws);
+    for (int cc = 0; cc < ncols; cc += RP_WC) {
+        const int nc = ncols - cc < RP_WC ? ncols - cc : RP_WC;
+        rp_apply<T>(mr, sb, nc, Vs, Tl, Wp, Ws, C + (i64)cc * lda, lda);
     }
 }
 
@@ -911,8 +1064,9 @@ template <typename T> __device__ void batched_geqrt(const BatchProblem<T> &P, T 
     for (int i = 0; i < k; i += ib) {
         const int sb = ib < k - i ? ib : k - i;
         T *panel = P.A + i + (i64)i * P.lda;
-        if (bq_fits<T>(m - i, sb)) {
-            bq_block<T>(m - i, sb, n - i - sb, panel, P.lda, P.Tm + (i64)i * P.ldt, P.ldt, smem);
+        if (rp_fits(m - i, sb)) {
+            rp_block<T>(m - i, sb, n - i - sb, panel, P.lda, P.Tm + (i64)i * P.ldt, P.ldt, smem);
+            __syncthreads();
             continue;
         }
         panel_qr_dev<T>(m - i, sb, panel, P.lda, tau, P.Tm + (i64)i * P.ldt, P.ldt, z, tcol, red, sh);
@@ -928,41 +1082,89 @@ template <typename T> __device__ void batched_geqrt(const BatchProblem<T> &P, T 
     }
 }
 
-// X = L^-1 B (L nt x nt lower, B nt x nrhs): column-oriented forward substitution on LDS-
-// staged blocks of BATCH_RHS right-hand sides, L read straight from memory.
+// X = L^-1 B (L nt x nt lower, B nt x nrhs), blocked by 16 columns of L.  A block of
+// BATCH_RHS right-hand sides lives in LDS (row-major, stride BATCH_RHS + 1); for each 16-column
+// block of L its rows are staged in LDS TR_CH at a time, the 16 x 16 diagonal block is solved by
+// one lane per right-hand side (forward substitution), and the rows below take
+// X[p, :] -= L[p, block] X[block, :] with thread t on right-hand side t % 32 and rows t / 32 + 8 k.
+constexpr int TR_XLD = BATCH_RHS + 1;
+constexpr int TR_CH = 128;
+
 template <typename T> __device__ void batched_trsm(const BatchProblem<T> &P, T *smem) {
-    const int n = (int)P.nt, t = threadIdx.x;
+    const int n = (int)P.nt, t = threadIdx.x, c = t & (BATCH_RHS - 1), g = t / BATCH_RHS;
+    constexpr int NG = 256 / BATCH_RHS;
+    T *X = smem, *Lp = smem + BATCH_MAXN * TR_XLD;
     for (i64 r0 = 0; r0 < P.nrhs; r0 += BATCH_RHS) {
         const int nr = P.nrhs - r0 < BATCH_RHS ? (int)(P.nrhs - r0) : BATCH_RHS;
         T *Bb = P.B + r0 * P.ldb;
         __syncthreads();
-        for (int idx = t; idx < n * nr; idx += 256) smem[idx] = Bb[(idx % n) + (i64)(idx / n) * P.ldb];
-        __syncthreads();
-        for (int i = 0; i < n; ++i) {
-            const T d = P.L[i + (i64)i * P.ldl];
-            if (t < nr) smem[i + t * n] /= d;
-            __syncthreads();
-            const int rows = n - i - 1;
-            for (int idx = t; idx < rows * nr; idx += 256) {
-                const int p = i + 1 + idx % rows, c = idx / rows;
-                smem[p + c * n] -= P.L[p + (i64)i * P.ldl] * smem[i + c * n];
+        for (int cc = 0; cc < BATCH_RHS; ++cc)
+            for (int p = t; p < n; p += 256) X[p * TR_XLD + cc] = cc < nr ? Bb[p + (i64)cc * P.ldb] : T(0);
+        for (int k0 = 0; k0 < n; k0 += 16) {
+            const int kw = n - k0 < 16 ? n - k0 : 16;
+            for (int q0 = k0; q0 < n; q0 += TR_CH) {
+                const int q1 = n - q0 < TR_CH ? n : q0 + TR_CH;
+                __syncthreads();
+                for (int idx = t; idx < TR_CH * 16; idx += 256) {
+                    const int rr = idx & (TR_CH - 1), j = idx / TR_CH;
+                    if (q0 + rr < q1 && j < kw) Lp[rr * 16 + j] = P.L[q0 + rr + (i64)(k0 + j) * P.ldl];
+                }
+                __syncthreads();
+                if (q0 == k0) {
+                    if (t < BATCH_RHS) {
+                        T x[16];
+#pragma unroll
+                        for (int r = 0; r < 16; ++r) x[r] = r < kw ? X[(k0 + r) * TR_XLD + c] : T(0);
+#pragma unroll
+                        for (int r = 0; r < 16; ++r) {
+                            if (r < kw) {
+                                T sum = x[r];
+#pragma unroll
+                                for (int q = 0; q < r; ++q) sum -= Lp[r * 16 + q] * x[q];
+                                x[r] = sum / Lp[r * 16 + r];
+                            }
+                        }
+#pragma unroll
+                        for (int r = 0; r < 16; ++r)
+                            if (r < kw) X[(k0 + r) * TR_XLD + c] = x[r];
+                    }
+                    __syncthreads();
+                }
+                const int u0 = q0 > k0 + kw ? q0 : k0 + kw;
+                if (u0 < q1) {
+                    T xb[16];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) xb[j] = j < kw ? X[(k0 + j) * TR_XLD + c] : T(0);
+                    for (int p = u0 + g; p < q1; p += NG) {
+                        T sum = X[p * TR_XLD + c];
+                        const T *Lr = Lp + (p - q0) * 16;
+#pragma unroll
+                        for (int j = 0; j < 16; ++j)
+                            if (j < kw) sum -= Lr[j] * xb[j];
+                        X[p * TR_XLD + c] = sum;
+                    }
+                }
             }
-            __syncthreads();
         }
-        for (int idx = t; idx < n * nr; idx += 256) Bb[(idx % n) + (i64)(idx / n) * P.ldb] = smem[idx];
+        __syncthreads();
+        for (int cc = 0; cc < nr; ++cc)
+            for (int p = t; p < n; p += 256) Bb[p + (i64)cc * P.ldb] = X[p * TR_XLD + cc];
     }
 }
 
 template <typename T> static size_t batched_lds_bytes() {
     const size_t geq = (size_t)KBM * RC + (size_t)CW * RC + 2 * (size_t)KBM * CW + 64 + 64 + 16 + 4 + 64;
-    const size_t trs = (size_t)BATCH_MAXN * BATCH_RHS;
+    const size_t trs = (size_t)BATCH_MAXN * TR_XLD + TR_CH * 16;
     size_t e = geq > trs ? geq : trs;
-    if (bq_lds_elems<T>() > e) e = bq_lds_elems<T>();
+    if (rp_lds_elems<T>() > e) e = rp_lds_elems<T>();
     return sizeof(T) * e;
 }
 
+// Float32 runs two workgroups per CU (LDS and registers sized for it); Float64 runs one.
+template <typename T> struct BatchOcc { static constexpr int waves_per_eu = sizeof(T) == 4 ? 2 : 1; };
+
 template <typename T>
-__global__ __launch_bounds__(256) void batched_kernel(int count, const BatchProblem<T> *__restrict__ probs, int *counter) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BatchOcc<T>::waves_per_eu))) void batched_kernel(int count, const BatchProblem<T> *__restrict__ probs, int *counter) {
     extern __shared__ __align__(16) unsigned char dyn_raw[];
     T *smem = reinterpret_cast<T *>(dyn_raw);
     __shared__ int sh_task;
@@ -972,8 +1174,10 @@ __global__ __launch_bounds__(256) void batched_kernel(int count, const BatchProb
         __syncthreads();
         const int task = sh_task;
         if (task >= 2 * count) break;
-        const BatchProblem<T> P = probs[task < count ? task : task - count];
-        if (task < count) {
+        // task 2i: the QR of problem i, task 2i + 1: its solve (independent of each other), so
+        // latency-bound solves share the CUs with the QRs instead of forming a tail
+        const BatchProblem<T> P = probs[task >> 1];
+        if ((task & 1) == 0) {
             if (P.m > 0 && P.n > 0) batched_geqrt<T>(P, smem);
         } else if (P.nt > 0 && P.nrhs > 0) {
             batched_trsm<T>(P, smem);
