@@ -217,4 +217,81 @@ template <typename T> __device__ __forceinline__ typename Acc<T>::t acc_zero() {
     return z;
 }
 
+// Global-memory view of a generic pointer.  Pointers read from memory (work lists, problem
+// descriptors) are generic, and generic loads are flat loads, which also count against the LDS
+// wait counter: every wait for LDS data would then wait for all outstanding memory loads too.
+template <typename T> using GPtr = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ GPtr<T> to_global(T *p) { return (GPtr<T>)p; }
+
+// Workgroup barrier ordering LDS accesses only: global stores issued before it may still be in
+// flight (a __syncthreads() waits for them), so use it only where no thread reads memory that
+// another thread wrote to global memory since the last full barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Cross-lane exchanges of gfx950 without the LDS crossbar.  Lane layouts (checked on the
+// hardware): v_permlane32_swap(x, y) leaves x = [x(0:31), y(0:31)], y = [x(32:63), y(32:63)];
+// v_permlane16_swap does the same per 32-lane half with 16-lane rows; DPP row_ror:8 gives lane
+// i the value of lane i ^ 8 of its row, row_shl:4 / row_shr:4 that of lane i + 4 / i - 4,
+// quad_perm [2,3,0,1] / [1,0,3,2] that of lane i ^ 2 / i ^ 1.
+__device__ __forceinline__ void pl32_swap(float &x, float &y) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void pl16_swap(float &x, float &y) {
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void pl32_swap(double &x, double &y) {
+    const unsigned long long xb = (unsigned long long)__double_as_longlong(x), yb = (unsigned long long)__double_as_longlong(y);
+    auto lo = __builtin_amdgcn_permlane32_swap((unsigned)xb, (unsigned)yb, false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(xb >> 32), (unsigned)(yb >> 32), false, false);
+    x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ void pl16_swap(double &x, double &y) {
+    const unsigned long long xb = (unsigned long long)__double_as_longlong(x), yb = (unsigned long long)__double_as_longlong(y);
+    auto lo = __builtin_amdgcn_permlane16_swap((unsigned)xb, (unsigned)yb, false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(xb >> 32), (unsigned)(yb >> 32), false, false);
+    x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+template <int CTRL> __device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int DPP_ROW_SHL4 = 0x104, DPP_ROW_SHR4 = 0x114, DPP_ROW_ROR4 = 0x124, DPP_ROW_ROR8 = 0x128;
+constexpr int DPP_QUAD_X2 = 0x4E, DPP_QUAD_X1 = 0xB1;
+
+// (x, y) -> x summed over lanes {l, l ^ 32} in lanes < 32, y summed likewise in lanes >= 32
+template <typename T> __device__ __forceinline__ T fold32(T x, T y) {
+    pl32_swap(x, y);
+    return x + y;
+}
+// (x, y) -> x summed over lanes {l, l ^ 16} where bit 4 of l is clear, y where it is set
+template <typename T> __device__ __forceinline__ T fold16(T x, T y) {
+    pl16_swap(x, y);
+    return x + y;
+}
+// sum over the wave; lane 0 holds it (the other lanes hold sums of the same terms)
+template <typename T> __device__ __forceinline__ T wave_sum_fast(T s) {
+    s = fold32(s, s);
+    s = fold16(s, s);
+    s += dpp_mov<DPP_ROW_ROR8>(s);
+    s += dpp_mov<DPP_ROW_ROR4>(s);
+    s += dpp_mov<DPP_QUAD_X2>(s);
+    s += dpp_mov<DPP_QUAD_X1>(s);
+    return s;
+}
+
 }  // namespace nla

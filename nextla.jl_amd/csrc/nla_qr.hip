@@ -778,140 +778,138 @@ constexpr int BATCH_MAXN = 512;  // largest triangular order / panel width handl
 // t + 512 and t + 768 of the block, so a column step costs two workgroup reductions (the
 // norm below the diagonal; the products v^T a_j of the new reflector with every other
 // column, geqr2!, src/geqr2.jl:23-105) and no LDS traffic for the panel itself.  T
-// (larft!('F','C'), src/larft.jl:55-126) is built in the same pass: thread j < 16 keeps row j
-// of T in registers.  The trailing columns then take W = V^T C, W = T^T W, C -= V W
-// (larfb!('L','T'), src/larfb.jl:57-98) on MFMA 16x16x4 tiles, V staged in LDS, C read and
-// written in place, W in LDS.  Real element types only (the batched entry points are s / d).
+// (larft!('F','C'), src/larft.jl:55-126) is formed after the last column from the Gram matrix
+// V^T V, one MFMA product over the block's rows.  The trailing columns then take W = V^T C, W = T^T W,
+// C -= V W (larfb!('L','T'), src/larfb.jl:57-98) on MFMA 16x16x4 tiles, V staged in LDS and
+// each wave's rows of C held in registers between the two products.  Real element types only
+// (the batched entry points are s / d).
 constexpr int RP_SB = 16;            // widest block of the register path
 constexpr int RP_NQ = 4;             // rows per thread: problems of up to 1024 rows
 constexpr int RP_VLD = RP_SB + 1;    // odd row stride of V in LDS (spreads the MFMA reads over banks)
-constexpr int RP_WC = 32;            // trailing columns per W chunk (two column tiles)
+// trailing columns per update chunk: two column tiles in Float32, one in Float64 (a wave keeps
+// 16 RP_NQ x tiles x 4 elements of C in registers)
+template <typename T> struct RpWc { static constexpr int v = sizeof(T) == 4 ? 32 : 16; };
 
-__device__ __forceinline__ bool rp_fits(int mr, int sb) { return sb <= RP_SB && mr <= 256 * RP_NQ; }
+__device__ __forceinline__ bool rp_fits(int mr, int sb, i64 lda, i64 n) {
+    return sb <= RP_SB && mr <= 256 * RP_NQ && lda * n < ((i64)1 << 31);
+}
 template <typename T> static constexpr size_t rp_lds_elems() {
-    // zred, two norm slots, T, W partials (4 waves x RP_WC / 16 tiles), W, V
-    return 64 + 16 + 256 + 4 * 16 * RP_WC + 16 * RP_WC + (size_t)256 * RP_NQ * RP_VLD;
+    // zred, two norm slots, T, W partials (4 waves x tiles; first the Gram partials), W (first
+    // the Gram matrix), V.  Float32: 81216 bytes, so two workgroups share a CU.
+    return 64 + 16 + 256 + 4 * 16 * RpWc<T>::v + 16 * RpWc<T>::v + (size_t)256 * RP_NQ * RP_VLD;
 }
 
-// C (mr x nc, nc <= RP_WC) -= V (T^T (V^T C)); Vs: V (rows padded with zeros to a multiple of
-// 256, columns sb..15 zero) row-major with stride RP_VLD; Tl: T row-major 16 x 16.
-// Both products stream C from memory; each wave keeps two 16-row steps of every column tile
-// in flight, so a wave has up to 32 independent loads outstanding.
-template <typename T>
-__device__ void rp_apply(int mr, int sb, int nc, const T *Vs, const T *Tl, T *Wp, T *Ws, T *C, i64 ldc) {
+// C (mr x nc, nc <= RpWc) -= V (T^T (V^T C)) for mr <= 256 NQ; Vs: V (rows padded with zeros
+// to 256 NQ, columns sb..15 zero) row-major with stride RP_VLD; Tl: T row-major 16 x 16.  Wave
+// w owns the 16-row steps w, w + 4, ..., 16 NQ - 4 + w; lane (lr, lk) holds rows
+// 16 s + acc_row(lk, q) of column 16 ct + lr, which is the B operand of V^T C (the row order
+// inside a step is free) and the D layout of V W, so C is read once, all loads in flight
+// together, and written once.  Offsets are 32-bit (the entry point checks lda * n < 2^31).
+template <typename T, int NQ>
+__device__ __forceinline__ void rp_apply(int mr, int sb, int nc, const T *Vs, const T *Tl, T *Wp, T *Ws, GPtr<T> C,
+                                         int ldc) {
     typedef typename Acc<T>::t AT;
-    constexpr int NCT = RP_WC / 16;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, lr = lane & 15, lk = lane >> 4;
+    constexpr int WC = RpWc<T>::v, NCT = WC / 16, NU = 4 * NQ;
+    const int t = threadIdx.x, lane = t & 63, lr = lane & 15, lk = lane >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int nct = (nc + 15) >> 4;           // column tiles, 1..NCT
-    const int nkt = (mr + 15) >> 4;           // 16-row steps
-    // W partials: wave w sums row steps w, w + 4, ... of V^T C for every column tile.
-    // Lane (lr, lk) takes rows 16 s + 4 lk + q of step s in the q-th MFMA (4 consecutive rows).
+    // Unconditional loads at addresses clamped into the block, so no branch splits them: rows
+    // >= mr meet zero rows of V in V^T C and are not stored back, columns >= nc only reach
+    // columns of W that are never used.
+    T cv[NU][NCT][4];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int s = w + 4 * u;
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) {
+            const int c = min(16 * ct + lr, nc - 1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cv[u][ct][q] = C[min(16 * s + acc_row<T>(lk, q), mr - 1) + c * ldc];
+        }
+    }
     {
         AT acc[NCT];
 #pragma unroll
         for (int ct = 0; ct < NCT; ++ct) acc[ct] = acc_zero<T>();
-        for (int s = w; s < nkt; s += 8) {
-            T a[8], b[NCT][8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int st = s + 4 * (q >> 2), r = 16 * st + 4 * lk + (q & 3);
-                const bool ok = st < nkt;
-                a[q] = ok ? Vs[r * RP_VLD + lr] : T(0);
+        for (int u = 0; u < NU; ++u) {
+            const int s = w + 4 * u;
 #pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) {
-                    const int c = 16 * ct + lr;
-                    b[ct][q] = (ok && r < mr && c < nc) ? C[r + (i64)c * ldc] : T(0);
-                }
+            for (int q = 0; q < 4; ++q) {
+                const T a = Vs[(16 * s + acc_row<T>(lk, q)) * RP_VLD + lr];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) mfma16(acc[ct], a, cv[u][ct][q]);
             }
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct)
-                if (ct < nct) {
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) mfma16(acc[ct], a[q], b[ct][q]);
-                }
         }
 #pragma unroll
         for (int ct = 0; ct < NCT; ++ct)
-            if (ct < nct) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) Wp[(w * NCT + ct) * 256 + acc_row<T>(lk, e) * 16 + lr] = acc[ct][e];
-            }
+            for (int e = 0; e < 4; ++e) Wp[(w * NCT + ct) * 256 + acc_row<T>(lk, e) * 16 + lr] = acc[ct][e];
     }
-    __syncthreads();
+    lds_barrier();
     // W = the four waves' partials summed in a fixed order, then W := T^T W into the partials' space
     for (int idx = t; idx < 16 * nc; idx += 256) {
         const int p = idx & 15, c = idx >> 4;
         const int o = (c >> 4) * 256 + p * 16 + (c & 15);
-        Ws[p * RP_WC + c] = (Wp[o] + Wp[NCT * 256 + o]) + (Wp[2 * NCT * 256 + o] + Wp[3 * NCT * 256 + o]);
+        Ws[p * WC + c] = (Wp[o] + Wp[NCT * 256 + o]) + (Wp[2 * NCT * 256 + o] + Wp[3 * NCT * 256 + o]);
     }
-    __syncthreads();
+    lds_barrier();
     for (int idx = t; idx < 16 * nc; idx += 256) {
         const int j = idx & 15, c = idx >> 4;
         T s = T(0);
         if (j < sb)
-            for (int p = 0; p <= j; ++p) s += Tl[p * 16 + j] * Ws[p * RP_WC + c];
-        Wp[j * RP_WC + c] = s;
+            for (int p = 0; p <= j; ++p) s += Tl[p * 16 + j] * Ws[p * WC + c];
+        Wp[j * WC + c] = s;
     }
-    __syncthreads();
-    // C -= V W as (C^T -= W^T V^T) tiles: lane (lr, lk) updates row 16 rt + lr of columns
-    // 16 ct + acc_row(lk, e), so each column's 16 rows are contiguous.  Two row tiles per pass,
-    // their loads issued before the products.
-    for (int rt0 = w; rt0 < nkt; rt0 += 8) {
-        T cv[2][NCT][4];
+    lds_barrier();
+    // C -= V W: A = V rows of the step (lane: V[16 s + lr][k0 + lk]), B = W (lane: W[k0 + lk][c]);
+    // W columns past nc are whatever the partials left there, and their results are not stored.
+    T wf[NCT][4];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-            const int r = 16 * (rt0 + 4 * rb) + lr;
-            const bool rv = rt0 + 4 * rb < nkt && r < mr;
+    for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct)
+        for (int k = 0; k < 4; ++k) wf[ct][k] = Wp[(4 * k + lk) * WC + 16 * ct + lr];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int c = 16 * ct + acc_row<T>(lk, e);
-                    cv[rb][ct][e] = (rv && c < nc) ? C[r + (i64)c * ldc] : T(0);
-                }
-        }
+    for (int u = 0; u < NU; ++u) {
+        const int s = w + 4 * u;
+        if (16 * s >= mr) break;   // wave-uniform
+        T va[4];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-            const int rt = rt0 + 4 * rb;
-            if (rt >= nkt) continue;
-            const int r = 16 * rt + lr;
-            const bool rv = r < mr;
+        for (int k = 0; k < 4; ++k) va[k] = Vs[(16 * s + lr) * RP_VLD + 4 * k + lk];
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) {
-                if (ct >= nct) continue;
-                AT acc = acc_zero<T>();
+        for (int ct = 0; ct < NCT; ++ct) {
+            AT acc = acc_zero<T>();
 #pragma unroll
-                for (int k0 = 0; k0 < 16; k0 += 4)
-                    mfma16(acc, Wp[(k0 + lk) * RP_WC + 16 * ct + lr], Vs[r * RP_VLD + k0 + lk]);
+            for (int k = 0; k < 4; ++k) mfma16(acc, va[k], wf[ct][k]);
+            const int c = 16 * ct + lr;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int c = 16 * ct + acc_row<T>(lk, e);
-                    if (rv && c < nc) C[r + (i64)c * ldc] = cv[rb][ct][e] - acc[e];
-                }
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * s + acc_row<T>(lk, e);
+                if (r < mr && c < nc) C[r + c * ldc] = cv[u][ct][e] - acc[e];
             }
         }
     }
-    __syncthreads();
+    lds_barrier();   // Wp is rewritten by the next chunk
 }
 
 // Column I of a register-resident block (see rp_block); a template so that every register
-// index is a constant.
+// index is a constant.  taus[I] receives the reflector's tau.
 template <typename T, int I>
-__device__ __forceinline__ void rp_column(T (&a)[RP_NQ][RP_SB], T (&Trow)[RP_SB], int mr, int sb, int nq, T *nred,
+__device__ __forceinline__ void rp_column(T (&a)[RP_NQ][RP_SB], T (&taus)[RP_SB], int mr, int sb, int nq, T *nred,
                                           T *zred) {
     constexpr int i = I;
     if (i >= sb) return;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
     // sum of squares below the diagonal (rows >= mr hold zeros) and alpha = A[i, i]
     T xs = T(0);
 #pragma unroll
     for (int q = 0; q < RP_NQ; ++q)
         if (q > 0 || t > i) xs += a[q][i] * a[q][i];
-    xs = wave_sum(xs);
+    xs = wave_sum_fast(xs);
     T *nr = nred + 8 * (i & 1);   // alternating slots: a step without a reflector has one barrier
     if (lane == 0) nr[w] = xs;
     if (t == i) nr[4] = a[0][i];
-    __syncthreads();
+    lds_barrier();
     const T xsum = (nr[0] + nr[1]) + (nr[2] + nr[3]);
     const T alpha = nr[4];
     // the reflector, identically in every thread (src/geqr2.jl:38-73)
@@ -921,6 +919,7 @@ __device__ __forceinline__ void rp_column(T (&a)[RP_NQ][RP_SB], T (&Trow)[RP_SB]
         tau = (beta - alpha) / beta;
         scale = T(1) / (alpha - beta);
     }
+    taus[i] = tau;
     if (tau != T(0)) {
         T v[RP_NQ];
 #pragma unroll
@@ -933,7 +932,7 @@ __device__ __forceinline__ void rp_column(T (&a)[RP_NQ][RP_SB], T (&Trow)[RP_SB]
                 a[q][i] = v[q];
             }
         }
-        // p_j = v^T a_j for every j != i: j > i feeds the update, j < i the T column
+        // p_j = v^T a_j for j > i (rows past mr hold zeros)
         T p[RP_SB];
 #pragma unroll
         for (int j = 0; j < RP_SB; ++j) p[j] = T(0);
@@ -941,82 +940,78 @@ __device__ __forceinline__ void rp_column(T (&a)[RP_NQ][RP_SB], T (&Trow)[RP_SB]
         for (int q = 0; q < RP_NQ; ++q)
             if (q < nq) {
 #pragma unroll
-                for (int j = 0; j < RP_SB; ++j)
-                    if (j != i) p[j] += v[q] * a[q][j];
+                for (int j = i + 1; j < RP_SB; ++j) p[j] += v[q] * a[q][j];
             }
-        // transposing butterfly: after it lane l holds the wave's sum of p_{(l >> 2) & 15}
+        // transposing butterfly: afterwards lane l holds the wave's sum of p_{(l >> 2) & 15}
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const bool hi = lane & 32;
-            const T send = hi ? p[k] : p[k + 8];
-            p[k] = (hi ? p[k + 8] : p[k]) + __shfl_xor(send, 32, WAVE);
-        }
+        for (int k = 0; k < 8; ++k) p[k] = k + 8 <= i ? T(0) : fold32(p[k], p[k + 8]);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const bool hi = lane & 16;
-            const T send = hi ? p[k] : p[k + 4];
-            p[k] = (hi ? p[k + 4] : p[k]) + __shfl_xor(send, 16, WAVE);
-        }
+        for (int k = 0; k < 4; ++k) p[k] = fold16(p[k], p[k + 4]);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const bool hi = lane & 8;
             const T send = hi ? p[k] : p[k + 2];
-            p[k] = (hi ? p[k + 2] : p[k]) + __shfl_xor(send, 8, WAVE);
+            p[k] = (hi ? p[k + 2] : p[k]) + dpp_mov<DPP_ROW_ROR8>(send);
         }
         {
             const bool hi = lane & 4;
             const T send = hi ? p[0] : p[1];
-            p[0] = (hi ? p[1] : p[0]) + __shfl_xor(send, 4, WAVE);
+            const T from_up = dpp_mov<DPP_ROW_SHL4>(send), from_down = dpp_mov<DPP_ROW_SHR4>(send);
+            p[0] = (hi ? p[1] : p[0]) + (hi ? from_down : from_up);
         }
-        p[0] += __shfl_xor(p[0], 2, WAVE);
-        p[0] += __shfl_xor(p[0], 1, WAVE);
+        p[0] += dpp_mov<DPP_QUAD_X2>(p[0]);
+        p[0] += dpp_mov<DPP_QUAD_X1>(p[0]);
         if ((lane & 3) == 0) zred[((lane >> 2) & 15) * 4 + w] = p[0];
-        __syncthreads();
+        lds_barrier();
         // A[:, j] -= tau v z_j for j > i (H^T applied to the block's later columns)
+        T u[RP_SB];
 #pragma unroll
-        for (int j = i + 1; j < RP_SB; ++j) {
-            const T u = tau * ((zred[4 * j] + zred[4 * j + 1]) + (zred[4 * j + 2] + zred[4 * j + 3]));
+        for (int j = i + 1; j < RP_SB; ++j)
+            u[j] = tau * ((zred[4 * j] + zred[4 * j + 1]) + (zred[4 * j + 2] + zred[4 * j + 3]));
 #pragma unroll
-            for (int q = 0; q < RP_NQ; ++q)
-                if (q < nq) a[q][j] -= v[q] * u;
-        }
-        // T[0:i, i] = T[0:i, 0:i] (-tau z[0:i]), T[i, i] = tau
-        T s = T(0);
+        for (int q = 0; q < RP_NQ; ++q)
+            if (q < nq) {
 #pragma unroll
-        for (int q = 0; q < i; ++q)
-            if (q >= t) s += Trow[q] * (-tau * ((zred[4 * q] + zred[4 * q + 1]) + (zred[4 * q + 2] + zred[4 * q + 3])));
-        if (t < i) Trow[i] = s;
-        if (t == i) Trow[i] = tau;
+                for (int j = i + 1; j < RP_SB; ++j) a[q][j] -= v[q] * u[j];
+            }
     }
     if (t == i) a[0][i] = beta;
-    }
+}
 
 template <typename T, int... I>
-__device__ __forceinline__ void rp_columns(T (&a)[RP_NQ][RP_SB], T (&Trow)[RP_SB], int mr, int sb, int nq, T *nred,
+__device__ __forceinline__ void rp_columns(T (&a)[RP_NQ][RP_SB], T (&taus)[RP_SB], int mr, int sb, int nq, T *nred,
                                            T *zred, std::integer_sequence<int, I...>) {
-    (rp_column<T, I>(a, Trow, mr, sb, nq, nred, zred), ...);
+    (rp_column<T, I>(a, taus, mr, sb, nq, nred, zred), ...);
 }
 
 // One ib-block (mr x sb at `panel`, ncols trailing columns to its right) on the register path.
 template <typename T>
-__device__ void rp_block(int mr, int sb, int ncols, T *panel, i64 lda, T *Tg, i64 ldt, T *smem) {
+__device__ __forceinline__ void rp_block(int mr, int sb, int ncols, T *panel_, i64 lda, T *Tg_, i64 ldt, T *smem) {
     static_assert(!IsCplx<T>::v, "real element types only");
-    T *zred = smem, *nred = zred + 64, *Tl = nred + 16, *Wp = Tl + 256, *Ws = Wp + 4 * 16 * RP_WC, *Vs = Ws + 16 * RP_WC;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const GPtr<T> panel = to_global(panel_), Tg = to_global(Tg_);
+    constexpr int WC = RpWc<T>::v;
+    T *zred = smem, *nred = zred + 64, *Tl = nred + 16, *Wp = Tl + 256, *Ws = Wp + 4 * 16 * WC;
+    T *Vs = Ws + 16 * WC;
+    static_assert(4 * 16 * WC >= 4 * 256 && 16 * WC >= 256, "the Gram partials and sum fit the W space");
+    const int t = threadIdx.x, lane = t & 63, lr = lane & 15, lk = lane >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int nq = (mr + 255) >> 8;
     T a[RP_NQ][RP_SB];
 #pragma unroll
     for (int q = 0; q < RP_NQ; ++q) {
         const int r = t + 256 * q;
 #pragma unroll
-        for (int j = 0; j < RP_SB; ++j) a[q][j] = (r < mr && j < sb) ? panel[r + (i64)j * lda] : T(0);
+        for (int j = 0; j < RP_SB; ++j) {
+            // unconditional load at a clamped address, then the zero padding (no divergent branch)
+            const T x = panel[(r < mr ? r : mr - 1) + (i64)(j < sb ? j : sb - 1) * lda];
+            a[q][j] = (r < mr && j < sb) ? x : T(0);
+        }
     }
-    T Trow[RP_SB];
-#pragma unroll
-    for (int j = 0; j < RP_SB; ++j) Trow[j] = T(0);
-    rp_columns<T>(a, Trow, mr, sb, nq, nred, zred, std::make_integer_sequence<int, RP_SB>{});
+    T taus[RP_SB];
+    rp_columns<T>(a, taus, mr, sb, nq, nred, zred, std::make_integer_sequence<int, RP_SB>{});
 
-    // R and V back to memory, T to memory (upper triangle; zeros below) and to LDS
+    // R and V back to memory; V (unit lower trapezoid, zero columns past sb, zero rows up to
+    // 256 nq) to LDS
 #pragma unroll
     for (int q = 0; q < RP_NQ; ++q) {
         const int r = t + 256 * q;
@@ -1026,19 +1021,6 @@ __device__ void rp_block(int mr, int sb, int ncols, T *panel, i64 lda, T *Tg, i6
                 if (j < sb) panel[r + (i64)j * lda] = a[q][j];
         }
     }
-    if (t < sb) {
-#pragma unroll
-        for (int j = 0; j < RP_SB; ++j) {
-            const T v = j >= t ? Trow[j] : T(0);
-            if (j < sb) Tg[t + (i64)j * ldt] = v;
-            Tl[t * 16 + j] = v;
-        }
-    } else if (t < 16) {
-#pragma unroll
-        for (int j = 0; j < RP_SB; ++j) Tl[t * 16 + j] = T(0);
-    }
-    if (ncols <= 0) return;
-    // V (unit lower trapezoid, zero columns past sb) to LDS for the update
 #pragma unroll
     for (int q = 0; q < RP_NQ; ++q) {
         if (q < nq) {
@@ -1048,15 +1030,62 @@ __device__ void rp_block(int mr, int sb, int ncols, T *panel, i64 lda, T *Tg, i6
                 Vs[r * RP_VLD + j] = j < sb ? (r > j ? a[q][j] : (r == j ? T(1) : T(0))) : T(0);
         }
     }
-    __syncthreads();
-    T *C = panel + (i64)sb * lda;
-    for (int cc = 0; cc < ncols; cc += RP_WC) {
-        const int nc = ncols - cc < RP_WC ? ncols - cc : RP_WC;
-        rp_apply<T>(mr, sb, nc, Vs, Tl, Wp, Ws, C + (i64)cc * lda, lda);
+    lds_barrier();
+    // G = V^T V on MFMA (both operands are the same V fragment), four partials summed in order
+    {
+        typedef typename Acc<T>::t AT;
+        AT acc = acc_zero<T>();
+        const int nkt = (mr + 15) >> 4;
+        for (int s = w; s < nkt; s += 4) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const T x = Vs[(16 * s + acc_row<T>(lk, q)) * RP_VLD + lr];
+                mfma16(acc, x, x);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Wp[w * 256 + acc_row<T>(lk, e) * 16 + lr] = acc[e];
+    }
+    lds_barrier();
+    Ws[t] = (Wp[t] + Wp[256 + t]) + (Wp[512 + t] + Wp[768 + t]);
+    lds_barrier();
+    // T (larft!('F','C')), row t by thread t < 16: T[t, i] = sum_{q=t}^{i-1} T[t, q] (-tau_i G[q, i]),
+    // T[i, i] = tau_i; to memory (zeros below the diagonal) and to LDS for the update
+    if (t < 16) {
+        T Trow[RP_SB];
+#pragma unroll
+        for (int i = 0; i < RP_SB; ++i) {
+            T s = T(0);
+            if (i < sb && taus[i] != T(0)) {
+                if (t == i) s = taus[i];
+#pragma unroll
+                for (int q = 0; q < i; ++q)
+                    if (q >= t) s += Trow[q] * (-taus[i] * Ws[q * 16 + i]);
+            }
+            Trow[i] = s;
+        }
+#pragma unroll
+        for (int j = 0; j < RP_SB; ++j) {
+            const T v = (j >= t && j < sb && t < sb) ? Trow[j] : T(0);
+            if (t < sb && j < sb) Tg[t + (i64)j * ldt] = v;
+            Tl[t * 16 + j] = v;
+        }
+    }
+    // (Tl is read after the first barrier of the update, which wave 0 reaches after writing it)
+    const GPtr<T> C = panel + (i64)sb * lda;
+    for (int cc = 0; cc < ncols; cc += WC) {
+        const int nc = ncols - cc < WC ? ncols - cc : WC;
+        const GPtr<T> Cc = C + (i64)cc * lda;
+        switch (nq) {
+        case 1: rp_apply<T, 1>(mr, sb, nc, Vs, Tl, Wp, Ws, Cc, (int)lda); break;
+        case 2: rp_apply<T, 2>(mr, sb, nc, Vs, Tl, Wp, Ws, Cc, (int)lda); break;
+        case 3: rp_apply<T, 3>(mr, sb, nc, Vs, Tl, Wp, Ws, Cc, (int)lda); break;
+        default: rp_apply<T, 4>(mr, sb, nc, Vs, Tl, Wp, Ws, Cc, (int)lda); break;
+        }
     }
 }
 
-template <typename T> __device__ void batched_geqrt(const BatchProblem<T> &P, T *smem) {
+template <typename T> __device__ __forceinline__ void batched_geqrt(const BatchProblem<T> &P, T *smem) {
     T *sV = smem, *sC = sV + KBM * RC, *sW0 = sC + CW * RC, *sW = sW0 + KBM * CW;
     T *z = sW + KBM * CW, *tcol = z + 64, *red = tcol + 64, *sh = red + 16, *tau = sh + 4;
     const int m = (int)P.m, n = (int)P.n, ib = (int)P.ib;
@@ -1064,7 +1093,7 @@ template <typename T> __device__ void batched_geqrt(const BatchProblem<T> &P, T 
     for (int i = 0; i < k; i += ib) {
         const int sb = ib < k - i ? ib : k - i;
         T *panel = P.A + i + (i64)i * P.lda;
-        if (rp_fits(m - i, sb)) {
+        if (rp_fits(m - i, sb, P.lda, n)) {
             rp_block<T>(m - i, sb, n - i - sb, panel, P.lda, P.Tm + (i64)i * P.ldt, P.ldt, smem);
             __syncthreads();
             continue;
@@ -1090,26 +1119,27 @@ template <typename T> __device__ void batched_geqrt(const BatchProblem<T> &P, T 
 constexpr int TR_XLD = BATCH_RHS + 1;
 constexpr int TR_CH = 128;
 
-template <typename T> __device__ void batched_trsm(const BatchProblem<T> &P, T *smem) {
+template <typename T> __device__ __forceinline__ void batched_trsm(const BatchProblem<T> &P, T *smem) {
     const int n = (int)P.nt, t = threadIdx.x, c = t & (BATCH_RHS - 1), g = t / BATCH_RHS;
     constexpr int NG = 256 / BATCH_RHS;
     T *X = smem, *Lp = smem + BATCH_MAXN * TR_XLD;
+    const GPtr<const T> L = to_global(P.L);
     for (i64 r0 = 0; r0 < P.nrhs; r0 += BATCH_RHS) {
         const int nr = P.nrhs - r0 < BATCH_RHS ? (int)(P.nrhs - r0) : BATCH_RHS;
-        T *Bb = P.B + r0 * P.ldb;
-        __syncthreads();
+        const GPtr<T> Bb = to_global(P.B + r0 * P.ldb);
+        lds_barrier();
         for (int cc = 0; cc < BATCH_RHS; ++cc)
             for (int p = t; p < n; p += 256) X[p * TR_XLD + cc] = cc < nr ? Bb[p + (i64)cc * P.ldb] : T(0);
         for (int k0 = 0; k0 < n; k0 += 16) {
             const int kw = n - k0 < 16 ? n - k0 : 16;
             for (int q0 = k0; q0 < n; q0 += TR_CH) {
                 const int q1 = n - q0 < TR_CH ? n : q0 + TR_CH;
-                __syncthreads();
+                lds_barrier();
                 for (int idx = t; idx < TR_CH * 16; idx += 256) {
                     const int rr = idx & (TR_CH - 1), j = idx / TR_CH;
-                    if (q0 + rr < q1 && j < kw) Lp[rr * 16 + j] = P.L[q0 + rr + (i64)(k0 + j) * P.ldl];
+                    if (q0 + rr < q1 && j < kw) Lp[rr * 16 + j] = L[q0 + rr + (i64)(k0 + j) * P.ldl];
                 }
-                __syncthreads();
+                lds_barrier();
                 if (q0 == k0) {
                     if (t < BATCH_RHS) {
                         T x[16];
@@ -1128,7 +1158,7 @@ template <typename T> __device__ void batched_trsm(const BatchProblem<T> &P, T *
                         for (int r = 0; r < 16; ++r)
                             if (r < kw) X[(k0 + r) * TR_XLD + c] = x[r];
                     }
-                    __syncthreads();
+                    lds_barrier();
                 }
                 const int u0 = q0 > k0 + kw ? q0 : k0 + kw;
                 if (u0 < q1) {
@@ -1146,11 +1176,14 @@ template <typename T> __device__ void batched_trsm(const BatchProblem<T> &P, T *
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         for (int cc = 0; cc < nr; ++cc)
             for (int p = t; p < n; p += 256) Bb[p + (i64)cc * P.ldb] = X[p * TR_XLD + cc];
     }
 }
+
+static_assert(2 * (sizeof(float) * rp_lds_elems<float>() + 64) <= 160 * 1024, "two Float32 workgroups per CU");
+static_assert((size_t)BATCH_MAXN * TR_XLD + TR_CH * 16 <= rp_lds_elems<float>(), "the solve fits the QR's LDS");
 
 template <typename T> static size_t batched_lds_bytes() {
     const size_t geq = (size_t)KBM * RC + (size_t)CW * RC + 2 * (size_t)KBM * CW + 64 + 64 + 16 + 4 + 64;
@@ -1172,7 +1205,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BatchOcc<T>
         __syncthreads();
         if (threadIdx.x == 0) sh_task = atomicAdd(counter, 1);
         __syncthreads();
-        const int task = sh_task;
+        const int task = __builtin_amdgcn_readfirstlane(sh_task);   // uniform: scalar loads of P
         if (task >= 2 * count) break;
         // task 2i: the QR of problem i, task 2i + 1: its solve (independent of each other), so
         // latency-bound solves share the CUs with the QRs instead of forming a tail
