@@ -304,7 +304,7 @@ def lu_factor(torch, nl, gen, n=8192):
             "flops": "2n^3/3 (LAPACK count)", "info": holder["r"][2]}
 
 
-def batched(torch, nl, gen, count=4096, nrhs=32):
+def batched(torch, nl, gen, count=4096, nrhs=32, fixed_ib=0):
     """Batched uneven workload: `count` Float32 problems, m uniform over multiples of 32 in
     [64, 1024], n uniform over multiples of 16 in [16, min(128, m)]; geqrt! of each A and a
     lower solve with 32 right-hand sides, all in one persistent launch, replayed from a hipGraph.
@@ -329,9 +329,12 @@ def batched(torch, nl, gen, count=4096, nrhs=32):
     A0 = [a.clone() for a in As]
     B0 = [b.clone() for b in Bs]
     t0 = time.perf_counter()
-    tune = bt.autotune(torch.float32, [(512, n) for n in range(16, 129, 16)], samples=32, candidates=(8, 16, 32))
+    if fixed_ib > 0:
+        ibs = {n: fixed_ib for n in range(16, 129, 16)}
+    else:
+        tune = bt.autotune(torch.float32, [(512, n) for n in range(16, 129, 16)], samples=64, candidates=(8, 16, 32))
+        ibs = {n: ib for (m, n), (ib, ms) in tune.items()}
     t_tune = time.perf_counter() - t0
-    ibs = {n: ib for (m, n), (ib, ms) in tune.items()}
     plan = bt.BatchedQRTRSM(As, Ls, Bs, ib=ibs)
     qflops = sum(2.0 * m * n * n - 2.0 / 3.0 * n ** 3 for m, n in shapes)
     sflops = sum(float(n) * n * nrhs for m, n in shapes)
@@ -419,6 +422,10 @@ def main():
     ap.add_argument("--qr-no-check", action="store_true", help="skip the tile-QR backward-error check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--only", default="",
+                    help="run only these extra workloads (comma-separated, as for --extras) and print their JSON; "
+                         "no headline solve, no CPU baseline (for profiling one workload)")
+    ap.add_argument("--batched-ib", type=int, default=0, help="fixed ib for the batched workload (0: autotune)")
     ap.add_argument("--cutoff-sweep", action="store_true",
                     help="only time TRSM and TRMM (Float64, Float32) at n x m over cut-offs 16..512 and print JSON")
     ap.add_argument("--sweep-funcs", default="SM", help="functions of the cut-off sweep: S and/or M")
@@ -436,6 +443,20 @@ def main():
 
     nl.load()
     n, m, cut = args.n, args.m, args.cutoff
+    if args.only:
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(8192)
+        only = set(x for x in args.only.split(",") if x)
+        runners = {"batched": lambda: batched(torch, nl, gen, fixed_ib=args.batched_ib),
+                   "lu": lambda: lu_factor(torch, nl, gen), "adjoints": lambda: adjoints(torch, nl, gen),
+                   "ls": lambda: least_squares(torch, nl, gen),
+                   "qr_solve": lambda: qr_solve_equal_error(torch, nl, gen, n=args.qr_n)}
+        unknown = only - set(runners)
+        if unknown:
+            print(f"--only supports {sorted(runners)}, not {sorted(unknown)}", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps({k: runners[k]() for k in sorted(only)}))
+        return
     if args.cutoff_sweep:
         cutoff_sweep(torch, nl, n, m, args.sweep_funcs, args.sweep_dtypes.split(","),
                      [int(c) for c in args.sweep_cuts.split(",")])
@@ -588,7 +609,7 @@ def main():
     if "lu" in chosen:
         extras["lu"] = lu_factor(torch, nl, gen)
     if "batched" in chosen:
-        extras["batched"] = batched(torch, nl, gen)
+        extras["batched"] = batched(torch, nl, gen, fixed_ib=args.batched_ib)
     if "adjoints" in chosen:
         extras["adjoints"] = adjoints(torch, nl, gen)
     if "ls" in chosen:
