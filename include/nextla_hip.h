@@ -327,6 +327,21 @@ int nla_stile_ormqr(char trans, int64_t m, int64_t n, int64_t nrhs, int64_t nb, 
                     const float *A, int64_t lda, const float *T, int64_t ldt, float *C, int64_t ldc,
                     void *stream);
 
+/* The same factorization (same A and T layout as nla_?tile_geqrf) issued the way the
+ * reference's callers issue it: the flat-tree loop of one geqrt!/unmqr!/tsqrt!/tsmqr! call per
+ * tile or tile pair (about (m/nb)(n/nb)^2/3 tsmqr! calls), each through the per-call entry
+ * point above (nla_?geqrt, nla_?unmqr, nla_?tsqrt, nla_?tsmqr), enqueued without
+ * synchronising.  No reference function: it is the caller loop of the reference's tile-QR
+ * use, provided to run and time that call sequence natively. */
+int nla_dtile_geqrf_calls(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
+                          int64_t ldt, void *stream);
+int nla_stile_geqrf_calls(int64_t m, int64_t n, int64_t nb, int64_t ib, float *A, int64_t lda, float *T,
+                          int64_t ldt, void *stream);
+int nla_ztile_geqrf_calls(int64_t m, int64_t n, int64_t nb, int64_t ib, nla_zcomplex *A, int64_t lda,
+                          nla_zcomplex *T, int64_t ldt, void *stream);
+int nla_ctile_geqrf_calls(int64_t m, int64_t n, int64_t nb, int64_t ib, nla_ccomplex *A, int64_t lda,
+                          nla_ccomplex *T, int64_t ldt, void *stream);
+
 /* ---- Mixed precision and least squares (no reference function: config "mixed-precision
  * tile QR + least squares"; the reference stops at the tile kernels) --------------------- */
 /* nla_dtile_geqrf with the trailing updates (unmqr!/tsmqr!) on MFMA with Float32

@@ -173,13 +173,13 @@ __device__ void panel_qr_dev(int m, int n, T *__restrict__ A, i64 lda, T *__rest
         }
         __syncthreads();
         if (tv != T(0)) {
-            // trailing in-panel update A[i+p, j] -= conj(tau) v_p z_j
+            // trailing in-panel update A[i+p, j] -= conj(tau) v_p z_j (thread per row: no index
+            // division, the row's factor formed once)
             const T ct = conj_(tv);
-            const int nc = n - i - 1;
-            for (int idx = t; idx < nc * len; idx += 256) {
-                const int p = idx % len, j = i + 1 + idx / len;
-                const T vp = p == 0 ? T(1) : col[p];
-                A[i + p + (i64)j * lda] -= ct * vp * z[j];
+            for (int p = t; p < len; p += 256) {
+                const T cv = ct * (p == 0 ? T(1) : col[p]);
+                T *row = A + i + p;
+                for (int j = i + 1; j < n; ++j) row[(i64)j * lda] -= cv * z[j];
             }
         }
         if (Tm) {
@@ -257,18 +257,15 @@ __global__ __launch_bounds__(256) void ts_panel_kernel(int m, int ii, int sb, T 
         __syncthreads();
         if (tv != T(0)) {
             const T ct = conj_(tv);
-            const int nc = sb - i - 1;
-            // w = A1[r, r+1+j] + z ;  A2[0:mi, r+1+j] -= conj(tau) x w ;  A1[r, r+1+j] -= conj(tau) w
-            for (int idx = t; idx < nc * mi; idx += 256) {
-                const int p = idx % mi, j = i + 1 + idx / mi;
-                const T wj = A1[r + (i64)(ii + j) * lda1] + z[j];
-                A2[p + (i64)(ii + j) * lda2] -= ct * x[p] * wj;
-            }
+            // w_j = A1[r, r+1+j] + z_j ;  A2[0:mi, r+1+j] -= conj(tau) x w ;  A1[r, r+1+j] -= conj(tau) w
+            for (int j = i + 1 + t; j < sb; j += 256) z[j] += A1[r + (i64)(ii + j) * lda1];
             __syncthreads();
-            for (int j = i + 1 + t; j < sb; j += 256) {
-                T *a1 = &A1[r + (i64)(ii + j) * lda1];
-                *a1 -= ct * (*a1 + z[j]);
+            for (int p = t; p < mi; p += 256) {
+                const T cx = ct * x[p];
+                T *row = A2 + p;
+                for (int j = i + 1; j < sb; ++j) row[(i64)(ii + j) * lda2] -= cx * z[j];
             }
+            for (int j = i + 1 + t; j < sb; j += 256) A1[r + (i64)(ii + j) * lda1] -= ct * z[j];
         }
         // T column: T[p, r] = -tau * sum_q T[p, ii+q] z_q  (p < i), T[i, r] = tau
         if (t < i) tcol[t] = tv != T(0) ? -tv * z[t] : T(0);
@@ -617,11 +614,241 @@ __global__ __launch_bounds__(256) void pemv_kernel(int m, int n, int l, bool con
 
 // ------------------------------------------------------------------------------------
 // Host launchers (no validation; used by the entry points below and by the tile driver).
+// ------------------------------------------------------------------------------------
+// Register-resident panels of the per-call geqrt!/tsqrt! (real element types, panel rows
+// mr <= 256, block width sb <= SB in {16, 32}): thread t holds row t of the block's columns.
+// A column step is two workgroup reductions (the norm; the products of the new reflector with
+// the block's later columns, folded across the wave with v_permlane / DPP and across the four
+// waves through LDS); the reflector is formed identically in every thread.  GE form (geqr2!
+// inside geqrt!, src/geqr2.jl:23-105): the block's own rows, v_i = 1 on row i.  TS form
+// (tsqrt_block_kernel!, src/tsqrt.jl:46-126): alpha = A1[i, i] and the block's A1 row i is held
+// by thread i, the reflector's lower part is all of A2's rows.  T (larft!('F','C'),
+// src/larft.jl:55-126) from the Gram matrix of the block's V (one MFMA product) after the last
+// column: T[t, i] = sum_{q=t}^{i-1} T[t, q] (-tau_i G[q, i]), T[i, i] = tau_i.
+
+// Transposing butterfly over the wave: afterwards lane l holds the wave's sum of p[idx(l)],
+// idx(l) = (l >> 2) & 15 for SB = 16, (l >> 1) & 31 for SB = 32.
+template <typename T, int SB> __device__ __forceinline__ T fold_values(T (&p)[SB], int lane) {
+    if constexpr (SB == 32) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) p[k] = fold32(p[k], p[k + 16]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) p[k] = fold16(p[k], p[k + 8]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool hi = lane & 8;
+            const T send = hi ? p[k] : p[k + 4];
+            p[k] = (hi ? p[k + 4] : p[k]) + dpp_mov<DPP_ROW_ROR8>(send);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const bool hi = lane & 4;
+            const T send = hi ? p[k] : p[k + 2];
+            const T from_up = dpp_mov<DPP_ROW_SHL4>(send), from_down = dpp_mov<DPP_ROW_SHR4>(send);
+            p[k] = (hi ? p[k + 2] : p[k]) + (hi ? from_down : from_up);
+        }
+        {
+            const bool hi = lane & 2;
+            const T send = hi ? p[0] : p[1];
+            p[0] = (hi ? p[1] : p[0]) + dpp_mov<DPP_QUAD_X2>(send);
+        }
+        return p[0] + dpp_mov<DPP_QUAD_X1>(p[0]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) p[k] = fold32(p[k], p[k + 8]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p[k] = fold16(p[k], p[k + 4]);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const bool hi = lane & 8;
+            const T send = hi ? p[k] : p[k + 2];
+            p[k] = (hi ? p[k + 2] : p[k]) + dpp_mov<DPP_ROW_ROR8>(send);
+        }
+        {
+            const bool hi = lane & 4;
+            const T send = hi ? p[0] : p[1];
+            const T from_up = dpp_mov<DPP_ROW_SHL4>(send), from_down = dpp_mov<DPP_ROW_SHR4>(send);
+            p[0] = (hi ? p[1] : p[0]) + (hi ? from_down : from_up);
+        }
+        p[0] += dpp_mov<DPP_QUAD_X2>(p[0]);
+        return p[0] + dpp_mov<DPP_QUAD_X1>(p[0]);
+    }
+}
+
+template <typename T, bool TS, int SB, int I>
+__device__ __forceinline__ void rpc_column(T (&a)[SB], T (&a1)[SB], T (&taus)[SB], int mr, int sb, T *nred, T *zred) {
+    constexpr int i = I;
+    if (i >= sb) return;
+    const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+    // sum of squares of the reflector's lower part (rows >= mr hold zeros) and alpha
+    T xs = (TS || t > i) ? a[i] * a[i] : T(0);
+    xs = wave_sum_fast(xs);
+    T *nr = nred + 8 * (i & 1);
+    if (lane == 0) nr[w] = xs;
+    if (t == i) nr[4] = TS ? a1[i] : a[i];
+    lds_barrier();
+    const T xsum = (nr[0] + nr[1]) + (nr[2] + nr[3]);
+    const T alpha = nr[4];
+    T tau = T(0), beta = alpha, scale = T(0);
+    if ((TS || mr - i > 1) && xsum != T(0)) {
+        beta = -copysign(sqrt(alpha * alpha + xsum), alpha);
+        tau = (beta - alpha) / beta;
+        scale = T(1) / (alpha - beta);
+    }
+    taus[i] = tau;
+    if (tau != T(0)) {
+        T v;
+        if (TS) {
+            v = a[i] * scale;
+            a[i] = v;
+        } else {
+            v = t > i ? a[i] * scale : (t == i ? T(1) : T(0));
+            if (t > i) a[i] = v;
+        }
+        T p[SB];
+#pragma unroll
+        for (int j = 0; j < SB; ++j) p[j] = T(0);
+#pragma unroll
+        for (int j = i + 1; j < SB; ++j) p[j] = v * a[j] + ((TS && t == i) ? a1[j] : T(0));
+        const T zsum = fold_values<T, SB>(p, lane);
+        constexpr int SH = SB == 32 ? 1 : 2;
+        if ((lane & ((1 << SH) - 1)) == 0) zred[((lane >> SH) & (SB - 1)) * 4 + w] = zsum;
+        lds_barrier();
+        T u[SB];
+#pragma unroll
+        for (int j = i + 1; j < SB; ++j)
+            u[j] = tau * ((zred[4 * j] + zred[4 * j + 1]) + (zred[4 * j + 2] + zred[4 * j + 3]));
+#pragma unroll
+        for (int j = i + 1; j < SB; ++j) a[j] -= v * u[j];
+        if (TS && t == i) {
+#pragma unroll
+            for (int j = i + 1; j < SB; ++j) a1[j] -= u[j];
+        }
+    }
+    if (t == i) {
+        if (TS) a1[i] = beta;
+        else a[i] = beta;
+    }
+}
+
+template <typename T, bool TS, int SB, int... I>
+__device__ __forceinline__ void rpc_columns(T (&a)[SB], T (&a1)[SB], T (&taus)[SB], int mr, int sb, T *nred, T *zred,
+                                            std::integer_sequence<int, I...>) {
+    (rpc_column<T, TS, SB, I>(a, a1, taus, mr, sb, nred, zred), ...);
+}
+
+// One ib-block panel: GE form on P (mr x sb, the block's rows), TS form on P = the block's
+// columns of A2 (mr x sb) with A1 = the block's diagonal block of A1 (upper triangle read and
+// written).  Writes the factors back, tau[0:sb] and T (sb x sb, zeros below the diagonal).
+template <typename T, bool TS, int SB>
+__global__ __launch_bounds__(256) void rpc_panel_kernel(int mr, int sb, T *P_, i64 ldp, T *A1_, i64 lda1, T *Tg_,
+                                                        i64 ldt, T *tau_) {
+    static_assert(!IsCplx<T>::v, "real element types only");
+    constexpr int VLD = SB + 1;
+    __shared__ T zred[SB * 4];
+    __shared__ T nred[16];
+    __shared__ T Gs[SB * SB];
+    __shared__ T Vs[256 * VLD];
+    const GPtr<T> P = to_global(P_), A1 = to_global(A1_), Tg = to_global(Tg_), tau = to_global(tau_);
+    const int t = threadIdx.x, lane = t & 63, lr = lane & 15, lk = lane >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    T a[SB], a1[SB], taus[SB];
+#pragma unroll
+    for (int j = 0; j < SB; ++j) {
+        const T x = P[min(t, mr - 1) + (i64)min(j, sb - 1) * ldp];
+        a[j] = (t < mr && j < sb) ? x : T(0);
+        if (TS) {
+            const T y = A1[min(t, sb - 1) + (i64)min(j, sb - 1) * lda1];
+            a1[j] = (t < sb && j < sb && j >= t) ? y : T(0);
+        }
+    }
+    rpc_columns<T, TS, SB>(a, a1, taus, mr, sb, nred, zred, std::make_integer_sequence<int, SB>{});
+    if (t < mr) {
+#pragma unroll
+        for (int j = 0; j < SB; ++j)
+            if (j < sb) P[t + (i64)j * ldp] = a[j];
+    }
+    if (TS && t < sb) {
+#pragma unroll
+        for (int j = 0; j < SB; ++j)
+            if (j < sb && j >= t) A1[t + (i64)j * lda1] = a1[j];
+    }
+    if (t < sb) {
+        T tv = T(0);
+#pragma unroll
+        for (int j = 0; j < SB; ++j)
+            if (j == t) tv = taus[j];
+        tau[t] = tv;
+    }
+    // V (GE: unit lower trapezoid; TS: A2's part) to LDS, then G = V^T V on MFMA
+#pragma unroll
+    for (int j = 0; j < SB; ++j)
+        Vs[t * VLD + j] = (t < mr && j < sb) ? (TS ? a[j] : (t > j ? a[j] : (t == j ? T(1) : T(0)))) : T(0);
+    lds_barrier();
+    constexpr int NGR = SB / 16;
+    if (w < NGR * NGR) {
+        typedef typename Acc<T>::t AT;
+        const int g1 = w / NGR, g2 = w % NGR;
+        AT acc = acc_zero<T>();
+        const int nkt = (mr + 15) >> 4;
+        for (int s = 0; s < nkt; ++s) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = 16 * s + acc_row<T>(lk, q);
+                mfma16(acc, Vs[r * VLD + 16 * g1 + lr], Vs[r * VLD + 16 * g2 + lr]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Gs[(16 * g1 + acc_row<T>(lk, e)) * SB + 16 * g2 + lr] = acc[e];
+    }
+    lds_barrier();
+    if (t < sb) {
+        T Trow[SB];
+#pragma unroll
+        for (int i = 0; i < SB; ++i) {
+            T s = T(0);
+            if (i < sb && taus[i] != T(0)) {
+                if (t == i) s = taus[i];
+#pragma unroll
+                for (int q = 0; q < i; ++q)
+                    if (q >= t) s += Trow[q] * (-taus[i] * Gs[q * SB + i]);
+            }
+            Trow[i] = s;
+        }
+#pragma unroll
+        for (int j = 0; j < SB; ++j)
+            if (j < sb) Tg[t + (i64)j * ldt] = j >= t ? Trow[j] : T(0);
+    }
+}
+
+// The register panel takes real element types with at most 256 rows and 32 columns; it returns
+// false (nothing launched) otherwise.
+template <typename T>
+static bool launch_rpc_panel(bool ts, i64 mr, i64 sb, T *P, i64 ldp, T *A1, i64 lda1, T *Tg, i64 ldt, T *tau,
+                             hipStream_t s, hipError_t *err) {
+    if constexpr (IsCplx<T>::v) {
+        return false;
+    } else {
+        if (mr <= 0 || mr > 256 || sb <= 0 || sb > 32 || !Tg) return false;
+        if (ts) {
+            if (sb <= 16) rpc_panel_kernel<T, true, 16><<<1, 256, 0, s>>>((int)mr, (int)sb, P, ldp, A1, lda1, Tg, ldt, tau);
+            else rpc_panel_kernel<T, true, 32><<<1, 256, 0, s>>>((int)mr, (int)sb, P, ldp, A1, lda1, Tg, ldt, tau);
+        } else {
+            if (sb <= 16) rpc_panel_kernel<T, false, 16><<<1, 256, 0, s>>>((int)mr, (int)sb, P, ldp, A1, lda1, Tg, ldt, tau);
+            else rpc_panel_kernel<T, false, 32><<<1, 256, 0, s>>>((int)mr, (int)sb, P, ldp, A1, lda1, Tg, ldt, tau);
+        }
+        *err = hipGetLastError();
+        return true;
+    }
+}
+
 template <typename T>
 hipError_t launch_panel_qr(i64 m, i64 n, T *A, i64 lda, T *tau, T *Tm, i64 ldt, hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;
     size_t sh = (size_t)n * sizeof(T) * (Tm ? 2 : 1);
     ProfScope prof(PK_PANEL, s);
+    hipError_t e = hipSuccess;
+    if (launch_rpc_panel<T>(false, m, n, A, lda, (T *)nullptr, 0, Tm, ldt, tau, s, &e)) return e;
     panel_qr_kernel<T><<<1, 256, sh, s>>>((int)m, (int)n, A, lda, tau, Tm, ldt);
     return hipGetLastError();
 }
@@ -630,18 +857,289 @@ template <typename T>
 hipError_t launch_ts_panel(bool tt, i64 m, i64 ii, i64 sb, T *A1, i64 lda1, T *A2, i64 lda2, T *Tm, i64 ldt, T *tau,
                            hipStream_t s) {
     ProfScope prof(PK_TS_PANEL, s);
+    hipError_t e = hipSuccess;
+    if (!tt && launch_rpc_panel<T>(true, m, sb, A2 + ii * lda2, lda2, A1 + ii + ii * lda1, lda1, Tm + ii * ldt, ldt,
+                                   tau + ii, s, &e))
+        return e;
     if (tt) ts_panel_kernel<T, true><<<1, 256, 0, s>>>((int)m, (int)ii, (int)sb, A1, lda1, A2, lda2, Tm, ldt, tau);
     else ts_panel_kernel<T, false><<<1, 256, 0, s>>>((int)m, (int)ii, (int)sb, A1, lda1, A2, lda2, Tm, ldt, tau);
     return hipGetLastError();
 }
 
 // Left-form GE apply on an m x ncols view (trc: C stored conjugate-transposed).
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ int cc_of(int c, int ncols) { return c < ncols ? c : ncols - 1; }
+
+// MFMA block-reflector apply of the per-call path (left side, real types): for each ib-block
+// (forward or backward), W = V_b^T C (+ the block's rows of A1 for the TS form), W = op(T_b) W,
+// A1 rows -= W (TS form), C -= V_b W: larfb!/unmqr! (GE: V_b the unit lower trapezoid stored in
+// the tile, src/larfb.jl:57-98, src/unmqr.jl:127-148) and tsmqr!/parfb! with l = 0 (TS: V_b full,
+// src/tsmqr.jl:104-145, src/parfb.jl:121-129, src/pamm.jl:104-127,306-325).  A workgroup owns 16
+// columns of C for all blocks; wave w keeps rows 16 s + acc_row(lk, q) (s = w, w + 4, ...) of its
+// columns in registers from the first block to the last, in the B layout of V^T C and the D
+// layout of V W.  Rows <= 64 NU, blocks of at most 16 NG reflectors.  Global memory is read in
+// the prologue (C, the A1 rows, T) and, one block ahead, for V_b, so a block's phases wait only on
+// LDS, MFMA and barriers.  Every product is unconditional (rows past `rows` meet zero rows of V_b,
+// reflectors past kb zero columns of V_b and zero rows of T_b), so the accumulators never cross a
+// branch.
+constexpr int MA_KB = 64;      // widest block
+constexpr int MA_VCAP = 8448;  // V_b elements staged in LDS (row stride kb + 1)
+constexpr int MA_VX = 16;      // V_b elements per thread prefetched a block ahead (rows x kb <= 4096)
+constexpr int MA_TCAP = 4096;  // T elements staged in LDS at the start (ib x k)
+constexpr int MA_AX = 8;       // A1 elements per thread prefetched at the start (k <= 128)
+template <typename T, bool TS, int NU, int NG>
+__global__ __launch_bounds__(256) void mfma_apply_kernel(int rows, int ncols, int k, int ib, const T *__restrict__ V_,
+                                                         i64 ldv, const T *__restrict__ Tm_, i64 ldt, bool notrans,
+                                                         bool forward, T *__restrict__ A1_, i64 lda1, T *__restrict__ C_,
+                                                         i64 ldc) {
+    typedef typename Acc<T>::t AT;
+    __shared__ T Wp[4 * MA_KB * 16];
+    __shared__ T Ws[MA_KB * 16];
+    __shared__ T Wt[MA_KB * 16];
+    __shared__ T Vl[MA_VCAP];
+    __shared__ T Tl[MA_TCAP];
+    const GPtr<const T> V = to_global(V_), Tm = to_global(Tm_);
+    const GPtr<T> A1 = to_global(A1_), C = to_global(C_);
+    const int t = threadIdx.x, lane = t & 63, lr = lane & 15, lk = lane >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int c0 = blockIdx.x * 16;
+    const int nkt = (rows + 15) >> 4;
+    const int cc = min(c0 + lr, ncols - 1);
+    const int nblk = (k + ib - 1) / ib, kbmax = min(ib, k);
+    // V_b[r][j] (block origin column i, width kb): zero outside 0 <= r < rows, 0 <= j < kb; the GE
+    // form is unit lower trapezoidal below row i + j.  The load is unconditional at a clamped address.
+    auto vb = [&](int r, int j, int i, int kb) -> T {
+        const T x = V[min(r, rows - 1) + (i64)(i + min(j, kb - 1)) * ldv];
+        if (r >= rows || j >= kb) return T(0);
+        if (TS) return x;
+        return r > i + j ? x : (r == i + j ? T(1) : T(0));
+    };
+    auto blk_i = [&](int bb) { return (forward ? bb : nblk - 1 - bb) * ib; };
+    // prologue loads, all in flight together: this workgroup's columns of C
+    T cv[NU][4];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int s = w + 4 * u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cv[u][q] = C[min(16 * s + acc_row<T>(lk, q), rows - 1) + (i64)cc * ldc];
+    }
+    // the A1 rows of every block (TS form), element (row (t >> 4) + 16 x, column t & 15)
+    const bool a1pre = TS && k <= 16 * MA_AX && ib % 16 == 0;   // block rows start on a multiple of 16
+    T a1r[MA_AX];
+    if (a1pre) {
+#pragma unroll
+        for (int x = 0; x < MA_AX; ++x)
+            a1r[x] = A1[min((t >> 4) + 16 * x, k - 1) + (i64)cc_of(c0 + (t & 15), ncols) * lda1];
+    }
+    // T (ib x k) to LDS
+    const bool tpre = kbmax * k <= MA_TCAP;
+    if (tpre)
+        for (int col = t; col < k; col += 256)
+            for (int p0 = 0; p0 < kbmax; p0 += 16) {
+                T x[16];   // 16 loads in flight, then the stores
+#pragma unroll
+                for (int u = 0; u < 16; ++u) x[u] = Tm[min(p0 + u, kbmax - 1) + (i64)col * ldt];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (p0 + u < kbmax) Tl[(p0 + u) * k + col] = x[u];
+            }
+    // V_b of the first block into registers: element e = t + 256 x of the block, row e % rows,
+    // column e / rows
+    const bool vpre = NU <= 4 && rows * kbmax <= 256 * MA_VX && rows * (kbmax + 1) <= MA_VCAP;
+    int vr[MA_VX], vj[MA_VX];
+    T vx[MA_VX];
+    if (vpre) {
+#pragma unroll
+        for (int x = 0; x < MA_VX; ++x) {
+            const int e = t + 256 * x;
+            vj[x] = e / rows;
+            vr[x] = e - vj[x] * rows;
+            vx[x] = vb(vr[x], vj[x], blk_i(0), kbmax);
+        }
+    }
+    for (int bb = 0; bb < nblk; ++bb) {
+        const int i = blk_i(bb);
+        const int kb = min(ib, k - i);
+        const int vld = kb + 1;
+        const bool staged = rows * vld <= MA_VCAP;   // uniform
+        if (bb > 0) lds_barrier();   // the previous block's readers of Vl and Wt are done
+        if (vpre) {
+#pragma unroll
+            for (int x = 0; x < MA_VX; ++x)
+                if (vj[x] < kb) Vl[vr[x] * vld + vj[x]] = vx[x];
+            if (bb + 1 < nblk) {
+                const int i1 = blk_i(bb + 1), kb1 = min(ib, k - i1);
+#pragma unroll
+                for (int x = 0; x < MA_VX; ++x) vx[x] = vb(vr[x], vj[x], i1, kb1);
+            }
+        } else if (staged) {
+            for (int r = t; r < rows; r += 256)
+                for (int j0 = 0; j0 < kb; j0 += 16) {
+                    T x[16];   // 16 loads in flight, then the stores
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) x[u] = vb(r, j0 + u, i, kb);
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        if (j0 + u < kb) Vl[r * vld + j0 + u] = x[u];
+                }
+        }
+        lds_barrier();
+        auto vv = [&](int r, int j) -> T {
+            if (staged) return (r < rows && j < kb) ? Vl[min(r, rows - 1) * vld + min(j, kb - 1)] : T(0);
+            return vb(r, j, i, kb);
+        };
+        // W partials of this wave's rows
+        // one accumulator chain per (group, step pair) so that consecutive products are independent
+        constexpr int NC = NU >= 2 ? 2 : 1;
+        AT acc[NG][NC];
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[g][c] = acc_zero<T>();
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int s = w + 4 * u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = 16 * s + acc_row<T>(lk, q);
+#pragma unroll
+                for (int g = 0; g < NG; ++g) mfma16(acc[g][u % NC], vv(r, 16 * g + lr), cv[u][q]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                T sum = acc[g][0][e];
+#pragma unroll
+                for (int c = 1; c < NC; ++c) sum += acc[g][c][e];
+                Wp[(w * MA_KB + 16 * g + acc_row<T>(lk, e)) * 16 + lr] = sum;
+            }
+        lds_barrier();
+        // W = the partials (fixed order) + the block's A1 rows; element (j, c) = (t >> 4 + 16 x, t & 15)
+        T a1v[NG];
+#pragma unroll
+        for (int x = 0; x < NG; ++x) {
+            const int j = (t >> 4) + 16 * x, c = t & 15;
+            {
+                T s = T(0);
+                if (j < kb) {
+                    s = (Wp[j * 16 + c] + Wp[(MA_KB + j) * 16 + c]) + (Wp[(2 * MA_KB + j) * 16 + c] + Wp[(3 * MA_KB + j) * 16 + c]);
+                    if (TS) {
+                        T a;
+                        if (a1pre) {
+                            a = T(0);
+#pragma unroll
+                            for (int y = 0; y < MA_AX; ++y)
+                                if (16 * y == i + 16 * x) a = a1r[y];   // row i + j = (t >> 4) + 16 y
+                        } else {
+                            a = A1[i + j + (i64)cc_of(c0 + c, ncols) * lda1];
+                        }
+                        a1v[x] = a;
+                        s += a;
+                    }
+                }
+                Ws[j * 16 + c] = s;
+            }
+        }
+        lds_barrier();
+        // W := op(T_b) W on MFMA, wave w for reflector group w
+        if (w < NG) {
+            AT d = acc_zero<T>();
+            const int j = 16 * w + lr;
+            T ta[4 * NG];   // op(T_b)[j][4 q + lk]
+#pragma unroll
+            for (int q = 0; q < 4 * NG; ++q) {
+                const int p = 4 * q + lk;
+                const int jc = min(j, kb - 1), pc = min(p, kb - 1);
+                const int rr = notrans ? jc : pc, col = i + (notrans ? pc : jc);
+                const T x = tpre ? Tl[rr * k + col] : Tm[rr + (i64)col * ldt];
+                ta[q] = (j < kb && p < kb && (notrans ? p >= j : p <= j)) ? x : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4 * NG; ++q) mfma16(d, ta[q], Ws[(4 * q + lk) * 16 + lr]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) Wt[(16 * w + acc_row<T>(lk, e)) * 16 + lr] = d[e];
+        }
+        lds_barrier();
+        // the TS form's A1 rows -= W, by the threads that read them
+        if (TS) {
+#pragma unroll
+            for (int x = 0; x < NG; ++x) {
+                const int j = (t >> 4) + 16 * x, c = t & 15;
+                if (j < kb && c0 + c < ncols) A1[i + j + (i64)(c0 + c) * lda1] = a1v[x] - Wt[j * 16 + c];
+            }
+        }
+        // C -= V_b W: A = V_b rows of the step, B = W
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int s = w + 4 * u;
+            AT d0 = acc_zero<T>(), d1 = acc_zero<T>();
+#pragma unroll
+            for (int q = 0; q < 4 * NG; q += 2) {
+                mfma16(d0, vv(16 * s + lr, 4 * q + lk), Wt[(4 * q + lk) * 16 + lr]);
+                mfma16(d1, vv(16 * s + lr, 4 * q + 4 + lk), Wt[(4 * q + 4 + lk) * 16 + lr]);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cv[u][e] -= d0[e] + d1[e];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int s = w + 4 * u;
+        if (s < nkt) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * s + acc_row<T>(lk, e);
+                if (r < rows && c0 + lr < ncols) C[r + (i64)(c0 + lr) * ldc] = cv[u][e];
+            }
+        }
+    }
+}
+
+// The MFMA apply takes the left form of real element types with blocks of at most 64
+// reflectors and at most 512 rows; it returns false (nothing launched) otherwise.
+template <typename T>
+static bool launch_mfma_apply(bool ts, int rows, int ncols, int k, int ib, const T *V, i64 ldv, const T *Tm, i64 ldt,
+                              bool notrans, bool forward, T *A1, i64 lda1, T *C, i64 ldc, hipStream_t s,
+                              hipError_t *err) {
+    if constexpr (IsCplx<T>::v) {
+        return false;
+    } else {
+        if (ib > MA_KB || rows > 512 || rows <= 0 || ncols <= 0 || k <= 0) return false;
+        dim3 grid((unsigned)((ncols + 15) / 16)), block(256);
+        const int nu = rows <= 128 ? 2 : rows <= 256 ? 4 : 8;
+        const int kbm = ib < k ? ib : k;
+        const int ng = kbm <= 16 ? 1 : kbm <= 32 ? 2 : 4;
+#define NLA_MA(TSV, NUV, NGV)                                                                                         \
+    mfma_apply_kernel<T, TSV, NUV, NGV><<<grid, block, 0, s>>>(rows, ncols, k, ib, V, ldv, Tm, ldt, notrans, forward, \
+                                                              A1, lda1, C, ldc)
+#define NLA_MA_NU(TSV, NGV)                                                                                           \
+    do {                                                                                                              \
+        if (nu == 2) NLA_MA(TSV, 2, NGV); else if (nu == 4) NLA_MA(TSV, 4, NGV); else NLA_MA(TSV, 8, NGV);            \
+    } while (0)
+        if (ts) {
+            if (ng == 1) NLA_MA_NU(true, 1); else if (ng == 2) NLA_MA_NU(true, 2); else NLA_MA_NU(true, 4);
+        } else {
+            if (ng == 1) NLA_MA_NU(false, 1); else if (ng == 2) NLA_MA_NU(false, 2); else NLA_MA_NU(false, 4);
+        }
+#undef NLA_MA_NU
+#undef NLA_MA
+        *err = hipGetLastError();
+        return true;
+    }
+}
+
 template <typename T>
 hipError_t launch_ge_apply(bool trc, i64 m, i64 ncols, i64 k, i64 ib, const T *V, i64 ldv, const T *Tm, i64 ldt,
                            bool notrans, bool forward, T *C, i64 ldc, hipStream_t s) {
     if (m <= 0 || ncols <= 0 || k <= 0) return hipSuccess;
     dim3 grid((unsigned)((ncols + CW - 1) / CW)), block(256);
     ProfScope prof(PK_GE_APPLY, s);
+    hipError_t e = hipSuccess;
+    if (!trc && launch_mfma_apply<T>(false, (int)m, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward,
+                                     (T *)nullptr, 0, C, ldc, s, &e))
+        return e;
     if (trc) ge_apply_kernel<T, true><<<grid, block, 0, s>>>((int)m, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, C, ldc);
     else ge_apply_kernel<T, false><<<grid, block, 0, s>>>((int)m, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, C, ldc);
     return hipGetLastError();
@@ -654,6 +1152,10 @@ hipError_t launch_ts_apply_mode(bool trc, i64 m2, i64 ncols, i64 k, i64 ib, cons
     if (m2 <= 0 || ncols <= 0 || k <= 0) return hipSuccess;
     dim3 grid((unsigned)((ncols + CW - 1) / CW)), block(256);
     ProfScope prof(PK_TS_APPLY, s);
+    hipError_t e = hipSuccess;
+    if (!trc && mode == 0 && lfix == 0 && launch_mfma_apply<T>(true, (int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans,
+                                                  forward, A1, lda1, A2, lda2, s, &e))
+        return e;
     if (trc) ts_apply_kernel<T, true><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2, mode, lfix);
     else ts_apply_kernel<T, false><<<grid, block, 0, s>>>((int)m2, (int)ncols, (int)k, (int)ib, V, ldv, Tm, ldt, notrans, forward, A1, lda1, A2, lda2, mode, lfix);
     return hipGetLastError();
@@ -1469,6 +1971,45 @@ static int tsmqr_entry(char side, char trans, i64 m1, i64 n1, i64 m2, i64 n2, i6
                         as_stream(stream));
 }
 
+// The reference's tile-QR call sequence (its callers' flat-tree loop over geqrt!/unmqr!/tsqrt!/
+// tsmqr!, one call per tile or tile pair), issued through the same per-call entry points a
+// drop-in caller reaches.  Same output layout as nla_?tile_geqrf: T tile (i,k) at
+// T[i*ib, k*nb].  Mirrors the caller loop; the library's own tile_geqrf runs the same
+// factorization as one persistent launch.
+template <typename T>
+static int tile_geqrf_calls_entry(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, void *stream) {
+    if (m < 0) return arg_error(1, "m must be non-negative");
+    if (n < 0) return arg_error(2, "n must be non-negative");
+    if (nb <= 0) return arg_error(3, "nb must be positive");
+    if (ib <= 0 || ib > nb) return arg_error(4, "ib must be in 1..nb");
+    if (m % nb || n % nb) return arg_error(1, "m and n must be multiples of nb");
+    if (m == 0 || n == 0) return 0;
+    if (lda < m) return arg_error(6, "lda must be >= m");
+    const i64 mt = m / nb, nt = n / nb, kt = mt < nt ? mt : nt;
+    if (ldt < mt * ib) return arg_error(8, "ldt must be >= (m/nb)*ib");
+    hipStream_t s = as_stream(stream);
+    int st = 0;
+    T *tau = (T *)scratch(SLOT_AUX, sizeof(T) * (size_t)nb, s, &st);
+    if (!tau) return st;
+    auto tile = [&](i64 i, i64 j) { return A + i * nb + j * nb * lda; };
+    auto ttile = [&](i64 i, i64 j) { return Tm + i * ib + j * nb * ldt; };
+    for (i64 k = 0; k < kt; ++k) {
+        if ((st = geqrt_entry<T>(nb, nb, ib, tile(k, k), lda, ttile(k, k), ldt, tau, s))) return st;
+        for (i64 j = k + 1; j < nt; ++j)
+            if ((st = unmqr_entry<T>('L', 'C', nb, nb, nb, ib, tile(k, k), lda, nb, ttile(k, k), ldt, tile(k, j), lda, s)))
+                return st;
+        for (i64 i = k + 1; i < mt; ++i) {
+            if ((st = tsqrt_entry<T>(nb, nb, ib, tile(k, k), lda, nb, tile(i, k), lda, ttile(i, k), ldt, tau, s)))
+                return st;
+            for (i64 j = k + 1; j < nt; ++j)
+                if ((st = tsmqr_entry<T>('L', 'C', nb, nb, nb, nb, nb, ib, tile(k, j), lda, tile(i, j), lda, tile(i, k),
+                                         lda, nb, ttile(i, k), ldt, nb, s)))
+                    return st;
+        }
+    }
+    return 0;
+}
+
 // ttmqr!(side, trans, m1, n1, m2, n2, k, ib, A1, A2, V, T, work): src/ttmqr.jl:1-95.
 template <typename T>
 static int ttmqr_entry(char side, char trans, i64 m1, i64 n1, i64 m2, i64 n2, i64 k, i64 ib, T *A1, i64 lda1, T *A2,
@@ -1635,6 +2176,10 @@ using namespace nla;
         (void)work;                                                                                                    \
         return tsmqr_entry<T>(side, trans, m1, n1, m2, n2, k, ib, A1, lda1, A2, lda2, V, ldv, v_cols, Tm, ldt, t_cols, \
                               stream);                                                                                 \
+    }                                                                                                                  \
+    NLA_API int nla_##p##tile_geqrf_calls(int64_t m, int64_t n, int64_t nb, int64_t ib, T *A, int64_t lda, T *Tm,    \
+                                          int64_t ldt, void *stream) {                                                 \
+        return tile_geqrf_calls_entry<T>(m, n, nb, ib, A, lda, Tm, ldt, stream);                                       \
     }                                                                                                                  \
     NLA_API int nla_##p##ttqrt(int64_t m, int64_t n, int64_t ib, T *A1, int64_t lda1, T *A2, int64_t lda2, T *Tm,     \
                                int64_t ldt, T *tau, T *work, void *stream) {                                           \

@@ -37,6 +37,7 @@ _SIGS = {
     "trmm_base": [ch, ch, i64, i64, vp, i64, vp, i64, vp],
     "gemm_acc": [i64, i64, i64, "R", vp, i64, vp, i64, vp, i64, vp],
     "tile_geqrf": [i64, i64, i64, i64, vp, i64, vp, i64, vp],
+    "tile_geqrf_calls": [i64, i64, i64, i64, vp, i64, vp, i64, vp],
     "tile_ormqr": [ch, i64, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp],
     "trsm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
     "trmm_rrule": [ch, ch, ch, "R", i64, i64, vp, i64, vp, i64, vp, i64, vp, i64, i64, vp],
@@ -56,7 +57,7 @@ _SIGS_D = {
 
 # the QR-family routines also exist for complex element types ('z' ComplexF64, 'c' ComplexF32)
 QR_FAMILY = ("larfg", "geqr2", "larft", "geqrt", "larfb", "unmqr", "tsqrt", "tsmqr", "parfb", "ttqrt", "ttmqr",
-             "pamm", "pemv")
+             "pamm", "pemv", "tile_geqrf_calls")
 
 
 class ZComplex(ctypes.Structure):

@@ -19,7 +19,8 @@ __all__ = [
     "ttmqr", "pamm_", "pemv_", "pemv",
     "unified_rectrxm_", "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_", "trsm", "trmm",
-    "GEMM_ADD_", "GEMM_SUB_", "tile_geqrf", "tile_ormqr", "tile_geqrf_mixed", "tile_gels", "rectrsm_mixed_",
+    "GEMM_ADD_", "GEMM_SUB_", "tile_geqrf", "tile_geqrf_calls", "tile_ormqr", "tile_geqrf_mixed", "tile_gels",
+    "rectrsm_mixed_",
     "getrf2_", "lu_", "laswp_",
     "NextLAMatrix",
     "synchronize",
@@ -526,6 +527,19 @@ def tile_geqrf(A, nb, ib):
     pa, lda = _m(A)
     pt, ldt = _m(T)
     call(p, "tile_geqrf", m, n, nb, ib, pa, lda, pt, ldt, _stream())
+    return T
+
+
+def tile_geqrf_calls(A, nb, ib):
+    """The factorization of tile_geqrf issued as the reference's callers issue it: one
+    geqrt!/unmqr!/tsqrt!/tsmqr! call per tile or tile pair (flat tree, PLASMA order), through
+    the per-call entry points.  Returns T in tile_geqrf's layout."""
+    p = _p(A)
+    m, n = A.shape
+    T = zeros_cm((m // nb) * ib, n, dtype=A.dtype, device=A.device)
+    pa, lda = _m(A)
+    pt, ldt = _m(T)
+    call(p, "tile_geqrf_calls", m, n, nb, ib, pa, lda, pt, ldt, _stream())
     return T
 
 
