@@ -10,7 +10,9 @@ Extra fields: Float32 TRSM and Float64 TRMM of the same shape, the solve's backw
 the mixed-precision solve (Float32 + Float64 refinement), Float64 and reduced-precision-update
 tile QR (flat-tree geqrt/unmqr/tsqrt/tsmqr) at 16384^2 with backward errors, the same
 factorizations compared at equal backward error as refined solves of a 16384^2 system, the
-batched uneven workload (4096 problems, one graph-replayed launch), adjoint timings, the
+recursive LU (getrf2!) at 8192^2, the batched uneven workload (4096 problems, one
+graph-replayed launch), the reference's per-tile call sequence (one geqrt!/unmqr!/tsqrt!/
+tsmqr! launch per tile, 8192^2) beside the persistent tile QR, adjoint timings, the
 32768 x 8192 least-squares workload; the roofline of the dominant kernel (the 128 x 128-tile
 MFMA GEMM of the level-1/2 off-diagonal updates, timed live with hipEvents on its launch stream
 in a second pass of the same steps, so that the event packets do not slow the pass that gives
@@ -19,7 +21,7 @@ in a second pass of the same steps, so that the event packets do not slow the pa
 median of 5 runs after a warm-up.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--qr-n N]
-                    [--extras trsm32,trmm,trsm_mixed,qr,qr_solve,batched,adjoints,ls] [--no-extras]
+                    [--extras trsm32,trmm,trsm_mixed,qr,qr_solve,lu,batched,percall,adjoints,ls] [--no-extras]
 """
 import argparse
 import json
@@ -369,6 +371,35 @@ def batched(torch, nl, gen, count=4096, nrhs=32, fixed_ib=0):
             "launch": "one persistent kernel replayed from a hipGraph"}
 
 
+def percall(torch, nl, gen, n=8192, nb=128, ib=32):
+    """The reference's tile-QR call sequence, one geqrt!/unmqr!/tsqrt!/tsmqr! launch per tile or
+    tile pair in flat-tree order (nla_dtile_geqrf_calls: the loop a NextLA caller writes), beside
+    the persistent nla_dtile_geqrf, Float64 n x n, nb = 128, ib = 32."""
+    nt = n // nb
+    calls = {"geqrt": nt, "unmqr": nt * (nt - 1) // 2, "tsqrt": nt * (nt - 1) // 2,
+             "tsmqr": sum(j * j for j in range(nt))}
+    A0 = nl.colmajor(torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64))
+    warm = nl.colmajor(A0[:1024, :1024].clone())
+    nl.tile_geqrf_calls(warm, nb, ib)
+    torch.cuda.synchronize()
+    res = {}
+    for name, fn in (("persistent", nl.tile_geqrf), ("per_call", nl.tile_geqrf_calls)):
+        A = nl.colmajor(A0.clone())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(A, nb, ib)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        res[name] = {"ms": round(t * 1e3, 2), "tflops": round(4.0 / 3.0 * n ** 3 / t / 1e12, 3)}
+        del A
+    res["per_call"]["launches"] = calls
+    res["per_call"]["us_per_call"] = round(res["per_call"]["ms"] * 1e3 / sum(calls.values()), 2)
+    res["persistent_vs_per_call"] = round(res["per_call"]["ms"] / res["persistent"]["ms"], 1)
+    log(f"[bench] tile QR call sequence n={n}: per-call {res['per_call']['ms']} ms "
+        f"({res['per_call']['us_per_call']} us per call), persistent {res['persistent']['ms']} ms")
+    return {"n": n, "nb": nb, "ib": ib, "dtype": "f64", **res}
+
+
 def adjoints(torch, nl, gen):
     """Adjoint kernels at the differentiation workload's sizes: TRSM A 4096, B 4096 x 512;
     thin QR of 8192 x 1024 (Float64); time of the backward pass alone."""
@@ -416,9 +447,9 @@ def main():
     ap.add_argument("--cutoff", type=int, default=256)
     ap.add_argument("--qr-n", type=int, default=16384)
     ap.add_argument("--ls", action="store_true", help="also run the 32768 x 8192 least-squares workload")
-    ap.add_argument("--extras", default="trsm32,trmm,trsm_mixed,qr,qr_solve,lu,batched,adjoints,ls",
+    ap.add_argument("--extras", default="trsm32,trmm,trsm_mixed,qr,qr_solve,lu,batched,percall,adjoints,ls",
                     help="comma-separated extra workloads: trsm32, trmm, trsm_mixed, qr, qr_solve, lu, batched, "
-                         "adjoints, ls")
+                         "percall, adjoints, ls")
     ap.add_argument("--qr-no-check", action="store_true", help="skip the tile-QR backward-error check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -448,6 +479,7 @@ def main():
         gen.manual_seed(8192)
         only = set(x for x in args.only.split(",") if x)
         runners = {"batched": lambda: batched(torch, nl, gen, fixed_ib=args.batched_ib),
+                   "percall": lambda: percall(torch, nl, gen),
                    "lu": lambda: lu_factor(torch, nl, gen), "adjoints": lambda: adjoints(torch, nl, gen),
                    "ls": lambda: least_squares(torch, nl, gen),
                    "qr_solve": lambda: qr_solve_equal_error(torch, nl, gen, n=args.qr_n)}
@@ -610,6 +642,8 @@ def main():
         extras["lu"] = lu_factor(torch, nl, gen)
     if "batched" in chosen:
         extras["batched"] = batched(torch, nl, gen, fixed_ib=args.batched_ib)
+    if "percall" in chosen:
+        extras["tile_qr_call_sequence"] = percall(torch, nl, gen)
     if "adjoints" in chosen:
         extras["adjoints"] = adjoints(torch, nl, gen)
     if "ls" in chosen:

@@ -23,17 +23,21 @@ def rtol_for(dtype):
     return RTOL[np.dtype(dtype).type]
 
 
+def _wide(x):
+    # complex inputs keep their imaginary parts
+    x = np.asarray(x)
+    return x.astype(np.complex128 if np.iscomplexobj(x) else np.float64)
+
+
 def isapprox(x, y, rtol):
     """Julia's isapprox for arrays: norm(x - y) <= rtol * max(norm(x), norm(y))."""
-    x = np.asarray(x, dtype=np.float64)
-    y = np.asarray(y, dtype=np.float64)
+    x, y = _wide(x), _wide(y)
     d = np.linalg.norm(x - y)
     return d <= rtol * max(np.linalg.norm(x), np.linalg.norm(y))
 
 
 def relerr(x, y):
-    x = np.asarray(x, dtype=np.float64)
-    y = np.asarray(y, dtype=np.float64)
+    x, y = _wide(x), _wide(y)
     den = max(np.linalg.norm(y), 1e-300)
     return np.linalg.norm(x - y) / den
 
