@@ -577,22 +577,30 @@ def main():
             f"{extras['trmm_f64_tflops_cutoff16']} (the reference's 16)")
 
     if "trsm_mixed" in chosen:
-        # mixed-precision TRSM (Float32 solve + Float64 refinement), same problem
+        # mixed-precision TRSM (Float32-storage recursive solve whose GEMM updates take Float32,
+        # FP16 or BF16 operands, + Float64 residual refinement), same problem
         Bm = nl.colmajor(torch.empty_like(B0))
-        info = {}
+        res = {}
+        for prec in ("S", "H", "B"):
+            info = {}
 
-        def smix():
-            Bm.copy_(B0)
-            _, info["it"], info["last"] = nl.rectrsm_mixed_("L", "L", "N", 1.0, A, Bm, cutoff=cut, maxit=6, tol=1e-15)
-        tmx = time_steps(torch, smix, 3, 1)
-        Rm = torch.tril(A) @ Bm - B0
-        berr_m = (torch.linalg.norm(Rm) / (torch.linalg.norm(torch.tril(A)) * torch.linalg.norm(Bm))).item()
-        del Rm, Bm
-        extras["trsm_mixed"] = {"scheme": "Float32 recursive solve + Float64 residual refinement",
-                                "ms": round(tmx * 1e3, 3), "tflops": round(flops / tmx / 1e12, 3),
-                                "vs_f64": round(sec / tmx, 3), "iterations": info["it"],
-                                "backward_error": berr_m, "backward_error_f64": berr}
-        log(f"[bench] TRSM mixed: {tmx*1e3:.2f} ms, {info['it']} refinement steps, berr {berr_m:.2e}")
+            def smix():
+                Bm.copy_(B0)
+                _, info["it"], info["last"] = nl.rectrsm_mixed_("L", "L", "N", 1.0, A, Bm, cutoff=cut, maxit=12,
+                                                                tol=1e-15, prec=prec)
+            tmx = time_steps(torch, smix, 3, 1)
+            Rm = torch.tril(A) @ Bm - B0
+            berr_m = (torch.linalg.norm(Rm) / (torch.linalg.norm(torch.tril(A)) * torch.linalg.norm(Bm))).item()
+            del Rm
+            res[{"S": "f32", "H": "f16", "B": "bf16"}[prec]] = {
+                "ms": round(tmx * 1e3, 3), "tflops": round(flops / tmx / 1e12, 3), "vs_f64": round(sec / tmx, 3),
+                "iterations": info["it"], "backward_error": berr_m}
+            log(f"[bench] TRSM mixed ({prec} updates): {tmx*1e3:.2f} ms, {info['it']} refinement steps, "
+                f"berr {berr_m:.2e}")
+        del Bm
+        extras["trsm_mixed"] = {"scheme": "Float32-storage recursive solve (GEMM updates with Float32 / FP16 / "
+                                          "BF16 operands, Float32 accumulation) + Float64 residual refinement",
+                                "backward_error_f64": berr, **res}
 
     if "qr" in chosen:
         # tile QR at the north-star size: Float64 and Float32-update mixed, nb = 128, ib = 32

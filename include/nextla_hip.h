@@ -374,12 +374,16 @@ int nla_stile_gels(int64_t m, int64_t n, int64_t nb, int64_t ib, const float *A,
                    float *x, char prec, int64_t maxit, double tol, int64_t *iters, double *last_correction,
                    char *prec_used, void *stream);
 /* Triangular solve op(A) X = alpha B / X op(A) = alpha B (arguments as nla_drectrxm with
- * func 'S'; X overwrites B) in Float32 with Float64 iterative refinement: X0 from the
- * Float32 recursion (cutoff as there), then up to maxit steps X += op(A)^-1 (alpha B - op(A) X)
- * with the residual in Float64 and the correction in Float32, stopping when
- * ||dX|| <= tol ||X|| or when a step shrinks ||dX|| by less than half.  iters / last_correction may be NULL.  Synchronises the stream. */
+ * func 'S'; X overwrites B) in Float32 storage with Float64 iterative refinement: X0 from the
+ * Float32 recursion (cutoff as there) whose GEMM updates take Float32 (prec 'S'), FP16 ('H')
+ * or BF16 ('B') operands with Float32 accumulation (diagonal blocks stay Float32), then up to
+ * maxit steps X += op(A)^-1 (alpha B - op(A) X) with the residual in Float64 and the correction
+ * solved as X0 was, stopping when ||dX|| <= tol ||X|| or when a step shrinks ||dX|| by less than
+ * half.  'H' needs |A| and the corrections below the FP16 range (65504).  iters /
+ * last_correction may be NULL.  Synchronises the stream.  No reference function: the
+ * reference's unified_rectrxm! (src/rectrxm.jl:43-76) runs in one precision. */
 int nla_drectrsm_mixed(char side, char uplo, char trans, double alpha, int64_t n, int64_t m, const double *A,
-                       int64_t lda, double *B, int64_t ldb, int64_t cutoff, int64_t maxit, double tol,
+                       int64_t lda, double *B, int64_t ldb, int64_t cutoff, int64_t maxit, double tol, char prec,
                        int64_t *iters, double *last_correction, void *stream);
 
 /* ---- Batched uneven problems (no reference function: the batched workload) ------------- */

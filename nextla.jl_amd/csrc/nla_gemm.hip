@@ -14,6 +14,8 @@
 #include "nla_common.hpp"
 #include "nla_gemm.hpp"
 
+#include <type_traits>
+
 namespace nla {
 
 template <typename T> struct GemmPad { static constexpr int v = 2; };
@@ -189,6 +191,126 @@ static hipError_t launch_sized(bool ta, bool tb, int M, int N, int K, T alpha, c
     return launch_tiles<T, BM, BN, BK, true, NSET>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
+// Float32 GEMM with 16-bit operands: the slabs of op(A) and op(B) are rounded to FP16 or BF16
+// (round to nearest even) as they are staged in LDS, and v_mfma_f32_16x16x32_{f16,bf16}
+// accumulates in Float32.  LDS holds op(A) as As[m][k] and op(B) as Bs[n][k] (k contiguous, 8
+// elements per lane and product); as in gemm_kernel the MFMA's A slot takes the op(B) fragment
+// so that a lane's accumulator rows run down a column of C.  128 x 128 tiles, BK = 32, bounds
+// checked.  Used for the reduced-precision updates of the mixed-precision triangular solve.
+template <typename Tl> struct Mfma16x32;
+template <> struct Mfma16x32<_Float16> {
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ void run(fx4 &c, v8 a, v8 b) { c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+template <> struct Mfma16x32<__bf16> {
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ void run(fx4 &c, v8 a, v8 b) { c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+
+template <typename Tl, bool TA, bool TB>
+__global__ __launch_bounds__(256, 2) void gemm_lowp_kernel(int M, int N, int K, float alpha, const float *__restrict__ A,
+                                                          i64 lda, const float *__restrict__ B, i64 ldb, float beta,
+                                                          float *__restrict__ C, i64 ldc, int tiles_m) {
+    constexpr int BM = 128, BN = 128, BK = 32, LD = BK + 8;
+    typedef typename Mfma16x32<Tl>::v8 v8;
+    __shared__ __align__(16) Tl As[BM * LD];
+    __shared__ __align__(16) Tl Bs[BN * LD];
+    int bid = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    const int tm = bid % tiles_m, tn = bid / tiles_m;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w & 1, wn = w >> 1;
+    const int lr = lane & 15, lk = lane >> 4;
+    // slab element i of this thread: op(A)(m, k) and op(B)(k, n), 16 of each
+    const int am = TA ? t / BK : t % BM, ak = TA ? t % BK : t / BM;
+    const int bk = TB ? t / BN : t % BK, bn = TB ? t % BN : t / BK;
+    constexpr int AMS = TA ? 256 / BK : 0, AKS = TA ? 0 : 256 / BM;
+    constexpr int BKS = TB ? 256 / BN : 0, BNS = TB ? 0 : 256 / BK;
+    float ra[16], rb[16];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int gm = m0 + am + i * AMS, gk = k0 + ak + i * AKS;
+            const bool ok = gm < M && gk < K;
+            const i64 off = TA ? (i64)(ok ? gk : 0) + (i64)(ok ? gm : 0) * lda : (i64)(ok ? gm : 0) + (i64)(ok ? gk : 0) * lda;
+            const float x = A[off];
+            ra[i] = ok ? x : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int gn = n0 + bn + i * BNS, gk = k0 + bk + i * BKS;
+            const bool ok = gn < N && gk < K;
+            const i64 off = TB ? (i64)(ok ? gn : 0) + (i64)(ok ? gk : 0) * ldb : (i64)(ok ? gk : 0) + (i64)(ok ? gn : 0) * ldb;
+            const float x = B[off];
+            rb[i] = ok ? x : 0.f;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) As[(am + i * AMS) * LD + ak + i * AKS] = (Tl)ra[i];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Bs[(bn + i * BNS) * LD + bk + i * BKS] = (Tl)rb[i];
+    };
+    fx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_zero<float>();
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (k0 + BK < K) load(k0 + BK);   // the next slab's loads overlap this slab's products
+        v8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const v8 *>(&As[(wm * 64 + 16 * i + lr) * LD + 8 * lk]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const v8 *>(&Bs[(wn * 64 + 16 * j + lr) * LD + 8 * lk]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Mfma16x32<Tl>::run(acc[i][j], b[j], a[i]);
+    }
+    // acc[i][j] element r of lane l: C[m0 + wm 64 + 16 i + l % 16][n0 + wn 64 + 16 j + 4 (l / 16) + r]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gm = m0 + wm * 64 + 16 * i + lr;
+        if (gm >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gn = n0 + wn * 64 + 16 * j + 4 * lk + r;
+                if (gn < N) {
+                    float *c = &C[gm + (i64)gn * ldc];
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.f) v += beta * *c;
+                    *c = v;
+                }
+            }
+    }
+}
+
+// Operand precision of the Float32 GEMMs issued by the calling host thread: 'S' (Float32,
+// the default), 'H' (FP16 operands) or 'B' (BF16 operands), set for a scope by GemmLowp.
+static thread_local char g_gemm_lowp = 'S';
+char gemm_lowp_get() { return g_gemm_lowp; }
+void gemm_lowp_set(char p) { g_gemm_lowp = p; }
+
+template <typename Tl>
+static hipError_t launch_lowp(bool ta, bool tb, int M, int N, int K, float alpha, const float *A, i64 lda,
+                              const float *B, i64 ldb, float beta, float *C, i64 ldc, hipStream_t s) {
+    const int tm = (M + 127) / 128, tn = (N + 127) / 128;
+    dim3 grid(tm * tn), block(256);
+    if (!ta && !tb) gemm_lowp_kernel<Tl, false, false><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else if (ta && !tb) gemm_lowp_kernel<Tl, true, false><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else if (!ta && tb) gemm_lowp_kernel<Tl, false, true><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    else gemm_lowp_kernel<Tl, true, true><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t gemm(char ta, char tb, i64 M, i64 N, i64 K, T alpha, const T *A, i64 lda, const T *B, i64 ldb, T beta,
                 T *C, i64 ldc, hipStream_t s) {
@@ -201,6 +323,12 @@ hipError_t gemm(char ta, char tb, i64 M, i64 N, i64 K, T alpha, const T *A, i64 
     const i64 big_tiles = ((M + 127) / 128) * ((N + 127) / 128);
     // "gemm" counts every launch; "gemm128" the 128 x 128-tile launches alone (one kernel)
     ProfScope prof(PK_GEMM, s);
+    if constexpr (std::is_same<T, float>::value) {
+        if (g_gemm_lowp == 'H' && K > 0)
+            return launch_lowp<_Float16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+        if (g_gemm_lowp == 'B' && K > 0)
+            return launch_lowp<__bf16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    }
     if (big_tiles >= 512) {
         ProfScope prof128(PK_GEMM128, s);
         return launch_sized<T, 128, 128, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);

@@ -1,7 +1,9 @@
 // Mixed-precision recursive triangular solve with iterative refinement (no reference
 // function: the reference's unified_rectrxm!, src/rectrxm.jl:43-76, runs in one precision).
-//   X0 = op(A)^-1 (alpha B) solved entirely in Float32 (the recursion's GEMM updates on
-//        v_mfma_f32_16x16x4, the 32x32 diagonal-block inverses and base solves in Float32);
+//   X0 = op(A)^-1 (alpha B) solved in Float32 storage: the diagonal-block inverses and base
+//        solves in Float32, the recursion's GEMM updates (the GEMM_SUB! call sites,
+//        src/rectrxm.jl:164,176,183,192) with Float32 ('S', v_mfma_f32_16x16x4f32), FP16 ('H')
+//        or BF16 ('B', v_mfma_f32_16x16x32_{f16,bf16}) operands and Float32 accumulation;
 //   R_k = alpha B - op(A) X_k in Float64 (a Float64 recursive TRMM, cut-off 256);
 //   X_{k+1} = X_k + op(A)^-1 R_k solved in Float32.
 // Each step multiplies the error by about kappa(A) u_32.  The Float64 residual costs as
@@ -9,6 +11,7 @@
 // differ by 2x this path cannot beat the plain Float64 solve at Float64 accuracy; it is
 // the measured comparison point for that claim.
 #include "nla_common.hpp"
+#include "nla_gemm.hpp"
 
 #include <cmath>
 #include <vector>
@@ -61,9 +64,10 @@ __global__ void update_kernel(int rows, int cols, const float *D32, i64 ldd, dou
 }
 
 static int rectrsm_mixed_entry(char side, char uplo, char trans, double alpha, i64 n, i64 m, const double *A, i64 lda,
-                               double *B, i64 ldb, i64 cutoff, i64 maxit, double tol, i64 *iters, double *last,
-                               void *stream) {
-    side = up(side), uplo = up(uplo), trans = up(trans);
+                               double *B, i64 ldb, i64 cutoff, i64 maxit, double tol, char prec, i64 *iters,
+                               double *last, void *stream) {
+    side = up(side), uplo = up(uplo), trans = up(trans), prec = up(prec);
+    if (prec != 'S' && prec != 'H' && prec != 'B') return arg_error(14, "prec must be 'S', 'H' or 'B', got '%c'", prec);
     if (side != 'L' && side != 'R') return arg_error(1, "side must be 'L' or 'R', got '%c'", side);
     if (uplo != 'L' && uplo != 'U') return arg_error(2, "uplo must be 'L' or 'U', got '%c'", uplo);
     if (trans != 'N' && trans != 'T' && trans != 'C') return arg_error(3, "transpose must be 'N', 'T' or 'C'");
@@ -93,7 +97,10 @@ static int rectrsm_mixed_entry(char side, char uplo, char trans, double alpha, i
     NLA_LAUNCH_CHECK();
     convert_kernel<float, double><<<gb, 256, 0, s>>>((int)rows, (int)cols, 0, 1.0, B0, rows, R32, rows);
     NLA_LAUNCH_CHECK();
-    NLA_ST(rectrxm_async<float>(side, uplo, trans, 1.0f, 'S', n, m, A32, n, R32, rows, cutoff, s));
+    {
+        GemmLowp lowp(prec);
+        NLA_ST(rectrxm_async<float>(side, uplo, trans, 1.0f, 'S', n, m, A32, n, R32, rows, cutoff, s));
+    }
     convert_kernel<double, float><<<gb, 256, 0, s>>>((int)rows, (int)cols, 0, 1.0f, R32, rows, B, ldb);
     NLA_LAUNCH_CHECK();
     double prev = 0;
@@ -104,8 +111,11 @@ static int rectrsm_mixed_entry(char side, char uplo, char trans, double alpha, i
         NLA_ST(rectrxm_async<double>(side, uplo, trans, 1.0, 'M', n, m, A, lda, Y, rows, 256, s));
         residual_kernel<<<gb, 256, 0, s>>>((int)rows, (int)cols, B0, rows, Y, rows, R32, rows);
         NLA_LAUNCH_CHECK();
-        // D = op(A)^-1 R in Float32; X += D
-        NLA_ST(rectrxm_async<float>(side, uplo, trans, 1.0f, 'S', n, m, A32, n, R32, rows, cutoff, s));
+        // D = op(A)^-1 R in Float32 storage with the chosen update precision; X += D
+        {
+            GemmLowp lowp(prec);
+            NLA_ST(rectrxm_async<float>(side, uplo, trans, 1.0f, 'S', n, m, A32, n, R32, rows, cutoff, s));
+        }
         NLA_HIP(hipMemsetAsync(sums, 0, 2 * sizeof(double), s));
         update_kernel<<<gb, 256, 0, s>>>((int)rows, (int)cols, R32, rows, B, ldb, sums);
         NLA_LAUNCH_CHECK();
@@ -125,8 +135,8 @@ static int rectrsm_mixed_entry(char side, char uplo, char trans, double alpha, i
 }  // namespace nla
 
 NLA_API int nla_drectrsm_mixed(char side, char uplo, char trans, double alpha, int64_t n, int64_t m, const double *A,
-                               int64_t lda, double *B, int64_t ldb, int64_t cutoff, int64_t maxit, double tol,
+                               int64_t lda, double *B, int64_t ldb, int64_t cutoff, int64_t maxit, double tol, char prec,
                                int64_t *iters, double *last_correction, void *stream) {
-    return nla::rectrsm_mixed_entry(side, uplo, trans, alpha, n, m, A, lda, B, ldb, cutoff, maxit, tol, iters,
+    return nla::rectrsm_mixed_entry(side, uplo, trans, alpha, n, m, A, lda, B, ldb, cutoff, maxit, tol, prec, iters,
                                     last_correction, stream);
 }

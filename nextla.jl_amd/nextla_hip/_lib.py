@@ -51,7 +51,7 @@ _SIGS = {
 # Float64-only entry points
 _SIGS_D = {
     "tile_geqrf_mixed": [i64, i64, i64, i64, vp, i64, vp, i64, ch, vp],
-    "rectrsm_mixed": [ch, ch, ch, dbl, i64, i64, vp, i64, vp, i64, i64, i64, dbl, vp, vp, vp],
+    "rectrsm_mixed": [ch, ch, ch, dbl, i64, i64, vp, i64, vp, i64, i64, i64, dbl, ch, vp, vp, vp],
     "rectrxm_noalloc": [ch, ch, ch, dbl, ch, i64, i64, vp, i64, vp, i64, i64, vp],
 }
 

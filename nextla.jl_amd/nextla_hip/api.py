@@ -554,9 +554,10 @@ def tile_ormqr(trans, A, T, C, nb, ib):
     return C
 
 
-def rectrsm_mixed_(side, uplo, transpose, alpha, A, B, cutoff=0, maxit=5, tol=1e-15):
+def rectrsm_mixed_(side, uplo, transpose, alpha, A, B, cutoff=0, maxit=5, tol=1e-15, prec="S"):
     """unified_rectrxm!(side, uplo, transpose, alpha, 'S', A, B) for Float64 data solved in
-    Float32 with Float64 iterative refinement.  Returns (B, iterations, last ||dX||/||X||)."""
+    Float32 storage (GEMM updates with Float32 'S', FP16 'H' or BF16 'B' operands) with Float64
+    iterative refinement.  Returns (B, iterations, last ||dX||/||X||)."""
     import ctypes
 
     if A.dtype != torch.float64 or B.dtype != torch.float64:
@@ -568,7 +569,7 @@ def rectrsm_mixed_(side, uplo, transpose, alpha, A, B, cutoff=0, maxit=5, tol=1e
     it = ctypes.c_int64(0)
     last = ctypes.c_double(0)
     call("d", "rectrsm_mixed", _c(side), _c(uplo), _c(transpose), float(alpha), n, m, pa, lda, pb, ldb, int(cutoff),
-         int(maxit), float(tol), ctypes.cast(ctypes.pointer(it), ctypes.c_void_p),
+         int(maxit), float(tol), _c(prec), ctypes.cast(ctypes.pointer(it), ctypes.c_void_p),
          ctypes.cast(ctypes.pointer(last), ctypes.c_void_p), _stream())
     return B, it.value, last.value
 

@@ -124,3 +124,10 @@ def test_noalloc_entry_rejects_oversize_on_host():
     r = st("nla_drectrxm_noalloc", b"L", b"L", b"N", ctypes.c_double(1.0), b"S", 16384, 4, NULL, 16384, NULL,
            16384, 0, NULL)
     assert r[0] == -6 and "37748736 bytes" in r[1] and "holds 18874368" in r[1], r
+
+
+def test_mixed_solve_rejects_unknown_update_precision():
+    # the precision check runs before any HIP call
+    r = st("nla_drectrsm_mixed", b"L", b"L", b"N", ctypes.c_double(1.0), 64, 8, NULL, 64, NULL, 64, 0, 3,
+           ctypes.c_double(1e-15), b"X", NULL, NULL, NULL)
+    assert r == (-14, "prec must be 'S', 'H' or 'B', got 'X'"), r

@@ -173,9 +173,14 @@ def test_gels_argument_errors():
         nl.tile_geqrf_mixed(nl.zeros_cm(256, 128, dtype=torch.float32), 128, 32)
 
 
+E0_RANGE = {"S": (1e-9, 1e-5), "H": (1e-6, 1e-2), "B": (1e-5, 1e-1)}   # before refinement
+
+
+@pytest.mark.parametrize("prec", ["S", "H", "B"])
 @pytest.mark.parametrize("side,uplo,trans", [("L", "L", "N"), ("L", "U", "T"), ("R", "L", "N"), ("R", "U", "N")])
-def test_rectrsm_mixed_refinement(side, uplo, trans):
-    """Float32 solve + Float64 refinement reaches the Float64 solve's backward error."""
+def test_rectrsm_mixed_refinement(side, uplo, trans, prec):
+    """Float32-storage solve (GEMM updates with Float32 / FP16 / BF16 operands) + Float64
+    refinement reaches the Float64 solve's backward error."""
     nl = _nl()
     rng = np.random.default_rng(21)
     n, m = 1024, 96
@@ -187,9 +192,9 @@ def test_rectrsm_mixed_refinement(side, uplo, trans):
     X64 = nl.to_device(B)
     nl.unified_rectrxm_(side, uplo, trans, 0.5, "S", dA, X64)
     X0 = nl.to_device(B)
-    _, it0, _ = nl.rectrsm_mixed_(side, uplo, trans, 0.5, dA, X0, maxit=0)
+    _, it0, _ = nl.rectrsm_mixed_(side, uplo, trans, 0.5, dA, X0, maxit=0, prec=prec)
     Xr = nl.to_device(B)
-    _, it, last = nl.rectrsm_mixed_(side, uplo, trans, 0.5, dA, Xr, maxit=6, tol=1e-15)
+    _, it, last = nl.rectrsm_mixed_(side, uplo, trans, 0.5, dA, Xr, maxit=12, tol=1e-15, prec=prec)
     torch.cuda.synchronize()
     opA = A if trans == "N" else A.T
 
@@ -198,5 +203,6 @@ def test_rectrsm_mixed_refinement(side, uplo, trans):
         return np.linalg.norm(R) / (np.linalg.norm(A) * np.linalg.norm(X))
 
     e64, e0, er = berr(nl.to_host(X64)), berr(nl.to_host(X0)), berr(nl.to_host(Xr))
-    assert it0 == 0 and 1e-9 < e0 < 1e-5, e0           # Float32-level before refinement
+    lo, hi = E0_RANGE[prec]
+    assert it0 == 0 and lo < e0 < hi, e0               # the update precision's level before refinement
     assert it >= 1 and er < max(4 * e64, 1e-16), (er, e64, it, last)
