@@ -400,6 +400,39 @@ def percall(torch, nl, gen, n=8192, nb=128, ib=32):
     return {"n": n, "nb": nb, "ib": ib, "dtype": "f64", **res}
 
 
+def gemm_rates(torch, nl, sizes=(4096, 8192)):
+    """Square GEMM rates that bound an integer-sliced (Ozaki-type) Float64 emulation of the
+    TRSM's off-diagonal updates: int8 x int8 -> int32 and FP16 through torch's library GEMMs,
+    Float64 through torch and through this library's MFMA GEMM (GEMM_SUB!)."""
+    def bench_fn(fn, reps=10):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / reps
+    out = {}
+    for n in sizes:
+        fl = 2.0 * n ** 3
+        a8 = torch.randint(-127, 128, (n, n), dtype=torch.int8, device="cuda")
+        b8 = torch.randint(-127, 128, (n, n), dtype=torch.int8, device="cuda")
+        x = nl.colmajor(torch.randn((n, n), device="cuda", dtype=torch.float64))
+        y = nl.colmajor(torch.randn((n, n), device="cuda", dtype=torch.float64))
+        c = nl.colmajor(torch.zeros((n, n), device="cuda", dtype=torch.float64))
+        r = {"int8_tops": fl / bench_fn(lambda: torch._int_mm(a8, b8)) / 1e12,
+             "f16_tflops": fl / bench_fn(lambda: x.half() @ y.half()) / 1e12,
+             "f64_torch_tflops": fl / bench_fn(lambda: x @ y) / 1e12,
+             "f64_mfma_gemm_tflops": fl / bench_fn(lambda: nl.GEMM_SUB_(x, y, c)) / 1e12}
+        out[str(n)] = {k: round(v, 1) for k, v in r.items()}
+        log(f"[bench] GEMM rates n={n}: {out[str(n)]}")
+        del a8, b8, x, y, c
+    return out
+
+
 def adjoints(torch, nl, gen):
     """Adjoint kernels at the differentiation workload's sizes: TRSM A 4096, B 4096 x 512;
     thin QR of 8192 x 1024 (Float64); time of the backward pass alone."""
@@ -480,6 +513,7 @@ def main():
         only = set(x for x in args.only.split(",") if x)
         runners = {"batched": lambda: batched(torch, nl, gen, fixed_ib=args.batched_ib),
                    "percall": lambda: percall(torch, nl, gen),
+                   "gemm_rates": lambda: gemm_rates(torch, nl),
                    "lu": lambda: lu_factor(torch, nl, gen), "adjoints": lambda: adjoints(torch, nl, gen),
                    "ls": lambda: least_squares(torch, nl, gen),
                    "qr_solve": lambda: qr_solve_equal_error(torch, nl, gen, n=args.qr_n)}
