@@ -6,9 +6,11 @@
 // A22, swap the left part's rows.  On the GPU the two updates are this library's recursive
 // TRSM (on a unit-diagonal copy of L11) and MFMA GEMM; the recursion stops at leaves of at
 // most LEAF columns, factored by one workgroup (a block-wide pivot search per column, the
-// row swap, the scaling and the rank-1 update of the leaf's remaining columns).  Pivots are
-// stored 1-based and absolute (row indices of the whole matrix), so every level's swaps read
-// them directly.  info: the first zero pivot (1-based), 0 if none, written on the device.
+// row swap, the scaling and the rank-1 update of the leaf's remaining columns), and each
+// leaf's interchanges are applied to all other columns right after it, in place of the
+// recursion's laswp steps (the same interchanges reach every column in the same order).
+// Pivots are stored 1-based and absolute (row indices of the whole matrix).  info: the first
+// zero pivot (1-based), 0 if none, written on the device.
 #include "nla_common.hpp"
 #include "nla_gemm.hpp"
 
@@ -18,8 +20,9 @@ template <typename T>
 int rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
                   i64 ldb, i64 cutoff, hipStream_t s);
 
-constexpr int LU_LEAF = 32;     // widest leaf factored by one workgroup
+constexpr int LU_LEAF = 8;     // widest leaf factored by one workgroup
 constexpr int LU_THREADS = 1024;
+static_assert(2 * LU_LEAF <= 64, "a leaf's interchanges touch at most one wave's worth of rows");
 constexpr unsigned long long LU_NO_ZERO = ~0ull;
 
 template <typename T> __device__ __forceinline__ T lu_sfmin();
@@ -112,6 +115,64 @@ __global__ __launch_bounds__(256) void laswp_kernel(int ncols, T *__restrict__ A
     }
 }
 
+// A leaf's row interchanges applied to the matrix's other columns: rows base..base+kb-1 were
+// swapped, in order, with rows ipiv(k) - 1 (absolute).  Every workgroup forms the composed
+// permutation of the (at most 2 kb) rows involved in LDS, then each thread gathers one column's
+// values of those rows (all loads in flight together) and writes them back permuted, so a leaf
+// costs one round trip instead of kb dependent swaps per column.  Columns [skip0, skip1) (the
+// leaf's own, swapped by the leaf) are left alone.
+template <typename T>
+__global__ __launch_bounds__(256) void lu_leaf_swaps_kernel(int ncols, T *__restrict__ A, i64 lda, i64 base, int kb,
+                                                            const int64_t *__restrict__ ipiv, int skip0, int skip1) {
+    __shared__ i64 rows[2 * LU_LEAF];   // distinct rows involved
+    __shared__ int src[2 * LU_LEAF];    // final content of rows[i] = old row rows[src[i]]
+    __shared__ int nrows;
+    const int t = threadIdx.x;
+    if (t < 64) {
+        // wave 0 composes the swaps: lane j holds slot j (its row and the slot whose old value it
+        // now holds); slots 0..kb-1 are rows base..base+kb-1, later slots the pivot rows below
+        i64 row = t < kb ? base + t : -1;
+        int content = t;
+        int n = kb;
+        for (int k = 0; k < kb; ++k) {
+            const i64 r = ipiv[base + k] - 1;
+            int b;
+            if (r < base + kb) {
+                b = (int)(r - base);
+            } else {
+                const unsigned long long hit = __ballot(row == r);
+                if (hit) {
+                    b = __ffsll((long long)hit) - 1;
+                } else {
+                    b = n++;
+                    if (t == b) row = r;
+                }
+            }
+            const int ca = __shfl(content, k, 64), cb = __shfl(content, b, 64);
+            if (t == k) content = cb;
+            else if (t == b) content = ca;
+        }
+        if (t < n) {
+            rows[t] = row;
+            src[t] = content;
+        }
+        if (t == 0) nrows = n;
+    }
+    __syncthreads();
+    int c = blockIdx.x * 256 + t;
+    if (c >= ncols - (skip1 - skip0)) return;
+    if (c >= skip0) c += skip1 - skip0;
+    T *a = A + (i64)c * lda;
+    const int n = nrows;
+    T v[2 * LU_LEAF];
+#pragma unroll
+    for (int i = 0; i < 2 * LU_LEAF; ++i)
+        if (i < n) v[i] = a[rows[src[i]]];   // every load before any store
+#pragma unroll
+    for (int i = 0; i < 2 * LU_LEAF; ++i)
+        if (i < n) a[rows[i]] = v[i];
+}
+
 // L (n x n, ld n) = unit lower triangle of A's leading n x n block (zeros above).
 template <typename T>
 __global__ void unit_lower_copy_kernel(int n, const T *__restrict__ A, i64 lda, T *__restrict__ L) {
@@ -136,19 +197,29 @@ static hipError_t laswp_launch(i64 ncols, T *A, i64 lda, i64 k1, i64 k2, const i
 }
 
 // getrf2 on the m x n block at A (rows base.. of the matrix, which is also its first column).
+// Each leaf's interchanges are applied to every other column of the whole matrix (A0, ncols0
+// columns) as soon as the leaf is factored: every column then sees every interchange once, in
+// order, as after the recursion's own laswp steps (src/lu.jl:263-293), which are therefore not
+// repeated.
 template <typename T>
 static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsigned long long *flag, T *Lbuf,
-                      hipStream_t s) {
+                      T *A0, i64 ncols0, hipStream_t s) {
     if (m <= 0 || n <= 0) return 0;
     if (n <= LU_LEAF || m == 1) {
-        lu_leaf_kernel<T><<<1, LU_THREADS, 0, s>>>((int)m, (int)(n < LU_LEAF ? n : LU_LEAF), A, lda, ipiv, base, flag);
+        const int kb = (int)(n < LU_LEAF ? n : LU_LEAF);
+        lu_leaf_kernel<T><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         NLA_LAUNCH_CHECK();
+        const int kpiv = (int)(m < kb ? m : kb);
+        const i64 others = ncols0 - kb;
+        if (others > 0 && kpiv > 0) {
+            lu_leaf_swaps_kernel<T><<<(unsigned)((others + 255) / 256), 256, 0, s>>>(
+                (int)ncols0, A0, lda, base, kpiv, ipiv, (int)base, (int)(base + kb));
+            NLA_LAUNCH_CHECK();
+        }
         return 0;
     }
     const i64 mn = m < n ? m : n, n1 = mn / 2, n2 = n - n1;
-    NLA_ST(getrf2_rec<T>(m, n1, A, lda, ipiv, base, flag, Lbuf, s));
-    // interchanges on [A12; A22]
-    NLA_HIP(laswp_launch<T>(n2, A + n1 * lda, lda, base + 1, base + n1, ipiv, base, 1, s));
+    NLA_ST(getrf2_rec<T>(m, n1, A, lda, ipiv, base, flag, Lbuf, A0, ncols0, s));
     // A12 = L11^-1 A12 (L11 unit lower): the recursive TRSM on a unit-diagonal copy
     {
         const i64 tot = n1 * n1;
@@ -159,9 +230,7 @@ static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsi
     }
     // A22 -= A21 A12
     NLA_HIP(gemm<T>('N', 'N', m - n1, n2, n1, T(-1), A + n1, lda, A + n1 * lda, lda, T(1), A + n1 + n1 * lda, lda, s));
-    NLA_ST(getrf2_rec<T>(m - n1, n2, A + n1 + n1 * lda, lda, ipiv, base + n1, flag, Lbuf, s));
-    // interchanges on A21
-    NLA_HIP(laswp_launch<T>(n1, A, lda, base + n1 + 1, base + mn, ipiv, base, 1, s));
+    NLA_ST(getrf2_rec<T>(m - n1, n2, A + n1 + n1 * lda, lda, ipiv, base + n1, flag, Lbuf, A0, ncols0, s));
     return 0;
 }
 
@@ -186,7 +255,7 @@ static int getrf2_entry(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, int64_t *inf
     T *Lbuf = reinterpret_cast<T *>(base + 64);
     lu_info_init_kernel<<<1, 1, 0, s>>>(flag);
     NLA_LAUNCH_CHECK();
-    NLA_ST(getrf2_rec<T>(m, n, A, lda, ipiv, 0, flag, Lbuf, s));
+    NLA_ST(getrf2_rec<T>(m, n, A, lda, ipiv, 0, flag, Lbuf, A, n, s));
     lu_info_finish_kernel<<<1, 1, 0, s>>>(flag, info);
     NLA_LAUNCH_CHECK();
     return 0;
