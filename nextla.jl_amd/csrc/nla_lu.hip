@@ -30,6 +30,14 @@ template <> __device__ __forceinline__ double lu_sfmin<double>() { return 2.2250
 template <> __device__ __forceinline__ float lu_sfmin<float>() { return 1.17549435e-38f; }
 
 // One leaf: columns 0..nb-1 of the m-row block at A (its row 0 is row `base` of the matrix).
+// Per column j: the pivot (the first row of largest |a| in rows j..m-1, ties to the smaller row
+// as idamax: per-thread candidates, a wave reduction, then every thread reduces the waves'
+// candidates from LDS), the swap of rows j and p by one thread per column (which also puts the
+// new row j into LDS), then the scaling and rank-1 update of rows j+1..m-1.  A thread takes its
+// rows LU_RB at a time with all their loads in flight before any arithmetic, and notes its
+// candidate for column j+1 while it has the updated values (its rows of column j+1 are exactly
+// the rows of its next pivot search), so a column costs about three memory round trips.
+constexpr int LU_RB = 4;
 template <typename T>
 __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *__restrict__ A, i64 lda,
                                                              int64_t *__restrict__ ipiv, i64 base,
@@ -37,17 +45,22 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *_
     __shared__ T sval[LU_THREADS / 64];
     __shared__ int sidx[LU_THREADS / 64];
     __shared__ T prow[LU_LEAF];
-    __shared__ int sp;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int kmax = min(m, nb);
-    for (int j = 0; j < kmax; ++j) {
-        T *col = A + (i64)j * lda;
-        // the first row of largest |a| in rows j..m-1 (ties keep the smaller row, as idamax)
-        T best = T(-1);
-        int bi = m;
-        for (int r = j + t; r < m; r += LU_THREADS) {
-            const T v = fabs(col[r]);
-            if (v > best) { best = v; bi = r; }
+    T best = T(-1);
+    int bi = m;
+    bool have = false;  // best / bi hold this thread's candidate for column j
+#pragma unroll
+    for (int j = 0; j < LU_LEAF; ++j) {
+        if (j >= kmax) break;
+        if (!have) {
+            const T *col = A + (i64)j * lda;
+            best = T(-1);
+            bi = m;
+            for (int r = j + t; r < m; r += LU_THREADS) {
+                const T v = fabs(col[r]);
+                if (v > best) { best = v; bi = r; }
+            }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -57,40 +70,69 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *_
         }
         if (lane == 0) { sval[w] = best; sidx[w] = bi; }
         __syncthreads();
-        if (t == 0) {
-            T b = sval[0];
-            int i0 = sidx[0];
-            for (int q = 1; q < LU_THREADS / 64; ++q)
-                if (sval[q] > b || (sval[q] == b && sidx[q] < i0)) { b = sval[q]; i0 = sidx[q]; }
-            sp = i0;
-            ipiv[base + j] = base + i0 + 1;
-        }
-        __syncthreads();
-        const int p = sp;
-        const T piv = col[p];
-        if (piv == T(0)) {
+        T b = sval[0];
+        int p = sidx[0];
+#pragma unroll
+        for (int q = 1; q < LU_THREADS / 64; ++q)
+            if (sval[q] > b || (sval[q] == b && sidx[q] < p)) { b = sval[q]; p = sidx[q]; }
+        if (t == 0) ipiv[base + j] = base + p + 1;
+        if (b == T(0)) {
             // exact zero pivot (the whole column below the diagonal is zero): no swap, no scaling
             if (t == 0) atomicMin(info, (unsigned long long)(base + j + 1));
-            __syncthreads();
+            have = false;
+            __syncthreads();  // the waves' candidates are read before the next column's
             continue;
         }
-        if (p != j && t < nb) {
+        if (t < nb) {
             T *a = A + (i64)t * lda;
-            const T x = a[j];
-            a[j] = a[p];
-            a[p] = x;
+            const T x = a[j], y = a[p];
+            if (p != j) {
+                a[j] = y;
+                a[p] = x;
+            }
+            prow[t] = y;
         }
-        __syncthreads();
-        if (t < nb) prow[t] = A[j + (i64)t * lda];
         __syncthreads();
         const T d = prow[j];
         const bool recip = fabs(d) >= lu_sfmin<T>();   // src/lu.jl:238-242
         const T rd = T(1) / d;
-        for (int r = j + 1 + t; r < m; r += LU_THREADS) {
-            const T l = recip ? col[r] * rd : col[r] / d;
-            col[r] = l;
-            for (int c = j + 1; c < nb; ++c) A[r + (i64)c * lda] -= l * prow[c];
+        T nbest = T(-1);
+        int nbi = m;
+        for (int r0 = j + 1 + t; r0 < m; r0 += LU_RB * LU_THREADS) {
+            T v[LU_RB][LU_LEAF];
+#pragma unroll
+            for (int q = 0; q < LU_RB; ++q) {
+                const int r = r0 + q * LU_THREADS;
+#pragma unroll
+                for (int c = j; c < LU_LEAF; ++c)
+                    if (r < m && c < nb) v[q][c] = A[r + (i64)c * lda];
+            }
+#pragma unroll
+            for (int q = 0; q < LU_RB; ++q) {
+                const int r = r0 + q * LU_THREADS;
+                if (r < m) {
+                    const T l = recip ? v[q][j] * rd : v[q][j] / d;
+                    v[q][j] = l;
+#pragma unroll
+                    for (int c = j + 1; c < LU_LEAF; ++c)
+                        if (c < nb) v[q][c] -= l * prow[c];
+                    if (j + 1 < LU_LEAF && j + 1 < nb) {
+                        const T a1 = fabs(v[q][j + 1 < LU_LEAF ? j + 1 : j]);
+                        if (a1 > nbest) { nbest = a1; nbi = r; }
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < LU_RB; ++q) {
+                const int r = r0 + q * LU_THREADS;
+#pragma unroll
+                for (int c = j; c < LU_LEAF; ++c)
+                    if (r < m && c < nb) A[r + (i64)c * lda] = v[q][c];
+            }
         }
+        best = nbest;
+        bi = nbi;
+        have = true;
         __syncthreads();
     }
 }
@@ -173,6 +215,40 @@ __global__ __launch_bounds__(256) void lu_leaf_swaps_kernel(int ncols, T *__rest
         if (i < n) a[rows[i]] = v[i];
 }
 
+// A12 := L11^-1 A12 for the recursion's small nodes (n1 <= N1 <= 32): L11 the unit lower
+// triangle of A's leading n1 x n1 block, read in place; one thread per column of A12 holds the
+// column in registers and runs the forward substitution in LAPACK's column order (for k: for
+// i > k: x_i -= L_ik x_k), L11 staged in LDS.  Replaces the unit-diagonal copy and the blocked
+// recursive solve (three launches) at these sizes.
+template <typename T, int N1>
+__global__ __launch_bounds__(256) void lu_trsm_small_kernel(int n1, int n2, const T *__restrict__ L, i64 lda,
+                                                            T *__restrict__ B, i64 ldb) {
+    __shared__ T Ls[N1 * N1];
+    for (int idx = threadIdx.x; idx < n1 * n1; idx += 256) {
+        const int r = idx % n1, c = idx / n1;
+        Ls[c * N1 + r] = L[r + (i64)c * lda];
+    }
+    __syncthreads();
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= n2) return;
+    T *b = B + (i64)col * ldb;
+    T x[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) x[i] = i < n1 ? b[i] : T(0);
+#pragma unroll
+    for (int k = 0; k < N1; ++k) {
+        if (k < n1) {
+            const T xk = x[k];
+#pragma unroll
+            for (int i = k + 1; i < N1; ++i)
+                if (i < n1) x[i] -= Ls[k * N1 + i] * xk;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N1; ++i)
+        if (i < n1) b[i] = x[i];
+}
+
 // L (n x n, ld n) = unit lower triangle of A's leading n x n block (zeros above).
 template <typename T>
 __global__ void unit_lower_copy_kernel(int n, const T *__restrict__ A, i64 lda, T *__restrict__ L) {
@@ -220,8 +296,15 @@ static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsi
     }
     const i64 mn = m < n ? m : n, n1 = mn / 2, n2 = n - n1;
     NLA_ST(getrf2_rec<T>(m, n1, A, lda, ipiv, base, flag, Lbuf, A0, ncols0, s));
-    // A12 = L11^-1 A12 (L11 unit lower): the recursive TRSM on a unit-diagonal copy
-    {
+    // A12 = L11^-1 A12 (L11 unit lower): in place for small n1, else the recursive TRSM on a
+    // unit-diagonal copy
+    if (n1 <= 32) {
+        const unsigned g = (unsigned)((n2 + 255) / 256);
+        if (n1 <= 8) lu_trsm_small_kernel<T, 8><<<g, 256, 0, s>>>((int)n1, (int)n2, A, lda, A + n1 * lda, lda);
+        else if (n1 <= 16) lu_trsm_small_kernel<T, 16><<<g, 256, 0, s>>>((int)n1, (int)n2, A, lda, A + n1 * lda, lda);
+        else lu_trsm_small_kernel<T, 32><<<g, 256, 0, s>>>((int)n1, (int)n2, A, lda, A + n1 * lda, lda);
+        NLA_LAUNCH_CHECK();
+    } else {
         const i64 tot = n1 * n1;
         const int blocks = (int)((tot + 255) / 256 < 2048 ? (tot + 255) / 256 : 2048);
         unit_lower_copy_kernel<T><<<blocks, 256, 0, s>>>((int)n1, A, lda, Lbuf);
