@@ -29,6 +29,36 @@ template <typename T> __device__ __forceinline__ T lu_sfmin();
 template <> __device__ __forceinline__ double lu_sfmin<double>() { return 2.2250738585072014e-308; }
 template <> __device__ __forceinline__ float lu_sfmin<float>() { return 1.17549435e-38f; }
 
+// (v, i) over the wave: the largest v, ties to the smaller i, in every lane.  Cross-lane steps
+// without the LDS crossbar: v_permlane32_swap / v_permlane16_swap pair lanes l and l ^ 32 /
+// l ^ 16, DPP rotations by 8 and 4 within each row of 16 and quad permutations do the rest; the
+// selection is a maximum over (v, -i), so the order of the pairings does not change it.
+template <typename T> __device__ __forceinline__ void lu_pick(T &v, int &i, T ov, int oi) {
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+template <typename T> __device__ __forceinline__ void lu_wave_argmax(T &v, int &i) {
+    {
+        T x = v, y = v;
+        float fi = __int_as_float(i), fj = fi;
+        pl32_swap(x, y);
+        pl32_swap(fi, fj);
+        v = x; i = __float_as_int(fi);
+        lu_pick(v, i, y, __float_as_int(fj));
+    }
+    {
+        T x = v, y = v;
+        float fi = __int_as_float(i), fj = fi;
+        pl16_swap(x, y);
+        pl16_swap(fi, fj);
+        v = x; i = __float_as_int(fi);
+        lu_pick(v, i, y, __float_as_int(fj));
+    }
+    lu_pick(v, i, dpp_mov<DPP_ROW_ROR8>(v), __float_as_int(dpp_mov<DPP_ROW_ROR8>(__int_as_float(i))));
+    lu_pick(v, i, dpp_mov<DPP_ROW_ROR4>(v), __float_as_int(dpp_mov<DPP_ROW_ROR4>(__int_as_float(i))));
+    lu_pick(v, i, dpp_mov<DPP_QUAD_X2>(v), __float_as_int(dpp_mov<DPP_QUAD_X2>(__int_as_float(i))));
+    lu_pick(v, i, dpp_mov<DPP_QUAD_X1>(v), __float_as_int(dpp_mov<DPP_QUAD_X1>(__int_as_float(i))));
+}
+
 // One leaf: columns 0..nb-1 of the m-row block at A (its row 0 is row `base` of the matrix).
 // Per column j: the pivot (the first row of largest |a| in rows j..m-1, ties to the smaller row
 // as idamax: per-thread candidates, a wave reduction, then every thread reduces the waves'
@@ -62,12 +92,7 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *_
                 if (v > best) { best = v; bi = r; }
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
+        lu_wave_argmax(best, bi);
         if (lane == 0) { sval[w] = best; sidx[w] = bi; }
         __syncthreads();
         T b = sval[0];
