@@ -191,3 +191,48 @@ def test_host_pointer_rectrxm(dt, side, uplo, func):
     want2 = B0.copy(order="F")
     O.rectrxm(side, uplo, "N", 1.0, func, Au, want2)
     assert relerr(Bm.data, want2) < (1e-12 if dt == np.float64 else 1e-4)
+
+
+def _kahan_lower(k, theta):
+    # transpose of Kahan's upper triangular matrix: diag sin(theta)^i, below -cos(theta) sin(theta)^j
+    s, c = np.sin(theta), np.cos(theta)
+    K = np.triu(-c * np.ones((k, k)), 1) + np.eye(k)
+    return (np.diag(s ** np.arange(k)) @ K).T
+
+
+def _ill_conditioned_lower(kind, n, rng):
+    A = np.tril(rng.standard_normal((n, n)) / np.sqrt(n), -1)
+    A[np.diag_indices(n)] = rng.uniform(1, 2, n)
+    if kind == "graded_rows":    # row scaling 1 .. 1e-10 inside every 256-row block
+        A *= (10.0 ** (-10.0 * ((np.arange(n) % 256) / 256)))[:, None]
+    elif kind == "graded_cols":  # column scaling 1 .. 1e-10 inside every 256-column block
+        A *= (10.0 ** (-10.0 * ((np.arange(n) % 256) / 256)))[None, :]
+    else:                        # Kahan diagonal blocks (cond of a block ~1e14 / ~1e20)
+        theta = 1.45 if kind == "kahan_1e14" else 1.3
+        for b0 in range(0, n, 256):
+            A[b0:b0 + 256, b0:b0 + 256] = _kahan_lower(256, theta)
+    return A
+
+
+@pytest.mark.parametrize("kind", ["graded_rows", "graded_cols", "kahan_1e14", "kahan_1e20"])
+@pytest.mark.parametrize("cut", [128, 256, 512])
+def test_trsm_ill_conditioned_bases(kind, cut):
+    """The 256-row bases apply explicitly inverted diagonal blocks (cut-offs 256 and 512; at 512
+    also the fused two-block base) where the reference and the oracle substitute; cut-off 128
+    substitutes.  On ill-conditioned diagonal blocks (graded rows / columns down to 1e-10, Kahan
+    blocks of condition ~1e14 and ~1e20) the solve must stay backward stable: backward error
+    below 10 u (a CPU model of the inverse-apply gives at most 6e-18 on these classes, within
+    10x of substitution), and the solution must agree with the oracle's substitution to 1e-11
+    relative (the model: at most 5e-15)."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    n, m = 1024, 32
+    A = F(_ill_conditioned_lower(kind, n, rng))
+    B0 = F(rng.standard_normal((n, m)))
+    dA, dB = nl.to_device(A), nl.to_device(B0)
+    nl.unified_rectrxm_("L", "L", "N", 1.0, "S", dA, dB, cutoff=cut)
+    X = host(dB)
+    be = np.linalg.norm(A @ X - B0) / (np.linalg.norm(A) * np.linalg.norm(X))
+    assert be < 10 * np.finfo(np.float64).eps, be
+    Xo = B0.copy(order="F")
+    O.rectrxm("L", "L", "N", 1.0, "S", A, Xo, threshold=cut)
+    assert relerr(X, Xo) < 1e-11, relerr(X, Xo)
