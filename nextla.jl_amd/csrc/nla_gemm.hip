@@ -18,13 +18,19 @@
 
 namespace nla {
 
-template <typename T> struct GemmPad { static constexpr int v = 2; };
+// LDS row padding: rows stay 16-byte aligned (for the vector slab stores) and a wave's MFMA
+// operand reads spread over the banks.
+template <typename T> struct GemmPad { static constexpr int v = 16 / (int)sizeof(T); };
 
 // EDGE = false: every tile is interior (M % BM == N % BN == K % BK == 0): no bounds checks.
 // NSET register sets for the slabs in flight: 1 fetches slab s+1 while slab s multiplies;
 // 2 fetches slab s+2, so a load has two slabs of products to arrive (small tiles, whose
 // slab is too short to cover a cache miss).
-template <typename T, int BM, int BN, int BK, bool TA, bool TB, bool EDGE, int NSET = 1>
+// VL (interior tiles of op = 'N' operands with 16-byte aligned columns): each thread moves 16
+// contiguous bytes of a column per global load (A along m, B along k), a quarter (Float32) or
+// half (Float64) of the load instructions of one element per load; A's slab goes to LDS as
+// 16-byte stores.
+template <typename T, int BM, int BN, int BK, bool TA, bool TB, bool EDGE, int NSET = 1, bool VL = false>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alpha, const T *__restrict__ A,
                                                       i64 lda, const T *__restrict__ B, i64 ldb, T beta,
                                                       T *__restrict__ C, i64 ldc, int tiles_m) {
@@ -33,8 +39,8 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
     constexpr int LB = (BK * BN) / 256;
     constexpr int FM = BM / 32;          // 16-row blocks per wave (2 waves along M)
     constexpr int FN = BN / 32;
-    __shared__ T As[2][BK][BM + PAD];
-    __shared__ T Bs[2][BK][BN + PAD];
+    __shared__ __align__(16) T As[2][BK][BM + PAD];
+    __shared__ __align__(16) T Bs[2][BK][BN + PAD];
 
     // XCD-aware tile numbering: hardware deals workgroups round-robin over 8 XCDs, so the
     // ids are first made contiguous per XCD; within that order tiles go in groups of GM tile
@@ -66,9 +72,32 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
     const i64 a_step_k = TA ? BK : (i64)BK * lda;               // advance per K slab
     const i64 b_step_k = TB ? (i64)BK * ldb : BK;
 
+    // vector slab layout (VL): V elements per load; A: m = am_v .. am_v + V - 1 of slab column
+    // ak_v + i * AKV; B: k = bk_v .. bk_v + V - 1 of column bn_v + i * BNV
+    constexpr int V = 16 / (int)sizeof(T);
+    constexpr int LAV = VL ? BM * BK / (256 * V) : 1, LBV = VL ? BK * BN / (256 * V) : 1;
+    constexpr int AKV = 256 / (BM / V), BNV = 256 / (BK / V);
+    static_assert(!VL || (BM % V == 0 && BK % V == 0 && 256 % (BM / V) == 0 && 256 % (BK / V) == 0 &&
+                          BK % AKV == 0 && BN % BNV == 0 && LAV >= 1 && LBV >= 1), "vector slab shape");
+    typedef T tvec __attribute__((ext_vector_type(V)));
+    const int am_v = (t % (BM / V)) * V, ak_v = t / (BM / V);
+    const int bk_v = (t % (BK / V)) * V, bn_v = t / (BK / V);
+    const T *pav = A + (m0 + am_v) + (i64)ak_v * lda;
+    const T *pbv = B + bk_v + (i64)(n0 + bn_v) * ldb;
+    tvec va[NSET][LAV], vb[NSET][LBV];
+
     T ra[NSET][LA], rb[NSET][LB];
     // slabs are loaded in order: the k0 argument only serves the edge checks
     auto load = [&](int k0, int set) {
+        if constexpr (VL) {
+#pragma unroll
+            for (int i = 0; i < LAV; ++i) va[set][i] = *reinterpret_cast<const tvec *>(pav + (i64)i * AKV * lda);
+#pragma unroll
+            for (int i = 0; i < LBV; ++i) vb[set][i] = *reinterpret_cast<const tvec *>(pbv + (i64)i * BNV * ldb);
+            pav += (i64)BK * lda;
+            pbv += BK;
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             if (EDGE) {
@@ -91,6 +120,15 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
         pb += b_step_k;
     };
     auto store = [&](int buf, int set) {
+        if constexpr (VL) {
+#pragma unroll
+            for (int i = 0; i < LAV; ++i) *reinterpret_cast<tvec *>(&As[buf][ak_v + i * AKV][am_v]) = va[set][i];
+#pragma unroll
+            for (int i = 0; i < LBV; ++i)
+#pragma unroll
+                for (int e = 0; e < V; ++e) Bs[buf][bk_v + e][bn_v + i * BNV] = vb[set][i][e];
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < LA; ++i) As[buf][ak + i * ASK][am + i * ASM] = ra[set][i];
 #pragma unroll
@@ -176,6 +214,15 @@ static hipError_t launch_tiles(bool ta, bool tb, int M, int N, int K, T alpha, c
                                i64 ldb, T beta, T *C, i64 ldc, hipStream_t s) {
     const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
     dim3 grid(tm * tn), block(256);
+    constexpr int V = 16 / (int)sizeof(T);
+    const bool aligned = !(reinterpret_cast<uintptr_t>(A) & 15) && !(reinterpret_cast<uintptr_t>(B) & 15) &&
+                         lda % V == 0 && ldb % V == 0;
+    if constexpr (!EDGE) {
+        if (!ta && !tb && aligned) {
+            gemm_kernel<T, BM, BN, BK, false, false, false, NSET, true><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
+            return hipGetLastError();
+        }
+    }
     if (!ta && !tb) gemm_kernel<T, BM, BN, BK, false, false, EDGE, NSET><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
     else if (ta && !tb) gemm_kernel<T, BM, BN, BK, true, false, EDGE, NSET><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
     else if (!ta && tb) gemm_kernel<T, BM, BN, BK, false, true, EDGE, NSET><<<grid, block, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm);
@@ -329,11 +376,12 @@ hipError_t gemm(char ta, char tb, i64 M, i64 N, i64 K, T alpha, const T *A, i64 
         if (g_gemm_lowp == 'B' && K > 0)
             return launch_lowp<__bf16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
     }
+    constexpr int BK = 16;
     if (big_tiles >= 512) {
         ProfScope prof128(PK_GEMM128, s);
-        return launch_sized<T, 128, 128, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+        return launch_sized<T, 128, 128, BK>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
     }
-    return launch_sized<T, 64, 64, 16>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
+    return launch_sized<T, 64, 64, BK>(TA, TB, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
 template hipError_t gemm<double>(char, char, i64, i64, i64, double, const double *, i64, const double *, i64, double,
