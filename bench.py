@@ -717,7 +717,7 @@ def main():
                      "peak": PEAK["f64"], "unit": "TFLOP/s",
                      "frac": None if achieved is None else round(achieved / PEAK["f64"], 4),
                      "traffic": traffic,
-                     "kernel": "nla::gemm_kernel<double, 128, 128, 16, false, false, false, 1> (the level-1/2 "
+                     "kernel": "nla::gemm_kernel<double, 128, 128, 16, false, false, false, 1, true> (the level-1/2 "
                                "off-diagonal GEMM_SUB! updates of the recursion)",
                      "launches_per_step": g128_n // max(1, args.steps),
                      "flop_per_step": g128_flops,
