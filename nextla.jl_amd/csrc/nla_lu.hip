@@ -162,6 +162,113 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *_
     }
 }
 
+// The leaf for m <= LU_THREADS * Q rows (m >= nb), its rows held in registers for all its
+// columns: thread t owns rows t + LU_THREADS q.  Per column one barrier: each wave's winning
+// lane puts its candidate row (all eight columns) into LDS and thread j its row j (row j < 8
+// always lives in thread j), so after the barrier every thread has the pivot row and row j
+// without a memory round trip; the swap, scaling and rank-1 update are register moves.  The
+// LDS slots alternate between even and odd columns, so a fast wave writing column j+1's
+// candidates cannot overwrite what a slow one still reads for column j.  Same operations in
+// the same order as lu_leaf_kernel, so the factors are identical.
+template <typename T, int Q>
+__global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb, T *__restrict__ A, i64 lda,
+                                                                 int64_t *__restrict__ ipiv, i64 base,
+                                                                 unsigned long long *info) {
+    constexpr int NW = LU_THREADS / 64;
+    __shared__ T cval[2][NW];
+    __shared__ int cidx[2][NW];
+    __shared__ T crow[2][NW][LU_LEAF];
+    __shared__ T jrow[2][LU_LEAF];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    T a[Q][LU_LEAF];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int r = t + q * LU_THREADS;
+#pragma unroll
+        for (int c = 0; c < LU_LEAF; ++c) a[q][c] = r < m && c < nb ? A[r + (i64)c * lda] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < LU_LEAF; ++j) {
+        if (j >= nb) break;
+        const int par = j & 1;
+        T best = T(-1);
+        int bi = m, bq = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = t + q * LU_THREADS;
+            if (r >= j && r < m) {
+                const T v = fabs(a[q][j]);
+                if (v > best) { best = v; bi = r; bq = q; }
+            }
+        }
+        T wb = best;
+        int wi = bi;
+        lu_wave_argmax(wb, wi);
+        if (wi < m && bi == wi) {  // this wave's winning lane (row indices are distinct)
+            cval[par][w] = wb;
+            cidx[par][w] = wi;
+#pragma unroll
+            for (int c = 0; c < LU_LEAF; ++c) {
+                T x = a[0][c];
+#pragma unroll
+                for (int q = 1; q < Q; ++q) x = bq == q ? a[q][c] : x;
+                crow[par][w][c] = x;
+            }
+        } else if (wi >= m && lane == 0) {  // no rows of this wave in range
+            cval[par][w] = T(-1);
+            cidx[par][w] = m;
+        }
+        if (t == j) {
+#pragma unroll
+            for (int c = 0; c < LU_LEAF; ++c) jrow[par][c] = a[0][c];
+        }
+        __syncthreads();
+        T b = cval[par][0];
+        int p = cidx[par][0], pw = 0;
+#pragma unroll
+        for (int i = 1; i < NW; ++i)
+            if (cval[par][i] > b || (cval[par][i] == b && cidx[par][i] < p)) { b = cval[par][i]; p = cidx[par][i]; pw = i; }
+        if (t == 0) ipiv[base + j] = base + p + 1;
+        if (b == T(0)) {
+            // exact zero pivot (the whole column below the diagonal is zero): no swap, no scaling
+            if (t == 0) atomicMin(info, (unsigned long long)(base + j + 1));
+            continue;
+        }
+        T prow[LU_LEAF];
+#pragma unroll
+        for (int c = 0; c < LU_LEAF; ++c) prow[c] = crow[par][pw][c];
+        const T d = prow[j];
+        const bool recip = fabs(d) >= lu_sfmin<T>();   // src/lu.jl:238-242
+        const T rd = T(1) / d;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int r = t + q * LU_THREADS;
+            if (p != j) {
+                if (r == j) {
+#pragma unroll
+                    for (int c = 0; c < LU_LEAF; ++c) a[q][c] = prow[c];
+                } else if (r == p) {
+#pragma unroll
+                    for (int c = 0; c < LU_LEAF; ++c) a[q][c] = jrow[par][c];
+                }
+            }
+            if (r > j && r < m) {
+                const T l = recip ? a[q][j] * rd : a[q][j] / d;
+                a[q][j] = l;
+#pragma unroll
+                for (int c = j + 1; c < LU_LEAF; ++c) a[q][c] -= l * prow[c];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int r = t + q * LU_THREADS;
+#pragma unroll
+        for (int c = 0; c < LU_LEAF; ++c)
+            if (r < m && c < nb) A[r + (i64)c * lda] = a[q][c];
+    }
+}
+
 // Row interchanges k = k1..k2 (absolute, 1-based; reverse order for inc < 0) with ipiv(k) on
 // ncols columns of the block at A whose row 0 is row `base` of the matrix: thread per column.
 template <typename T>
@@ -308,7 +415,11 @@ static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsi
     if (m <= 0 || n <= 0) return 0;
     if (n <= LU_LEAF || m == 1) {
         const int kb = (int)(n < LU_LEAF ? n : LU_LEAF);
-        lu_leaf_kernel<T><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
+        if (m >= kb && m <= LU_THREADS) lu_leaf_rows_kernel<T, 1><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
+        else if (m >= kb && m <= 2 * LU_THREADS) lu_leaf_rows_kernel<T, 2><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
+        else if (sizeof(T) == 4 && m >= kb && m <= 4 * LU_THREADS)  // (Float64 at four rows per thread spills)
+            lu_leaf_rows_kernel<T, sizeof(T) == 4 ? 4 : 1><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
+        else lu_leaf_kernel<T><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         NLA_LAUNCH_CHECK();
         const int kpiv = (int)(m < kb ? m : kb);
         const i64 others = ncols0 - kb;
