@@ -66,6 +66,10 @@ static int tile_geqrf_entry(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, 
 }
 
 template <typename T>
+static int tile_qt_vec(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, const T *Tm, i64 ldt, T *x, hipStream_t s,
+                       bool *handled);
+
+template <typename T>
 static int tile_ormqr_entry(char trans, i64 m, i64 n, i64 nrhs, i64 nb, i64 ib, const T *A, i64 lda, const T *Tm,
                             i64 ldt, T *C, i64 ldc, void *stream) {
     trans = up(trans);
@@ -81,6 +85,11 @@ static int tile_ormqr_entry(char trans, i64 m, i64 n, i64 nrhs, i64 nb, i64 ib, 
     if (ldt < mt * ib) return arg_error(10, "ldt must be >= (m/nb)*ib");
     const bool notrans = trans == 'N';
     hipStream_t s = as_stream(stream);
+    if (nrhs == 1 && !notrans) {  // one column: the single-vector wavefront
+        bool handled = false;
+        const int r = tile_qt_vec<T>(m, n, nb, ib, A, lda, Tm, ldt, C, s, &handled);
+        if (handled) return r;
+    }
     {
         // Q^T C / Q C by the dataflow kernel's walkers alone (C padded to a multiple of 64 columns)
         constexpr i64 TWC = 64;
@@ -156,6 +165,254 @@ template <typename T> __device__ __forceinline__ T ld_sc1(const T *p) {
 template <typename T> __device__ __forceinline__ void st_sc1(T *p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// ---------------------------------------------------------------------------------
+// Q^T x for a single vector x (m entries) with the factors of tile_geqrf, in the flat-tree order
+// of the factorization (src/unmqr.jl, src/tsmqr.jl with trans 'C', one column).  One workgroup
+// per step k, claimed in order from a counter: unmqr('L','C') of x_k with the blocks of
+// geqrt(A_kk), then for every tile row i > k the TS apply of the blocks of tile (i, k) to
+// (x_k, x_i), each row published when stored (prog[k] = rows of x final for step k + 1; step k
+// waits on prog[k - 1] only, so the lowest unfinished step always runs).  A row is a few
+// microseconds of latency, where the dataflow kernel's walker spends a 64-column strip's
+// products on it; the wavefront over the kt + mt steps and rows is correspondingly shorter.
+//
+// Four waves apply the blocks; their only global accesses are the loads of the reflectors and
+// T (final before the launch), issued three blocks ahead into registers and staged into the
+// second of two LDS buffers while the current block is applied.  A fifth wave does all the
+// traffic with the other steps: it waits on the previous step's progress, loads the rows of x
+// into LDS (the next row's during the current one, into the second of two row buffers), stores
+// each finished row and publishes it.  A wave's memory counter drains in issue order, so this
+// keeps the progress waits and the publications from waiting on the block loads in flight, and
+// the block loads from being drained by a wait on a row.  Bounded waits as in the dataflow
+// kernel.
+template <typename T, int NB, int IBV>
+__global__ __launch_bounds__(320) void qt_vec_kernel(int mt, int kt, const T *__restrict__ A, i64 lda,
+                                                     const T *__restrict__ Tm, i64 ldt, T *x, int *counter,
+                                                     int *prog, int *err) {
+    constexpr int NBLK = NB / IBV, G = 256 / IBV, RPG = NB / G, XPL = NB / 64, TPR = 256 / NB;
+    constexpr int NV = NB * IBV / 256, NT = IBV * IBV / 256;
+    static_assert(NB % IBV == 0 && NBLK % 2 == 0 && (G == 8 || G == 16) && (IBV * IBV) % 256 == 0 &&
+                      NB % G == 0 && (NB * IBV) % 256 == 0 && NB % 64 == 0,
+                  "block shape");
+    // LDS row strides: the G lanes of a reflector read G consecutive entries and the next
+    // reflector's lanes start G entries further along the 64 banks, so a half wave's reads
+    // are free of bank conflicts
+    constexpr int BW = 64 * 4 / (int)sizeof(T);  // entries per pass over the banks
+    constexpr int VLD = NB + ((G - NB) % BW + BW) % BW, TLD = IBV + ((G - IBV) % BW + BW) % BW;
+    __shared__ T xk[NB], xi[2][NB];
+    __shared__ T Vs[2][IBV][VLD];
+    __shared__ T Ts[2][IBV][TLD];  // Ts[.][j][q] = T(q, j)
+    __shared__ T w0[IBV], wv[IBV];
+    __shared__ int sh_k;
+    const int t = threadIdx.x, jr = t / G, g = t % G, lane = t & 63;
+    const bool ctl = t >= 256;  // the fifth wave
+    T pv[2][NV], pt[2][NT];     // register buffers: block j in buffer j % 2
+    using R0 = std::integral_constant<int, 0>;
+    using R1 = std::integral_constant<int, 1>;
+    // sum over the G lanes of a reflector (quad swaps, half-row mirror, row rotation by 8)
+    auto group_sum = [](T v) {
+        v += dpp_mov<DPP_QUAD_X1>(v);
+        v += dpp_mov<DPP_QUAD_X2>(v);
+        v += dpp_mov<0x141>(v);
+        if constexpr (G == 16) v += dpp_mov<DPP_ROW_ROR8>(v);
+        return v;
+    };
+    // item s of a step: s < NBLK the geqrt block s of A_kk, else block (s - NBLK) % NBLK of tile
+    // row k + 1 + (s - NBLK) / NBLK; tile row i > k works in row buffer (i - k - 1) % 2
+    auto fetch = [&](auto R, int k, int s) {
+        constexpr int r = decltype(R)::value;
+        const bool ge = s < NBLK;
+        const int i = ge ? k : k + 1 + (s - NBLK) / NBLK, b = ge ? s : (s - NBLK) % NBLK;
+        const T *Vg = A + (i64)i * NB + ((i64)k * NB + b * IBV) * lda;
+        const T *Tg = Tm + (i64)i * IBV + ((i64)k * NB + b * IBV) * ldt;
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int idx = t + 256 * e, pr = idx % NB, jj = idx / NB;
+            pv[r][e] = Vg[pr + (i64)jj * lda];
+        }
+#pragma unroll
+        for (int e = 0; e < NT; ++e) {
+            const int idx = t + 256 * e, q = idx % IBV, jj = idx / IBV;
+            pt[r][e] = Tg[q + (i64)jj * ldt];
+        }
+    };
+    // the loaded values are used only here, so the loads stay in flight until the block is
+    // staged; a geqrt block gets its unit diagonal and zeros above it here
+    auto stage = [&](auto R, int s) {
+        constexpr int r = decltype(R)::value;
+        const bool ge = s < NBLK;
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int idx = t + 256 * e, pr = idx % NB, jj = idx / NB, ja = s * IBV + jj;
+            T v = pv[r][e];
+            if (ge) v = pr < ja ? T(0) : (pr == ja ? T(1) : v);
+            Vs[r][jj][pr] = v;
+        }
+#pragma unroll
+        for (int e = 0; e < NT; ++e) {
+            const int idx = t + 256 * e;
+            Ts[r][idx / IBV][idx % IBV] = idx % IBV <= idx / IBV ? pt[r][e] : T(0);
+        }
+    };
+    // fifth wave: spin until *p >= v (first: a value of *p read earlier).  On a timeout, or
+    // once another workgroup has raised the error flag, it raises the flag and returns at once;
+    // the launch then runs to its end without further waits (a result the host discards).  No
+    // early exit: the block loop keeps a single shape, which keeps the compiler's waits for the
+    // block registers exact.
+    auto ctl_wait = [&](const int *p, int v, int first) {
+        if (first >= v || ld_sc1(p) >= v) return;
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        while (ld_sc1(p) < v) {
+            if (ld_sc1(err) || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                if (lane == 0) st_sc1(err, 1);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    };
+    // fifth wave: store tile row i from row buffer c, then publish it
+    auto ctl_store = [&](int k, int i, int c) {
+#pragma unroll
+        for (int e = 0; e < XPL; ++e) st_sc1(&x[(i64)i * NB + lane + 64 * e], xi[c][lane + 64 * e]);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (lane == 0) st_sc1(&prog[k], i + 1);
+    };
+    // The fifth wave runs its own loop: its loads and stores then never share a code path with
+    // the block loads, so the compiler's waits for a block's registers count only block loads.
+    // Both loops pass the same barriers: three per block, three per step.
+    if (ctl) {
+        T xn[XPL];  // the next tile row's x
+        int pp = 0;
+        for (;;) {
+            __syncthreads();
+            __syncthreads();
+            const int k = sh_k;
+            if (k >= kt) break;
+            if (k > 0) ctl_wait(&prog[k - 1], k + 1, 0);
+#pragma unroll
+            for (int e = 0; e < XPL; ++e) xk[lane + 64 * e] = ld_sc1(&x[(i64)k * NB + lane + 64 * e]);
+            __syncthreads();
+            const int nitems = NBLK * (mt - k);
+            for (int s = 0; s < nitems; ++s) {
+                const bool ge = s < NBLK;
+                const int b = ge ? s : (s - NBLK) % NBLK, i = ge ? k : k + 1 + (s - NBLK) / NBLK;
+                const bool more = i + 1 < mt;
+                if (!ge && b == 0 && i - 1 > k) ctl_store(k, i - 1, ((i - k - 1) & 1) ^ 1);
+                if (b == 0 && more && k > 0) pp = ld_sc1(&prog[k - 1]);
+                if (b == NBLK / 2 && more) {  // row i + 1 once the previous step has released it
+                    if (k > 0) ctl_wait(&prog[k - 1], i + 2, pp);
+#pragma unroll
+                    for (int e = 0; e < XPL; ++e) xn[e] = ld_sc1(&x[(i64)(i + 1) * NB + lane + 64 * e]);
+                }
+                __syncthreads();
+                __syncthreads();
+                if (b == NBLK - 1 && more) {
+#pragma unroll
+                    for (int e = 0; e < XPL; ++e) xi[(i - k) & 1][lane + 64 * e] = xn[e];
+                }
+                __syncthreads();
+            }
+            if (mt - 1 > k) ctl_store(k, mt - 1, (mt - k) & 1);
+#pragma unroll
+            for (int e = 0; e < XPL; ++e) st_sc1(&x[(i64)k * NB + lane + 64 * e], xk[lane + 64 * e]);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+        return;
+    }
+    int k = 0, nitems = 0;
+    // H^T of block s (staged in Vs[s % 2]) on S: W0 = V^T S (+ the A1 rows), W = T^T W0,
+    // A1 -= W, S -= V W; block s + 1 is staged and block s + 3 fetched in between
+    auto item = [&](auto R, int s) {
+        constexpr int r = decltype(R)::value, o = 1 - r;
+        const bool ge = s < NBLK;
+        const int b = ge ? s : (s - NBLK) % NBLK, i = ge ? k : k + 1 + (s - NBLK) / NBLK;
+        T *S = ge ? xk : xi[(i - k - 1) & 1], *A1 = ge ? nullptr : xk + b * IBV;
+        T acc = 0;
+#pragma unroll
+        for (int q = 0; q < RPG; ++q) acc += Vs[r][jr][q * G + g] * S[q * G + g];
+        acc = group_sum(acc);
+        if (g == 0) w0[jr] = A1 ? acc + A1[jr] : acc;
+        __syncthreads();
+        // W[jr] = sum_{q <= jr} T(q, jr) w0[q]: the G threads of reflector jr take every G-th q
+        T w = 0;
+#pragma unroll
+        for (int q = g; q < IBV; q += G)
+            if (q <= jr) w += Ts[r][jr][q] * w0[q];
+        w = group_sum(w);
+        if (g == 0) {
+            wv[jr] = w;
+            if (A1) A1[jr] -= w;
+        }
+        // unconditional (the last items re-load the step's last block), so the number of loads
+        // in flight behind each block is the same on every path and the waits for a block's
+        // registers stop at that block
+        stage(std::integral_constant<int, o>(), s + 1 < nitems ? s + 1 : nitems - 1);
+        fetch(std::integral_constant<int, o>(), k, s + 3 < nitems ? s + 3 : nitems - 1);
+        __syncthreads();
+        {  // S -= V W: the TPR lanes of a row take every TPR-th reflector, summed by quad swaps
+            const int row = t / TPR, h = t % TPR;
+            T v = 0;
+#pragma unroll 8
+            for (int jj = h; jj < IBV; jj += TPR) v += Vs[r][jj][row] * wv[jj];
+            v += dpp_mov<DPP_QUAD_X1>(v);
+            if constexpr (TPR == 4) v += dpp_mov<DPP_QUAD_X2>(v);
+            if (h == 0) S[row] -= v;
+        }
+        __syncthreads();
+    };
+    for (;;) {
+        __syncthreads();
+        if (t == 0) sh_k = atomicAdd(counter, 1);
+        __syncthreads();
+        k = sh_k;
+        if (k >= kt) break;
+        nitems = NBLK * (mt - k);
+        fetch(R0(), k, 0);
+        fetch(R1(), k, 1);
+        stage(R0(), 0);
+        fetch(R0(), k, nitems > 2 ? 2 : nitems - 1);
+        __syncthreads();
+        for (int s = 0; s < nitems; s += 2) {
+            item(R0(), s);
+            item(R1(), s + 1);
+        }
+    }
+}
+
+__global__ void qt_vec_clear_kernel(int *p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// x := Q^T x (x of m entries, device) with the tile_geqrf factors; *handled = false for the
+// shapes it does not take (nb 64 / 128 with ib 32 or 16).
+template <typename T>
+static int tile_qt_vec(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, const T *Tm, i64 ldt, T *x, hipStream_t s,
+                       bool *handled) {
+    *handled = (nb == 128 || nb == 64) && (ib == 32 || ib == 16) && m % nb == 0 && n % nb == 0 && m >= n;
+    if (!*handled) return 0;
+    const int mt = (int)(m / nb), kt = (int)(n / nb);
+    int st = 0;
+    int *mem = (int *)scratch(SLOT_SCHED, sizeof(int) * (size_t)(kt + 2), s, &st);
+    if (!mem) return st;
+    int *counter = mem, *err = mem + 1, *prog = mem + 2;
+    qt_vec_clear_kernel<<<(kt + 2 + 255) / 256, 256, 0, s>>>(mem, kt + 2);
+    NLA_LAUNCH_CHECK();
+    int dev = 0, ncu = 0;
+    NLA_HIP(hipGetDevice(&dev));
+    NLA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = kt < ncu ? kt : ncu;
+    if (nb == 128 && ib == 32) qt_vec_kernel<T, 128, 32><<<grid, 320, 0, s>>>(mt, kt, A, lda, Tm, ldt, x, counter, prog, err);
+    else if (nb == 128) qt_vec_kernel<T, 128, 16><<<grid, 320, 0, s>>>(mt, kt, A, lda, Tm, ldt, x, counter, prog, err);
+    else if (ib == 32) qt_vec_kernel<T, 64, 32><<<grid, 320, 0, s>>>(mt, kt, A, lda, Tm, ldt, x, counter, prog, err);
+    else qt_vec_kernel<T, 64, 16><<<grid, 320, 0, s>>>(mt, kt, A, lda, Tm, ldt, x, counter, prog, err);
+    NLA_LAUNCH_CHECK();
+    int herr = 0;
+    NLA_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    NLA_HIP(hipStreamSynchronize(s));
+    if (herr) return wait_timeout_error("tile Q^T x");
+    return 0;
+}
+
 
 // Inverses of the n/64 diagonal 64 x 64 blocks of the upper triangle R (one workgroup per
 // block, thread j forms column j by back substitution): Dinv[b][i + 64 j] = inv(R_bb)[i][j].
@@ -500,9 +757,8 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     };
     // x0 = R^-1 (Q^T b)[0:n]
     auto direct_start = [&]() -> int {
-        NLA_HIP(hipMemsetAsync(Cb, 0, sizeof(T) * (size_t)m * W, s));
         NLA_HIP(hipMemcpyAsync(Cb, b, sizeof(T) * m, hipMemcpyDeviceToDevice, s));
-        NLA_ST(tile_ormqr_entry<T>('T', m, n, W, nb, ib, AF, m, Tm, ldt, Cb, m, s));
+        NLA_ST(tile_ormqr_entry<T>('T', m, n, 1, nb, ib, AF, m, Tm, ldt, Cb, m, s));
         NLA_ST(rsolve(false, Cb));
         NLA_HIP(hipMemcpyAsync(x, Cb, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
         return 0;

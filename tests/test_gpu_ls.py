@@ -52,6 +52,34 @@ def test_ormqr_walkers_vs_oracle(m, n, nb, nrhs, trans):
     assert relerr(nl.to_host(dC), C) < 1e-13
 
 
+@pytest.mark.parametrize("dt,tol", [(np.float64, 1e-13), (np.float32, 2e-6)])
+@pytest.mark.parametrize("nb,ib", [(64, 16), (64, 32), (128, 16), (128, 32)])
+@pytest.mark.parametrize("m,n", [(512, 512), (1024, 256), (768, 384), (2048, 1024)])
+def test_ormqr_single_vector_vs_oracle(m, n, nb, ib, dt, tol):
+    """Q^T x for one column (the single-vector wavefront) against the oracle's tile_ormqr, and
+    against the multi-column walkers on the same factors with x as one of several columns."""
+    nl = _nl()
+    import oracle as O
+
+    rng = np.random.default_rng(m + 7 * n + ib)
+    A0 = np.asfortranarray(rng.standard_normal((m, n)).astype(dt))
+    x0 = np.asfortranarray(rng.standard_normal((m, 1)).astype(dt))
+    dA = nl.to_device(A0)
+    dT = nl.tile_geqrf(dA, nb, ib)
+    dx = nl.to_device(x0)
+    nl.tile_ormqr("T", dA, dT, dx, nb, ib)
+    dC = nl.to_device(np.asfortranarray(np.repeat(x0, 3, axis=1)))
+    nl.tile_ormqr("T", dA, dT, dC, nb, ib)
+    torch.cuda.synchronize()
+    x = nl.to_host(dx)
+    xo = x0.copy(order="F")
+    O.tile_ormqr("T", nl.to_host(dA), nl.to_host(dT), xo, nb, ib)
+    assert relerr(x, xo) < tol
+    assert relerr(x, nl.to_host(dC)[:, :1]) < tol
+    # Q is orthogonal: ||Q^T x|| = ||x||
+    assert abs(np.linalg.norm(x.astype(np.float64)) / np.linalg.norm(x0.astype(np.float64)) - 1) < 10 * tol
+
+
 # backward error bounds of the update precisions: Float32 / FP16 / BF16 unit roundoff
 # (6e-8 / 4.9e-4 / 3.9e-3) times a modest growth factor, and well above the Float64 path
 MIXED_BERR = {"S": (1e-9, 2e-5), "H": (1e-6, 2e-2), "B": (1e-5, 1e-1)}
