@@ -35,6 +35,7 @@ for p in (os.path.join(REPO, "nextla.jl_amd"), os.path.join(REPO, "oracle")):
         sys.path.insert(0, p)
 
 PEAK = {"f64": 78.6, "f32": 157.3, "f16": 2516.6, "bf16": 2516.6}   # TFLOP/s, MI355X dense MFMA
+PEAK["bf16x3"] = PEAK["bf16"] / 3   # split BF16: three BF16 products per product
 
 
 def log(*a):
@@ -202,7 +203,7 @@ def cutoff_sweep(torch, nl, n, m, funcs="SM", dtypes=("f64", "f32"), cuts=(16, 3
 def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
     """Mixed-precision tile QR + least squares: A = U diag(s) V^T (U, V from QR of Gaussian,
     s geometric from 1 to 1/kappa), b = A x0 + 1e-8 N(0,1); tile QR with Float64 / Float32 /
-    FP16 / BF16 trailing updates, each refined by
+    FP16 / BF16 / split-BF16 trailing updates, each refined by
     R-preconditioned LSQR (products by A in Float64) towards the Float64 solution."""
     out = {"m": m, "n": n}
     for kappa in (1e2, 1e6):
@@ -214,7 +215,7 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
         x0 = torch.randn(n, generator=gen, device="cuda", dtype=torch.float64)
         b = A @ x0 + 1e-8 * torch.randn(m, generator=gen, device="cuda", dtype=torch.float64)
         row = {}
-        for prec in ("D", "S", "H", "B"):
+        for prec in ("D", "S", "H", "B", "X"):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             info = {}
@@ -228,12 +229,13 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
             log(f"[bench] LS kappa={kappa:g} prec {prec}: {dt:.3f} s, {it} steps, eta {eta}, "
                 f"factor {info['prec_used']}")
         xd = row["D"]["x"]
-        for p in ("S", "H", "B"):
+        for p in ("S", "H", "B", "X"):
             row[p]["rel_diff_vs_f64_solution"] = (torch.linalg.norm(row[p]["x"] - xd) / torch.linalg.norm(xd)).item()
             row[p]["vs_f64_time"] = round(row["D"]["s"] / row[p]["s"], 3)
         for v in row.values():
             v.pop("x")
-        out[f"kappa_{kappa:g}"] = {"f64": row["D"], "mixed_f32": row["S"], "mixed_f16": row["H"], "mixed_bf16": row["B"]}
+        out[f"kappa_{kappa:g}"] = {"f64": row["D"], "mixed_f32": row["S"], "mixed_f16": row["H"], "mixed_bf16": row["B"],
+                                   "mixed_bf16x3": row["X"]}
         del A, b
     return out
 
@@ -241,7 +243,7 @@ def least_squares(torch, nl, gen, m=32768, n=8192, nb=128, ib=32):
 def qr_solve_equal_error(torch, nl, gen, n=16384, kappa=1e2, nb=128, ib=32):
     """Tile QR at 16384^2 compared at equal backward error: solve A x = b (A = U diag(s) V^T, s
     geometric from 1 to 1/kappa, b = A x0) with the Float64 factorization and with the Float32 /
-    FP16 / BF16-update factorizations, each refined by R-preconditioned LSQR (products by A in
+    FP16 / BF16 / split-BF16-update factorizations, each refined by R-preconditioned LSQR (products by A in
     Float64) until LSQR's estimate reaches 1e-15; the time covers factorization + refinement
     and every row states the normwise backward error ||b - A x|| / (||A||_2 ||x|| + ||b||)."""
     U = torch.linalg.qr(torch.randn((n, n), generator=gen, device="cuda", dtype=torch.float64))[0]
@@ -253,7 +255,7 @@ def qr_solve_equal_error(torch, nl, gen, n=16384, kappa=1e2, nb=128, ib=32):
     b = A @ x0
     out = {"n": n, "kappa": kappa, "nb": nb, "ib": ib,
            "time": "factorization + direct start + LSQR refinement, one call, wall clock"}
-    for prec, tag in (("D", "f64"), ("S", "mixed_f32"), ("H", "mixed_f16"), ("B", "mixed_bf16")):
+    for prec, tag in (("D", "f64"), ("S", "mixed_f32"), ("H", "mixed_f16"), ("B", "mixed_bf16"), ("X", "mixed_bf16x3")):
         info = {}
         nl.tile_gels(A, b, nb, ib, prec, maxit=100, tol=1e-15, info=info)   # warm-up
         times = []
@@ -269,7 +271,7 @@ def qr_solve_equal_error(torch, nl, gen, n=16384, kappa=1e2, nb=128, ib=32):
                     "backward_error": be, "forward_error": (torch.linalg.norm(x - x0) / torch.linalg.norm(x0)).item()}
         log(f"[bench] QR solve {n}^2 kappa {kappa:g} prec {prec}: {t*1e3:.1f} ms, {it} its, berr {be:.2e}, "
             f"factor {info['prec_used']}")
-    for tag in ("mixed_f32", "mixed_f16", "mixed_bf16"):
+    for tag in ("mixed_f32", "mixed_f16", "mixed_bf16", "mixed_bf16x3"):
         out[tag]["vs_f64"] = round(out["f64"]["ms"] / out[tag]["ms"], 3)
     del A, b
     return out
@@ -649,7 +651,8 @@ def main():
         variants = (("f64", "f64", lambda: nl.tile_geqrf(Qw, nb, ib)),
                     ("mixed_f32", "f32", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "S")),
                     ("mixed_f16", "f16", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "H")),
-                    ("mixed_bf16", "bf16", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "B")))
+                    ("mixed_bf16", "bf16", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "B")),
+                    ("mixed_bf16x3", "bf16x3", lambda: nl.tile_geqrf_mixed(Qw, nb, ib, "X")))
         for tag, upd, fn in variants:
             holder = {}
 

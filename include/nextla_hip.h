@@ -345,23 +345,25 @@ int nla_ctile_geqrf_calls(int64_t m, int64_t n, int64_t nb, int64_t ib, nla_ccom
 /* ---- Mixed precision and least squares (no reference function: config "mixed-precision
  * tile QR + least squares"; the reference stops at the tile kernels) --------------------- */
 /* nla_dtile_geqrf with the trailing updates (unmqr!/tsmqr!) on MFMA with Float32
- * accumulation: prec 'S' Float32 operands, 'H' FP16 operands, 'B' BF16 operands (the strips
+ * accumulation: prec 'S' Float32 operands, 'H' FP16 operands, 'B' BF16 operands, 'X' split
+ * BF16 (each operand as two BF16 parts, hi = bf16(x), lo = bf16(x - hi), three products
+ * hi*hi + hi*lo + lo*hi: about 2^-16 per product at three BF16 products' cost) (the strips
  * stay Float32 between products; V, T and each product's operands are rounded); the panels,
  * larfg norms/scaling and the T factors stay Float64.  prec 'D' is nla_dtile_geqrf.  'H'
  * assumes the matrix's entries are within FP16 range.  Needs nb in {64, 128}, ib = 32 and
  * 16-byte aligned columns.  Holds in the device scratch pool a Float32 copy of the trailing
  * matrix (4 lda n bytes) and the panels' per-block V/T images (~21 KB per ib-block of each
- * panel tile).  Synchronises the stream. */
+ * panel tile; 'X': ~42 KB).  Synchronises the stream. */
 int nla_dtile_geqrf_mixed(int64_t m, int64_t n, int64_t nb, int64_t ib, double *A, int64_t lda, double *T,
                           int64_t ldt, char prec, void *stream);
 /* min ||A x - b|| for A m x n (m >= n, multiples of nb, full rank), A and b unchanged, x (n)
- * out.  Tile QR of a copy of A (prec 'D' plain; 'S' / 'H' / 'B' as nla_dtile_geqrf_mixed,
+ * out.  Tile QR of a copy of A (prec 'D' plain; 'S' / 'H' / 'B' / 'X' as nla_dtile_geqrf_mixed,
  * Float64 data only), the direct solution x0 = R^-1 (Q^T b)[0:n] (returned for maxit = 0; with
  * prec 'D' and maxit > 0, x0 = R^-1 R^-T (A^T b), the seminormal equations), then up to maxit
  * iterations of LSQR (Paige & Saunders) on the correction, right-preconditioned by R:
  * min ||(A R^-1) y - (b - A x0)||, x = x0 + R^-1 y, with every product by A in the working
  * precision.  Stops when LSQR's estimate of ||M^T r|| / (||M||_F ||r||) (M = A R^-1) reaches
- * tol or the residual estimate falls to tol ||b - A x0||.  With a 16-bit factor ('H', 'B')
+ * tol or the residual estimate falls to tol ||b - A x0||.  With a 16-bit factor ('H', 'B', 'X')
  * the contraction of that estimate is watched from iteration 12 on: when the iterations
  * still needed at the observed rate exceed what is left of maxit, A is factored again with
  * Float32 updates and LSQR restarts from that factor's direct solution.  iters / last_correction / prec_used

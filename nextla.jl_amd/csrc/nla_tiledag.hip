@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 namespace nla {
@@ -32,6 +33,15 @@ namespace dag {
 constexpr int IB = 32;   // inner block (reflectors per T block)
 constexpr int SW = 16;   // strip width = one MFMA column tile
 constexpr int NTH = 256;
+
+// Walker element type of the split-BF16 updates (prec 'X'): every product operand as two BF16
+// parts, x = hi + lo with hi = bf16(x), lo = bf16(x - hi) (16 mantissa bits together), and a
+// product as hi*hi + hi*lo + lo*hi on the BF16 matrix cores with Float32 accumulation.  The
+// dropped lo*lo term and the parts' rounding leave a product good to about 2^-16 (FP16
+// operands: 2^-11; Float32: 2^-24) at three 16-bit products, 16/3 times the Float32 MFMA rate.
+struct bf16x3 { __bf16 h; };
+static_assert(sizeof(bf16x3) == 2, "a 16-bit walker type");
+template <typename Tw> constexpr int lp_parts() { return std::is_same<Tw, bf16x3>::value ? 2 : 1; }
 
 enum : int { TASK_PANEL = 0, TASK_WALK = 1 };
 __host__ __device__ inline int enc(int type, int k, int s) { return (type << 30) | (k << 18) | s; }
@@ -813,6 +823,7 @@ __host__ __device__ inline i64 lp_tile_index(int k, int i, int mt);
 template <typename T, typename Tw> struct PanelTwT { using type = T; };
 template <typename T> struct PanelTwT<T, _Float16> { using type = _Float16; };
 template <typename T> struct PanelTwT<T, __bf16> { using type = __bf16; };
+template <typename T> struct PanelTwT<T, bf16x3> { using type = bf16x3; };
 template <> struct PanelTwT<double, float> { using type = float; };
 template <typename T, typename Tw> using PanelTw = typename PanelTwT<T, Tw>::type;
 // Panel side of F32Img: the block just factored (V, T of the panel's LDS, V unit lower for a
@@ -1320,11 +1331,14 @@ template <typename Tw> __device__ __forceinline__ typename LpMfma<Tw>::v4 lp_cvt
 // row's strip prefetch (StripStage) follows the two buffers.
 // Each buffer is padded to whole KB: the panel writes exactly this image (IMG_BYTES) to HBM
 // once per (tile, block) and walkers copy it into a buffer with 1 KB LDS-DMA instructions.
+// Split BF16 ('X') images hold the hi parts and then, LO elements further, the lo parts.
 template <typename Tw, int NB> struct LdsLp {
     static constexpr int LDC = NB + 8, LDR = IB + 8;
     static constexpr int VC = 0, VR = VC + IB * LDC, TC = VR + NB * LDR, BUF = TC + IB * LDR;
-    static constexpr int IMG_BYTES = (BUF * (int)sizeof(Tw) + 1023) / 1024 * 1024;
-    static constexpr int BUFS = IMG_BYTES / (int)sizeof(Tw);  // buffer stride in Tw elements
+    static constexpr int PART_BYTES = (BUF * 2 + 1023) / 1024 * 1024;
+    static constexpr int LO = PART_BYTES / 2;  // the lo part's offset in 16-bit elements
+    static constexpr int IMG_BYTES = lp_parts<Tw>() * PART_BYTES;
+    static constexpr int BUFS = IMG_BYTES / 2;  // buffer stride in 16-bit elements
     static constexpr int SSTG_BYTES = 2 * IMG_BYTES;
     static_assert(SSTG_BYTES % 16 == 0, "staging columns must start 16-byte aligned");
 };
@@ -1353,8 +1367,8 @@ template <typename T, typename Tw, int NB> __device__ void write_lp_image(unsign
     const T *Vs = lds_base<T>() + Lds<T, NB>::V;
     const T *Ts = lds_base<T>() + Lds<T, NB>::TT;
     unsigned *d32 = reinterpret_cast<unsigned *>(dst);
-    for (int wd = threadIdx.x; wd < L::BUFS / 2; wd += NTH) {
-        unsigned short h[2];
+    for (int wd = threadIdx.x; wd < L::PART_BYTES / 4; wd += NTH) {
+        unsigned short h[2], l[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int e = 2 * wd + q;
@@ -1369,9 +1383,18 @@ template <typename T, typename Tw, int NB> __device__ void write_lp_image(unsign
                 const int f = e - L::TC, j = f / L::LDR, r = f % L::LDR;
                 if (r < IB && r <= j) v = Ts[r * (IB + 1) + j];
             }
-            h[q] = __builtin_bit_cast(unsigned short, to_lp<Tw>(v));
+            if constexpr (lp_parts<Tw>() == 2) {
+                float f = (float)v;
+                asm volatile("" : "+v"(f));
+                const __bf16 hi = (__bf16)f;
+                h[q] = __builtin_bit_cast(unsigned short, hi);
+                l[q] = __builtin_bit_cast(unsigned short, (__bf16)(f - (float)hi));  // f - hi is exact
+            } else {
+                h[q] = __builtin_bit_cast(unsigned short, to_lp<Tw>(v));
+            }
         }
         cst(&d32[wd], (unsigned)h[0] | ((unsigned)h[1] << 16));
+        if constexpr (lp_parts<Tw>() == 2) cst(&d32[wd + L::PART_BYTES / 4], (unsigned)l[0] | ((unsigned)l[1] << 16));
     }
 }
 // Walker side: one block image into LDS buffer buf with 1 KB LDS-DMA instructions (waves
@@ -1466,6 +1489,98 @@ __device__ __forceinline__ void apply_lp(WStrip<float, NB> *S, WStrip<float, NB>
     }
 }
 
+// The split-BF16 apply: apply_lp's products with both operands in two parts.  A operands (V, T)
+// come from the image's hi and lo parts; B operands (the Float32 strips, W0, W) are split in
+// registers right before their products.
+__device__ __forceinline__ void bf16_split(const fx4 &x, float sign, LpMfma<__bf16>::v4 &hi, LpMfma<__bf16>::v4 &lo) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float f = sign * x[q];
+        const __bf16 h = (__bf16)f;
+        hi[q] = h;
+        lo[q] = (__bf16)(f - (float)h);
+    }
+}
+// c += A B as lo*hi + hi*lo + hi*hi (the small terms first)
+__device__ __forceinline__ void mma8x3(fx4 &c, const LpMfma<__bf16>::v4 *a, const LpMfma<__bf16>::v4 *b) {
+    // a / b: [0] hi of K 0..3, [1] hi of K 4..7, [2] lo of K 0..3, [3] lo of K 4..7
+    LpMfma<__bf16>::mma8(c, a[2], a[3], b[0], b[1]);
+    LpMfma<__bf16>::mma8(c, a[0], a[1], b[2], b[3]);
+    LpMfma<__bf16>::mma8(c, a[0], a[1], b[0], b[1]);
+}
+template <int NB, bool GE, int NS>
+__device__ __forceinline__ void apply_lp3(WStrip<float, NB> *S, WStrip<float, NB> *A1, int B, int buf) {
+    using L = LdsLp<bf16x3, NB>;
+    typedef LpMfma<__bf16>::v4 v4;
+    const __bf16 *sm = lds_base<__bf16>() + buf * L::BUFS;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+    auto rd = [&](int off) { return *reinterpret_cast<const v4 *>(sm + off); };
+    auto rd4 = [&](v4 *a, int o) {  // hi and lo parts of a lane's eight K values at o, o + 16
+        a[0] = rd(o);
+        a[1] = rd(o + 16);
+        a[2] = rd(o + L::LO);
+        a[3] = rd(o + L::LO + 16);
+    };
+    fx4 w0[NS][2];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) w0[q][0] = w0[q][1] = acc_zero<float>();
+#pragma unroll
+    for (int t = 0; t < NB / 32; ++t) {
+        v4 a[2][4];
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) rd4(a[ot], L::VC + (16 * ot + lr) * L::LDC + 32 * t + 4 * lk);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            v4 b[4];
+            bf16_split(S[q].a[2 * t], 1.0f, b[0], b[2]);
+            bf16_split(S[q].a[2 * t + 1], 1.0f, b[1], b[3]);
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) mma8x3(w0[q][ot], a[ot], b);
+        }
+    }
+    if (!GE) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) w0[q][ot] += A1[q].a[2 * B + ot];
+    }
+    fx4 wv[NS][2];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) wv[q][0] = wv[q][1] = acc_zero<float>();
+    {
+        v4 a[2][4];
+#pragma unroll
+        for (int ot = 0; ot < 2; ++ot) rd4(a[ot], L::TC + (16 * ot + lr) * L::LDR + 4 * lk);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            v4 b[4];
+            bf16_split(w0[q][0], 1.0f, b[0], b[2]);
+            bf16_split(w0[q][1], 1.0f, b[1], b[3]);
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) mma8x3(wv[q][ot], a[ot], b);
+        }
+    }
+    if (!GE) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int ot = 0; ot < 2; ++ot) A1[q].a[2 * B + ot] -= wv[q][ot];
+    }
+    v4 b[NS][4];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        bf16_split(wv[q][0], -1.0f, b[q][0], b[q][2]);
+        bf16_split(wv[q][1], -1.0f, b[q][1], b[q][3]);
+    }
+#pragma unroll
+    for (int rt = 0; rt < NB / 16; ++rt) {
+        v4 a[4];
+        rd4(a, L::VR + (16 * rt + lr) * L::LDR + 4 * lk);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) mma8x3(S[q].a[rt], a, b[q]);
+    }
+}
+
 template <typename T, typename Tw, int NB>
 __device__ __forceinline__ void run_walk_lp(int k, int s, T * /*A*/, i64 /*lda*/, T * /*Tm*/, i64 /*ldt*/, DagArgs g,
                                             int *sh_flag) {
@@ -1523,7 +1638,8 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T * /*A*/, i64 /*lda*/
             have0 = ready_ge(pdone, k + 2, &sh_ready);
             if (have0) lp_img_load<Tw, NB>(img(k + 1, 0), 0);
         }
-        apply_lp<Tw, NB, true, NS>(A1, nullptr, B, B & 1);
+        if constexpr (lp_parts<Tw>() == 2) apply_lp3<NB, true, NS>(A1, nullptr, B, B & 1);
+        else apply_lp<Tw, NB, true, NS>(A1, nullptr, B, B & 1);
     }
     using Stg = StripStage<LdsLp<Tw, NB>::SSTG_BYTES, NB, NS>;
     static_assert(WAVES * Stg::NG * Stg::GRP <= TW * Lds<T, NB>::LDS_S * (int)sizeof(T), "staging fits the strip region");
@@ -1579,7 +1695,8 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T * /*A*/, i64 /*lda*/
                 haveS = i + 1 < g.mt && (!prev || sh_nx[1] != 0);
                 if (haveS && src32) Stg::load(sh32 + (i64)(i + 1) * NB + col * ldc, ldc);
             }
-            apply_lp<Tw, NB, false, NS>(S, A1, B, B & 1);
+            if constexpr (lp_parts<Tw>() == 2) apply_lp3<NB, false, NS>(S, A1, B, B & 1);
+            else apply_lp<Tw, NB, false, NS>(S, A1, B, B & 1);
         }
         have0 = next0;
         store_strips(S, i, dst64);
@@ -1590,12 +1707,26 @@ __device__ __forceinline__ void run_walk_lp(int k, int s, T * /*A*/, i64 /*lda*/
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
+// Dynamic LDS of a launch: the larger of the T and Tw layouts, and for the 16-bit walkers the
+// two block-image buffers with the next row's strip staging after them (split-BF16 images are
+// twice as large, which takes the launch past the Float64 layout's 150 KB).
+template <typename T, typename Tw, int NB> constexpr size_t dag_lds_bytes() {
+    const size_t t = sizeof(T) * (size_t)Lds<T, NB>::TOTAL, w = sizeof(Tw) * (size_t)Lds<Tw, NB>::TOTAL;
+    size_t b = t > w ? t : w;
+    if constexpr (sizeof(Tw) == 2) {
+        const size_t lp = (size_t)LdsLp<Tw, NB>::SSTG_BYTES + TW * Lds<T, NB>::LDS_S * sizeof(T);
+        if (lp > b) b = lp;
+    }
+    return b;
+}
+
 template <typename T, int NB, typename Tw>
 __global__ __launch_bounds__(NTH) void tileqr_dag_kernel(T *A, i64 lda, T *Tm, i64 ldt, DagArgs g) {
     static_assert((sizeof(Tw) == 2 ? (size_t)LdsLp<Tw, NB>::SSTG_BYTES : Lds<Tw, NB>::SSTG * sizeof(Tw)) +
                           TW * Lds<T, NB>::LDS_S * sizeof(T) <=
-                      Lds<T, NB>::TOTAL * sizeof(T),
+                      dag_lds_bytes<T, Tw, NB>(),
                   "the walker's staging columns must fit the launch's LDS");
+    static_assert(dag_lds_bytes<T, Tw, NB>() <= 160 * 1024, "a CU's LDS");
     __shared__ int sh_task, sh_flag;
     for (;;) {
         __syncthreads();
@@ -1685,8 +1816,7 @@ static int dag_run(DagMode mode, i64 m, i64 n, T *A, i64 lda, T *Tm, i64 ldt, T 
     int dev = 0, ncu = 0;
     NLA_HIP(hipGetDevice(&dev));
     NLA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t sh_t = sizeof(T) * (size_t)Lds<T, NB>::TOTAL, sh_w = sizeof(Tw) * (size_t)Lds<Tw, NB>::TOTAL;
-    const size_t sh = sh_t > sh_w ? sh_t : sh_w;
+    constexpr size_t sh = dag_lds_bytes<T, Tw, NB>();
     if (sh > 64 * 1024)
         NLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&tileqr_dag_kernel<T, NB, Tw>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
@@ -1753,12 +1883,16 @@ int tile_geqrf_dag(i64 m, i64 n, i64 nb, i64 ib, T *A, i64 lda, T *Tm, i64 ldt, 
     return dag_run<T, 128, T>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
 }
 
-// Float64 panels, Float32 trailing updates.
-// Trailing updates in Float32 (prec 'S'), FP16 ('H') or BF16 ('B'); the panels in Float64.
+// Trailing updates in Float32 (prec 'S'), FP16 ('H'), BF16 ('B') or split BF16 ('X'); the panels
+// in Float64.
 int tile_geqrf_dag_mixed(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 lda, double *Tm, i64 ldt, char prec,
                          hipStream_t s, bool *handled) {
     *handled = dag_shape_ok<double>(m, n, nb, ib, A, lda);
     if (!*handled) return 0;
+    if (prec == 'X') {
+        if (nb == 64) return dag_run<double, 64, dag::bf16x3>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+        return dag_run<double, 128, dag::bf16x3>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
+    }
     if (prec == 'H') {
         if (nb == 64) return dag_run<double, 64, _Float16>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);
         return dag_run<double, 128, _Float16>(DAG_FACTOR, m, n, A, lda, Tm, ldt, A, lda, n, s);

@@ -128,11 +128,13 @@ static int tile_ormqr_entry(char trans, i64 m, i64 n, i64 nrhs, i64 nb, i64 ib, 
     return 0;
 }
 
-// Float64 tile QR with Float32 ('S'), FP16 ('H') or BF16 ('B') trailing updates, or plain ('D').
+// Float64 tile QR with Float32 ('S'), FP16 ('H'), BF16 ('B') or split-BF16 ('X') trailing
+// updates, or plain ('D').
 static int tile_geqrf_mixed_entry(i64 m, i64 n, i64 nb, i64 ib, double *A, i64 lda, double *Tm, i64 ldt, char prec,
                                   void *stream) {
     prec = up(prec);
-    if (prec != 'D' && prec != 'S' && prec != 'H' && prec != 'B') return arg_error(9, "prec must be 'D', 'S', 'H' or 'B'");
+    if (prec != 'D' && prec != 'S' && prec != 'H' && prec != 'B' && prec != 'X')
+        return arg_error(9, "prec must be 'D', 'S', 'H', 'B' or 'X'");
     if (prec == 'D') return tile_geqrf_entry<double>(m, n, nb, ib, A, lda, Tm, ldt, stream);
     if (m < 0) return arg_error(1, "m must be non-negative");
     if (n < 0) return arg_error(2, "n must be non-negative");
@@ -674,8 +676,8 @@ template <typename T> __global__ void axpy1_kernel(i64 n, const T *x, T *y) {
 }
 
 // Least squares min ||A x - b|| (A m x n, m >= n, full rank).
-// Tile QR A = Q R (prec 'D': Float64; 'S', 'H', 'B': Float32, FP16 or BF16 trailing updates,
-// Float64 data only).  x0 = R^-1 (Q^T b)[0:n], the direct solution (returned for maxit = 0);
+// Tile QR A = Q R (prec 'D': Float64; 'S', 'H', 'B', 'X': Float32, FP16, BF16 or split-BF16
+// trailing updates, Float64 data only).  x0 = R^-1 (Q^T b)[0:n], the direct solution (returned for maxit = 0);
 // with the Float64 factor and maxit > 0, x0 = R^-1 R^-T (A^T b) instead (the seminormal
 // equations: Q^T b = R^-T A^T b for A = QR; one product by A^T and two triangular solves,
 // ~1.2 ms at 32768 x 8192, where Q^T b takes 7.4 ms because its walker launch has a single
@@ -690,7 +692,7 @@ template <typename T> __global__ void axpy1_kernel(i64 n, const T *x, T *y) {
 // (x += R^-1 (Q^T r)[0:n]) diverges: kappa(A) times the factorization error near or above one.
 // Stops after maxit iterations or when LSQR's estimate of ||M^T r|| / (||M|| ||r||)
 // (M = A R^-1) falls to tol, or the residual estimate to tol ||b - A x0||.
-// A 16-bit factor ('H', 'B') is a usable preconditioner only while kappa(A) times its
+// A 16-bit factor ('H', 'B', 'X') is a usable preconditioner only while kappa(A) times its
 // backward error stays well below one.  While refining with one, the contraction of LSQR's
 // estimate is watched: from iteration STALL_START on, every STALL_SPAN iterations the
 // iterations still needed at the observed rate are projected, and when they exceed what is
@@ -709,7 +711,8 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     if (ib <= 0 || ib > nb) return arg_error(4, "ib must be in 1..nb");
     if (m % nb || n % nb) return arg_error(1, "m and n must be multiples of nb");
     if (lda < m) return arg_error(6, "lda must be >= m");
-    if (prec != 'D' && prec != 'S' && prec != 'H' && prec != 'B') return arg_error(9, "prec must be 'D', 'S', 'H' or 'B'");
+    if (prec != 'D' && prec != 'S' && prec != 'H' && prec != 'B' && prec != 'X')
+        return arg_error(9, "prec must be 'D', 'S', 'H', 'B' or 'X'");
     if (prec != 'D' && sizeof(T) != 8) return arg_error(9, "prec '%c' needs Float64 data", prec);
     if (maxit < 0) return arg_error(10, "maxit must be non-negative");
     if (iters) *iters = 0;
@@ -875,7 +878,7 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
         return 0;
     };
     bool stalled = false;
-    NLA_ST(lsqr(maxit, prec == 'H' || prec == 'B', &stalled));
+    NLA_ST(lsqr(maxit, prec == 'H' || prec == 'B' || prec == 'X', &stalled));
     if (stalled && done < maxit) {
         // the 16-bit factor does not precondition A well enough: Float32 updates instead,
         // from that factor's own direct solution (a restart from the stalled iterate leaves

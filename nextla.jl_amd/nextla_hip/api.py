@@ -576,8 +576,9 @@ def rectrsm_mixed_(side, uplo, transpose, alpha, A, B, cutoff=0, maxit=5, tol=1e
 
 def tile_geqrf_mixed(A, nb, ib, prec="S"):
     """tile_geqrf of a Float64 A with the trailing updates on MFMA with Float32 accumulation
-    and Float32 ('S'), FP16 ('H') or BF16 ('B') operands; the panels, Householder
-    norms/scaling and T stay Float64.  Returns T."""
+    and Float32 ('S'), FP16 ('H'), BF16 ('B') or split-BF16 ('X': each operand as two BF16
+    parts, three products) operands; the panels, Householder norms/scaling and T stay
+    Float64.  Returns T."""
     if A.dtype != torch.float64:
         raise ArgumentError("tile_geqrf_mixed needs Float64 data")
     m, n = A.shape
@@ -589,8 +590,8 @@ def tile_geqrf_mixed(A, nb, ib, prec="S"):
 
 
 def tile_gels(A, b, nb=128, ib=32, prec="D", maxit=100, tol=1e-15, info=None):
-    """Least-squares solution of min ||A x - b||: tile QR (prec 'S' / 'H' / 'B', Float64
-    only: Float32 / FP16 / BF16 trailing updates); the direct solution x0 = R^-1 (Q^T b)[0:n]
+    """Least-squares solution of min ||A x - b||: tile QR (prec 'S' / 'H' / 'B' / 'X', Float64
+    only: Float32 / FP16 / BF16 / split-BF16 trailing updates); the direct solution x0 = R^-1 (Q^T b)[0:n]
     (returned for maxit = 0; prec 'D' with maxit > 0 starts from R^-1 R^-T A^T b instead),
     corrected by R-preconditioned LSQR with products by A in the working precision.  A 16-bit
     factor whose refinement stalls is replaced by a Float32-update factor (the library does

@@ -128,7 +128,7 @@ def _eta(A, b, x):
 
 
 @pytest.mark.parametrize("kappa", [1e2, 1e6])
-@pytest.mark.parametrize("prec", ["D", "S", "H", "B"])
+@pytest.mark.parametrize("prec", ["D", "S", "H", "B", "X"])
 def test_config_least_squares_32768x8192(ls_problems, kappa, prec):
     P = ls_problems(kappa)
     A, b = P["A"], P["b"]

@@ -80,12 +80,13 @@ def test_ormqr_single_vector_vs_oracle(m, n, nb, ib, dt, tol):
     assert abs(np.linalg.norm(x.astype(np.float64)) / np.linalg.norm(x0.astype(np.float64)) - 1) < 10 * tol
 
 
-# backward error bounds of the update precisions: Float32 / FP16 / BF16 unit roundoff
-# (6e-8 / 4.9e-4 / 3.9e-3) times a modest growth factor, and well above the Float64 path
-MIXED_BERR = {"S": (1e-9, 2e-5), "H": (1e-6, 2e-2), "B": (1e-5, 1e-1)}
+# backward error bounds of the update precisions: Float32 / FP16 / BF16 / split-BF16 unit
+# roundoff (6e-8 / 4.9e-4 / 3.9e-3 / 2^-17 = 7.6e-6) times a modest growth factor, and well above
+# the Float64 path
+MIXED_BERR = {"S": (1e-9, 2e-5), "H": (1e-6, 2e-2), "B": (1e-5, 1e-1), "X": (1e-7, 2e-3)}
 
 
-@pytest.mark.parametrize("prec", ["S", "H", "B"])
+@pytest.mark.parametrize("prec", ["S", "H", "B", "X"])
 @pytest.mark.parametrize("m,n,nb", [(1024, 512, 128), (512, 512, 64), (2048, 256, 128), (1152, 640, 128), (640, 1152, 128),
                                     (576, 448, 64)])
 def test_tile_geqrf_mixed_backward_error(m, n, nb, prec):
@@ -114,7 +115,7 @@ def test_tile_geqrf_mixed_backward_error(m, n, nb, prec):
     assert lo < d < 10 * hi, d
 
 
-@pytest.mark.parametrize("prec", ["S", "H", "B"])
+@pytest.mark.parametrize("prec", ["S", "H", "B", "X"])
 def test_tile_geqrf_mixed_large_backward_error(prec):
     """The benchmark size (16384^2, 128 tile rows and columns): every strip goes through the
     Float32 shadow for up to 127 steps, with the launch's full concurrency.  A transport
@@ -138,13 +139,13 @@ def test_tile_geqrf_mixed_large_backward_error(prec):
 
 
 def test_tile_geqrf_mixed_precisions_order():
-    """Float32 updates are more accurate than FP16, FP16 more than BF16 (unit roundoffs
-    6e-8 < 4.9e-4 < 3.9e-3), on the same matrix."""
+    """Float32 updates are more accurate than split BF16, split BF16 than FP16, FP16 than BF16
+    (unit roundoffs 6e-8 < 7.6e-6 < 4.9e-4 < 3.9e-3), on the same matrix."""
     nl = _nl()
     rng = np.random.default_rng(10)
     A0 = np.asfortranarray(rng.standard_normal((1024, 1024)))
     errs = {}
-    for prec in ("S", "H", "B"):
+    for prec in ("S", "X", "H", "B"):
         dA = nl.to_device(A0)
         dT = nl.tile_geqrf_mixed(dA, 128, 32, prec)
         QR = nl.zeros_cm(1024, 1024)
@@ -152,14 +153,14 @@ def test_tile_geqrf_mixed_precisions_order():
         nl.tile_ormqr("N", dA, dT, QR, 128, 32)
         torch.cuda.synchronize()
         errs[prec] = relerr(nl.to_host(QR), A0)
-    assert errs["S"] < errs["H"] / 100 < errs["B"] / 100, errs
+    assert errs["S"] < errs["X"] / 4 < errs["H"] / 100 < errs["B"] / 100, errs
 
 
 @pytest.mark.parametrize("prec,kappa", [("D", 1e2), ("D", 1e6), ("S", 1e2), ("S", 1e6), ("H", 1e2), ("B", 1e2),
-                                        ("H", 1e6), ("B", 1e6)])
+                                        ("H", 1e6), ("B", 1e6), ("X", 1e2), ("X", 1e6)])
 def test_gels_refinement_vs_lapack(prec, kappa):
     nl = _nl()
-    rng = np.random.default_rng(int(kappa) % 1000 + "DSHB".index(prec))
+    rng = np.random.default_rng(int(kappa) % 1000 + "DSHBX".index(prec))
     m, n, nb = 2048, 512, 128
     A, b = ls_problem(rng, m, n, kappa)
     xref = np.linalg.lstsq(A, b, rcond=None)[0]
@@ -194,7 +195,7 @@ def test_gels_argument_errors():
     A = nl.zeros_cm(256, 128)
     b = torch.zeros(256, dtype=torch.float64, device="cuda")
     with pytest.raises(nl.ArgumentError, match="prec must be"):
-        nl.tile_gels(A, b, 128, 32, "X")
+        nl.tile_gels(A, b, 128, 32, "Q")
     with pytest.raises(nl.ArgumentError, match="needs Float64"):
         nl.tile_gels(nl.zeros_cm(256, 128, dtype=torch.float32), b.float(), 128, 32, "H")
     with pytest.raises(nl.ArgumentError, match="Float64"):
