@@ -436,20 +436,53 @@ template <typename T> __global__ __launch_bounds__(64) void diag_block_inv_kerne
     for (int i = 0; i < BS; ++i) Dinv[(i64)b * BS * BS + i + (i64)j * BS] = Y[i][j];
 }
 
-// Blocks finish in claim order (each depends on every block claimed before it), so one
-// counter `fin` (blocks finished) replaces per-block flags: block d of a claim is ready when
-// fin > d.  Thread 0 keeps the last value read; only when the loop reaches a block beyond it
-// does the workgroup wait (thread 0 polls, a barrier shares the new value), so the products
-// with long-finished blocks run without flag reads or barriers.
+// The solved values are published through a second vector y that starts out filled with a NaN
+// of a fixed payload (arithmetic on numbers yields the default quiet NaN, never this one): a
+// block's values are ready when none of its words reads as the mark.  A 64-bit (32-bit) word
+// is stored and loaded whole, so no flag, no store-completion wait before it and no second
+// round trip after it are needed between the workgroup that solves a block and the ones that
+// use it.
+template <typename T> struct TrsvMark;
+template <> struct TrsvMark<double> {
+    using U = unsigned long long;
+    static constexpr U bits = 0x7ff4a5a5a5a5a5a5ull;
+};
+template <> struct TrsvMark<float> {
+    using U = unsigned;
+    static constexpr U bits = 0x7fa5a5a5u;
+};
+template <typename T> __device__ __forceinline__ bool trsv_unset(T v) {
+    return __builtin_bit_cast(typename TrsvMark<T>::U, v) == TrsvMark<T>::bits;
+}
+
+// Blocks are claimed in dependency order (each depends on every block claimed before it); a
+// workgroup accumulates the products of its block row with the solved blocks as they appear,
+// every thread polling the words of y it uses (bounded: a timeout or another workgroup's
+// error raises the error flag and the launch runs to its end), then solves its diagonal block
+// with the precomputed inverse and writes the block to y (for the later blocks) and to x.
+// x_b is read only by block b, so the result overwrites it in place.
 template <typename T, bool TRANS>
-__global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 lda, const T *Dinv, T *x, int *counter,
-                                                          int *fin, int *err) {
+__global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 lda, const T *Dinv, T *x, T *y,
+                                                          int *counter, int *err) {
     constexpr int BS = TRSV_BS;
     __shared__ T red[4][BS + 1];
     __shared__ T ys[BS];
     __shared__ T tr[TRANS ? 4 * 16 * 64 : 1];  // TRANS: the lanes' partial sums, transposed
-    __shared__ int sh_b, sh_known[2];
+    __shared__ int sh_b;
     const int nblk = n / BS, t = threadIdx.x, lane = t & 63, g = t >> 6;
+    auto solved = [&](const T *p) -> T {
+        T val = ld_sc1(p);
+        if (!trsv_unset(val)) return val;
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        while (trsv_unset(val = ld_sc1(p))) {
+            if (ld_sc1(err) || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                st_sc1(err, 1);
+                return T(0);
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        return val;
+    };
     for (;;) {
         __syncthreads();
         if (t == 0) sh_b = atomicAdd(counter, 1);
@@ -461,57 +494,57 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
         T acc[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[q] = 0;
-        const int nc = claim;  // finished blocks this one depends on (claims 0 .. claim-1)
-        int known = 0, par = 0;
-        bool stop = false;
-        for (int d = 0; d < nc; ++d) {
-            const int c = TRANS ? d : nblk - 1 - d;
-            const i64 c0 = (i64)c * BS;
-            // this tile of R does not depend on the counter: load it first
-            T rv[16];
-            if (!TRANS) {
-                const T *Rc = R + r0 + lane + (c0 + g * 16) * lda;  // R[r0 + lane][c0 + g*16 + q]
-#pragma unroll
-                for (int q = 0; q < 16; ++q) rv[q] = Rc[q * lda];
+        // TPS tiles per step, all their loads in flight together (past the end a tile is the
+        // last one again, with weight 0); !TRANS: lane l < 16 of a wave loads the wave's x value
+        // l and the products take it by readlane.  Two tiles: 0.62 / 0.80 ms (plain /
+        // transposed) at n = 16384 against 0.78 / 0.92 with one; four were slower again.
+        constexpr int TPS = 2;
+        auto bcast = [](T v, int l) {
+            if constexpr (sizeof(T) == 8) {
+                const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+                const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l), hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+                return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
             } else {
-                const T *Rc = R + c0 + lane + (r0 + g * 16) * lda;  // R[c0 + lane][r0 + g*16 + q]
-#pragma unroll
-                for (int q = 0; q < 16; ++q) rv[q] = Rc[q * lda];
+                return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(unsigned, v), l));
             }
-            if (d >= known) {  // workgroup-uniform
-                if (t == 0) {
-                    int f = ld_sc1(fin);
-                    int st = 0;
-                    if (f <= d) {
-                        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-                        while ((f = ld_sc1(fin)) <= d) {
-                            if (ld_sc1(err)) { st = 1; break; }
-                            if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
-                                st_sc1(err, 1);
-                                st = 1;
-                                break;
-                            }
-                            __builtin_amdgcn_s_sleep(1);
-                        }
-                    }
-                    sh_known[par] = st ? -1 : f;
+        };
+        const int xo = TRANS ? lane : g * 16 + (lane & 15);
+        for (int d = 0; d < claim; d += TPS) {
+            i64 c0[TPS];
+            T rv[TPS][16], xv[TPS];
+#pragma unroll
+            for (int u = 0; u < TPS; ++u) {
+                const int du = d + u < claim ? d + u : claim - 1;
+                c0[u] = (i64)(TRANS ? du : nblk - 1 - du) * BS;
+                const T *Rc = TRANS ? R + c0[u] + lane + (r0 + g * 16) * lda : R + r0 + lane + (c0[u] + g * 16) * lda;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) rv[u][q] = Rc[q * lda];
+            }
+            bool unset = false;
+#pragma unroll
+            for (int u = 0; u < TPS; ++u) {
+                xv[u] = ld_sc1(&y[c0[u] + xo]);
+                unset = unset || trsv_unset(xv[u]);
+            }
+            if (unset) {
+#pragma unroll
+                for (int u = 0; u < TPS; ++u) xv[u] = solved(&y[c0[u] + xo]);
+            }
+#pragma unroll
+            for (int u = 1; u < TPS; ++u)
+                if (d + u >= claim) xv[u] = T(0);
+#pragma unroll
+            for (int u = 0; u < TPS; ++u) {
+                if (!TRANS) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) acc[0] += rv[u][q] * bcast(xv[u], q);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) acc[q] += rv[u][q] * xv[u];
                 }
-                __syncthreads();
-                known = sh_known[par];  // alternate slots: a wave may still read the last one
-                par ^= 1;
-                if (known < 0) { stop = true; break; }
-            }
-            if (!TRANS) {
-#pragma unroll
-                for (int q = 0; q < 16; ++q) acc[0] += rv[q] * ld_sc1(&x[c0 + g * 16 + q]);
-            } else {
-                const T xv = ld_sc1(&x[c0 + lane]);
-#pragma unroll
-                for (int q = 0; q < 16; ++q) acc[q] += rv[q] * xv;
             }
         }
-        if (stop) break;
-        // the diagonal block's inverse (also independent of the counter)
+        // the diagonal block's inverse (independent of the other blocks)
         const T *D = Dinv + (i64)b * BS * BS;
         T dv[16];
 #pragma unroll
@@ -544,7 +577,7 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
             __syncthreads();
         }
         if (g == 0) {
-            // y = v_b - sum of the finished blocks' products (v_b not yet overwritten)
+            // y = v_b - sum of the solved blocks' products
             const T sum = TRANS ? red[0][lane] : (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
             ys[lane] = x[r0 + lane] - sum;
         }
@@ -556,33 +589,33 @@ __global__ __launch_bounds__(256) void trsv_upper_kernel(int n, const T *R, i64 
         red[g][lane] = p;
         __syncthreads();
         if (g == 0) {
-            st_sc1(&x[r0 + lane], (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            const T xb = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+            st_sc1(&y[r0 + lane], xb);
+            x[r0 + lane] = xb;
         }
-        __syncthreads();
-        if (t == 0) st_sc1(fin, claim + 1);
     }
 }
-template <typename T> __global__ void trsv_reset_kernel(int *counter, int *fin) {
-    if (threadIdx.x == 0) {
-        *counter = 0;
-        *fin = 0;
-    }
+// y := the mark (n words), the claim counter := 0
+template <typename T> __global__ void trsv_reset_kernel(T *y, int n, int *counter) {
+    const T mark = __builtin_bit_cast(T, TrsvMark<T>::bits);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = mark;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
 }
-// x := R^-1 x or R^-T x (R upper n x n at R, lda; Dinv from diag_block_inv_kernel); sync =
-// the counter, an error flag and nblk flags (device ints, n/64 + 2 of them)
+// x := R^-1 x or R^-T x (R upper n x n at R, lda; Dinv from diag_block_inv_kernel); y: n words
+// of device scratch; sync: the claim counter and the error flag (device ints)
 template <typename T>
-static hipError_t trsv_upper(bool trans, int n, const T *R, i64 lda, const T *Dinv, T *x, int *sync, hipStream_t s) {
+static hipError_t trsv_upper(bool trans, int n, const T *R, i64 lda, const T *Dinv, T *x, T *y, int *sync,
+                             hipStream_t s) {
     const int nblk = n / TRSV_BS;
-    trsv_reset_kernel<T><<<1, 64, 0, s>>>(sync, sync + 2);
+    trsv_reset_kernel<T><<<(n + 255) / 256 < 256 ? (n + 255) / 256 : 256, 256, 0, s>>>(y, n, sync);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     int dev = 0, ncu = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
     const int grid = std::min(nblk, ncu);
-    if (trans) trsv_upper_kernel<T, true><<<grid, 256, 0, s>>>(n, R, lda, Dinv, x, sync, sync + 2, sync + 1);
-    else trsv_upper_kernel<T, false><<<grid, 256, 0, s>>>(n, R, lda, Dinv, x, sync, sync + 2, sync + 1);
+    if (trans) trsv_upper_kernel<T, true><<<grid, 256, 0, s>>>(n, R, lda, Dinv, x, y, sync, sync + 1);
+    else trsv_upper_kernel<T, false><<<grid, 256, 0, s>>>(n, R, lda, Dinv, x, y, sync, sync + 1);
     return hipGetLastError();
 }
 
@@ -721,27 +754,26 @@ static int tile_gels_entry(i64 m, i64 n, i64 nb, i64 ib, const T *A, i64 lda, co
     if (m == 0 || n == 0) return 0;
     hipStream_t s = as_stream(stream);
     const i64 mt = m / nb, ldt = mt * ib;
-    constexpr i64 W = 64;  // right-hand side block width of the Q^T walkers
     const int rb = (int)((m + 511) / 512);
     const int nsplit = std::max(1, std::min(32, 2048 / rb));
     const int gm = (int)((m + 255) / 256), gn = (int)((n + 255) / 256);
     int st = 0;
-    const size_t ne = (size_t)m * n + (size_t)ldt * n + (size_t)m * W + (size_t)m * (1 + nsplit) + 5 * (size_t)n;
+    const size_t ne = (size_t)m * n + (size_t)ldt * n + (size_t)m + (size_t)m * (1 + nsplit) + 6 * (size_t)n;
     const size_t nd = (size_t)std::max(gm, gn) + 8;
     const size_t nsync = (size_t)n / TRSV_BS + 2;
     const size_t ndinv = n % TRSV_BS == 0 ? (size_t)n * TRSV_BS : 0;
     char *base = (char *)scratch(SLOT_LS, sizeof(T) * (ne + ndinv) + sizeof(double) * nd + sizeof(int) * (nsync + 2), s, &st);
     if (!base) return st;
-    T *AF = (T *)base, *Tm = AF + (size_t)m * n, *Cb = Tm + (size_t)ldt * n, *u = Cb + (size_t)m * W;
-    T *part = u + m, *v = part + (size_t)m * nsplit, *w = v + n, *y = w + n, *t = y + n, *q = t + n;
-    double *bsum = (double *)(q + n), *dnorm = bsum + nd - 8;
+    T *AF = (T *)base, *Tm = AF + (size_t)m * n, *Cb = Tm + (size_t)ldt * n, *u = Cb + (size_t)m;
+    T *part = u + m, *v = part + (size_t)m * nsplit, *w = v + n, *y = w + n, *t = y + n, *q = t + n, *ty = q + n;
+    double *bsum = (double *)(ty + n), *dnorm = bsum + nd - 8;
     int *sync = (int *)(bsum + nd);
     T *Dinv = (T *)(sync + nsync + (nsync & 1));  // 8-byte aligned
     NLA_HIP(hipMemsetAsync(sync, 0, sizeof(int) * nsync, s));
     // x := R^-1 x / R^-T x with R the upper triangle of the factor
     auto rsolve = [&](bool trans, T *vec) -> int {
         if (n % TRSV_BS == 0) {
-            const hipError_t e = trsv_upper<T>(trans, (int)n, AF, m, Dinv, vec, sync, s);
+            const hipError_t e = trsv_upper<T>(trans, (int)n, AF, m, Dinv, vec, ty, sync, s);
             return e == hipSuccess ? 0 : hip_error(e, "trsv_upper");
         }
         return rectrxm_async<T>('L', 'U', trans ? 'T' : 'N', T(1), 'S', n, 1, AF, m, vec, n, 0, s);
