@@ -269,6 +269,216 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb,
     }
 }
 
+// The leaf for tall panels (more rows than the one-workgroup register leaf takes): P
+// workgroups of 256 threads, workgroup g holding rows g RW .. g RW + RW - 1 (RW = 256 LU_CQ) in
+// registers for all the leaf's columns.  Per column: each workgroup takes its candidate as
+// lu_leaf_rows_kernel does and publishes it, with the candidate's row (and workgroup 0 row j),
+// in a slot; an arrival counter per column (arrive[base + j], cleared by the entry point)
+// gathers the P workgroups; then every workgroup reads the P candidates, takes the same pivot
+// (the largest, ties to the smaller row: idamax over the whole column) and applies the swap,
+// the scaling and the rank-1 update to its rows, as the one-workgroup leaves do, so the factors
+// are the same.  One exchange between the workgroups per column replaces a pass of the panel
+// through one CU's memory path.  P <= the CU count, so the P workgroups are resident together;
+// the wait is bounded (2 s, then the error flag, after which nothing waits).
+// Wave 0 of a workgroup: the kb interchanges of a leaf (row base + k with row piv(k), absolute,
+// in order) composed into a permutation of the rows involved: afterwards rows[i] must receive
+// the old content of row rows[src[i]], i < *nrows (at most 2 kb rows).  Lane j holds slot j (its
+// row and the slot whose old value it now holds); slots 0..kb-1 are rows base..base+kb-1, later
+// slots the pivot rows below.
+template <typename F>
+__device__ __forceinline__ void lu_compose_swaps(int kb, i64 base, F piv, i64 *rows, int *src, int *nrows) {
+    const int t = threadIdx.x;
+    i64 row = t < kb ? base + t : -1;
+    int content = t;
+    int n = kb;
+    for (int k = 0; k < kb; ++k) {
+        const i64 r = piv(k);
+        int b;
+        if (r < base + kb) {
+            b = (int)(r - base);
+        } else {
+            const unsigned long long hit = __ballot(row == r);
+            if (hit) {
+                b = __ffsll((long long)hit) - 1;
+            } else {
+                b = n++;
+                if (t == b) row = r;
+            }
+        }
+        const int ca = __shfl(content, k, 64), cb = __shfl(content, b, 64);
+        if (t == k) content = cb;
+        else if (t == b) content = ca;
+    }
+    if (t < n) {
+        rows[t] = row;
+        src[t] = content;
+    }
+    if (t == 0) *nrows = n;
+}
+// One column a of the matrix: the composed interchanges, every load before any store.
+template <typename T> __device__ __forceinline__ void lu_swap_column(T *a, const i64 *rows, const int *src, int n) {
+    T v[2 * LU_LEAF];
+#pragma unroll
+    for (int i = 0; i < 2 * LU_LEAF; ++i)
+        if (i < n) v[i] = a[rows[src[i]]];
+#pragma unroll
+    for (int i = 0; i < 2 * LU_LEAF; ++i)
+        if (i < n) a[rows[i]] = v[i];
+}
+
+// device-coherent (agent scope) loads and stores of the slots and counters: they bypass the
+// non-coherent caches, so a value written on one XCD is read correctly on another
+template <typename T> __device__ __forceinline__ T lu_ld(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ void lu_st(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// f(integral_constant<int, J>) for J = 0, 1, ... in order
+template <typename F, int... J> __device__ __forceinline__ void lu_for_cols(F &&f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>()), ...);
+}
+constexpr int LU_CQ = 1;   // rows per thread (8192 rows: 32 workgroups; 2: +2 ms, 4: +6.5 ms per 8192^2 LU)
+constexpr int LU_CRW = 256 * LU_CQ;   // rows per workgroup
+constexpr int LU_CSL = 2 + LU_LEAF;   // slot: |candidate|, its row, the row's values
+template <typename T>
+__global__ __launch_bounds__(256) void lu_leaf_coop_kernel(int m, int nb, T *__restrict__ A, i64 lda,
+                                                           int64_t *__restrict__ ipiv, i64 base,
+                                                           unsigned long long *info, T *slots, int *arrive, int *err) {
+    const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    __shared__ T cval[4];
+    __shared__ int cidx[4];
+    __shared__ T prow_s[LU_LEAF], jrow_s[LU_LEAF];
+    __shared__ T gb_s;
+    __shared__ int gp_s;
+    T a[LU_CQ][LU_LEAF];
+#pragma unroll
+    for (int q = 0; q < LU_CQ; ++q) {
+        const int r = g * LU_CRW + q * 256 + t;
+#pragma unroll
+        for (int c = 0; c < LU_LEAF; ++c) a[q][c] = r < m && c < nb ? A[r + (i64)c * lda] : T(0);
+    }
+    T *rowj = slots + (i64)LU_LEAF * P * LU_CSL;  // row j of each column (workgroup 0 holds rows 0..7)
+    // one column; a lambda over a constant column index so that every register index is constant
+    auto column = [&](auto J) __attribute__((always_inline)) {
+        constexpr int j = decltype(J)::value;
+        if (j >= nb) return;
+        T best = T(-1);
+        int bi = m;
+#pragma unroll
+        for (int q = 0; q < LU_CQ; ++q) {
+            const int r = g * LU_CRW + q * 256 + t;
+            if (r >= j && r < m) {
+                const T v = fabs(a[q][j]);
+                if (v > best) { best = v; bi = r; }
+            }
+        }
+        T wb = best;
+        int wi = bi;
+        lu_wave_argmax(wb, wi);
+        if (lane == 0) {
+            cval[w] = wb;
+            cidx[w] = wi;
+        }
+        __syncthreads();
+        T b = cval[0];
+        int p = cidx[0];
+#pragma unroll
+        for (int i = 1; i < 4; ++i) lu_pick(b, p, cval[i], cidx[i]);
+        T *S = slots + ((i64)j * P + g) * LU_CSL;
+        if (t == 0) {
+            lu_st(&S[0], b);
+            lu_st(&S[1], (T)p);
+        }
+#pragma unroll
+        for (int q = 0; q < LU_CQ; ++q)  // the thread holding the workgroup's candidate row
+            if (g * LU_CRW + q * 256 + t == p) {
+#pragma unroll
+                for (int c = 0; c < LU_LEAF; ++c) lu_st(&S[2 + c], a[q][c]);
+            }
+        if (g == 0 && t == j) {
+#pragma unroll
+            for (int c = 0; c < LU_LEAF; ++c) lu_st(&rowj[j * LU_LEAF + c], a[0][c]);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (t == 0) {
+            int *cnt = &arrive[base + j];
+            __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lu_ld(cnt) < P) {
+                const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+                while (lu_ld(cnt) < P) {
+                    if (lu_ld(err) || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                        lu_st(err, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        __syncthreads();
+        if (w == 0) {  // the pivot over the P candidates, its row and row j
+            T cb = T(-1);
+            int cp = m;
+            for (int i = lane; i < P; i += 64) {
+                const T *Si = slots + ((i64)j * P + i) * LU_CSL;
+                lu_pick(cb, cp, lu_ld(&Si[0]), (int)lu_ld(&Si[1]));
+            }
+            lu_wave_argmax(cb, cp);
+            if (lane < LU_LEAF) {
+                prow_s[lane] = cp < m ? lu_ld(&slots[((i64)j * P + cp / LU_CRW) * LU_CSL + 2 + lane]) : T(0);
+                jrow_s[lane] = lu_ld(&rowj[j * LU_LEAF + lane]);
+            }
+            if (lane == 0) {
+                gb_s = cb;
+                gp_s = cp;
+            }
+        }
+        __syncthreads();
+        const T gb = gb_s;
+        const int gp = gp_s;
+        if (g == 0 && t == 0) ipiv[base + j] = base + gp + 1;
+        if (gb == T(0)) {
+            // exact zero pivot (the whole column below the diagonal is zero): no swap, no scaling
+            if (g == 0 && t == 0) atomicMin(info, (unsigned long long)(base + j + 1));
+            return;
+        }
+        T prow[LU_LEAF];
+#pragma unroll
+        for (int c = 0; c < LU_LEAF; ++c) prow[c] = prow_s[c];
+        const T d = prow[j];
+        const bool recip = fabs(d) >= lu_sfmin<T>();   // src/lu.jl:238-242
+        const T rd = T(1) / d;
+#pragma unroll
+        for (int q = 0; q < LU_CQ; ++q) {
+            const int r = g * LU_CRW + q * 256 + t;
+            if (gp != j) {
+                if (r == j) {
+#pragma unroll
+                    for (int c = 0; c < LU_LEAF; ++c) a[q][c] = prow[c];
+                } else if (r == gp) {
+#pragma unroll
+                    for (int c = 0; c < LU_LEAF; ++c) a[q][c] = jrow_s[c];
+                }
+            }
+            if (r > j && r < m) {
+                const T l = recip ? a[q][j] * rd : a[q][j] / d;
+                a[q][j] = l;
+#pragma unroll
+                for (int c = j + 1; c < LU_LEAF; ++c) a[q][c] -= l * prow[c];
+            }
+        }
+    };
+    lu_for_cols(column, std::make_integer_sequence<int, LU_LEAF>());
+#pragma unroll
+    for (int q = 0; q < LU_CQ; ++q) {
+        const int r = g * LU_CRW + q * 256 + t;
+#pragma unroll
+        for (int c = 0; c < LU_LEAF; ++c)
+            if (r < m && c < nb) A[r + (i64)c * lda] = a[q][c];
+    }
+}
+
 // Row interchanges k = k1..k2 (absolute, 1-based; reverse order for inc < 0) with ipiv(k) on
 // ncols columns of the block at A whose row 0 is row `base` of the matrix: thread per column.
 template <typename T>
@@ -302,49 +512,12 @@ __global__ __launch_bounds__(256) void lu_leaf_swaps_kernel(int ncols, T *__rest
     __shared__ int src[2 * LU_LEAF];    // final content of rows[i] = old row rows[src[i]]
     __shared__ int nrows;
     const int t = threadIdx.x;
-    if (t < 64) {
-        // wave 0 composes the swaps: lane j holds slot j (its row and the slot whose old value it
-        // now holds); slots 0..kb-1 are rows base..base+kb-1, later slots the pivot rows below
-        i64 row = t < kb ? base + t : -1;
-        int content = t;
-        int n = kb;
-        for (int k = 0; k < kb; ++k) {
-            const i64 r = ipiv[base + k] - 1;
-            int b;
-            if (r < base + kb) {
-                b = (int)(r - base);
-            } else {
-                const unsigned long long hit = __ballot(row == r);
-                if (hit) {
-                    b = __ffsll((long long)hit) - 1;
-                } else {
-                    b = n++;
-                    if (t == b) row = r;
-                }
-            }
-            const int ca = __shfl(content, k, 64), cb = __shfl(content, b, 64);
-            if (t == k) content = cb;
-            else if (t == b) content = ca;
-        }
-        if (t < n) {
-            rows[t] = row;
-            src[t] = content;
-        }
-        if (t == 0) nrows = n;
-    }
+    if (t < 64) lu_compose_swaps(kb, base, [&](int k) { return (i64)(ipiv[base + k] - 1); }, rows, src, &nrows);
     __syncthreads();
     int c = blockIdx.x * 256 + t;
     if (c >= ncols - (skip1 - skip0)) return;
     if (c >= skip0) c += skip1 - skip0;
-    T *a = A + (i64)c * lda;
-    const int n = nrows;
-    T v[2 * LU_LEAF];
-#pragma unroll
-    for (int i = 0; i < 2 * LU_LEAF; ++i)
-        if (i < n) v[i] = a[rows[src[i]]];   // every load before any store
-#pragma unroll
-    for (int i = 0; i < 2 * LU_LEAF; ++i)
-        if (i < n) a[rows[i]] = v[i];
+    lu_swap_column(A + (i64)c * lda, rows, src, nrows);
 }
 
 // A12 := L11^-1 A12 for the recursion's small nodes (n1 <= N1 <= 32): L11 the unit lower
@@ -392,6 +565,9 @@ __global__ void unit_lower_copy_kernel(int n, const T *__restrict__ A, i64 lda, 
 }
 
 __global__ void lu_info_init_kernel(unsigned long long *info) { *info = LU_NO_ZERO; }
+__global__ void lu_int_fill_kernel(int *p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
 __global__ void lu_info_finish_kernel(const unsigned long long *flag, int64_t *info) {
     *info = *flag == LU_NO_ZERO ? 0 : (int64_t)*flag;
 }
@@ -409,17 +585,32 @@ static hipError_t laswp_launch(i64 ncols, T *A, i64 lda, i64 k1, i64 k2, const i
 // columns) as soon as the leaf is factored: every column then sees every interchange once, in
 // order, as after the recursion's own laswp steps (src/lu.jl:263-293), which are therefore not
 // repeated.
+// Device state of the multi-workgroup leaves: the candidate slots, one arrival counter per
+// column (cleared by the entry point), the error flag of their bounded wait; ncu caps the
+// workgroups of one leaf; used is set once such a leaf has been launched.
+template <typename T> struct LuCoop {
+    T *slots;
+    int *arrive, *err;
+    int ncu;
+    bool used;
+};
+
 template <typename T>
 static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsigned long long *flag, T *Lbuf,
-                      T *A0, i64 ncols0, hipStream_t s) {
+                      T *A0, i64 ncols0, LuCoop<T> &co, hipStream_t s) {
     if (m <= 0 || n <= 0) return 0;
     if (n <= LU_LEAF || m == 1) {
         const int kb = (int)(n < LU_LEAF ? n : LU_LEAF);
+        const i64 pw = (m + LU_CRW - 1) / LU_CRW;  // workgroups of a multi-workgroup leaf
         if (m >= kb && m <= LU_THREADS) lu_leaf_rows_kernel<T, 1><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         else if (m >= kb && m <= 2 * LU_THREADS) lu_leaf_rows_kernel<T, 2><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         else if (sizeof(T) == 4 && m >= kb && m <= 4 * LU_THREADS)  // (Float64 at four rows per thread spills)
             lu_leaf_rows_kernel<T, sizeof(T) == 4 ? 4 : 1><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
-        else lu_leaf_kernel<T><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
+        else if (m >= kb && pw <= co.ncu) {
+            lu_leaf_coop_kernel<T><<<(unsigned)pw, 256, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag, co.slots, co.arrive,
+                                                               co.err);
+            co.used = true;
+        } else lu_leaf_kernel<T><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         NLA_LAUNCH_CHECK();
         const int kpiv = (int)(m < kb ? m : kb);
         const i64 others = ncols0 - kb;
@@ -431,7 +622,7 @@ static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsi
         return 0;
     }
     const i64 mn = m < n ? m : n, n1 = mn / 2, n2 = n - n1;
-    NLA_ST(getrf2_rec<T>(m, n1, A, lda, ipiv, base, flag, Lbuf, A0, ncols0, s));
+    NLA_ST(getrf2_rec<T>(m, n1, A, lda, ipiv, base, flag, Lbuf, A0, ncols0, co, s));
     // A12 = L11^-1 A12 (L11 unit lower): in place for small n1, else the recursive TRSM on a
     // unit-diagonal copy
     if (n1 <= 32) {
@@ -449,7 +640,7 @@ static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsi
     }
     // A22 -= A21 A12
     NLA_HIP(gemm<T>('N', 'N', m - n1, n2, n1, T(-1), A + n1, lda, A + n1 * lda, lda, T(1), A + n1 + n1 * lda, lda, s));
-    NLA_ST(getrf2_rec<T>(m - n1, n2, A + n1 + n1 * lda, lda, ipiv, base + n1, flag, Lbuf, A0, ncols0, s));
+    NLA_ST(getrf2_rec<T>(m - n1, n2, A + n1 + n1 * lda, lda, ipiv, base + n1, flag, Lbuf, A0, ncols0, co, s));
     return 0;
 }
 
@@ -466,17 +657,34 @@ static int getrf2_entry(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, int64_t *inf
     if (m > INT32_MAX || n > INT32_MAX) return arg_error(1, "dimension exceeds 2^31-1");
     if (!ipiv || !info) return arg_error(5, "ipiv and info must be device pointers");
     const i64 mn = m < n ? m : n, h = mn / 2;
+    int dev = 0, ncu = 0;
+    NLA_HIP(hipGetDevice(&dev));
+    NLA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     int st = 0;
-    // the unit-lower copies of L11 (at most (min(m,n)/2)^2) and the zero-pivot flag
-    char *base = (char *)scratch(SLOT_LU, sizeof(T) * (size_t)(h * h + 1) + 64, s, &st);
+    // the zero-pivot flag, the unit-lower copies of L11 (at most (min(m,n)/2)^2), the
+    // multi-workgroup leaves' slots, arrival counters (one per column) and error flag
+    const size_t nslot = (size_t)LU_LEAF * ncu * LU_CSL + LU_LEAF * LU_LEAF;
+    const size_t lbytes = (sizeof(T) * (size_t)(h * h + 1) + 63) / 64 * 64, sbytes = (sizeof(T) * nslot + 63) / 64 * 64;
+    char *base = (char *)scratch(SLOT_LU, 64 + lbytes + sbytes + sizeof(int) * (size_t)(mn + 1), s, &st);
     if (!base) return st;
     unsigned long long *flag = reinterpret_cast<unsigned long long *>(base);
     T *Lbuf = reinterpret_cast<T *>(base + 64);
+    LuCoop<T> co{reinterpret_cast<T *>(base + 64 + lbytes), reinterpret_cast<int *>(base + 64 + lbytes + sbytes), nullptr,
+                 ncu, false};
+    co.err = co.arrive + mn;
     lu_info_init_kernel<<<1, 1, 0, s>>>(flag);
     NLA_LAUNCH_CHECK();
-    NLA_ST(getrf2_rec<T>(m, n, A, lda, ipiv, 0, flag, Lbuf, A, n, s));
+    lu_int_fill_kernel<<<(unsigned)((mn + 1 + 255) / 256 < 1024 ? (mn + 1 + 255) / 256 : 1024), 256, 0, s>>>(co.arrive, (int)(mn + 1));
+    NLA_LAUNCH_CHECK();
+    NLA_ST(getrf2_rec<T>(m, n, A, lda, ipiv, 0, flag, Lbuf, A, n, co, s));
     lu_info_finish_kernel<<<1, 1, 0, s>>>(flag, info);
     NLA_LAUNCH_CHECK();
+    if (co.used) {  // the multi-workgroup leaves' bounded wait
+        int herr = 0;
+        NLA_HIP(hipMemcpyAsync(&herr, co.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        NLA_HIP(hipStreamSynchronize(s));
+        if (herr) return wait_timeout_error("LU leaf");
+    }
     return 0;
 }
 

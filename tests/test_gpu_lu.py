@@ -20,8 +20,10 @@ def rel(x, y):
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("m,n", [(1, 1), (1, 40), (7, 1), (10, 10), (64, 40), (40, 64), (129, 129), (300, 77),
-                                 (77, 300), (513, 512)])
+                                 (77, 300), (513, 512), (3000, 24), (5000, 9), (9000, 16)])
 def test_getrf2_vs_oracle(dt, m, n):
+    """Pivots identical to the oracle's; the tall shapes take the multi-workgroup leaves (more
+    than 2048 rows in Float64, 4096 in Float32, in the first leaves)."""
     rng = np.random.default_rng(m * 1000 + n)
     A = np.asfortranarray(rng.standard_normal((m, n)).astype(dt))
     Ao = A.copy(order="F")
@@ -32,6 +34,22 @@ def test_getrf2_vs_oracle(dt, m, n):
     assert info == info_o == 0
     assert np.array_equal(ipiv.cpu().numpy(), ipiv_o)
     assert rel(nl.to_host(dA), Ao) < TOL[dt] * (10 if m > 200 else 1)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_getrf2_tall_zero_column_info(dt):
+    """A zero column in a tall panel (the multi-workgroup leaf): U(3,3) = 0 exactly, info = 3,
+    no swap and no scaling for that column, as xgetrf."""
+    rng = np.random.default_rng(17)
+    A = np.asfortranarray(rng.standard_normal((6000, 12)).astype(dt))
+    A[:, 2] = 0
+    Ao = A.copy(order="F")
+    ipiv_o, info_o = Q.getrf2(Ao)
+    dA = nl.to_device(A)
+    _, ipiv, info = nl.lu_(dA)
+    assert info == info_o == 3
+    assert np.array_equal(ipiv.cpu().numpy(), ipiv_o)
+    assert rel(nl.to_host(dA), Ao) < TOL[dt]
 
 
 def test_getrf2_zero_pivot_info():
