@@ -289,7 +289,9 @@ int nla_sgemm_acc(int64_t N, int64_t R, int64_t M, float alpha, const float *X, 
  * A (m x n) is overwritten with L (unit lower, below the diagonal) and U; ipiv (DEVICE,
  * min(m,n) entries) receives the 1-based pivot rows; info (DEVICE, one entry) receives the
  * first i with U(i,i) exactly zero (1-based), 0 if none.  The off-diagonal updates run on
- * nla_?rectrxm's TRSM and the MFMA GEMM. */
+ * nla_?rectrxm's TRSM and the MFMA GEMM.  Leaves of more than 2048 rows (Float32: 4096) run on
+ * several workgroups with a bounded wait between them; then the call synchronises the stream
+ * to read its timeout flag. */
 int nla_dgetrf2(int64_t m, int64_t n, double *A, int64_t lda, int64_t *ipiv, int64_t *info, void *stream);
 int nla_sgetrf2(int64_t m, int64_t n, float *A, int64_t lda, int64_t *ipiv, int64_t *info, void *stream);
 /* laswp(A, first, last, ipiv, incx): src/lu.jl:470-522.  Rows first..last (1-based) of the n
