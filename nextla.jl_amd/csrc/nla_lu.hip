@@ -273,8 +273,8 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb,
 // workgroups of 256 threads, workgroup g holding rows g RW .. g RW + RW - 1 (RW = 256 LU_CQ) in
 // registers for all the leaf's columns.  Per column: each workgroup takes its candidate as
 // lu_leaf_rows_kernel does and publishes it, with the candidate's row (and workgroup 0 row j),
-// in a slot; an arrival counter per column (arrive[base + j], cleared by the entry point)
-// gathers the P workgroups; then every workgroup reads the P candidates, takes the same pivot
+// in a slot; wave 0 of every workgroup polls the P slots (marked words, below) until all are
+// complete, and every workgroup takes the same pivot
 // (the largest, ties to the smaller row: idamax over the whole column) and applies the swap,
 // the scaling and the rank-1 update to its rows, as the one-workgroup leaves do, so the factors
 // are the same.  One exchange between the workgroups per column replaces a pass of the panel
@@ -341,16 +341,49 @@ template <typename F, int... J> __device__ __forceinline__ void lu_for_cols(F &&
 constexpr int LU_CQ = 1;   // rows per thread (8192 rows: 32 workgroups; 2: +2 ms, 4: +6.5 ms per 8192^2 LU)
 constexpr int LU_CRW = 256 * LU_CQ;   // rows per workgroup
 constexpr int LU_CSL = 2 + LU_LEAF;   // slot: |candidate|, its row, the row's values
+// Every slot word starts as a NaN of a fixed payload (arithmetic on numbers yields the default
+// quiet NaN, never this one): a slot is complete when none of its words reads as the mark, and
+// a word is stored and loaded whole, so the workgroups need no flag, no counter and no wait for
+// their stores' completion.  Two slot sets alternate between consecutive leaves: a leaf marks
+// the other set, which the previous leaf (finished before this launch) used and the next uses.
+template <typename T> struct LuMark;
+template <> struct LuMark<double> {
+    using U = unsigned long long;
+    static constexpr U bits = 0x7ff4a5a5a5a5a5a5ull;
+};
+template <> struct LuMark<float> {
+    using U = unsigned;
+    static constexpr U bits = 0x7fa5a5a5u;
+};
+template <typename T> __device__ __forceinline__ bool lu_unset(T v) {
+    return __builtin_bit_cast(typename LuMark<T>::U, v) == LuMark<T>::bits;
+}
+// words of one slot set for at most pmax workgroups: LU_LEAF columns x pmax slots, then row j
+// of each column
+__host__ __device__ constexpr i64 lu_slot_set(int pmax) { return (i64)LU_LEAF * pmax * LU_CSL + LU_LEAF * LU_LEAF; }
+template <typename T> __global__ void lu_mark_kernel(T *p, i64 n) {
+    const T mark = __builtin_bit_cast(T, LuMark<T>::bits);
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) p[i] = mark;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void lu_leaf_coop_kernel(int m, int nb, T *__restrict__ A, i64 lda,
                                                            int64_t *__restrict__ ipiv, i64 base,
-                                                           unsigned long long *info, T *slots, int *arrive, int *err) {
+                                                           unsigned long long *info, T *slots, int pmax, int par,
+                                                           int *err) {
     const int P = gridDim.x, g = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
     __shared__ T cval[4];
     __shared__ int cidx[4];
     __shared__ T prow_s[LU_LEAF], jrow_s[LU_LEAF];
     __shared__ T gb_s;
     __shared__ int gp_s;
+    const i64 set = lu_slot_set(pmax);
+    T *R = slots + par * set;  // this leaf's slots
+    {                          // mark the other set for the next leaf
+        T *O = slots + (1 - par) * set;
+        const T mark = __builtin_bit_cast(T, LuMark<T>::bits);
+        for (i64 i = g * 256 + t; i < set; i += (i64)P * 256) O[i] = mark;
+    }
     T a[LU_CQ][LU_LEAF];
 #pragma unroll
     for (int q = 0; q < LU_CQ; ++q) {
@@ -358,7 +391,22 @@ __global__ __launch_bounds__(256) void lu_leaf_coop_kernel(int m, int nb, T *__r
 #pragma unroll
         for (int c = 0; c < LU_LEAF; ++c) a[q][c] = r < m && c < nb ? A[r + (i64)c * lda] : T(0);
     }
-    T *rowj = slots + (i64)LU_LEAF * P * LU_CSL;  // row j of each column (workgroup 0 holds rows 0..7)
+    T *rowj = R + (i64)LU_LEAF * pmax * LU_CSL;  // row j of each column (workgroup 0 holds rows 0..7)
+    // lane-level poll: *p once it is no longer the mark (bounded: a timeout or another
+    // workgroup's error raises the error flag and the value is taken as it is)
+    auto fetch = [&](const T *p) -> T {
+        T v = lu_ld(p);
+        if (!lu_unset(v)) return v;
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        while (lu_unset(v = lu_ld(p))) {
+            if (lu_ld(err) || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+                lu_st(err, 1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        return v;
+    };
     // one column; a lambda over a constant column index so that every register index is constant
     auto column = [&](auto J) __attribute__((always_inline)) {
         constexpr int j = decltype(J)::value;
@@ -385,7 +433,7 @@ __global__ __launch_bounds__(256) void lu_leaf_coop_kernel(int m, int nb, T *__r
         int p = cidx[0];
 #pragma unroll
         for (int i = 1; i < 4; ++i) lu_pick(b, p, cval[i], cidx[i]);
-        T *S = slots + ((i64)j * P + g) * LU_CSL;
+        T *S = R + ((i64)j * pmax + g) * LU_CSL;
         if (t == 0) {
             lu_st(&S[0], b);
             lu_st(&S[1], (T)p);
@@ -400,43 +448,54 @@ __global__ __launch_bounds__(256) void lu_leaf_coop_kernel(int m, int nb, T *__r
 #pragma unroll
             for (int c = 0; c < LU_LEAF; ++c) lu_st(&rowj[j * LU_LEAF + c], a[0][c]);
         }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();
-        if (t == 0) {
-            int *cnt = &arrive[base + j];
-            __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (lu_ld(cnt) < P) {
-                const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-                while (lu_ld(cnt) < P) {
-                    if (lu_ld(err) || __builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {
+        if (w == 0) {  // the pivot over the P candidates (each lane polls its slots), its row and row j
+            T cb = T(-1), crow[LU_LEAF];
+            int cp = m;
+            T jv = lane < LU_LEAF ? lu_ld(&rowj[j * LU_LEAF + lane]) : T(0);  // in flight with the slots
+#pragma unroll
+            for (int c = 0; c < LU_LEAF; ++c) crow[c] = T(0);
+            for (int i = lane; i < P; i += 64) {
+                const T *Si = R + ((i64)j * pmax + i) * LU_CSL;
+                T sv[LU_CSL];
+                unsigned long long t0 = 0;
+                for (;;) {  // the whole slot per round trip until no word is the mark (bounded)
+#pragma unroll
+                    for (int k = 0; k < LU_CSL; ++k) sv[k] = lu_ld(&Si[k]);
+                    bool unset = false;
+#pragma unroll
+                    for (int k = 0; k < LU_CSL; ++k) unset = unset || lu_unset(sv[k]);
+                    if (!unset) break;
+                    const unsigned long long tn = __builtin_amdgcn_s_memrealtime();
+                    if (t0 == 0) t0 = tn;
+                    else if (lu_ld(err) || tn - t0 > 200000000ull) {
                         lu_st(err, 1);
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
+                }
+                const int si = (int)sv[1];
+                if (sv[0] > cb || (sv[0] == cb && si < cp)) {
+                    cb = sv[0];
+                    cp = si;
+#pragma unroll
+                    for (int c = 0; c < LU_LEAF; ++c) crow[c] = sv[2 + c];
                 }
             }
-        }
-        __syncthreads();
-        if (w == 0) {  // the pivot over the P candidates, its row and row j
-            T cb = T(-1);
-            int cp = m;
-            for (int i = lane; i < P; i += 64) {
-                const T *Si = slots + ((i64)j * P + i) * LU_CSL;
-                lu_pick(cb, cp, lu_ld(&Si[0]), (int)lu_ld(&Si[1]));
+            T gb = cb;
+            int gp = cp;
+            lu_wave_argmax(gb, gp);
+            if (cp == gp && gp < m) {
+#pragma unroll
+                for (int c = 0; c < LU_LEAF; ++c) prow_s[c] = crow[c];
             }
-            lu_wave_argmax(cb, cp);
-            if (lane < LU_LEAF) {
-                prow_s[lane] = cp < m ? lu_ld(&slots[((i64)j * P + cp / LU_CRW) * LU_CSL + 2 + lane]) : T(0);
-                jrow_s[lane] = lu_ld(&rowj[j * LU_LEAF + lane]);
-            }
+            if (lane < LU_LEAF) jrow_s[lane] = lu_unset(jv) ? fetch(&rowj[j * LU_LEAF + lane]) : jv;
             if (lane == 0) {
-                gb_s = cb;
-                gp_s = cp;
+                gb_s = gb;
+                gp_s = gp;
             }
         }
         __syncthreads();
         const T gb = gb_s;
-        const int gp = gp_s;
+        const int gp = gp_s >= j && gp_s < m ? gp_s : j;  // (out of range only after a timed-out wait)
         if (g == 0 && t == 0) ipiv[base + j] = base + gp + 1;
         if (gb == T(0)) {
             // exact zero pivot (the whole column below the diagonal is zero): no swap, no scaling
@@ -585,13 +644,13 @@ static hipError_t laswp_launch(i64 ncols, T *A, i64 lda, i64 k1, i64 k2, const i
 // columns) as soon as the leaf is factored: every column then sees every interchange once, in
 // order, as after the recursion's own laswp steps (src/lu.jl:263-293), which are therefore not
 // repeated.
-// Device state of the multi-workgroup leaves: the candidate slots, one arrival counter per
-// column (cleared by the entry point), the error flag of their bounded wait; ncu caps the
-// workgroups of one leaf; used is set once such a leaf has been launched.
+// Device state of the multi-workgroup leaves: the two candidate slot sets (marked by the entry
+// point), the error flag of their bounded wait; ncu caps the workgroups of one leaf; par is the
+// slot set of the next such leaf; used is set once one has been launched.
 template <typename T> struct LuCoop {
     T *slots;
-    int *arrive, *err;
-    int ncu;
+    int *err;
+    int ncu, par;
     bool used;
 };
 
@@ -607,8 +666,9 @@ static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsi
         else if (sizeof(T) == 4 && m >= kb && m <= 4 * LU_THREADS)  // (Float64 at four rows per thread spills)
             lu_leaf_rows_kernel<T, sizeof(T) == 4 ? 4 : 1><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         else if (m >= kb && pw <= co.ncu) {
-            lu_leaf_coop_kernel<T><<<(unsigned)pw, 256, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag, co.slots, co.arrive,
-                                                               co.err);
+            lu_leaf_coop_kernel<T><<<(unsigned)pw, 256, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag, co.slots, co.ncu,
+                                                               co.par, co.err);
+            co.par ^= 1;
             co.used = true;
         } else lu_leaf_kernel<T><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         NLA_LAUNCH_CHECK();
@@ -662,19 +722,20 @@ static int getrf2_entry(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, int64_t *inf
     NLA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     int st = 0;
     // the zero-pivot flag, the unit-lower copies of L11 (at most (min(m,n)/2)^2), the
-    // multi-workgroup leaves' slots, arrival counters (one per column) and error flag
-    const size_t nslot = (size_t)LU_LEAF * ncu * LU_CSL + LU_LEAF * LU_LEAF;
+    // multi-workgroup leaves' two slot sets and error flag
+    const size_t nslot = 2 * (size_t)lu_slot_set(ncu);
     const size_t lbytes = (sizeof(T) * (size_t)(h * h + 1) + 63) / 64 * 64, sbytes = (sizeof(T) * nslot + 63) / 64 * 64;
-    char *base = (char *)scratch(SLOT_LU, 64 + lbytes + sbytes + sizeof(int) * (size_t)(mn + 1), s, &st);
+    char *base = (char *)scratch(SLOT_LU, 64 + lbytes + sbytes + 64, s, &st);
     if (!base) return st;
     unsigned long long *flag = reinterpret_cast<unsigned long long *>(base);
     T *Lbuf = reinterpret_cast<T *>(base + 64);
-    LuCoop<T> co{reinterpret_cast<T *>(base + 64 + lbytes), reinterpret_cast<int *>(base + 64 + lbytes + sbytes), nullptr,
-                 ncu, false};
-    co.err = co.arrive + mn;
+    LuCoop<T> co{reinterpret_cast<T *>(base + 64 + lbytes), reinterpret_cast<int *>(base + 64 + lbytes + sbytes), ncu, 0,
+                 false};
     lu_info_init_kernel<<<1, 1, 0, s>>>(flag);
     NLA_LAUNCH_CHECK();
-    lu_int_fill_kernel<<<(unsigned)((mn + 1 + 255) / 256 < 1024 ? (mn + 1 + 255) / 256 : 1024), 256, 0, s>>>(co.arrive, (int)(mn + 1));
+    lu_int_fill_kernel<<<1, 64, 0, s>>>(co.err, 1);
+    NLA_LAUNCH_CHECK();
+    lu_mark_kernel<T><<<(unsigned)((nslot + 255) / 256 < 1024 ? (nslot + 255) / 256 : 1024), 256, 0, s>>>(co.slots, (i64)nslot);
     NLA_LAUNCH_CHECK();
     NLA_ST(getrf2_rec<T>(m, n, A, lda, ipiv, 0, flag, Lbuf, A, n, co, s));
     lu_info_finish_kernel<<<1, 1, 0, s>>>(flag, info);
