@@ -560,20 +560,24 @@ __global__ __launch_bounds__(256) void laswp_kernel(int ncols, T *__restrict__ A
 
 // A leaf's row interchanges applied to the matrix's other columns: rows base..base+kb-1 were
 // swapped, in order, with rows ipiv(k) - 1 (absolute).  Every workgroup forms the composed
-// permutation of the (at most 2 kb) rows involved in LDS, then each thread gathers one column's
-// values of those rows (all loads in flight together) and writes them back permuted, so a leaf
-// costs one round trip instead of kb dependent swaps per column.  Columns [skip0, skip1) (the
-// leaf's own, swapped by the leaf) are left alone.
+// permutation of the (at most 2 kb) rows involved in LDS (the kb pivots read by one load per
+// lane, not one dependent load per interchange), then each thread gathers one column's values
+// of those rows (all loads in flight together) and writes them back permuted, so a leaf costs
+// one round trip instead of kb dependent swaps per column.  One-wave workgroups spread the
+// columns' scattered row accesses over four times as many CUs as 256-thread ones.  Columns
+// [skip0, skip1) (the leaf's own, swapped by the leaf) are left alone.
+constexpr int LU_SWT = 64;   // threads (columns) per workgroup of the interchanges
 template <typename T>
-__global__ __launch_bounds__(256) void lu_leaf_swaps_kernel(int ncols, T *__restrict__ A, i64 lda, i64 base, int kb,
-                                                            const int64_t *__restrict__ ipiv, int skip0, int skip1) {
+__global__ __launch_bounds__(LU_SWT) void lu_leaf_swaps_kernel(int ncols, T *__restrict__ A, i64 lda, i64 base, int kb,
+                                                               const int64_t *__restrict__ ipiv, int skip0, int skip1) {
     __shared__ i64 rows[2 * LU_LEAF];   // distinct rows involved
     __shared__ int src[2 * LU_LEAF];    // final content of rows[i] = old row rows[src[i]]
     __shared__ int nrows;
     const int t = threadIdx.x;
-    if (t < 64) lu_compose_swaps(kb, base, [&](int k) { return (i64)(ipiv[base + k] - 1); }, rows, src, &nrows);
+    const i64 mypiv = t < kb ? (i64)(ipiv[base + t] - 1) : 0;
+    lu_compose_swaps(kb, base, [&](int k) { return (i64)__shfl((long long)mypiv, k, 64); }, rows, src, &nrows);
     __syncthreads();
-    int c = blockIdx.x * 256 + t;
+    int c = blockIdx.x * LU_SWT + t;
     if (c >= ncols - (skip1 - skip0)) return;
     if (c >= skip0) c += skip1 - skip0;
     lu_swap_column(A + (i64)c * lda, rows, src, nrows);
@@ -675,7 +679,7 @@ static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsi
         const int kpiv = (int)(m < kb ? m : kb);
         const i64 others = ncols0 - kb;
         if (others > 0 && kpiv > 0) {
-            lu_leaf_swaps_kernel<T><<<(unsigned)((others + 255) / 256), 256, 0, s>>>(
+            lu_leaf_swaps_kernel<T><<<(unsigned)((others + LU_SWT - 1) / LU_SWT), LU_SWT, 0, s>>>(
                 (int)ncols0, A0, lda, base, kpiv, ipiv, (int)base, (int)(base + kb));
             NLA_LAUNCH_CHECK();
         }
