@@ -20,10 +20,17 @@ template <typename T>
 int rectrxm_async(char side, char uplo, char trans, T alpha, char func, i64 n, i64 m, const T *A, i64 lda, T *B,
                   i64 ldb, i64 cutoff, hipStream_t s);
 
-constexpr int LU_LEAF = 8;     // widest leaf factored by one workgroup
+constexpr int LU_LEAF = 16;    // widest leaf (the multi-workgroup leaf of tall panels)
+constexpr int LU_LEAF_R = 8;   // widest leaf of the one-workgroup kernels (registers: wider spills)
 constexpr int LU_THREADS = 1024;
 static_assert(2 * LU_LEAF <= 64, "a leaf's interchanges touch at most one wave's worth of rows");
 constexpr unsigned long long LU_NO_ZERO = ~0ull;
+
+// f(integral_constant<int, J>) for J = 0, 1, ... in order: a column loop whose register
+// indices are constants (the multi-workgroup leaf's sixteen columns)
+template <typename F, int... J> __device__ __forceinline__ void lu_for_cols(F &&f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>()), ...);
+}
 
 template <typename T> __device__ __forceinline__ T lu_sfmin();
 template <> __device__ __forceinline__ double lu_sfmin<double>() { return 2.2250738585072014e-308; }
@@ -74,14 +81,14 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *_
                                                              unsigned long long *info) {
     __shared__ T sval[LU_THREADS / 64];
     __shared__ int sidx[LU_THREADS / 64];
-    __shared__ T prow[LU_LEAF];
+    __shared__ T prow[LU_LEAF_R];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int kmax = min(m, nb);
     T best = T(-1);
     int bi = m;
     bool have = false;  // best / bi hold this thread's candidate for column j
 #pragma unroll
-    for (int j = 0; j < LU_LEAF; ++j) {
+    for (int j = 0; j < LU_LEAF_R; ++j) {
         if (j >= kmax) break;
         if (!have) {
             const T *col = A + (i64)j * lda;
@@ -124,12 +131,12 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *_
         T nbest = T(-1);
         int nbi = m;
         for (int r0 = j + 1 + t; r0 < m; r0 += LU_RB * LU_THREADS) {
-            T v[LU_RB][LU_LEAF];
+            T v[LU_RB][LU_LEAF_R];
 #pragma unroll
             for (int q = 0; q < LU_RB; ++q) {
                 const int r = r0 + q * LU_THREADS;
 #pragma unroll
-                for (int c = j; c < LU_LEAF; ++c)
+                for (int c = j; c < LU_LEAF_R; ++c)
                     if (r < m && c < nb) v[q][c] = A[r + (i64)c * lda];
             }
 #pragma unroll
@@ -139,10 +146,10 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *_
                     const T l = recip ? v[q][j] * rd : v[q][j] / d;
                     v[q][j] = l;
 #pragma unroll
-                    for (int c = j + 1; c < LU_LEAF; ++c)
+                    for (int c = j + 1; c < LU_LEAF_R; ++c)
                         if (c < nb) v[q][c] -= l * prow[c];
-                    if (j + 1 < LU_LEAF && j + 1 < nb) {
-                        const T a1 = fabs(v[q][j + 1 < LU_LEAF ? j + 1 : j]);
+                    if (j + 1 < LU_LEAF_R && j + 1 < nb) {
+                        const T a1 = fabs(v[q][j + 1 < LU_LEAF_R ? j + 1 : j]);
                         if (a1 > nbest) { nbest = a1; nbi = r; }
                     }
                 }
@@ -151,7 +158,7 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_kernel(int m, int nb, T *_
             for (int q = 0; q < LU_RB; ++q) {
                 const int r = r0 + q * LU_THREADS;
 #pragma unroll
-                for (int c = j; c < LU_LEAF; ++c)
+                for (int c = j; c < LU_LEAF_R; ++c)
                     if (r < m && c < nb) A[r + (i64)c * lda] = v[q][c];
             }
         }
@@ -177,18 +184,18 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb,
     constexpr int NW = LU_THREADS / 64;
     __shared__ T cval[2][NW];
     __shared__ int cidx[2][NW];
-    __shared__ T crow[2][NW][LU_LEAF];
-    __shared__ T jrow[2][LU_LEAF];
+    __shared__ T crow[2][NW][LU_LEAF_R];
+    __shared__ T jrow[2][LU_LEAF_R];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    T a[Q][LU_LEAF];
+    T a[Q][LU_LEAF_R];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int r = t + q * LU_THREADS;
 #pragma unroll
-        for (int c = 0; c < LU_LEAF; ++c) a[q][c] = r < m && c < nb ? A[r + (i64)c * lda] : T(0);
+        for (int c = 0; c < LU_LEAF_R; ++c) a[q][c] = r < m && c < nb ? A[r + (i64)c * lda] : T(0);
     }
 #pragma unroll
-    for (int j = 0; j < LU_LEAF; ++j) {
+    for (int j = 0; j < LU_LEAF_R; ++j) {
         if (j >= nb) break;
         const int par = j & 1;
         T best = T(-1);
@@ -208,7 +215,7 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb,
             cval[par][w] = wb;
             cidx[par][w] = wi;
 #pragma unroll
-            for (int c = 0; c < LU_LEAF; ++c) {
+            for (int c = 0; c < LU_LEAF_R; ++c) {
                 T x = a[0][c];
 #pragma unroll
                 for (int q = 1; q < Q; ++q) x = bq == q ? a[q][c] : x;
@@ -220,7 +227,7 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb,
         }
         if (t == j) {
 #pragma unroll
-            for (int c = 0; c < LU_LEAF; ++c) jrow[par][c] = a[0][c];
+            for (int c = 0; c < LU_LEAF_R; ++c) jrow[par][c] = a[0][c];
         }
         __syncthreads();
         T b = cval[par][0];
@@ -234,9 +241,9 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb,
             if (t == 0) atomicMin(info, (unsigned long long)(base + j + 1));
             continue;
         }
-        T prow[LU_LEAF];
+        T prow[LU_LEAF_R];
 #pragma unroll
-        for (int c = 0; c < LU_LEAF; ++c) prow[c] = crow[par][pw][c];
+        for (int c = 0; c < LU_LEAF_R; ++c) prow[c] = crow[par][pw][c];
         const T d = prow[j];
         const bool recip = fabs(d) >= lu_sfmin<T>();   // src/lu.jl:238-242
         const T rd = T(1) / d;
@@ -246,17 +253,17 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb,
             if (p != j) {
                 if (r == j) {
 #pragma unroll
-                    for (int c = 0; c < LU_LEAF; ++c) a[q][c] = prow[c];
+                    for (int c = 0; c < LU_LEAF_R; ++c) a[q][c] = prow[c];
                 } else if (r == p) {
 #pragma unroll
-                    for (int c = 0; c < LU_LEAF; ++c) a[q][c] = jrow[par][c];
+                    for (int c = 0; c < LU_LEAF_R; ++c) a[q][c] = jrow[par][c];
                 }
             }
             if (r > j && r < m) {
                 const T l = recip ? a[q][j] * rd : a[q][j] / d;
                 a[q][j] = l;
 #pragma unroll
-                for (int c = j + 1; c < LU_LEAF; ++c) a[q][c] -= l * prow[c];
+                for (int c = j + 1; c < LU_LEAF_R; ++c) a[q][c] -= l * prow[c];
             }
         }
     }
@@ -264,7 +271,7 @@ __global__ __launch_bounds__(LU_THREADS) void lu_leaf_rows_kernel(int m, int nb,
     for (int q = 0; q < Q; ++q) {
         const int r = t + q * LU_THREADS;
 #pragma unroll
-        for (int c = 0; c < LU_LEAF; ++c)
+        for (int c = 0; c < LU_LEAF_R; ++c)
             if (r < m && c < nb) A[r + (i64)c * lda] = a[q][c];
     }
 }
@@ -334,10 +341,6 @@ template <typename T> __device__ __forceinline__ T lu_ld(const T *p) {
 template <typename T> __device__ __forceinline__ void lu_st(T *p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// f(integral_constant<int, J>) for J = 0, 1, ... in order
-template <typename F, int... J> __device__ __forceinline__ void lu_for_cols(F &&f, std::integer_sequence<int, J...>) {
-    (f(std::integral_constant<int, J>()), ...);
-}
 constexpr int LU_CQ = 1;   // rows per thread (8192 rows: 32 workgroups; 2: +2 ms, 4: +6.5 ms per 8192^2 LU)
 constexpr int LU_CRW = 256 * LU_CQ;   // rows per workgroup
 constexpr int LU_CSL = 2 + LU_LEAF;   // slot: |candidate|, its row, the row's values
@@ -391,7 +394,7 @@ __global__ __launch_bounds__(256) void lu_leaf_coop_kernel(int m, int nb, T *__r
 #pragma unroll
         for (int c = 0; c < LU_LEAF; ++c) a[q][c] = r < m && c < nb ? A[r + (i64)c * lda] : T(0);
     }
-    T *rowj = R + (i64)LU_LEAF * pmax * LU_CSL;  // row j of each column (workgroup 0 holds rows 0..7)
+    T *rowj = R + (i64)LU_LEAF * pmax * LU_CSL;  // row j of each column (workgroup 0 holds rows 0..LU_LEAF-1)
     // lane-level poll: *p once it is no longer the mark (bounded: a timeout or another
     // workgroup's error raises the error flag and the value is taken as it is)
     auto fetch = [&](const T *p) -> T {
@@ -662,9 +665,13 @@ template <typename T>
 static int getrf2_rec(i64 m, i64 n, T *A, i64 lda, int64_t *ipiv, i64 base, unsigned long long *flag, T *Lbuf,
                       T *A0, i64 ncols0, LuCoop<T> &co, hipStream_t s) {
     if (m <= 0 || n <= 0) return 0;
-    if (n <= LU_LEAF || m == 1) {
-        const int kb = (int)(n < LU_LEAF ? n : LU_LEAF);
-        const i64 pw = (m + LU_CRW - 1) / LU_CRW;  // workgroups of a multi-workgroup leaf
+    // leaf width: LU_LEAF on the multi-workgroup leaf (panels taller than the one-workgroup
+    // register leaves take), LU_LEAF_R elsewhere
+    const i64 pw = (m + LU_CRW - 1) / LU_CRW;  // workgroups of a multi-workgroup leaf
+    const i64 rmax = (sizeof(T) == 4 ? 4 : 2) * (i64)LU_THREADS;
+    const i64 lw = m > rmax && pw <= co.ncu ? LU_LEAF : LU_LEAF_R;
+    if (n <= lw || m == 1) {
+        const int kb = (int)(n < lw ? n : lw);
         if (m >= kb && m <= LU_THREADS) lu_leaf_rows_kernel<T, 1><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         else if (m >= kb && m <= 2 * LU_THREADS) lu_leaf_rows_kernel<T, 2><<<1, LU_THREADS, 0, s>>>((int)m, kb, A, lda, ipiv, base, flag);
         else if (sizeof(T) == 4 && m >= kb && m <= 4 * LU_THREADS)  // (Float64 at four rows per thread spills)
