@@ -500,8 +500,8 @@ def main():
     ap.add_argument("--sweep-cuts", default="16,32,64,128,256,512", help="cut-offs of the sweep")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: the library has no multi-GPU path "
+              "(the workloads are measured on one GPU's share); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
